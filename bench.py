@@ -1,0 +1,183 @@
+"""Throughput of the single-device Fold3dPMM solve on one MI355X.
+
+Workload (the reference's own headline configuration): the fold model
+F|grad T| + a.grad T = 1 on the reference's implicit-surface input, 512^3
+nodes, FP64, 8 iterations of the 6 directional sweeps (reference
+Stencil-OpenACC/AN_ISO_1_OpenACC.c:1187-1237).  One step = one whole solve of a
+fresh copy of the input field (the device-to-device reset copy is inside the
+timed region).  Metric: G cell-updates/s, cell-updates = 8 * 6 * (n-1) * n^2.
+
+The JSON line also carries the roofline of the dominant kernel (HBM-bound
+algorithmic bytes: read + write of the target node, 16 B per cell-update in
+FP64), an FP32 run of the same workload, and the CPU baseline (the oracle's
+OpenMP restatement on this host's cores, on a bounded sample).
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes as C
+import json
+import os
+import sys
+import time
+
+REPO = os.path.dirname(os.path.abspath(__file__))
+for p in (REPO, os.path.join(REPO, "fgcs-openacc-advanced_amd")):
+    if p not in sys.path:
+        sys.path.insert(0, p)
+
+HBM_PEAK_GBS = 8000.0          # MI355X HBM3E
+A100_V2_512_GCUPS = 8 * 6 * 511 * 512 ** 2 / 3.038 / 1e9  # plots/perfect-2-gpu.py:20,24
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def cell_updates(n, iters):
+    return iters * 6 * (n - 1) * n * n
+
+
+def time_solves(solver, T0d, n, steps, warmup, dtype, torch):
+    T = torch.empty_like(T0d, dtype=dtype)
+    for _ in range(warmup):
+        T.copy_(T0d)
+        solver.solve(T, n, max_iters=8)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        T.copy_(T0d)
+        solver.solve(T, n, max_iters=8)
+    torch.cuda.synchronize()
+    return (time.perf_counter() - t0) * 1e3, T
+
+
+def kernel_launch_us(solver, T, n, torch, reps=40):
+    """Average duration of single plane-step launches of the dominant kernel,
+    bracketed by HIP events on the stream the kernel runs on, over planes of
+    all six directions of a mid-solve field."""
+    import numpy as np
+    import fold3d
+    h = fold3d.reference_spacing(n)
+    dg = float(np.sqrt(2 * h * h))
+    A, B, Cc = fold3d.REF_A
+    F = fold3d.REF_F
+    args = {0: (F, B, Cc, A), 1: (F, -B, -Cc, -A), 2: (F, A, B, Cc),
+            3: (F, -A, -B, -Cc), 4: (F, -Cc, -A, -B), 5: (F, Cc, A, B)}
+    stream = torch.cuda.current_stream()
+    total, count = 0.0, 0
+    for d in range(6):
+        for p in (n // 4, n // 2, 3 * n // 4):
+            e0 = torch.cuda.Event(enable_timing=True)
+            e1 = torch.cuda.Event(enable_timing=True)
+            e0.record(stream)
+            for _ in range(reps):
+                solver.plane_step(d, T, n, p, h, h, h, dg, dg, dg, *args[d],
+                                  stream=stream.cuda_stream)
+            e1.record(stream)
+            e1.synchronize()
+            total += e0.elapsed_time(e1) * 1e3
+            count += reps
+    return total / count
+
+
+def cpu_baseline(T0_host, n, threads):
+    """The oracle (FP64 OpenMP restatement, bit-identical to the reference)
+    on a bounded sample: one of the 8 iterations of the same 512^3 solve."""
+    from oracle.oracle import Oracle
+    o = Oracle()
+    o.set_threads(threads)
+    T = T0_host.copy()
+    t0 = time.perf_counter()
+    o.sweeps(T, n, 1)
+    dt = time.perf_counter() - t0
+    return cell_updates(n, 1) / dt / 1e9, dt
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--n", type=int, default=512)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-fp32", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
+              "the solve does not shard across GPUs", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+    import fold3d
+
+    n = args.n
+    log(f"bench: host initializer {n}^3")
+    t0 = time.perf_counter()
+    T0 = fold3d.implicit_initialiser(n)
+    log(f"bench: initializer took {time.perf_counter() - t0:.1f} s")
+    T0d = torch.from_numpy(T0).to("cuda")
+    solver = fold3d.Solver(0)
+
+    W = cell_updates(n, 8)
+    log(f"bench: FP64 solve x{args.steps} (+{args.warmup} warmup)")
+    ms, T = time_solves(solver, T0d, n, args.steps, args.warmup, torch.float64, torch)
+    ms_step = ms / args.steps
+    value = W * args.steps / (ms / 1e3) / 1e9
+    log(f"bench: FP64 {ms_step:.1f} ms/solve, {value:.2f} Gcell-upd/s")
+
+    log("bench: per-launch kernel timing")
+    us = kernel_launch_us(solver, T, n, torch)
+    bytes_per_launch = 16.0 * n * n
+    achieved = bytes_per_launch / (us * 1e-6) / 1e9
+
+    fp32 = None
+    if not args.no_fp32:
+        log("bench: FP32 solve")
+        ms32, T32 = time_solves(solver, T0d, n, args.steps, args.warmup, torch.float32, torch)
+        ref = T.cpu().numpy()
+        got = T32.double().cpu().numpy()
+        reached = np.abs(ref) < 1e15
+        rel = float(np.max(np.abs(got[reached] - ref[reached]) / np.abs(ref[reached])))
+        fp32 = {"value": W * args.steps / (ms32 / 1e3) / 1e9, "unit": "Gcell-updates/s",
+                "ms_per_step": ms32 / args.steps, "max_rel_err_vs_fp64": rel}
+        log(f"bench: FP32 {fp32['value']:.2f} Gcell-upd/s, max rel err {rel:.2e}")
+
+    cpu = None
+    if not args.no_cpu:
+        threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or min(16, os.cpu_count() or 1)
+        log(f"bench: CPU baseline, {threads} threads, 1 iteration of {n}^3")
+        v, dt = cpu_baseline(T0, n, threads)
+        cpu = {"value": v, "unit": "Gcell-updates/s", "cores": threads, "kind": "port",
+               "sample": f"1 of the 8 iterations (6 sweeps) of the same {n}^3 FP64 solve, "
+                         f"{cell_updates(n, 1):.3e} cell-updates in {dt:.1f} s"}
+        log(f"bench: CPU {v:.3f} Gcell-upd/s")
+
+    out = {
+        "metric": "Gcell-updates/s at 512^3 (fold-model PMM solve, 8 iterations)",
+        "value": value,
+        "unit": "Gcell-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": value / A100_V2_512_GCUPS if n == 512 else None,
+        "dtype": "f64",
+        "data": "synthetic: the reference's implicit-surface initial field (Fold3dPMM.c)",
+        "config": {"workload": f"fold-model PMM solve, {n}^3 nodes, FP64, 8 iterations x 6 sweeps",
+                   "n": n, "iterations": 8, "cell_updates_per_step": W},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "fold_plane_kernel<double>", "kernel_us": us,
+                     "bytes_per_launch": bytes_per_launch},
+        "fp32": fp32,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

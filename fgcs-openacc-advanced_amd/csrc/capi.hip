@@ -1,0 +1,227 @@
+// capi.hip — the extern "C" surface of libfold3d.so (include/fold3d.h and the
+// device half of include/Fold3dPMM.h).
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstring>
+
+#include "../../include/Fold3dPMM.h"
+#include "common.h"
+#include "solver.h"
+
+using namespace fold3d;
+
+struct fold3d_handle {
+  int device;
+  unsigned long long *d_max;  // max |dT| of the running iteration
+  hipEvent_t ev0, ev1;
+};
+
+extern "C" {
+
+const char *fold3d_version(void) { return "fold3d-mi355x 0.1 (gfx950)"; }
+
+void fold3d_default_opts(fold3d_opts *o) {
+  if (!o) return;
+  memset(o, 0, sizeof *o);
+  o->precision = FOLD3D_F64;
+  o->memory = FOLD3D_DEVICE;
+  o->max_iters = 8;  // AN_ISO_1_OpenACC.c:1187
+  o->tol = 0.0;
+  o->stream = nullptr;
+  o->engine = FOLD3D_ENGINE_AUTO;
+}
+
+void fold3d_reference_model(fold3d_model *m) {
+  if (!m) return;
+  memset(m, 0, sizeof *m);
+  m->kind = FOLD3D_MODEL_FOLD;
+  m->F = 1.1;  // Fold3dPMM.h:49-52
+  m->a[0] = -0.35;
+  m->a[1] = 0.4;
+  m->a[2] = 0.7;
+  m->speed = 1.0;
+}
+
+int fold3d_create(fold3d_handle **out, int device) {
+  F3D_CHECK(out != nullptr, FOLD3D_E_INVALID, "fold3d_create: out is NULL");
+  int count = 0;
+  F3D_HIP(hipGetDeviceCount(&count));
+  F3D_CHECK(device >= 0 && device < count, FOLD3D_E_INVALID,
+            "fold3d_create: device %d not in [0,%d)", device, count);
+  F3D_HIP(hipSetDevice(device));
+  fold3d_handle *h = new fold3d_handle();
+  h->device = device;
+  hipError_t e = hipMalloc((void **)&h->d_max, sizeof(unsigned long long));
+  if (e != hipSuccess) {
+    delete h;
+    set_error("fold3d_create: hipMalloc failed: %s", hipGetErrorString(e));
+    return FOLD3D_E_NOMEM;
+  }
+  F3D_HIP(hipEventCreate(&h->ev0));
+  F3D_HIP(hipEventCreate(&h->ev1));
+  *out = h;
+  return FOLD3D_OK;
+}
+
+int fold3d_destroy(fold3d_handle *h) {
+  if (!h) return FOLD3D_OK;
+  (void)hipSetDevice(h->device);
+  (void)hipFree(h->d_max);
+  (void)hipEventDestroy(h->ev0);
+  (void)hipEventDestroy(h->ev1);
+  delete h;
+  return FOLD3D_OK;
+}
+
+int fold3d_solve(fold3d_handle *h, void *T, int nx, int ny, int nz, double dx,
+                 double dy, double dz, const fold3d_model *model,
+                 const fold3d_opts *opts, fold3d_stats *stats) {
+  F3D_CHECK(h && T && model, FOLD3D_E_INVALID, "fold3d_solve: NULL handle, T or model");
+  F3D_CHECK(nx >= 3 && ny >= 3 && nz >= 3, FOLD3D_E_INVALID,
+            "fold3d_solve: grid %dx%dx%d too small (need >= 3 per axis)", nx, ny, nz);
+  F3D_CHECK(dx > 0 && dy > 0 && dz > 0 && std::isfinite(dx) && std::isfinite(dy) &&
+                std::isfinite(dz),
+            FOLD3D_E_INVALID, "fold3d_solve: spacings must be positive and finite");
+  fold3d_opts o;
+  fold3d_default_opts(&o);
+  if (opts) o = *opts;
+  F3D_CHECK(o.max_iters >= 1, FOLD3D_E_INVALID, "fold3d_solve: max_iters must be >= 1");
+  F3D_CHECK(o.tol >= 0 && std::isfinite(o.tol), FOLD3D_E_INVALID, "fold3d_solve: bad tol");
+  F3D_CHECK(o.precision == FOLD3D_F64 || o.precision == FOLD3D_F32, FOLD3D_E_INVALID,
+            "fold3d_solve: bad precision %d", o.precision);
+  F3D_CHECK(o.memory == FOLD3D_DEVICE || o.memory == FOLD3D_HOST, FOLD3D_E_INVALID,
+            "fold3d_solve: bad memory kind %d", o.memory);
+  F3D_CHECK(model->kind == FOLD3D_MODEL_FOLD, FOLD3D_E_UNSUPPORTED,
+            "fold3d_solve: model kind %d not available", model->kind);
+  F3D_CHECK(std::isfinite(model->F) && model->F > 0, FOLD3D_E_INVALID,
+            "fold3d_solve: F must be positive");
+
+  F3D_HIP(hipSetDevice(h->device));
+  hipStream_t st = (hipStream_t)o.stream;
+  const Grid G(nx, ny, nz);
+  const size_t esz = o.precision == FOLD3D_F64 ? 8 : 4;
+  const size_t bytes = (size_t)G.count * esz;
+  void *dT = T;
+  if (o.memory == FOLD3D_HOST) {
+    F3D_HIP(hipMalloc(&dT, bytes));
+    F3D_HIP(hipMemcpyAsync(dT, T, bytes, hipMemcpyHostToDevice, st));
+  }
+  const double hsp[3] = {dx, dy, dz};
+  int iters = 0;
+  double maxdt = -1;
+  F3D_HIP(hipEventRecord(h->ev0, st));
+  int rc = o.precision == FOLD3D_F64
+               ? fold_sweeps_plane_engine<double>(st, (double *)dT, G, *model, hsp,
+                                                  o.max_iters, o.tol, h->d_max,
+                                                  &iters, &maxdt)
+               : fold_sweeps_plane_engine<float>(st, (float *)dT, G, *model, hsp,
+                                                 o.max_iters, o.tol, h->d_max,
+                                                 &iters, &maxdt);
+  F3D_HIP(hipEventRecord(h->ev1, st));
+  if (rc != FOLD3D_OK) return rc;
+  if (o.memory == FOLD3D_HOST) {
+    F3D_HIP(hipMemcpyAsync(T, dT, bytes, hipMemcpyDeviceToHost, st));
+    F3D_HIP(hipStreamSynchronize(st));
+    F3D_HIP(hipFree(dT));
+  }
+  F3D_HIP(hipEventSynchronize(h->ev1));
+  float ms = 0;
+  F3D_HIP(hipEventElapsedTime(&ms, h->ev0, h->ev1));
+  if (stats) {
+    stats->iterations = iters;
+    stats->max_dT = maxdt;
+    double w = 0;
+    for (int d = 0; d < 3; ++d)
+      w += 2.0 * (G.n[d] - 1) * (double)G.n[(d + 1) % 3] * G.n[(d + 2) % 3];
+    stats->cell_updates = w * iters;
+    stats->sweep_ms = ms;
+  }
+  if (o.tol > 0 && !(maxdt < o.tol)) {
+    set_error("fold3d_solve: max|dT| = %g >= tol %g after %d iterations", maxdt, o.tol, iters);
+    return FOLD3D_E_NOT_CONVERGED;
+  }
+  return FOLD3D_OK;
+}
+
+int fold3d_plane_step(fold3d_handle *h, int direction, double *T, int nx, int ny,
+                      int nz, int plane, double dzz, double dxx, double dyy,
+                      double dxy, double dxz, double dyz, double F, double ay,
+                      double ax, double az, void *stream) {
+  F3D_CHECK(h && T, FOLD3D_E_INVALID, "fold3d_plane_step: NULL handle or T");
+  F3D_CHECK(direction >= 0 && direction < 6, FOLD3D_E_INVALID,
+            "fold3d_plane_step: direction %d not in [0,6)", direction);
+  F3D_CHECK(nx >= 3 && ny >= 3 && nz >= 3, FOLD3D_E_INVALID,
+            "fold3d_plane_step: grid too small");
+  const Grid G(nx, ny, nz);
+  const Frame &fr = kFrames[direction];
+  const int n = G.n[fr.m];
+  const bool ok = fr.s > 0 ? (plane >= 1 && plane <= n - 1) : (plane >= 2 && plane <= n);
+  F3D_CHECK(ok, FOLD3D_E_INVALID, "fold3d_plane_step: plane %d out of range for direction %d",
+            plane, direction);
+  F3D_HIP(hipSetDevice(h->device));
+  FoldFrame<double> f{F, ax, ay, az, dzz, dxx, dyy, dxy, dxz, dyz};
+  return fold_plane_step_f64((hipStream_t)stream, T, G, direction, plane, f);
+}
+
+int fold3d_device_alloc(void **ptr, size_t bytes) {
+  F3D_CHECK(ptr != nullptr, FOLD3D_E_INVALID, "fold3d_device_alloc: ptr is NULL");
+  F3D_HIP(hipMalloc(ptr, bytes));
+  return FOLD3D_OK;
+}
+int fold3d_device_free(void *ptr) {
+  F3D_HIP(hipFree(ptr));
+  return FOLD3D_OK;
+}
+int fold3d_copy_to_device(void *dst, const void *src, size_t bytes) {
+  F3D_HIP(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
+  return FOLD3D_OK;
+}
+int fold3d_copy_to_host(void *dst, const void *src, size_t bytes) {
+  F3D_HIP(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+  return FOLD3D_OK;
+}
+int fold3d_synchronize(void *stream) {
+  F3D_HIP(hipStreamSynchronize((hipStream_t)stream));
+  return FOLD3D_OK;
+}
+
+const char *fold3d_last_error(void);
+
+// Reference-signature plane steps (AN_ISO_1_OpenACC.c:180-1092): synchronous,
+// on the grid of the last ImplicitInitialiser()/fold3d_set_grid() call.  The
+// reference's functions return void, so a failure is reported on stderr.
+static void dropin_step(int dir, double *T, int plane, double dzz, double dxx,
+                        double dyy, double dxy, double dxz, double dyz, double F,
+                        double ay, double ax, double az) {
+  int nx, ny, nz;
+  fold3d_get_grid(&nx, &ny, &nz);
+  static fold3d_handle *h = nullptr;
+  int dev = 0;
+  (void)hipGetDevice(&dev);
+  if (!h || h->device != dev) {
+    if (h) fold3d_destroy(h);
+    if (fold3d_create(&h, dev) != FOLD3D_OK) {
+      fprintf(stderr, "fold3d: %s\n", fold3d_last_error());
+      return;
+    }
+  }
+  int rc = fold3d_plane_step(h, dir, T, nx, ny, nz, plane, dzz, dxx, dyy, dxy, dxz,
+                             dyz, F, ay, ax, az, nullptr);
+  if (rc == FOLD3D_OK) rc = fold3d_synchronize(nullptr);
+  if (rc != FOLD3D_OK) fprintf(stderr, "fold3d: %s\n", fold3d_last_error());
+}
+
+#define F3D_DROPIN(name, dir)                                                     \
+  void name(double *T, int p, double dzz, double dxx, double dyy, double dxy,     \
+            double dxz, double dyz, double F, double ay, double ax, double az) { \
+    dropin_step(dir, T, p, dzz, dxx, dyy, dxy, dxz, dyz, F, ay, ax, az);         \
+  }
+F3D_DROPIN(function_x_down, FOLD3D_X_DOWN)
+F3D_DROPIN(function_x_up, FOLD3D_X_UP)
+F3D_DROPIN(function_k_up, FOLD3D_K_UP)
+F3D_DROPIN(function_k_down, FOLD3D_K_DOWN)
+F3D_DROPIN(function_y_up, FOLD3D_Y_UP)
+F3D_DROPIN(function_y_down, FOLD3D_Y_DOWN)
+
+}  // extern "C"

@@ -1,0 +1,71 @@
+// common.h — host-side plumbing shared by the library's translation units:
+// status codes, the thread-local error message, HIP call checking, the sweep
+// direction frames and the padded grid geometry.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <string>
+
+#include "../../include/fold3d.h"
+
+namespace fold3d {
+
+void set_error(const char *fmt, ...);
+
+#define F3D_HIP(call)                                                       \
+  do {                                                                      \
+    hipError_t e_ = (call);                                                 \
+    if (e_ != hipSuccess) {                                                 \
+      ::fold3d::set_error("%s failed: %s (%s:%d)", #call,                   \
+                          hipGetErrorString(e_), __FILE__, __LINE__);       \
+      return e_ == hipErrorOutOfMemory ? FOLD3D_E_NOMEM : FOLD3D_E_HIP;     \
+    }                                                                       \
+  } while (0)
+
+#define F3D_CHECK(cond, code, ...)        \
+  do {                                    \
+    if (!(cond)) {                        \
+      ::fold3d::set_error(__VA_ARGS__);   \
+      return (code);                      \
+    }                                     \
+  } while (0)
+
+// Sweep frames.  Each direction marches along axis m in steps of s (+1/-1);
+// the upwind plane's local x^ is axis ux with sign sx, local y^ is uy with
+// sign sy.  Read off the loads and model constants of the reference kernels
+// (AN_ISO_1_OpenACC.c:195-212, 348-364, 500-516, 652-668, 805-821, 960-976)
+// and the spacing arguments of its driver (:1198-1235).
+struct Frame {
+  int m, s, ux, sx, uy, sy;
+};
+constexpr Frame kFrames[6] = {
+    {0, +1, 2, +1, 1, +1},  // x_down: i -> i+1, x^ = +z, y^ = +y
+    {0, -1, 2, -1, 1, -1},  // x_up:   i -> i-1, x^ = -z, y^ = -y
+    {2, +1, 1, +1, 0, +1},  // k_up:   k -> k+1, x^ = +y, y^ = +x
+    {2, -1, 1, -1, 0, -1},  // k_down: k -> k-1, x^ = -y, y^ = -x
+    {1, -1, 0, -1, 2, -1},  // y_up:   j -> j-1, x^ = -x, y^ = -z
+    {1, +1, 0, +1, 2, +1},  // y_down: j -> j+1, x^ = +x, y^ = +z
+};
+
+// Padded grid: nodes 1..n[a] on each axis, ghosts at 0 and n[a]+1.
+struct Grid {
+  int n[3];
+  long long stride[3];
+  long long count;  // (nx+2)(ny+2)(nz+2)
+  Grid(int nx, int ny, int nz) {
+    n[0] = nx;
+    n[1] = ny;
+    n[2] = nz;
+    stride[0] = 1;
+    stride[1] = (long long)nx + 2;
+    stride[2] = stride[1] * ((long long)ny + 2);
+    count = stride[2] * ((long long)nz + 2);
+  }
+  // Planes of a sweep along axis m: forward 1..n-1, backward n..2.
+  int planes(int m) const { return n[m] - 1; }
+};
+
+}  // namespace fold3d

@@ -1,0 +1,189 @@
+// fold_sweep.hip — plane-step kernels and the sweep driver of the fold-model
+// PMM solve on gfx950, behind the C-ABI of include/fold3d.h.
+//
+// Reference: Stencil-OpenACC/AN_ISO_1_OpenACC.c — plane kernels :180-1092,
+// driver :1150-1241 (6 directions x (n-1) planes per iteration, 8 iterations).
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdarg>
+#include <cstring>
+#include <vector>
+
+#include "common.h"
+#include "fold_local.h"
+#include "solver.h"
+
+namespace fold3d {
+
+static thread_local char g_err[512] = "";
+
+void set_error(const char *fmt, ...) {
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(g_err, sizeof g_err, fmt, ap);
+  va_end(ap);
+}
+
+}  // namespace fold3d
+
+extern "C" const char *fold3d_last_error(void) { return fold3d::g_err; }
+
+namespace fold3d {
+
+// ---------------------------------------------------------------------------
+// Plane-step kernel: one thread per node of the target plane.  Lanes run along
+// the in-plane axis with the smallest stride (x for y- and z-sweeps, which is
+// unit stride and coalesced; y for x-sweeps).  Blocks are 64 x 4: one
+// wavefront per row.
+// ---------------------------------------------------------------------------
+constexpr int kLaneX = 64;
+constexpr int kRowsY = 4;
+
+template <typename R, bool kReduce>
+__global__ __launch_bounds__(kLaneX *kRowsY) void fold_plane_kernel(
+    R *__restrict__ T, PlaneGeom g, FoldFrame<R> f,
+    unsigned long long *__restrict__ max_dt) {
+  const int a = 1 + blockIdx.x * kLaneX + threadIdx.x;
+  const int b = 1 + blockIdx.y * kRowsY + threadIdx.y;
+  R d = R(0);
+  if (a <= g.na && b <= g.nb) {
+    const long long c = g.base + a * g.Sa + b * g.Sb;
+    Patch<R> p;
+    p.c = fabs(T[c]);
+    p.px = fabs(T[c + g.su]);
+    p.py = fabs(T[c + g.sv]);
+    p.mx = fabs(T[c - g.su]);
+    p.my = fabs(T[c - g.sv]);
+    p.pxpy = fabs(T[c + g.su + g.sv]);
+    p.pxmy = fabs(T[c + g.su - g.sv]);
+    p.mxpy = fabs(T[c - g.su + g.sv]);
+    p.mxmy = fabs(T[c - g.su - g.sv]);
+    const R told = T[c + g.sm];
+    const R tn = fold_node_update(told, p, f);
+    T[c + g.sm] = tn;
+    if (kReduce) d = fabs(tn - told);
+  }
+  if (kReduce) {
+    // wavefront max, then one atomic per wave (|dT| >= 0, so the IEEE bit
+    // pattern orders like an unsigned integer)
+    double v = (double)d;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, 64));
+    if ((threadIdx.x & 63) == 0 && v > 0.0)
+      atomicMax(max_dt, (unsigned long long)__double_as_longlong(v));
+  }
+}
+
+__global__ void zero_u64_kernel(unsigned long long *p) { *p = 0ull; }
+
+// Local frame of direction `dir` from the global model (F, a) and spacings.
+template <typename R>
+static FoldFrame<R> fold_frame(int dir, double F, const double a[3],
+                               const double h[3]) {
+  const Frame &fr = kFrames[dir];
+  auto diag = [&](int u, int v) {
+    // DXYP/DXZP/DYZP of the reference driver (:1159-1161), lower axis first
+    int lo = u < v ? u : v, hi = u < v ? v : u;
+    return std::sqrt(h[lo] * h[lo] + h[hi] * h[hi]);
+  };
+  FoldFrame<R> f;
+  f.F = (R)F;
+  f.ax = (R)(fr.sx * a[fr.ux]);
+  f.ay = (R)(fr.sy * a[fr.uy]);
+  f.az = (R)(fr.s * a[fr.m]);
+  f.dzz = (R)h[fr.m];
+  f.dxx = (R)h[fr.ux];
+  f.dyy = (R)h[fr.uy];
+  f.dxy = (R)diag(fr.ux, fr.uy);
+  f.dxz = (R)diag(fr.ux, fr.m);
+  f.dyz = (R)diag(fr.uy, fr.m);
+  return f;
+}
+
+PlaneGeom plane_geom(const Grid &G, int dir, int plane) {
+  const Frame &fr = kFrames[dir];
+  // lane axis: the in-plane axis with the smaller stride
+  int u = fr.ux, v = fr.uy;
+  int la = u < v ? u : v, lb = u < v ? v : u;
+  PlaneGeom g;
+  g.base = (long long)plane * G.stride[fr.m];
+  g.Sa = G.stride[la];
+  g.Sb = G.stride[lb];
+  g.sm = fr.s * G.stride[fr.m];
+  g.su = fr.sx * G.stride[fr.ux];
+  g.sv = fr.sy * G.stride[fr.uy];
+  g.na = G.n[la];
+  g.nb = G.n[lb];
+  return g;
+}
+
+template <typename R>
+static void launch_plane(hipStream_t st, R *T, const PlaneGeom &g,
+                         const FoldFrame<R> &f, unsigned long long *max_dt) {
+  dim3 block(kLaneX, kRowsY);
+  dim3 grid((g.na + kLaneX - 1) / kLaneX, (g.nb + kRowsY - 1) / kRowsY);
+  if (max_dt)
+    fold_plane_kernel<R, true><<<grid, block, 0, st>>>(T, g, f, max_dt);
+  else
+    fold_plane_kernel<R, false><<<grid, block, 0, st>>>(T, g, f, nullptr);
+}
+
+// One direction: the reference's plane order (forward 1..n-1, backward n..2).
+template <typename R>
+static void fold_direction(hipStream_t st, R *T, const Grid &G, int dir,
+                           const FoldFrame<R> &f, unsigned long long *max_dt) {
+  const Frame &fr = kFrames[dir];
+  const int n = G.n[fr.m];
+  if (fr.s > 0)
+    for (int p = 1; p < n; ++p) launch_plane(st, T, plane_geom(G, dir, p), f, max_dt);
+  else
+    for (int p = n; p > 1; --p) launch_plane(st, T, plane_geom(G, dir, p), f, max_dt);
+}
+
+template <typename R>
+int fold_sweeps_plane_engine(hipStream_t st, R *T, const Grid &G,
+                             const fold3d_model &m, const double h[3],
+                             int max_iters, double tol,
+                             unsigned long long *d_max, int *iters_out,
+                             double *max_dt_out) {
+  FoldFrame<R> frames[6];
+  for (int d = 0; d < 6; ++d) frames[d] = fold_frame<R>(d, m.F, m.a, h);
+  int it = 0;
+  double last = -1.0;
+  for (; it < max_iters;) {
+    if (tol > 0) zero_u64_kernel<<<1, 1, 0, st>>>(d_max);
+    for (int d = 0; d < 6; ++d)
+      fold_direction<R>(st, T, G, d, frames[d], tol > 0 ? d_max : nullptr);
+    F3D_HIP(hipGetLastError());
+    ++it;
+    if (tol > 0) {
+      unsigned long long bits = 0;
+      F3D_HIP(hipMemcpyAsync(&bits, d_max, sizeof bits, hipMemcpyDeviceToHost, st));
+      F3D_HIP(hipStreamSynchronize(st));
+      memcpy(&last, &bits, sizeof last);
+      if (last < tol) break;
+    }
+  }
+  *iters_out = it;
+  *max_dt_out = last;
+  return FOLD3D_OK;
+}
+
+template int fold_sweeps_plane_engine<double>(hipStream_t, double *, const Grid &,
+                                              const fold3d_model &, const double *,
+                                              int, double, unsigned long long *,
+                                              int *, double *);
+template int fold_sweeps_plane_engine<float>(hipStream_t, float *, const Grid &,
+                                             const fold3d_model &, const double *,
+                                             int, double, unsigned long long *,
+                                             int *, double *);
+
+int fold_plane_step_f64(hipStream_t st, double *T, const Grid &G, int dir,
+                        int plane, const FoldFrame<double> &f) {
+  launch_plane(st, T, plane_geom(G, dir, plane), f, nullptr);
+  F3D_HIP(hipGetLastError());
+  return FOLD3D_OK;
+}
+
+}  // namespace fold3d

@@ -1,0 +1,33 @@
+// solver.h — internal interfaces between the C-ABI layer and the engines.
+#pragma once
+
+#include "common.h"
+#include "fold_local.h"
+
+namespace fold3d {
+
+// One plane step's addressing: node (a, b) of the plane, a, b in 1..na/nb,
+// sits at base + a*Sa + b*Sb; the target is +sm away, the local x^/y^
+// neighbours are +-su / +-sv away.
+struct PlaneGeom {
+  long long base, Sa, Sb;
+  long long sm, su, sv;
+  int na, nb;
+};
+
+PlaneGeom plane_geom(const Grid &G, int dir, int plane);
+
+template <typename R>
+int fold_sweeps_plane_engine(hipStream_t st, R *T, const Grid &G,
+                             const fold3d_model &m, const double h[3],
+                             int max_iters, double tol,
+                             unsigned long long *d_max, int *iters_out,
+                             double *max_dt_out);
+
+int fold_plane_step_f64(hipStream_t st, double *T, const Grid &G, int dir,
+                        int plane, const FoldFrame<double> &f);
+
+}  // namespace fold3d
+
+// Grid shared by the reference-signature entry points (host_init.c).
+extern "C" void fold3d_get_grid(int *nx, int *ny, int *nz);

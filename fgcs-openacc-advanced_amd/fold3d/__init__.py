@@ -1,0 +1,238 @@
+"""Python view of libfold3d.so (the MI355X Fold3dPMM solver) through its C-ABI.
+
+This is a thin ctypes layer over include/fold3d.h and include/Fold3dPMM.h;
+it performs no computation of its own.  Device buffers are torch tensors on
+``cuda`` (torch supplies memory and streams only); host buffers are numpy
+arrays.  Loading fails loudly when the shared library has not been built.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass
+
+import numpy as np
+
+PKG_DIR = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB_PATH = os.path.join(PKG_DIR, "libfold3d.so")
+REPO_DIR = os.path.dirname(PKG_DIR)
+INCLUDE_DIR = os.path.join(REPO_DIR, "include")
+
+FOLD3D_OK = 0
+FOLD3D_E_INVALID = -1
+FOLD3D_E_HIP = -2
+FOLD3D_E_NOMEM = -3
+FOLD3D_E_UNSUPPORTED = -4
+FOLD3D_E_NOT_CONVERGED = -5
+
+F64, F32 = 0, 1
+DEVICE, HOST = 0, 1
+MODEL_FOLD, MODEL_ELLIP = 0, 1
+ENGINE_AUTO, ENGINE_PLANE = 0, 1
+
+MAXT = 10e15 * 1.0
+REF_F = 1.1                      # reference Fold3dPMM.h:52
+REF_A = (-0.35, 0.4, 0.7)        # reference Fold3dPMM.h:49-51
+DIRECTIONS = ("x_down", "x_up", "k_up", "k_down", "y_up", "y_down")
+
+
+class Fold3dError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"fold3d error {code}: {msg}")
+        self.code = code
+
+
+class Model(C.Structure):
+    _fields_ = [
+        ("kind", C.c_int),
+        ("F", C.c_double),
+        ("a", C.c_double * 3),
+        ("M", C.c_double * 6),
+        ("speed", C.c_double),
+        ("speed_field", C.c_void_p),
+        ("theta", C.c_void_p),
+        ("phi", C.c_void_p),
+        ("ra", C.c_void_p),
+        ("rb", C.c_void_p),
+    ]
+
+
+class Opts(C.Structure):
+    _fields_ = [
+        ("precision", C.c_int),
+        ("memory", C.c_int),
+        ("max_iters", C.c_int),
+        ("tol", C.c_double),
+        ("stream", C.c_void_p),
+        ("engine", C.c_int),
+    ]
+
+
+class Stats(C.Structure):
+    _fields_ = [
+        ("iterations", C.c_int),
+        ("max_dT", C.c_double),
+        ("cell_updates", C.c_double),
+        ("sweep_ms", C.c_double),
+    ]
+
+
+_lib = None
+
+
+def lib() -> C.CDLL:
+    """Load libfold3d.so once; raise if it is missing (no fallback path)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(
+            f"{LIB_PATH} is not built; run __graft_entry__.build() (make -C csrc)")
+    L = C.CDLL(LIB_PATH)
+    d, i, vp = C.c_double, C.c_int, C.c_void_p
+    dp = C.POINTER(C.c_double)
+    L.fold3d_version.restype = C.c_char_p
+    L.fold3d_last_error.restype = C.c_char_p
+    L.fold3d_create.argtypes = [C.POINTER(vp), i]
+    L.fold3d_destroy.argtypes = [vp]
+    L.fold3d_default_opts.argtypes = [C.POINTER(Opts)]
+    L.fold3d_reference_model.argtypes = [C.POINTER(Model)]
+    L.fold3d_solve.argtypes = [vp, vp, i, i, i, d, d, d, C.POINTER(Model),
+                               C.POINTER(Opts), C.POINTER(Stats)]
+    L.fold3d_plane_step.argtypes = [vp, i, vp, i, i, i, i] + [d] * 10 + [vp]
+    L.fold3d_synchronize.argtypes = [vp]
+    L.ImplicitInitialiser.argtypes = [dp, i, i, i, d, d, d]
+    L.InitFoldFromPoint.restype = d
+    L.InitFoldFromPoint.argtypes = [d] * 7
+    L.ImpSurfF.restype = d
+    L.ImpSurfF.argtypes = [d] * 3
+    L.fold3d_set_grid.argtypes = [i, i, i]
+    for name in DIRECTIONS:
+        fn = getattr(L, "function_" + name)
+        fn.argtypes = [vp, i] + [d] * 10
+        fn.restype = None
+    _lib = L
+    return L
+
+
+def last_error() -> str:
+    return lib().fold3d_last_error().decode()
+
+
+def check(rc: int) -> int:
+    if rc != FOLD3D_OK:
+        raise Fold3dError(rc, last_error())
+    return rc
+
+
+def reference_spacing(n: int) -> float:
+    """Node spacing of the reference's [0,10] domain (AN_ISO_1_OpenACC.c:1154)."""
+    return (10.0 - 0.0) / (n - 1.0)
+
+
+def reference_model() -> Model:
+    m = Model()
+    lib().fold3d_reference_model(C.byref(m))
+    return m
+
+
+def fold_model(F: float = REF_F, a=REF_A) -> Model:
+    m = reference_model()
+    m.F = F
+    for k in range(3):
+        m.a[k] = a[k]
+    return m
+
+
+def implicit_initialiser(n: int) -> np.ndarray:
+    """The reference's input field on an n^3 grid (host C, Fold3dPMM.c:129)."""
+    h = reference_spacing(n)
+    T = np.empty((n + 2) ** 3, dtype=np.float64)
+    lib().ImplicitInitialiser(T.ctypes.data_as(C.POINTER(C.c_double)), n, n, n, h, h, h)
+    return T
+
+
+class Solver:
+    """One fold3d handle bound to a device."""
+
+    def __init__(self, device: int = 0):
+        h = C.c_void_p()
+        check(lib().fold3d_create(C.byref(h), device))
+        self.h = h
+
+    def close(self):
+        if self.h:
+            lib().fold3d_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def solve(self, T, n, spacing=None, model: Model | None = None,
+              max_iters: int = 8, tol: float = 0.0, stream=None,
+              engine: int = ENGINE_AUTO, raise_not_converged: bool = True) -> Stats:
+        """In-place solve of T (torch cuda tensor or numpy host array)."""
+        nx, ny, nz = (n, n, n) if np.isscalar(n) else n
+        if spacing is None:
+            spacing = reference_spacing(nx)
+        dx, dy, dz = (spacing,) * 3 if np.isscalar(spacing) else spacing
+        o = Opts()
+        lib().fold3d_default_opts(C.byref(o))
+        o.max_iters, o.tol, o.engine = max_iters, tol, engine
+        if isinstance(T, np.ndarray):
+            o.memory = HOST
+            o.precision = F64 if T.dtype == np.float64 else F32
+            ptr = T.ctypes.data
+        else:
+            import torch
+            o.memory = DEVICE
+            o.precision = F64 if T.dtype == torch.float64 else F32
+            ptr = T.data_ptr()
+            if stream is None:
+                stream = torch.cuda.current_stream(T.device).cuda_stream
+        o.stream = stream
+        st = Stats()
+        m = model if model is not None else reference_model()
+        rc = lib().fold3d_solve(self.h, C.c_void_p(ptr), nx, ny, nz, dx, dy, dz,
+                                C.byref(m), C.byref(o), C.byref(st))
+        if rc == FOLD3D_E_NOT_CONVERGED and not raise_not_converged:
+            return st
+        check(rc)
+        return st
+
+    def plane_step(self, direction: int, T, n, plane: int, dzz, dxx, dyy, dxy,
+                   dxz, dyz, F, ay, ax, az, stream=None):
+        nx, ny, nz = (n, n, n) if np.isscalar(n) else n
+        if stream is None:
+            import torch
+            stream = torch.cuda.current_stream(T.device).cuda_stream
+        check(lib().fold3d_plane_step(self.h, direction, C.c_void_p(T.data_ptr()),
+                                      nx, ny, nz, plane, dzz, dxx, dyy, dxy, dxz,
+                                      dyz, F, ay, ax, az, stream))
+
+
+def header_symbols() -> list[str]:
+    """Function names declared in include/*.h (for export checks)."""
+    import re
+    names = []
+    for fn in ("fold3d.h", "Fold3dPMM.h"):
+        src = open(os.path.join(INCLUDE_DIR, fn)).read()
+        src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+        src = re.sub(r"#define[^\n]*(\\\n[^\n]*)*", "", src)
+        for m in re.finditer(r"\b([A-Za-z_][A-Za-z0-9_]*)\s*\(", src):
+            name = m.group(1)
+            if name in ("if", "sizeof", "defined", "return"):
+                continue
+            # a declaration: preceded by a type on the same statement
+            names.append(name)
+    # drop macro-ish and type names
+    skip = {"_DOUBLE_", "sign", "fmin"}
+    out = []
+    for nm in names:
+        if nm in skip or nm in out:
+            continue
+        out.append(nm)
+    return out

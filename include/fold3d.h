@@ -1,0 +1,144 @@
+/*
+ * fold3d.h — C-ABI of the MI355X (gfx950) parallel marching method (PMM)
+ * solver for the 3D anisotropic eikonal equation.
+ *
+ * Replaces the single-device solve of the reference
+ *   Stencil-OpenACC/AN_ISO_1_OpenACC.c:1150-1241 (driver: device copy, sweep loop)
+ *   Stencil-OpenACC/AN_ISO_1_OpenACC.c:180-1092  (six plane-step kernels)
+ * behind plain C types: pointers, sizes, doubles.  No HIP or torch types appear
+ * in any signature; streams travel as `void *` (a hipStream_t, NULL = the
+ * legacy default stream).
+ *
+ * Grid layout (reference Fold3dPMM.h:27-30): T[i + (nx+2)*(j + (ny+2)*k)],
+ * i fastest, nodes 1..n per axis, one ghost layer on each side that the solver
+ * reads but never writes.  T is signed: the sign marks the side of the initial
+ * surface and is preserved; |T| >= FOLD3D_MAXT means "not reached".
+ *
+ * Conventions: every entry point returns FOLD3D_OK (0) or a negative
+ * FOLD3D_E* code, never exits.  The caller owns every buffer passed in; a
+ * handle owns its scratch memory.  A handle is used by one host thread at a
+ * time; separate handles are independent.
+ */
+#ifndef FOLD3D_H
+#define FOLD3D_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FOLD3D_MAXT (10e15 * 1.0) /* reference Fold3dPMM.h:6 */
+
+enum fold3d_status {
+  FOLD3D_OK = 0,
+  FOLD3D_E_INVALID = -1,     /* bad argument (size, pointer, option) */
+  FOLD3D_E_HIP = -2,         /* a HIP runtime call failed (see fold3d_last_error) */
+  FOLD3D_E_NOMEM = -3,       /* device allocation failed */
+  FOLD3D_E_UNSUPPORTED = -4, /* valid request this build does not implement */
+  FOLD3D_E_NOT_CONVERGED = -5 /* max_iters reached before tol (result still written) */
+};
+
+enum fold3d_precision { FOLD3D_F64 = 0, FOLD3D_F32 = 1 };
+
+enum fold3d_model_kind {
+  /* Reference model: F*|grad T| + a.grad T = 1, a flips sign where T < 0
+   * (AN_ISO_1_OpenACC.c:206-212). */
+  FOLD3D_MODEL_FOLD = 0,
+  /* Tilted elliptic model: sqrt(grad T^T M grad T) = 1/v(x), M symmetric
+   * positive definite, constant or per node. */
+  FOLD3D_MODEL_ELLIP = 1
+};
+
+/* Where the array T lives. */
+enum fold3d_memory { FOLD3D_DEVICE = 0, FOLD3D_HOST = 1 };
+
+typedef struct fold3d_model {
+  int kind;         /* enum fold3d_model_kind */
+  /* FOLD3D_MODEL_FOLD */
+  double F;         /* reference: 1.1  (Fold3dPMM.h:52) */
+  double a[3];      /* reference: (-0.35, 0.4, 0.7) (Fold3dPMM.h:49-51) */
+  /* FOLD3D_MODEL_ELLIP, constant part: M = [[m0 m3 m4][m3 m1 m5][m4 m5 m2]] */
+  double M[6];
+  double speed;     /* used when speed_field == NULL */
+  /* FOLD3D_MODEL_ELLIP, optional per-node fields on the padded grid, same
+   * precision as T, device pointers (NULL = use the constant):              */
+  const void *speed_field;            /* v(x) > 0 */
+  const void *theta, *phi, *ra, *rb;  /* tilt angles (rad) and axis ratios:
+                                         M = R diag(1, ra, rb)^2 R^T,
+                                         R = Rz(phi) Ry(theta); all four or none */
+} fold3d_model;
+
+/* Sweep order: the reference's (AN_ISO_1_OpenACC.c:1195-1236). */
+enum fold3d_direction {
+  FOLD3D_X_DOWN = 0, /* +x: reads plane i, writes i+1, i = 1..n-1 */
+  FOLD3D_X_UP = 1,   /* -x */
+  FOLD3D_K_UP = 2,   /* +z */
+  FOLD3D_K_DOWN = 3, /* -z */
+  FOLD3D_Y_UP = 4,   /* -y (reference name) */
+  FOLD3D_Y_DOWN = 5  /* +y */
+};
+
+typedef struct fold3d_opts {
+  int precision;    /* enum fold3d_precision of T (and of per-node fields) */
+  int memory;       /* enum fold3d_memory of T */
+  int max_iters;    /* outer iterations of 6 sweeps; reference: 8 */
+  double tol;       /* > 0: stop after the first iteration with max|dT| < tol;
+                       0: run exactly max_iters (the reference's behaviour) */
+  void *stream;     /* hipStream_t; NULL = default stream */
+  int engine;       /* 0 = auto; see enum fold3d_engine */
+} fold3d_opts;
+
+enum fold3d_engine {
+  FOLD3D_ENGINE_AUTO = 0,
+  FOLD3D_ENGINE_PLANE = 1  /* one kernel launch per plane */
+};
+
+typedef struct fold3d_stats {
+  int iterations;        /* outer iterations run */
+  double max_dT;         /* max |T_new - T_old| over the last iteration (tol > 0 only) */
+  double cell_updates;   /* iterations * 6 * (n-1) * n^2 for cubes */
+  double sweep_ms;       /* device time of the sweep loop (HIP events) */
+} fold3d_stats;
+
+typedef struct fold3d_handle fold3d_handle;
+
+/* Library ---------------------------------------------------------------- */
+const char *fold3d_version(void);
+const char *fold3d_last_error(void);     /* thread-local message of the last failure */
+
+int fold3d_create(fold3d_handle **out, int device);
+int fold3d_destroy(fold3d_handle *h);
+
+void fold3d_default_opts(fold3d_opts *o);           /* F64, device, 8 iters, tol 0 */
+void fold3d_reference_model(fold3d_model *m);       /* F = 1.1, a = (-0.35, 0.4, 0.7) */
+
+/* Whole solve, in place (replaces AN_ISO_1_OpenACC.c:1185-1241).
+ * dx, dy, dz: node spacings.  Returns FOLD3D_E_NOT_CONVERGED (with T and
+ * stats written) when tol > 0 and max_iters ran out first. */
+int fold3d_solve(fold3d_handle *h, void *T, int nx, int ny, int nz,
+                 double dx, double dy, double dz, const fold3d_model *model,
+                 const fold3d_opts *opts, fold3d_stats *stats);
+
+/* One plane step of one direction on a device FP64 field, fold model, with
+ * explicit local-frame arguments exactly as the reference kernels take them
+ * (AN_ISO_1_OpenACC.c:180-183; the CUDA twin Stencil-CUDA/AN_ISO_2_GPU.cu:151-155
+ * uses F, ay, ax, az as passed, as this does).  Asynchronous on `stream`. */
+int fold3d_plane_step(fold3d_handle *h, int direction, double *T, int nx,
+                      int ny, int nz, int plane, double dzz, double dxx,
+                      double dyy, double dxy, double dxz, double dyz, double F,
+                      double ay, double ax, double az, void *stream);
+
+/* Device memory helpers for C callers that have no HIP headers
+ * (stand in for acc_malloc / acc_memcpy_* at AN_ISO_1_OpenACC.c:1185-1186,1241). */
+int fold3d_device_alloc(void **ptr, size_t bytes);
+int fold3d_device_free(void *ptr);
+int fold3d_copy_to_device(void *dst, const void *src, size_t bytes);
+int fold3d_copy_to_host(void *dst, const void *src, size_t bytes);
+int fold3d_synchronize(void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,63 @@
+/*
+ * ORACLE — CPU restatement of the reference's single-device Fold3dPMM solve.
+ *
+ * TEST INFRASTRUCTURE ONLY.  Only tests/, __graft_entry__.smoke() and the
+ * cpu_baseline leg of bench.py may load this library, and only as the
+ * checker (or the timed CPU baseline).  The shipped library never links it.
+ *
+ * Grid layout (reference Fold3dPMM.h:27-30, AN_ISO_1_OpenACC.c:195):
+ *   T[i + H*(j + H*k)], H = n+2, nodes 1..n per axis, ghost layer 0 and n+1.
+ * Sweep direction numbering (reference driver order, AN_ISO_1_OpenACC.c:1195-1236):
+ *   0 x_down (+x)  1 x_up (-x)  2 k_up (+z)  3 k_down (-z)  4 y_up (-y)  5 y_down (+y)
+ *
+ * Pinning (what checks each function, see tests/test_oracle_*.py):
+ *   orc_edge, orc_surf           bitwise vs the compiled reference (oracle/_ref) on
+ *                                seeded tuples and committed KAT fixtures; a=0 case vs the
+ *                                textbook isotropic update; brute-force simplex minimum.
+ *   orc_fold_point               bitwise vs reference; closed form |d|/... properties.
+ *   orc_plane / orc_sweeps       bitwise vs reference plane kernels and full solves
+ *                                (committed fixtures at 16^3/24^3/32^3; _ref up to 128^3).
+ *   orc_initialiser              bitwise vs reference ImplicitInitialiser fixtures.
+ *   orc_ellip_*                  (elliptic model, no reference code) closed-form point
+ *                                source, isotropic reduction, brute-force simplex minimum.
+ */
+#ifndef FOLD3D_ORACLE_H
+#define FOLD3D_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- fold model (F|grad T| + a.grad T = 1), reference AN_ISO_1_OpenACC.c ---- */
+double orc_edge(double tnew, double st, double xt, double F, double ax,
+                double ay, double az, double dxx, double dzz);
+double orc_surf(double tnew, double st, double xt, double yt, double F,
+                double ax, double ay, double az, double dxx, double dyy,
+                double dzz, double dxz);
+double orc_fold_point(double F, double ax, double ay, double az, double x,
+                      double y, double z);
+double orc_impsurf(double x, double y, double z);
+
+/* Reference input on an n^3 grid over [0,10]^3 (Fold3dPMM.c:129-389). */
+void orc_initialiser(double *T, int n);
+
+/* One plane step: reads plane p along the march axis, writes plane p+step.
+ * F, a = (ax, ay, az) in global axes; h = spacings (dx, dy, dz). */
+void orc_plane(int dir, double *T, int n, int p, double F, const double a[3],
+               const double h[3]);
+/* Whole sweep along one direction in the reference plane order. */
+void orc_direction(int dir, double *T, int n, double F, const double a[3],
+                   const double h[3]);
+/* `iterations` x the six directions in reference order. */
+void orc_sweeps(double *T, int n, int iterations, double F, const double a[3],
+                const double h[3]);
+
+/* OpenMP thread count used by the plane loops (0 = OpenMP default). */
+void orc_set_threads(int nthreads);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,156 @@
+"""ctypes view of the oracle (oracle/liboracle.so) and of the reference checker
+builds (oracle/_ref/libfoldref_<N>.so).
+
+TEST INFRASTRUCTURE ONLY.  Imported by tests/, ``__graft_entry__.smoke()`` and
+the cpu_baseline leg of bench.py, always as the checker; the shipped library
+never imports it.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+REF_DIR = os.path.join(HERE, "_ref")
+
+# Reference model constants (reference Stencil-OpenACC/Fold3dPMM.h:49-52).
+REF_F = 1.1
+REF_A = (-0.35, 0.4, 0.7)
+MAXT = 10e15 * 1.0  # Fold3dPMM.h:6
+
+DIRECTIONS = ("x_down", "x_up", "k_up", "k_down", "y_up", "y_down")
+
+_d = C.c_double
+_dp = C.POINTER(C.c_double)
+_i = C.c_int
+
+
+def build(quiet: bool = True) -> None:
+    """Compile liboracle.so with the committed Makefile."""
+    subprocess.run(["make", "-C", HERE, "liboracle.so"], check=True,
+                   stdout=subprocess.DEVNULL if quiet else None)
+
+
+def _ptr(a: np.ndarray):
+    assert a.dtype == np.float64 and a.flags.c_contiguous
+    return a.ctypes.data_as(_dp)
+
+
+class Oracle:
+    """The CPU restatement (FP64, OpenMP over plane cells)."""
+
+    def __init__(self, path: str = LIB_PATH):
+        if not os.path.exists(path):
+            build()
+        lib = C.CDLL(path)
+        lib.orc_edge.restype = _d
+        lib.orc_edge.argtypes = [_d] * 9
+        lib.orc_surf.restype = _d
+        lib.orc_surf.argtypes = [_d] * 12
+        lib.orc_fold_point.restype = _d
+        lib.orc_fold_point.argtypes = [_d] * 7
+        lib.orc_impsurf.restype = _d
+        lib.orc_impsurf.argtypes = [_d] * 3
+        lib.orc_initialiser.argtypes = [_dp, _i]
+        lib.orc_plane.argtypes = [_i, _dp, _i, _i, _d, _dp, _dp]
+        lib.orc_direction.argtypes = [_i, _dp, _i, _d, _dp, _dp]
+        lib.orc_sweeps.argtypes = [_dp, _i, _i, _d, _dp, _dp]
+        lib.orc_set_threads.argtypes = [_i]
+        self.lib = lib
+
+    # local solvers -------------------------------------------------------
+    def edge(self, *args):
+        return self.lib.orc_edge(*map(float, args))
+
+    def surf(self, *args):
+        return self.lib.orc_surf(*map(float, args))
+
+    def fold_point(self, *args):
+        return self.lib.orc_fold_point(*map(float, args))
+
+    # grids ---------------------------------------------------------------
+    @staticmethod
+    def ref_spacing(n: int):
+        h = (10.0 - 0.0) / (n - 1.0)
+        return (h, h, h)
+
+    def initialiser(self, n: int) -> np.ndarray:
+        T = np.empty((n + 2) ** 3, dtype=np.float64)
+        self.lib.orc_initialiser(_ptr(T), n)
+        return T
+
+    def _model(self, n, F, a, h):
+        F = REF_F if F is None else F
+        a = np.ascontiguousarray(REF_A if a is None else a, dtype=np.float64)
+        h = np.ascontiguousarray(self.ref_spacing(n) if h is None else h, dtype=np.float64)
+        return F, a, h
+
+    def plane(self, d: int, T: np.ndarray, n: int, p: int, F=None, a=None, h=None):
+        F, a, h = self._model(n, F, a, h)
+        self.lib.orc_plane(d, _ptr(T), n, p, F, _ptr(a), _ptr(h))
+
+    def direction(self, d: int, T: np.ndarray, n: int, F=None, a=None, h=None):
+        F, a, h = self._model(n, F, a, h)
+        self.lib.orc_direction(d, _ptr(T), n, F, _ptr(a), _ptr(h))
+
+    def sweeps(self, T: np.ndarray, n: int, iterations: int, F=None, a=None, h=None):
+        F, a, h = self._model(n, F, a, h)
+        self.lib.orc_sweeps(_ptr(T), n, iterations, F, _ptr(a), _ptr(h))
+
+    def set_threads(self, k: int):
+        self.lib.orc_set_threads(k)
+
+
+class RefBuild:
+    """The reference's own solver compiled for an n^3 grid (oracle/_ref)."""
+
+    def __init__(self, n: int):
+        path = os.path.join(REF_DIR, f"libfoldref_{n}.so")
+        if not os.path.exists(path):
+            raise FileNotFoundError(path)
+        lib = C.CDLL(path)
+        lib.ref_edge.restype = _d
+        lib.ref_edge.argtypes = [_d] * 9
+        lib.ref_surf.restype = _d
+        lib.ref_surf.argtypes = [_d] * 12
+        lib.ref_init_fold.restype = _d
+        lib.ref_init_fold.argtypes = [_d] * 7
+        lib.ref_impsurf.restype = _d
+        lib.ref_impsurf.argtypes = [_d] * 3
+        lib.ref_initialiser.argtypes = [_dp]
+        lib.ref_plane.argtypes = [_i, _dp, _i]
+        lib.ref_direction.argtypes = [_i, _dp]
+        lib.ref_sweeps.argtypes = [_dp, _i]
+        assert lib.ref_n() == n
+        self.lib, self.n = lib, n
+
+    def edge(self, *args):
+        return self.lib.ref_edge(*map(float, args))
+
+    def surf(self, *args):
+        return self.lib.ref_surf(*map(float, args))
+
+    def fold_point(self, *args):
+        return self.lib.ref_init_fold(*map(float, args))
+
+    def initialiser(self) -> np.ndarray:
+        T = np.empty((self.n + 2) ** 3, dtype=np.float64)
+        self.lib.ref_initialiser(_ptr(T))
+        return T
+
+    def plane(self, d: int, T: np.ndarray, p: int):
+        self.lib.ref_plane(d, _ptr(T), p)
+
+    def direction(self, d: int, T: np.ndarray):
+        self.lib.ref_direction(d, _ptr(T))
+
+    def sweeps(self, T: np.ndarray, iterations: int):
+        self.lib.ref_sweeps(_ptr(T), iterations)
+
+
+def ref_available(n: int) -> bool:
+    return os.path.exists(os.path.join(REF_DIR, f"libfoldref_{n}.so"))

@@ -1,0 +1,103 @@
+"""The C-ABI library without a GPU: it loads, exports every symbol the headers
+declare, validates arguments before touching HIP, reports errors as codes,
+and its host-side initializer (the reference's ImplicitInitialiser, served by
+libfold3d.so) is bitwise equal to the reference's output."""
+import ctypes as C
+import subprocess
+
+import numpy as np
+import pytest
+
+import fold3d
+from conftest import bits_equal
+
+
+def test_library_loads():
+    L = fold3d.lib()
+    assert b"gfx950" in L.fold3d_version()
+
+
+def test_exports_every_header_symbol():
+    names = fold3d.header_symbols()
+    assert "fold3d_solve" in names and "ImplicitInitialiser" in names
+    out = subprocess.run(["nm", "-D", "--defined-only", fold3d.LIB_PATH],
+                         capture_output=True, text=True, check=True).stdout
+    exported = {line.split()[-1] for line in out.splitlines() if line.strip()}
+    missing = [n for n in names if n not in exported]
+    assert not missing, missing
+
+
+def test_code_object_targets_gfx950():
+    blob = open(fold3d.LIB_PATH, "rb").read()
+    assert b"amdgcn-amd-amdhsa--gfx950" in blob
+
+
+@pytest.mark.parametrize("n", [16, 24])
+def test_host_initialiser_matches_reference(golden, n):
+    assert bits_equal(fold3d.implicit_initialiser(n), golden[f"init_{n}"])
+
+
+def test_host_point_source_matches_reference(golden):
+    L = fold3d.lib()
+    P, want = golden["point_in"], golden["point_out"]
+    got = np.array([L.InitFoldFromPoint(*map(float, row)) for row in P])
+    assert bits_equal(got, want)
+
+
+def _solve(handle, T, n=16, dx=0.5, model=None, opts=None):
+    L = fold3d.lib()
+    m = model if model is not None else fold3d.reference_model()
+    o = opts if opts is not None else fold3d.Opts()
+    if opts is None:
+        L.fold3d_default_opts(C.byref(o))
+    st = fold3d.Stats()
+    return L.fold3d_solve(handle, T, n, n, n, dx, dx, dx, C.byref(m), C.byref(o), C.byref(st))
+
+
+def test_invalid_arguments_are_codes_not_crashes():
+    L = fold3d.lib()
+    dummy = C.c_void_p(0x1000)  # never dereferenced: validation fails first
+    assert _solve(None, dummy) == fold3d.FOLD3D_E_INVALID
+    assert "NULL" in fold3d.last_error()
+    assert _solve(dummy, None) == fold3d.FOLD3D_E_INVALID
+    assert _solve(dummy, dummy, n=2) == fold3d.FOLD3D_E_INVALID
+    assert "too small" in fold3d.last_error()
+    assert _solve(dummy, dummy, dx=-1.0) == fold3d.FOLD3D_E_INVALID
+    assert _solve(dummy, dummy, dx=float("nan")) == fold3d.FOLD3D_E_INVALID
+    o = fold3d.Opts()
+    L.fold3d_default_opts(C.byref(o))
+    o.max_iters = 0
+    assert _solve(dummy, dummy, opts=o) == fold3d.FOLD3D_E_INVALID
+    o.max_iters, o.tol = 8, -1.0
+    assert _solve(dummy, dummy, opts=o) == fold3d.FOLD3D_E_INVALID
+    o.tol, o.precision = 0.0, 7
+    assert _solve(dummy, dummy, opts=o) == fold3d.FOLD3D_E_INVALID
+    m = fold3d.reference_model()
+    m.F = 0.0
+    assert _solve(dummy, dummy, model=m) == fold3d.FOLD3D_E_INVALID
+    m = fold3d.reference_model()
+    m.kind = 9
+    assert _solve(dummy, dummy, model=m) in (fold3d.FOLD3D_E_INVALID, fold3d.FOLD3D_E_UNSUPPORTED)
+    # plane step: direction and plane range are checked
+    assert L.fold3d_plane_step(dummy, 6, dummy, 16, 16, 16, 3, *([1.0] * 10), None) == fold3d.FOLD3D_E_INVALID
+    assert L.fold3d_plane_step(dummy, 0, dummy, 16, 16, 16, 16, *([1.0] * 10), None) == fold3d.FOLD3D_E_INVALID
+    assert L.fold3d_plane_step(dummy, 1, dummy, 16, 16, 16, 1, *([1.0] * 10), None) == fold3d.FOLD3D_E_INVALID
+
+
+def test_create_without_gpu_is_an_error_code():
+    import torch
+    if torch.cuda.is_available():
+        pytest.skip("a GPU is present")
+    h = C.c_void_p()
+    rc = fold3d.lib().fold3d_create(C.byref(h), 0)
+    assert rc in (fold3d.FOLD3D_E_HIP, fold3d.FOLD3D_E_INVALID)
+    assert fold3d.last_error()
+
+
+def test_defaults_follow_the_reference():
+    L = fold3d.lib()
+    o = fold3d.Opts()
+    L.fold3d_default_opts(C.byref(o))
+    assert (o.precision, o.memory, o.max_iters, o.tol) == (fold3d.F64, fold3d.DEVICE, 8, 0.0)
+    m = fold3d.reference_model()
+    assert (m.F, tuple(m.a)) == (1.1, (-0.35, 0.4, 0.7))

@@ -1,0 +1,178 @@
+"""Parity of the HIP fold-model solve (through the C-ABI) with the reference.
+
+FP64 is held bit-for-bit: against golden vectors written from the reference's
+own solver and, at sizes the fixtures do not cover, against the oracle on the
+same inputs.  FP32 is held to a stated relative tolerance of the FP64 result.
+"""
+import hashlib
+
+import numpy as np
+import pytest
+
+from conftest import bits_equal
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+import fold3d  # noqa: E402
+
+# Driver arguments per direction for a cube with spacing h
+# (reference AN_ISO_1_OpenACC.c:1198-1235): (dzz, dxx, dyy, dxy, dxz, dyz, F, ay, ax, az)
+AXX, AYY, AZZ = fold3d.REF_A
+FC = fold3d.REF_F
+
+
+def driver_args(d, h):
+    dg = np.sqrt(h * h + h * h)
+    sp = (h, h, h, dg, dg, dg)
+    model = {
+        0: (FC, AYY, AZZ, AXX), 1: (FC, -AYY, -AZZ, -AXX),
+        2: (FC, AXX, AYY, AZZ), 3: (FC, -AXX, -AYY, -AZZ),
+        4: (FC, -AZZ, -AXX, -AYY), 5: (FC, AZZ, AXX, AYY),
+    }[d]
+    return sp + model
+
+
+def planes_of(d, n):
+    return range(1, n) if d in (0, 2, 5) else range(n, 1, -1)
+
+
+def to_dev(a, dtype=torch.float64):
+    return torch.from_numpy(np.ascontiguousarray(a)).to("cuda", dtype)
+
+
+@pytest.mark.parametrize("d", range(6))
+def test_plane_step_random_field(solver, golden, d):
+    print(f"plane step direction {d} on the 16^3 random field", flush=True)
+    planes = {0: 7, 1: 9, 2: 7, 3: 9, 4: 9, 5: 7}
+    T = to_dev(golden["plane_rand_in_16"])
+    h = fold3d.reference_spacing(16)
+    solver.plane_step(d, T, 16, planes[d], *driver_args(d, h))
+    torch.cuda.synchronize()
+    assert bits_equal(T.cpu().numpy(), golden[f"plane_rand_{d}_16"])
+
+
+@pytest.mark.parametrize("d", range(6))
+def test_direction_on_initial_field(solver, golden, d):
+    T = to_dev(golden["init_16"])
+    h = fold3d.reference_spacing(16)
+    for p in planes_of(d, 16):
+        solver.plane_step(d, T, 16, p, *driver_args(d, h))
+    torch.cuda.synchronize()
+    assert bits_equal(T.cpu().numpy(), golden[f"direction_init_{d}_16"])
+
+
+@pytest.mark.parametrize("n", [16, 24, 32])
+def test_full_solve_matches_reference(solver, golden, n):
+    print(f"full solve n={n}", flush=True)
+    T = to_dev(fold3d.implicit_initialiser(n))
+    solver.solve(T, n, max_iters=1)
+    assert bits_equal(T.cpu().numpy(), golden[f"solve1_{n}"])
+    solver.solve(T, n, max_iters=7)
+    assert bits_equal(T.cpu().numpy(), golden[f"solve8_{n}"])
+
+
+@pytest.mark.parametrize("n", [16, 24, 32])
+def test_convergence_stop_reaches_fixed_point(solver, golden, n):
+    T = to_dev(fold3d.implicit_initialiser(n))
+    st = solver.solve(T, n, max_iters=100, tol=1e-300)
+    assert st.iterations == int(golden[f"fixed_iters_{n}"][0])
+    assert st.max_dT == 0.0
+    assert bits_equal(T.cpu().numpy(), golden[f"fixed_{n}"])
+
+
+def test_not_converged_is_reported(solver):
+    n = 24
+    T = to_dev(fold3d.implicit_initialiser(n))
+    with pytest.raises(fold3d.Fold3dError) as e:
+        solver.solve(T, n, max_iters=2, tol=1e-300)
+    assert e.value.code == fold3d.FOLD3D_E_NOT_CONVERGED
+    st = solver.solve(T, n, max_iters=1, tol=1e-300, raise_not_converged=False)
+    assert st.iterations == 1 and st.max_dT > 0
+
+
+def test_solve_128_matches_reference_digest(solver, golden):
+    print("full solve n=128, 8 iterations", flush=True)
+    T = to_dev(fold3d.implicit_initialiser(128))
+    st = solver.solve(T, 128)
+    out = T.cpu().numpy()
+    assert np.array_equal(out[golden["solve8_128_idx"]], golden["solve8_128_val"])
+    assert hashlib.sha256(out.tobytes()).digest() == golden["solve8_128_sha256"].tobytes()
+    assert st.cell_updates == 8 * 6 * 127 * 128 * 128
+
+
+def test_solve_64_matches_oracle(solver, oracle):
+    n = 64
+    T0 = fold3d.implicit_initialiser(n)
+    want = T0.copy()
+    oracle.sweeps(want, n, 8)
+    T = to_dev(T0)
+    solver.solve(T, n)
+    assert bits_equal(T.cpu().numpy(), want)
+
+
+def test_random_field_and_model_match_oracle(solver, oracle):
+    """A model other than the reference's (F, a) and an unstructured field."""
+    n = 20
+    rng = np.random.default_rng(7)
+    H = n + 2
+    T0 = np.where(rng.random(H ** 3) < 0.3, -1.0, 1.0) * rng.uniform(1.0, 4.0, H ** 3)
+    F, a, h = 1.3, (0.2, -0.5, 0.3), (0.3, 0.3, 0.3)
+    want = T0.copy()
+    oracle.sweeps(want, n, 2, F=F, a=a, h=h)
+    T = to_dev(T0)
+    solver.solve(T, n, spacing=h, model=fold3d.fold_model(F, a), max_iters=2)
+    assert bits_equal(T.cpu().numpy(), want)
+
+
+def test_host_memory_solve(solver, golden):
+    T = fold3d.implicit_initialiser(24)
+    solver.solve(T, 24, max_iters=8)
+    assert bits_equal(T, golden["solve8_24"])
+
+
+def test_reference_signature_plane_steps(golden):
+    """function_x_down & co. with the reference's signatures on the grid set
+    by ImplicitInitialiser()."""
+    import ctypes as C
+    L = fold3d.lib()
+    n = 16
+    T = to_dev(fold3d.implicit_initialiser(n))  # sets the grid to 16^3
+    h = fold3d.reference_spacing(n)
+    for d, name in enumerate(fold3d.DIRECTIONS):
+        for p in planes_of(d, n):
+            getattr(L, "function_" + name)(C.c_void_p(T.data_ptr()), p, *driver_args(d, h))
+    torch.cuda.synchronize()
+    want = golden["init_16"].copy()
+    from oracle.oracle import Oracle
+    o = Oracle()
+    for d in range(6):
+        o.direction(d, want, n)
+    assert bits_equal(T.cpu().numpy(), want)
+    assert bits_equal(want, golden["solve1_16"])
+
+
+def test_deterministic(solver):
+    n = 48
+    T0 = fold3d.implicit_initialiser(n)
+    A, B = to_dev(T0), to_dev(T0)
+    solver.solve(A, n, max_iters=3)
+    solver.solve(B, n, max_iters=3)
+    assert torch.equal(A.view(torch.int64), B.view(torch.int64))
+
+
+@pytest.mark.parametrize("n", [32, 64])
+def test_fp32_within_tolerance_of_fp64(solver, oracle, n):
+    """FP32 field vs the FP64 oracle on the same input, relative error over
+    reached nodes.  Tolerance 1e-5 relative (sign must agree everywhere)."""
+    T0 = fold3d.implicit_initialiser(n)
+    want = T0.copy()
+    oracle.sweeps(want, n, 8)
+    T = to_dev(T0, torch.float32)
+    solver.solve(T, n, max_iters=8)
+    got = T.cpu().double().numpy()
+    assert np.array_equal(np.sign(got), np.sign(want))
+    reached = np.abs(want) < 1e15
+    rel = np.abs(got[reached] - want[reached]) / np.abs(want[reached])
+    print(f"FP32 n={n}: max rel err {rel.max():.3e}, mean {rel.mean():.3e}")
+    assert rel.max() < 1e-5

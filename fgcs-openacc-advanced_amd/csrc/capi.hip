@@ -15,7 +15,12 @@ struct fold3d_handle {
   int device;
   unsigned long long *d_max;  // max |dT| of the running iteration
   hipEvent_t ev0, ev1;
+  void *d_consts;             // per-direction constants of the running solve
+  void *h_consts;             // pinned staging for them
+  hipEvent_t consts_copied;   // staging may be rewritten once this completes
 };
+
+static constexpr size_t kConstBytes = 6 * sizeof(FoldFast<double>);
 
 extern "C" {
 
@@ -60,6 +65,9 @@ int fold3d_create(fold3d_handle **out, int device) {
   }
   F3D_HIP(hipEventCreate(&h->ev0));
   F3D_HIP(hipEventCreate(&h->ev1));
+  F3D_HIP(hipEventCreateWithFlags(&h->consts_copied, hipEventDisableTiming));
+  F3D_HIP(hipMalloc(&h->d_consts, kConstBytes));
+  F3D_HIP(hipHostMalloc(&h->h_consts, kConstBytes, hipHostMallocDefault));
   *out = h;
   return FOLD3D_OK;
 }
@@ -67,6 +75,10 @@ int fold3d_create(fold3d_handle **out, int device) {
 int fold3d_destroy(fold3d_handle *h) {
   if (!h) return FOLD3D_OK;
   (void)hipSetDevice(h->device);
+  (void)hipEventSynchronize(h->consts_copied);
+  (void)hipFree(h->d_consts);
+  (void)hipHostFree(h->h_consts);
+  (void)hipEventDestroy(h->consts_copied);
   (void)hipFree(h->d_max);
   (void)hipEventDestroy(h->ev0);
   (void)hipEventDestroy(h->ev1);
@@ -92,10 +104,31 @@ int fold3d_solve(fold3d_handle *h, void *T, int nx, int ny, int nz, double dx,
             "fold3d_solve: bad precision %d", o.precision);
   F3D_CHECK(o.memory == FOLD3D_DEVICE || o.memory == FOLD3D_HOST, FOLD3D_E_INVALID,
             "fold3d_solve: bad memory kind %d", o.memory);
-  F3D_CHECK(model->kind == FOLD3D_MODEL_FOLD, FOLD3D_E_UNSUPPORTED,
-            "fold3d_solve: model kind %d not available", model->kind);
-  F3D_CHECK(std::isfinite(model->F) && model->F > 0, FOLD3D_E_INVALID,
-            "fold3d_solve: F must be positive");
+  F3D_CHECK(model->kind == FOLD3D_MODEL_FOLD || model->kind == FOLD3D_MODEL_ELLIP,
+            FOLD3D_E_INVALID, "fold3d_solve: unknown model kind %d", model->kind);
+  if (model->kind == FOLD3D_MODEL_FOLD) {
+    F3D_CHECK(std::isfinite(model->F) && model->F > 0, FOLD3D_E_INVALID,
+              "fold3d_solve: F must be positive");
+  } else {
+    const int nf = (model->theta != nullptr) + (model->phi != nullptr) +
+                   (model->ra != nullptr) + (model->rb != nullptr);
+    F3D_CHECK(nf == 0 || nf == 4, FOLD3D_E_INVALID,
+              "fold3d_solve: give all four of theta, phi, ra, rb or none");
+    F3D_CHECK(model->speed_field != nullptr || (std::isfinite(model->speed) && model->speed > 0),
+              FOLD3D_E_INVALID, "fold3d_solve: speed must be positive");
+    if (nf == 0) {
+      const double *M = model->M;
+      bool fin = true;
+      for (int k = 0; k < 6; ++k) fin = fin && std::isfinite(M[k]);
+      const double m1 = M[0], m2 = M[0] * M[1] - M[3] * M[3];
+      const double m3 = M[0] * (M[1] * M[2] - M[5] * M[5]) - M[3] * (M[3] * M[2] - M[5] * M[4]) +
+                        M[4] * (M[3] * M[5] - M[1] * M[4]);
+      F3D_CHECK(fin && m1 > 0 && m2 > 0 && m3 > 0, FOLD3D_E_INVALID,
+                "fold3d_solve: metric M is not symmetric positive definite");
+    }
+    F3D_CHECK(o.memory == FOLD3D_DEVICE || (model->speed_field == nullptr && nf == 0),
+              FOLD3D_E_INVALID, "fold3d_solve: per-node fields need a device-resident T");
+  }
 
   F3D_HIP(hipSetDevice(h->device));
   hipStream_t st = (hipStream_t)o.stream;
@@ -111,13 +144,38 @@ int fold3d_solve(fold3d_handle *h, void *T, int nx, int ny, int nz, double dx,
   int iters = 0;
   double maxdt = -1;
   F3D_HIP(hipEventRecord(h->ev0, st));
-  int rc = o.precision == FOLD3D_F64
-               ? fold_sweeps_plane_engine<double>(st, (double *)dT, G, *model, hsp,
-                                                  o.max_iters, o.tol, h->d_max,
-                                                  &iters, &maxdt)
-               : fold_sweeps_plane_engine<float>(st, (float *)dT, G, *model, hsp,
-                                                 o.max_iters, o.tol, h->d_max,
-                                                 &iters, &maxdt);
+  int rc;
+  if (model->kind == FOLD3D_MODEL_FOLD && o.engine == FOLD3D_ENGINE_PLANE) {
+    rc = o.precision == FOLD3D_F64
+             ? fold_sweeps_plane_engine<double>(st, (double *)dT, G, *model, hsp, o.max_iters,
+                                                o.tol, h->d_max, &iters, &maxdt)
+             : fold_sweeps_plane_engine<float>(st, (float *)dT, G, *model, hsp, o.max_iters,
+                                               o.tol, h->d_max, &iters, &maxdt);
+  } else if (model->kind == FOLD3D_MODEL_FOLD) {
+    // stage the six directions' constants (the previous solve's copy must be done)
+    F3D_HIP(hipEventSynchronize(h->consts_copied));
+    if (o.precision == FOLD3D_F64) {
+      FoldFast<double> *c = (FoldFast<double> *)h->h_consts;
+      for (int d = 0; d < 6; ++d) build_fold_fast<double>(d, *model, hsp, c[d]);
+    } else {
+      FoldFast<float> *c = (FoldFast<float> *)h->h_consts;
+      for (int d = 0; d < 6; ++d) build_fold_fast<float>(d, *model, hsp, c[d]);
+    }
+    F3D_HIP(hipMemcpyAsync(h->d_consts, h->h_consts, kConstBytes, hipMemcpyHostToDevice, st));
+    F3D_HIP(hipEventRecord(h->consts_copied, st));
+    rc = o.precision == FOLD3D_F64
+             ? fold_sweeps_fast_engine<double>(st, (double *)dT, G,
+                                               (const FoldFast<double> *)h->d_consts,
+                                               o.max_iters, o.tol, h->d_max, &iters, &maxdt)
+             : fold_sweeps_fast_engine<float>(st, (float *)dT, G,
+                                              (const FoldFast<float> *)h->d_consts,
+                                              o.max_iters, o.tol, h->d_max, &iters, &maxdt);
+  } else
+    rc = o.precision == FOLD3D_F64
+             ? ellip_sweeps_plane_engine<double>(st, (double *)dT, G, *model, hsp, o.max_iters,
+                                                 o.tol, h->d_max, &iters, &maxdt)
+             : ellip_sweeps_plane_engine<float>(st, (float *)dT, G, *model, hsp, o.max_iters,
+                                                o.tol, h->d_max, &iters, &maxdt);
   F3D_HIP(hipEventRecord(h->ev1, st));
   if (rc != FOLD3D_OK) return rc;
   if (o.memory == FOLD3D_HOST) {

@@ -1,0 +1,185 @@
+// ellip_local.h — device local solver of the tilted elliptic model
+//     sqrt(grad T^T M grad T) = s(x) = 1/v(x),   M symmetric positive definite,
+// on the same 3x3 upwind patch and sweep frames as the reference's fold model
+// (AN_ISO_1_OpenACC.c:195-326).  The reference has no such model.
+//
+// With N = M^-1, a straight segment d costs s*|d|_N.  The new value of target X
+// is min(T_old, min over the patch's simplices S of min_{Y in S} T_lin(Y) +
+// s|X - Y|_N).  The simplices are the 9 nodes, 16 edges (8 spokes from the
+// centre, 8 rim edges) and 8 triangles fanned around the centre.  For a
+// simplex with base Y0, edge vectors E and D = X - Y0 the constants
+//     P = (E^T N E)^-1,  l0 = P E^T N D,  kappa = D^T N D - (E^T N D).l0
+// depend only on the metric and the spacings; per node only the value
+// differences delta_i = T_i - T_0 enter:
+//     mu = P delta, r = s^2 - delta.mu, root = sqrt(kappa r),
+//     lambda = l0 - (root/r) mu in the simplex  =>  candidate T_0 + delta.l0 + root.
+// A candidate of S is never below min over S's vertices, so a simplex whose
+// smallest vertex is not below the running minimum is skipped: pruning that
+// cannot change the result.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "fold_local.h"  // rmin
+
+namespace fold3d {
+
+// In-plane ring order around the centre, in units of (dxx, dyy):
+// +x, +x+y, +y, -x+y, -x, -x-y, -y, +x-y.
+constexpr int kRing[8][2] = {{1, 0}, {1, 1}, {0, 1}, {-1, 1},
+                             {-1, 0}, {-1, -1}, {0, -1}, {1, -1}};
+
+template <typename R>
+struct EllipConsts {
+  R vert[9];      // |X - Y|_N, centre then ring
+  R edge[16][3];  // P, l0, kappa; 0..7 spokes centre->ring k, 8..15 rims ring k -> k+1
+  R tri[8][6];    // P11, P12, P22, l1, l2, kappa; centre, ring k, ring k+1
+};
+
+template <typename R>
+__device__ __forceinline__ R ellip_sqrt(R x);
+template <>
+__device__ __forceinline__ double ellip_sqrt<double>(double x) { return sqrt(x); }
+template <>
+__device__ __forceinline__ float ellip_sqrt<float>(float x) { return __builtin_amdgcn_sqrtf(x); }
+
+// Unreached nodes carry MAXT (1e16); a simplex touching one is not used.
+template <typename R>
+__device__ __forceinline__ bool reached(R t) { return t < R(1e15); }
+
+template <typename R>
+__device__ __forceinline__ void ellip_edge(R &best, R t0, R t1, R s2, const R *E) {
+  if (!(rmin(t0, t1) < best) || !(reached(t0) && reached(t1))) return;
+  const R P = E[0], l0 = E[1], kap = E[2];
+  const R d = t1 - t0;
+  const R mu = P * d;
+  const R r = s2 - d * mu;
+  if (!(r > R(0))) return;
+  const R root = ellip_sqrt(kap * r);
+  if (!(l0 * r - root * mu >= R(0))) return;
+  if (!((l0 - R(1)) * r - root * mu <= R(0))) return;
+  const R c = t0 + d * l0 + root;
+  if (c < best) best = c;
+}
+
+template <typename R>
+__device__ __forceinline__ void ellip_tri(R &best, R t0, R t1, R t2, R s2, const R *Tr) {
+  if (!(rmin(t0, rmin(t1, t2)) < best) || !(reached(t0) && reached(t1) && reached(t2)))
+    return;
+  const R d1 = t1 - t0, d2 = t2 - t0;
+  const R m1 = Tr[0] * d1 + Tr[1] * d2;
+  const R m2 = Tr[1] * d1 + Tr[2] * d2;
+  const R r = s2 - (d1 * m1 + d2 * m2);
+  if (!(r > R(0))) return;
+  const R root = ellip_sqrt(Tr[5] * r);
+  if (!(Tr[3] * r - root * m1 >= R(0))) return;
+  if (!(Tr[4] * r - root * m2 >= R(0))) return;
+  if (!((Tr[3] + Tr[4] - R(1)) * r - root * (m1 + m2) <= R(0))) return;
+  const R c = t0 + (d1 * Tr[3] + d2 * Tr[4]) + root;
+  if (c < best) best = c;
+}
+
+// p[0] = centre, p[1..8] = ring.  Returns the new (non-increasing) value.
+template <typename R>
+__device__ __forceinline__ R ellip_node_update(R told, const R p[9], R s,
+                                               const EllipConsts<R> &K) {
+  R best = told;
+  const R s2 = s * s;
+#pragma unroll
+  for (int k = 0; k < 9; ++k)
+    if (reached(p[k])) best = rmin(p[k] + s * K.vert[k], best);
+#pragma unroll
+  for (int k = 0; k < 8; ++k) ellip_edge(best, p[0], p[1 + k], s2, K.edge[k]);
+#pragma unroll
+  for (int k = 0; k < 8; ++k) ellip_edge(best, p[1 + k], p[1 + (k + 1) % 8], s2, K.edge[8 + k]);
+#pragma unroll
+  for (int k = 0; k < 8; ++k) ellip_tri(best, p[0], p[1 + k], p[1 + (k + 1) % 8], s2, K.tri[k]);
+  return best;
+}
+
+// Local-frame constants for inverse metric N (xx, yy, zz, xy, xz, yz in global
+// axes), frame axes u = (x^, y^, z^) with signs sg, spacings (dxx, dyy, dzz).
+// Used on the host for a constant metric and on the device per node.
+template <typename R>
+__host__ __device__ inline void ellip_constants(const R N[6], const int u[3],
+                                                const R sg[3], const R hl[3],
+                                                EllipConsts<R> &K) {
+  const R Ng[3][3] = {{N[0], N[3], N[4]}, {N[3], N[1], N[5]}, {N[4], N[5], N[2]}};
+  R NL[3][3];
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) NL[i][j] = sg[i] * sg[j] * Ng[u[i]][u[j]];
+  const R dxx = hl[0], dyy = hl[1], dzz = hl[2];
+  R pos[9][3];
+  pos[0][0] = R(0);
+  pos[0][1] = R(0);
+  pos[0][2] = R(0);
+  for (int k = 0; k < 8; ++k) {
+    pos[k + 1][0] = kRing[k][0] * dxx;
+    pos[k + 1][1] = kRing[k][1] * dyy;
+    pos[k + 1][2] = R(0);
+  }
+  const R X[3] = {R(0), R(0), dzz};
+  auto ndot = [&](const R *p, const R *q) {
+    return p[0] * (NL[0][0] * q[0] + NL[0][1] * q[1] + NL[0][2] * q[2]) +
+           p[1] * (NL[1][0] * q[0] + NL[1][1] * q[1] + NL[1][2] * q[2]) +
+           p[2] * (NL[2][0] * q[0] + NL[2][1] * q[1] + NL[2][2] * q[2]);
+  };
+  for (int k = 0; k < 9; ++k) {
+    R D[3] = {X[0] - pos[k][0], X[1] - pos[k][1], X[2] - pos[k][2]};
+    K.vert[k] = sqrt(ndot(D, D));
+  }
+  for (int k = 0; k < 16; ++k) {
+    const int v0 = k < 8 ? 0 : 1 + (k - 8);
+    const int v1 = k < 8 ? 1 + k : 1 + ((k - 8 + 1) % 8);
+    R E[3], D[3];
+    for (int i = 0; i < 3; ++i) {
+      E[i] = pos[v1][i] - pos[v0][i];
+      D[i] = X[i] - pos[v0][i];
+    }
+    const R g = ndot(E, E), b = ndot(E, D), c = ndot(D, D);
+    const R P = R(1) / g;
+    const R l0 = P * b;
+    K.edge[k][0] = P;
+    K.edge[k][1] = l0;
+    K.edge[k][2] = c - b * l0;
+  }
+  for (int k = 0; k < 8; ++k) {
+    const int v1 = 1 + k, v2 = 1 + (k + 1) % 8;
+    const R *E1 = pos[v1], *E2 = pos[v2];
+    const R g11 = ndot(E1, E1), g12 = ndot(E1, E2), g22 = ndot(E2, E2);
+    const R b1 = ndot(E1, X), b2 = ndot(E2, X), c = ndot(X, X);
+    const R det = g11 * g22 - g12 * g12;
+    const R P11 = g22 / det, P12 = -g12 / det, P22 = g11 / det;
+    const R l1 = P11 * b1 + P12 * b2, l2 = P12 * b1 + P22 * b2;
+    K.tri[k][0] = P11;
+    K.tri[k][1] = P12;
+    K.tri[k][2] = P22;
+    K.tri[k][3] = l1;
+    K.tri[k][4] = l2;
+    K.tri[k][5] = c - (b1 * l1 + b2 * l2);
+  }
+}
+
+// M = R diag(1, ra, rb)^2 R^T with R = Rz(phi) Ry(theta); returns N = M^-1 =
+// R diag(1, 1/ra^2, 1/rb^2) R^T directly (no 3x3 inverse needed).
+template <typename R>
+__host__ __device__ inline void tilted_inverse_metric(R theta, R phi, R ra, R rb, R N[6]) {
+  const R ct = cos(theta), st = sin(theta), cp = cos(phi), sp = sin(phi);
+  const R Q[3][3] = {{cp * ct, -sp, cp * st}, {sp * ct, cp, sp * st}, {-st, R(0), ct}};
+  const R w[3] = {R(1), R(1) / (ra * ra), R(1) / (rb * rb)};
+  R F[3][3];
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) {
+      R acc = R(0);
+      for (int k = 0; k < 3; ++k) acc += Q[i][k] * w[k] * Q[j][k];
+      F[i][j] = acc;
+    }
+  N[0] = F[0][0];
+  N[1] = F[1][1];
+  N[2] = F[2][2];
+  N[3] = F[0][1];
+  N[4] = F[0][2];
+  N[5] = F[1][2];
+}
+
+}  // namespace fold3d

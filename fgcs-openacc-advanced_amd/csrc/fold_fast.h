@@ -1,0 +1,296 @@
+// fold_fast.h — the fold-model node update of fold_local.h with its
+// per-direction invariants hoisted out of the node loop.
+//
+// In the reference's local solvers (AN_ISO_1_OpenACC.c:59-175) every
+// parameter except the ten patch values is fixed per sweep direction and per
+// sign of the target node (a flips where T < 0):
+//   * each edge call site has fixed (ax, ay, az, dxx, dzz), so its one-point
+//     candidate xt + len2/c (:99-113) is xt + a constant;
+//   * the one-point update from the centre (:214-228) is st + a constant;
+//   * the rotated-frame parameters of the diagonal edges and of the triangle
+//     sub-edges ((ax*dxx + ay*dyy)/dxy, (ax*dzz + az*dxx)/dxz, ...) are constants;
+//   * every division by one of those constants (x/dxx, x/(ga - az^2), x/ga) is
+//     computed as a reciprocal multiply with two fused residual corrections,
+//     which returns the correctly rounded quotient (Markstein's theorem: once
+//     q is within one ulp of x/d, q + (x - q d) * RN(1/d) rounds to RN(x/d)),
+//     i.e. the same bits as the IEEE division.
+// Everything else keeps the reference's operation order, so results stay
+// bit-identical to fold_local.h (and to the reference) in FP64.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "fold_local.h"
+
+namespace fold3d {
+
+// x / d with r = RN(1/d): correctly rounded for finite x, d and normal results.
+__device__ __forceinline__ double div_const(double x, double d, double r) {
+  double q = x * r;
+  double e = fma(-q, d, x);
+  q = fma(e, r, q);
+  e = fma(-q, d, x);
+  return fma(e, r, q);
+}
+__device__ __forceinline__ float div_const(float x, float d, float r) {
+  float q = x * r;
+  float e = fmaf(-q, d, x);
+  q = fmaf(e, r, q);
+  e = fmaf(-q, d, x);
+  return fmaf(e, r, q);
+}
+
+template <typename R>
+__device__ __forceinline__ R fast_sqrt(R x);
+template <>
+__device__ __forceinline__ double fast_sqrt<double>(double x) { return sqrt(x); }
+template <>
+__device__ __forceinline__ float fast_sqrt<float>(float x) { return __builtin_amdgcn_sqrtf(x); }
+
+// One edge call site (FoldEdge2Points with fixed model/spacing arguments).
+template <typename R>
+struct EdgeSite {
+  R ax, az, dxx, dzz;
+  R ga, rga;        // F^2 - ay^2 and its reciprocal
+  R gaz, rgaz;      // ga - az^2 and its reciprocal
+  R rdxx;           // 1/dxx
+  R k1;             // one-point constant len2/c of the far end
+  int ax_pos;       // ax > 0
+  int k1_ok;        // the one-point candidate exists
+};
+
+// One triangle call site (FoldSurf3Points with fixed arguments) and its sub-edge.
+template <typename R>
+struct TriSite {
+  R ax, ay, dxx, dyy, rdxx, rdyy;
+  int ax_pos, ay_pos;
+  int sub_pos;      // ax*dzz + az*dxx > 0
+};
+
+// Everything one sign of one direction needs.
+template <typename R>
+struct FoldSign {
+  R F, az, dzz;
+  R ga, rga;        // triangles: F^2 - az^2 (shared by all eight) and reciprocal
+  R k0;             // centre one-point constant dzz^2/d
+  int k0_ok;
+  TriSite<R> tri[8];
+  EdgeSite<R> edge[16];  // 0..7 the node's edges, 8..15 the triangles' sub-edges
+};
+
+template <typename R>
+struct FoldFast {
+  FoldSign<R> sgn[2];  // [0]: T_target >= 0 (a as given), [1]: T_target < 0 (a negated)
+};
+
+template <typename R>
+__device__ __forceinline__ void fast_edge(R &tnew, R st, R xt, R F, const EdgeSite<R> &e) {
+  if (tnew < rmin(xt, st) || (e.ax_pos && xt > st)) return;
+  const R gx = div_const(xt - st, e.dxx, e.rdxx);
+  const R q = R(1) - gx * e.ax;
+  const R disc = e.ga * (q * q - gx * gx * e.gaz);
+  if (R(0) < disc) {
+    const R gz = div_const(-e.az * q + fast_sqrt(disc), e.gaz, e.rgaz);
+    const R cand = st + e.dzz * gz;
+    if (cand < tnew && cand > rmin(st, xt)) {
+      R t = (gx * gx + gz * gz) * F;
+      t = t * F;
+      const R g = fast_sqrt(div_const(t, e.ga, e.rga));
+      const R foot = -e.dzz * (gx + e.ax * g) / (gz + e.az * g);
+      if (R(0) < foot && foot < e.dxx) tnew = cand;
+    }
+  }
+  if (tnew > xt && e.k1_ok) tnew = rmin(tnew, xt + e.k1);
+}
+
+template <typename R>
+__device__ __forceinline__ void fast_tri(R &tnew, R st, R xt, R yt, const FoldSign<R> &S,
+                                         const TriSite<R> &t, const EdgeSite<R> &sub) {
+  if (tnew < rmin(st, rmin(xt, yt)) || (t.ax_pos && xt > st) || (t.ay_pos && yt > xt)) return;
+  const R F = S.F, az = S.az, dzz = S.dzz;
+  const R gx = div_const(xt - st, t.dxx, t.rdxx);
+  const R gy = div_const(yt - xt, t.dyy, t.rdyy);
+  const R g2 = gx * gx + gy * gy;
+  const R al = R(1) - t.ax * gx - t.ay * gy;
+  const R disc = al * al - g2 * S.ga;
+  if (disc > R(0)) {
+    const R gz = div_const(-az * al + F * fast_sqrt(disc), S.ga, S.rga);
+    const R cand = st + dzz * gz;
+    if (cand < tnew && cand > rmin(rmin(st, xt), yt)) {
+      const R g = fast_sqrt(g2 + gz * gz);
+      const R s = -dzz / (F * gz + az * g);
+      const R fx = s * (F * gx + t.ax * g);
+      const R fy = s * (F * gy + t.ay * g);
+      if (R(0) <= rmin(fx, fy) && fy * t.dxx <= fx * t.dyy && fx <= t.dxx) tnew = cand;
+    }
+  }
+  if (tnew > rmin(xt, yt) && (!(t.sub_pos && xt > yt))) fast_edge(tnew, xt, yt, F, sub);
+}
+
+// Same result as fold_node_update(), bit for bit.
+template <typename R>
+__device__ __forceinline__ R fold_node_fast(R told, const Patch<R> &p, const FoldSign<R> &S) {
+  R tnew = fabs(told);
+  const R st = p.c;
+  if (st < tnew && S.k0_ok) tnew = rmin(tnew, st + S.k0);
+  if (!(tnew < st && tnew < rmin(p.px, p.py) && tnew < rmin(p.pxpy, p.pxmy) &&
+        tnew < rmin(p.mxmy, p.mxpy))) {
+    // (centre, first, second) vertices of the eight triangles, reference order
+    fast_tri(tnew, st, p.px, p.pxpy, S, S.tri[0], S.edge[8]);
+    fast_tri(tnew, st, p.px, p.pxmy, S, S.tri[1], S.edge[9]);
+    fast_tri(tnew, st, p.mx, p.mxmy, S, S.tri[2], S.edge[10]);
+    fast_tri(tnew, st, p.mx, p.mxpy, S, S.tri[3], S.edge[11]);
+    fast_tri(tnew, st, p.my, p.mxmy, S, S.tri[4], S.edge[12]);
+    fast_tri(tnew, st, p.my, p.pxmy, S, S.tri[5], S.edge[13]);
+    fast_tri(tnew, st, p.py, p.pxpy, S, S.tri[6], S.edge[14]);
+    fast_tri(tnew, st, p.py, p.mxpy, S, S.tri[7], S.edge[15]);
+    fast_edge(tnew, st, p.px, S.F, S.edge[0]);
+    fast_edge(tnew, st, p.pxpy, S.F, S.edge[1]);
+    fast_edge(tnew, st, p.py, S.F, S.edge[2]);
+    fast_edge(tnew, st, p.mxpy, S.F, S.edge[3]);
+    fast_edge(tnew, st, p.mx, S.F, S.edge[4]);
+    fast_edge(tnew, st, p.mxmy, S.F, S.edge[5]);
+    fast_edge(tnew, st, p.my, S.F, S.edge[6]);
+    fast_edge(tnew, st, p.pxmy, S.F, S.edge[7]);
+  }
+  return ((told > R(0)) ? R(1) : R(-1)) * tnew;
+}
+
+// ---------------------------------------------------------------------------
+// Host-side construction (plain double arithmetic, same expressions as the
+// reference, so the hoisted values carry the bits the node loop would compute).
+
+inline void host_edge_site(double F, double ax, double ay, double az, double dxx,
+                           double dzz, EdgeSite<double> &e) {
+  e.ax = ax;
+  e.az = az;
+  e.dxx = dxx;
+  e.dzz = dzz;
+  e.ga = F * F - ay * ay;
+  e.rga = 1.0 / e.ga;
+  e.gaz = e.ga - az * az;
+  e.rgaz = 1.0 / e.gaz;
+  e.rdxx = 1.0 / dxx;
+  e.ax_pos = ax > 0.0;
+  double c = (-dxx * ax + dzz * az);
+  const double len2 = dxx * dxx + dzz * dzz;
+  const double d = (F * F - ax * ax - ay * ay - az * az) * len2 + c * c;
+  e.k1_ok = 0;
+  e.k1 = 0.0;
+  if (d > 0.0) {
+    c = c + std::sqrt(d);
+    if (c > 0.0) {
+      e.k1_ok = 1;
+      e.k1 = len2 / c;
+    }
+  }
+}
+
+// FoldFrame f holds (F, ax, ay, az, dzz, dxx, dyy, dxy, dxz, dyz) of the direction.
+inline void host_fold_sign(const FoldFrame<double> &f, double sgn, FoldSign<double> &S) {
+  const double F = f.F;
+  // a flips on the negative side exactly as `a *= -1` does
+  const double ax = f.ax * sgn, ay = f.ay * sgn, az = f.az * sgn;
+  const double dzz = f.dzz, dxx = f.dxx, dyy = f.dyy, dxy = f.dxy, dxz = f.dxz, dyz = f.dyz;
+  S.F = F;
+  S.az = az;
+  S.dzz = dzz;
+  S.ga = F * F - az * az;
+  S.rga = 1.0 / S.ga;
+  double d = (F * F - (ax * ax + ay * ay)) * dzz * dzz;
+  S.k0_ok = 0;
+  S.k0 = 0.0;
+  if (d > 0.0) {
+    d = az * dzz + std::sqrt(d);
+    if (d > 0.0) {
+      S.k0_ok = 1;
+      S.k0 = dzz * dzz / d;
+    }
+  }
+  // triangles (AN_ISO_1_OpenACC.c:233-240): (ax_t, ay_t, dxx_t, dyy_t, dxz_t)
+  const double tp[8][5] = {
+      {ax, ay, dxx, dyy, dxz},   {ax, -ay, dxx, dyy, dxz},  {-ax, -ay, dxx, dyy, dxz},
+      {-ax, ay, dxx, dyy, dxz},  {-ay, -ax, dyy, dxx, dyz}, {-ay, ax, dyy, dxx, dyz},
+      {ay, ax, dyy, dxx, dyz},   {ay, -ax, dyy, dxx, dyz}};
+  for (int k = 0; k < 8; ++k) {
+    TriSite<double> &t = S.tri[k];
+    t.ax = tp[k][0];
+    t.ay = tp[k][1];
+    t.dxx = tp[k][2];
+    t.dyy = tp[k][3];
+    t.rdxx = 1.0 / t.dxx;
+    t.rdyy = 1.0 / t.dyy;
+    t.ax_pos = t.ax > 0.0;
+    t.ay_pos = t.ay > 0.0;
+    t.sub_pos = t.ax * dzz + az * t.dxx > 0.0;
+    const double txz = tp[k][4];
+    // sub-edge xt -> yt in the rotated frame (:170-174)
+    host_edge_site(F, t.ay, (t.ax * dzz + az * t.dxx) / txz, (-t.ax * t.dxx + az * dzz) / txz,
+                   t.dyy, txz, S.edge[8 + k]);
+  }
+  // the node's own edges (:242-253), order px, pxpy, py, mxpy, mx, mxmy, my, pxmy
+  host_edge_site(F, ax, ay, az, dxx, dzz, S.edge[0]);
+  host_edge_site(F, (ax * dxx + ay * dyy) / dxy, (-ax * dyy + ay * dxx) / dxy, az, dxy, dzz,
+                 S.edge[1]);
+  host_edge_site(F, ay, ax, az, dyy, dzz, S.edge[2]);
+  host_edge_site(F, -(ax * dxx - ay * dyy) / dxy, (ax * dyy + ay * dxx) / dxy, az, dxy, dzz,
+                 S.edge[3]);
+  host_edge_site(F, -ax, ay, az, dxx, dzz, S.edge[4]);
+  host_edge_site(F, -(ax * dxx + ay * dyy) / dxy, (-ax * dyy + ay * dxx) / dxy, az, dxy, dzz,
+                 S.edge[5]);
+  host_edge_site(F, -ay, ax, az, dyy, dzz, S.edge[6]);
+  host_edge_site(F, (ax * dxx - ay * dyy) / dxy, (ax * dyy + ay * dxx) / dxy, az, dxy, dzz,
+                 S.edge[7]);
+}
+
+template <typename R>
+inline void narrow(const FoldFast<double> &in, FoldFast<R> &out);
+
+template <>
+inline void narrow<double>(const FoldFast<double> &in, FoldFast<double> &out) { out = in; }
+
+template <>
+inline void narrow<float>(const FoldFast<double> &in, FoldFast<float> &out) {
+  for (int s = 0; s < 2; ++s) {
+    const FoldSign<double> &a = in.sgn[s];
+    FoldSign<float> &b = out.sgn[s];
+    b.F = (float)a.F;
+    b.az = (float)a.az;
+    b.dzz = (float)a.dzz;
+    b.ga = (float)a.ga;
+    b.rga = 1.0f / b.ga;
+    b.k0 = (float)a.k0;
+    b.k0_ok = a.k0_ok;
+    for (int k = 0; k < 8; ++k) {
+      const TriSite<double> &x = a.tri[k];
+      TriSite<float> &y = b.tri[k];
+      y.ax = (float)x.ax;
+      y.ay = (float)x.ay;
+      y.dxx = (float)x.dxx;
+      y.dyy = (float)x.dyy;
+      y.rdxx = 1.0f / y.dxx;
+      y.rdyy = 1.0f / y.dyy;
+      y.ax_pos = x.ax_pos;
+      y.ay_pos = x.ay_pos;
+      y.sub_pos = x.sub_pos;
+    }
+    for (int k = 0; k < 16; ++k) {
+      const EdgeSite<double> &x = a.edge[k];
+      EdgeSite<float> &y = b.edge[k];
+      y.ax = (float)x.ax;
+      y.az = (float)x.az;
+      y.dxx = (float)x.dxx;
+      y.dzz = (float)x.dzz;
+      y.ga = (float)x.ga;
+      y.rga = 1.0f / y.ga;
+      y.gaz = (float)x.gaz;
+      y.rgaz = 1.0f / y.gaz;
+      y.rdxx = 1.0f / y.dxx;
+      y.k1 = (float)x.k1;
+      y.ax_pos = x.ax_pos;
+      y.k1_ok = x.k1_ok;
+    }
+  }
+}
+
+}  // namespace fold3d

@@ -11,6 +11,7 @@
 #include <vector>
 
 #include "common.h"
+#include "fold_fast.h"
 #include "fold_local.h"
 #include "solver.h"
 
@@ -185,5 +186,102 @@ int fold_plane_step_f64(hipStream_t st, double *T, const Grid &G, int dir,
   F3D_HIP(hipGetLastError());
   return FOLD3D_OK;
 }
+
+// ---------------------------------------------------------------------------
+// Hoisted-constant engine: same plane loop, node update from fold_fast.h.
+// C points at this direction's FoldFast in device memory; the two sign
+// variants sit in separate branches so a wave of one sign runs one of them
+// with its constants in scalar registers.
+// ---------------------------------------------------------------------------
+template <typename R, bool kReduce>
+__global__ __launch_bounds__(kLaneX *kRowsY) void fold_plane_fast_kernel(
+    R *__restrict__ T, PlaneGeom g, const FoldFast<R> *__restrict__ C,
+    unsigned long long *__restrict__ max_dt) {
+  const int a = 1 + blockIdx.x * kLaneX + threadIdx.x;
+  const int b = 1 + blockIdx.y * kRowsY + threadIdx.y;
+  R d = R(0);
+  if (a <= g.na && b <= g.nb) {
+    const long long c = g.base + a * g.Sa + b * g.Sb;
+    Patch<R> p;
+    p.c = fabs(T[c]);
+    p.px = fabs(T[c + g.su]);
+    p.py = fabs(T[c + g.sv]);
+    p.mx = fabs(T[c - g.su]);
+    p.my = fabs(T[c - g.sv]);
+    p.pxpy = fabs(T[c + g.su + g.sv]);
+    p.pxmy = fabs(T[c + g.su - g.sv]);
+    p.mxpy = fabs(T[c - g.su + g.sv]);
+    p.mxmy = fabs(T[c - g.su - g.sv]);
+    const R told = T[c + g.sm];
+    R tn;
+    if (told < R(0))
+      tn = fold_node_fast(told, p, C->sgn[1]);
+    else
+      tn = fold_node_fast(told, p, C->sgn[0]);
+    if (tn != told) T[c + g.sm] = tn;
+    if (kReduce) d = fabs(tn - told);
+  }
+  if (kReduce) {
+    double v = (double)d;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, 64));
+    if ((threadIdx.x & 63) == 0 && v > 0.0)
+      atomicMax(max_dt, (unsigned long long)__double_as_longlong(v));
+  }
+}
+
+template <typename R>
+void build_fold_fast(int dir, const fold3d_model &m, const double h[3], FoldFast<R> &out) {
+  const FoldFrame<double> f = fold_frame<double>(dir, m.F, m.a, h);
+  FoldFast<double> d;
+  host_fold_sign(f, 1.0, d.sgn[0]);
+  host_fold_sign(f, -1.0, d.sgn[1]);
+  narrow<R>(d, out);
+}
+template void build_fold_fast<double>(int, const fold3d_model &, const double *, FoldFast<double> &);
+template void build_fold_fast<float>(int, const fold3d_model &, const double *, FoldFast<float> &);
+
+template <typename R>
+int fold_sweeps_fast_engine(hipStream_t st, R *T, const Grid &G, const FoldFast<R> *d_consts,
+                            int max_iters, double tol, unsigned long long *d_max,
+                            int *iters_out, double *max_dt_out) {
+  int it = 0;
+  double last = -1.0;
+  const dim3 block(kLaneX, kRowsY);
+  for (; it < max_iters;) {
+    if (tol > 0) zero_u64_kernel<<<1, 1, 0, st>>>(d_max);
+    for (int d = 0; d < 6; ++d) {
+      const Frame &fr = kFrames[d];
+      const int n = G.n[fr.m];
+      for (int q = 0; q < n - 1; ++q) {
+        const int p = fr.s > 0 ? 1 + q : n - q;
+        const PlaneGeom g = plane_geom(G, d, p);
+        const dim3 grid((g.na + kLaneX - 1) / kLaneX, (g.nb + kRowsY - 1) / kRowsY);
+        if (tol > 0)
+          fold_plane_fast_kernel<R, true><<<grid, block, 0, st>>>(T, g, d_consts + d, d_max);
+        else
+          fold_plane_fast_kernel<R, false><<<grid, block, 0, st>>>(T, g, d_consts + d, nullptr);
+      }
+    }
+    F3D_HIP(hipGetLastError());
+    ++it;
+    if (tol > 0) {
+      unsigned long long bits = 0;
+      F3D_HIP(hipMemcpyAsync(&bits, d_max, sizeof bits, hipMemcpyDeviceToHost, st));
+      F3D_HIP(hipStreamSynchronize(st));
+      memcpy(&last, &bits, sizeof last);
+      if (last < tol) break;
+    }
+  }
+  *iters_out = it;
+  *max_dt_out = last;
+  return FOLD3D_OK;
+}
+template int fold_sweeps_fast_engine<double>(hipStream_t, double *, const Grid &,
+                                             const FoldFast<double> *, int, double,
+                                             unsigned long long *, int *, double *);
+template int fold_sweeps_fast_engine<float>(hipStream_t, float *, const Grid &,
+                                            const FoldFast<float> *, int, double,
+                                            unsigned long long *, int *, double *);
 
 }  // namespace fold3d

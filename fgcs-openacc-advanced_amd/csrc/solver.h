@@ -2,6 +2,7 @@
 #pragma once
 
 #include "common.h"
+#include "fold_fast.h"
 #include "fold_local.h"
 
 namespace fold3d {
@@ -23,6 +24,23 @@ int fold_sweeps_plane_engine(hipStream_t st, R *T, const Grid &G,
                              int max_iters, double tol,
                              unsigned long long *d_max, int *iters_out,
                              double *max_dt_out);
+
+template <typename R>
+int ellip_sweeps_plane_engine(hipStream_t st, R *T, const Grid &G,
+                              const fold3d_model &m, const double h[3],
+                              int max_iters, double tol,
+                              unsigned long long *d_max, int *iters_out,
+                              double *max_dt_out);
+
+void sym3_inverse(const double M[6], double N[6]);
+
+template <typename R>
+void build_fold_fast(int dir, const fold3d_model &m, const double h[3], FoldFast<R> &out);
+
+template <typename R>
+int fold_sweeps_fast_engine(hipStream_t st, R *T, const Grid &G, const FoldFast<R> *d_consts,
+                            int max_iters, double tol, unsigned long long *d_max,
+                            int *iters_out, double *max_dt_out);
 
 int fold_plane_step_f64(hipStream_t st, double *T, const Grid &G, int dir,
                         int plane, const FoldFrame<double> &f);

@@ -91,8 +91,11 @@ typedef struct fold3d_opts {
 } fold3d_opts;
 
 enum fold3d_engine {
-  FOLD3D_ENGINE_AUTO = 0,
-  FOLD3D_ENGINE_PLANE = 1  /* one kernel launch per plane */
+  FOLD3D_ENGINE_AUTO = 0,    /* the fastest engine available for the model */
+  FOLD3D_ENGINE_PLANE = 1,   /* one launch per plane, node update written as the
+                                reference's local solvers (fold model) */
+  FOLD3D_ENGINE_HOISTED = 2  /* one launch per plane, per-direction invariants
+                                hoisted out of the node update (same bits) */
 };
 
 typedef struct fold3d_stats {

@@ -1,0 +1,293 @@
+/*
+ * ORACLE — FP64 CPU restatement of the tilted elliptic PMM model
+ *     sqrt(grad T^T M grad T) = s(x) = 1/v(x),   M symmetric positive definite.
+ * TEST INFRASTRUCTURE ONLY (see oracle.h).  The reference has no such model;
+ * this file restates the project's own local solver (the sweep structure is
+ * the reference's: AN_ISO_1_OpenACC.c:1192-1237 plane order, :195-326 the
+ * 3x3 upwind patch of one target node).
+ *
+ * Local solver.  With N = M^-1 the traveltime of a straight segment d is
+ * s*|d|_N.  The new value of target X is the minimum over the current value
+ * and, for every simplex S of the upwind patch (9 vertices, 16 edges, 8
+ * triangles fanned around the centre node), of
+ *     min_{Y in S}  T_lin(Y) + s |X - Y|_N .
+ * For a simplex with base Y0, edge matrix E (k columns) and D = X - Y0:
+ *     G = E^T N E, b = E^T N D, c = D^T N D, P = G^-1, l0 = P b, kappa = c - b.l0
+ * and with the value differences delta_i = T_i - T_0:
+ *     mu = P delta, q = delta.mu, r = s^2 - q,  root = sqrt(kappa r)
+ *     barycentrics  lambda = l0 - (root/r) mu   (must lie in the simplex)
+ *     candidate     T_0 + delta.l0 + root                      (r > 0)
+ * (stationary point of the convex function above).  Here every candidate of
+ * every simplex is evaluated; the GPU kernel prunes, and must agree.
+ */
+#include "oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#define BIG 1e15 /* a vertex at or above this is unreached (MAXT = 1e16) */
+
+/* sweep frames of the reference's six kernels (m, s, ux, sx, uy, sy) */
+static const int FR[6][6] = {
+    {0, +1, 2, +1, 1, +1}, {0, -1, 2, -1, 1, -1}, {2, +1, 1, +1, 0, +1},
+    {2, -1, 1, -1, 0, -1}, {1, -1, 0, -1, 2, -1}, {1, +1, 0, +1, 2, +1},
+};
+
+/* ring order of the 8 neighbours around the centre (a, b) in units of
+ * (dxx, dyy): +x, +x+y, +y, -x+y, -x, -x-y, -y, +x-y */
+static const int RING[8][2] = {{1, 0}, {1, 1}, {0, 1}, {-1, 1},
+                               {-1, 0}, {-1, -1}, {0, -1}, {1, -1}};
+
+void orc_tilted_metric(double theta, double phi, double a1, double a2,
+                       double a3, double M[6])
+{
+  /* M = R diag(a1^2, a2^2, a3^2) R^T, R = Rz(phi) Ry(theta) */
+  const double ct = cos(theta), st = sin(theta), cp = cos(phi), sp = sin(phi);
+  const double R[3][3] = {{cp * ct, -sp, cp * st},
+                          {sp * ct, cp, sp * st},
+                          {-st, 0.0, ct}};
+  const double d[3] = {a1 * a1, a2 * a2, a3 * a3};
+  double F[3][3];
+  int i, j, k;
+  for (i = 0; i < 3; i++)
+    for (j = 0; j < 3; j++) {
+      double acc = 0.0;
+      for (k = 0; k < 3; k++) acc += R[i][k] * d[k] * R[j][k];
+      F[i][j] = acc;
+    }
+  M[0] = F[0][0]; M[1] = F[1][1]; M[2] = F[2][2];
+  M[3] = F[0][1]; M[4] = F[0][2]; M[5] = F[1][2];
+}
+
+/* inverse of a symmetric 3x3 given as (xx, yy, zz, xy, xz, yz) */
+void orc_sym3_inverse(const double M[6], double N[6])
+{
+  const double a = M[0], b = M[1], c = M[2], d = M[3], e = M[4], f = M[5];
+  const double A = b * c - f * f, B = a * c - e * e, C = a * b - d * d;
+  const double D = e * f - c * d, E = d * f - b * e, F = d * e - a * f;
+  const double det = a * A + d * D + e * E;
+  N[0] = A / det; N[1] = B / det; N[2] = C / det;
+  N[3] = D / det; N[4] = E / det; N[5] = F / det;
+}
+
+/* Local-frame constants of one direction for inverse metric N (global axes). */
+void orc_ellip_constants(int dir, const double N[6], const double h[3],
+                         orc_ellip_consts *K)
+{
+  const int *f = FR[dir];
+  const int u[3] = {f[2], f[4], f[0]};
+  const double sg[3] = {f[3], f[5], f[1]};
+  /* full symmetric N in global axes */
+  const double Ng[3][3] = {{N[0], N[3], N[4]}, {N[3], N[1], N[5]}, {N[4], N[5], N[2]}};
+  double NL[3][3];
+  int i, j, k;
+  for (i = 0; i < 3; i++)
+    for (j = 0; j < 3; j++) NL[i][j] = sg[i] * sg[j] * Ng[u[i]][u[j]];
+  const double dxx = h[u[0]], dyy = h[u[1]], dzz = h[u[2]];
+  double pos[9][3];
+  pos[0][0] = 0.0; pos[0][1] = 0.0; pos[0][2] = 0.0;
+  for (k = 0; k < 8; k++) {
+    pos[k + 1][0] = RING[k][0] * dxx;
+    pos[k + 1][1] = RING[k][1] * dyy;
+    pos[k + 1][2] = 0.0;
+  }
+  const double X[3] = {0.0, 0.0, dzz};
+#define NDOT(p, q) \
+  ((p)[0] * (NL[0][0] * (q)[0] + NL[0][1] * (q)[1] + NL[0][2] * (q)[2]) + \
+   (p)[1] * (NL[1][0] * (q)[0] + NL[1][1] * (q)[1] + NL[1][2] * (q)[2]) + \
+   (p)[2] * (NL[2][0] * (q)[0] + NL[2][1] * (q)[1] + NL[2][2] * (q)[2]))
+  for (k = 0; k < 9; k++) {
+    double D[3] = {X[0] - pos[k][0], X[1] - pos[k][1], X[2] - pos[k][2]};
+    K->vert[k] = sqrt(NDOT(D, D));
+  }
+  /* edges: 0..7 spokes centre -> ring k, 8..15 rims ring k -> ring k+1 */
+  for (k = 0; k < 16; k++) {
+    const int v0 = k < 8 ? 0 : 1 + (k - 8);
+    const int v1 = k < 8 ? 1 + k : 1 + ((k - 8 + 1) % 8);
+    double E[3], D[3];
+    for (i = 0; i < 3; i++) {
+      E[i] = pos[v1][i] - pos[v0][i];
+      D[i] = X[i] - pos[v0][i];
+    }
+    const double g = NDOT(E, E), b = NDOT(E, D), c = NDOT(D, D);
+    const double P = 1.0 / g;
+    const double l0 = P * b;
+    K->edge[k][0] = P;
+    K->edge[k][1] = l0;
+    K->edge[k][2] = c - b * l0;
+  }
+  /* triangles: centre, ring k, ring k+1 */
+  for (k = 0; k < 8; k++) {
+    const int v1 = 1 + k, v2 = 1 + (k + 1) % 8;
+    double E1[3], E2[3], D[3];
+    for (i = 0; i < 3; i++) {
+      E1[i] = pos[v1][i];
+      E2[i] = pos[v2][i];
+      D[i] = X[i];
+    }
+    const double g11 = NDOT(E1, E1), g12 = NDOT(E1, E2), g22 = NDOT(E2, E2);
+    const double b1 = NDOT(E1, D), b2 = NDOT(E2, D), c = NDOT(D, D);
+    const double det = g11 * g22 - g12 * g12;
+    const double P11 = g22 / det, P12 = -g12 / det, P22 = g11 / det;
+    const double l1 = P11 * b1 + P12 * b2, l2 = P12 * b1 + P22 * b2;
+    K->tri[k][0] = P11;
+    K->tri[k][1] = P12;
+    K->tri[k][2] = P22;
+    K->tri[k][3] = l1;
+    K->tri[k][4] = l2;
+    K->tri[k][5] = c - (b1 * l1 + b2 * l2);
+  }
+#undef NDOT
+}
+
+/* Edge candidate; t0 at the base, t1 at the far end. Returns +inf if invalid. */
+double orc_ellip_edge(double t0, double t1, double s2, const double E[3])
+{
+  const double P = E[0], l0 = E[1], kap = E[2];
+  if (!(t0 < BIG && t1 < BIG)) return INFINITY;
+  const double d = t1 - t0;
+  const double mu = P * d;
+  const double r = s2 - d * mu;
+  if (!(r > 0.0)) return INFINITY;
+  const double root = sqrt(kap * r);
+  if (!(l0 * r - root * mu >= 0.0)) return INFINITY;
+  if (!((l0 - 1.0) * r - root * mu <= 0.0)) return INFINITY;
+  return t0 + d * l0 + root;
+}
+
+/* Triangle candidate over (centre t0, ring t1, ring t2). */
+double orc_ellip_tri(double t0, double t1, double t2, double s2, const double Tr[6])
+{
+  if (!(t0 < BIG && t1 < BIG && t2 < BIG)) return INFINITY;
+  const double d1 = t1 - t0, d2 = t2 - t0;
+  const double m1 = Tr[0] * d1 + Tr[1] * d2;
+  const double m2 = Tr[1] * d1 + Tr[2] * d2;
+  const double r = s2 - (d1 * m1 + d2 * m2);
+  if (!(r > 0.0)) return INFINITY;
+  const double root = sqrt(Tr[5] * r);
+  if (!(Tr[3] * r - root * m1 >= 0.0)) return INFINITY;
+  if (!(Tr[4] * r - root * m2 >= 0.0)) return INFINITY;
+  if (!((Tr[3] + Tr[4] - 1.0) * r - root * (m1 + m2) <= 0.0)) return INFINITY;
+  return t0 + (d1 * Tr[3] + d2 * Tr[4]) + root;
+}
+
+/* New value of one node from its patch values p[9] (centre + ring order). */
+double orc_ellip_node(double told, const double p[9], double s,
+                      const orc_ellip_consts *K)
+{
+  double best = told;
+  const double s2 = s * s;
+  int k;
+  for (k = 0; k < 9; k++) {
+    if (p[k] < BIG) {
+      const double c = p[k] + s * K->vert[k];
+      if (c < best) best = c;
+    }
+  }
+  for (k = 0; k < 16; k++) {
+    const double t0 = k < 8 ? p[0] : p[1 + (k - 8)];
+    const double t1 = k < 8 ? p[1 + k] : p[1 + ((k - 8 + 1) % 8)];
+    const double c = orc_ellip_edge(t0, t1, s2, K->edge[k]);
+    if (c < best) best = c;
+  }
+  for (k = 0; k < 8; k++) {
+    const double c = orc_ellip_tri(p[0], p[1 + k], p[1 + (k + 1) % 8], s2, K->tri[k]);
+    if (c < best) best = c;
+  }
+  return best;
+}
+
+static int g_ell_threads = 0;
+void orc_ellip_set_threads(int t) { g_ell_threads = t; }
+
+/* One plane of one direction.  T, speed, theta.. on the padded grid
+ * (nx+2)(ny+2)(nz+2).  Constant model when the field pointers are NULL:
+ * M6 = metric, speed = v.  Returns max |dT| over the plane. */
+double orc_ellip_plane(int dir, double *T, const int n[3], int p,
+                       const double h[3], const double M6[6], double speed,
+                       const double *speed_field, const double *theta,
+                       const double *phi, const double *ra, const double *rb)
+{
+  const int *f = FR[dir];
+  const long H0 = n[0] + 2, H1 = n[1] + 2;
+  const long stride[3] = {1, H0, H0 * H1};
+  const long sm = f[1] * stride[f[0]], su = f[3] * stride[f[2]], sv = f[5] * stride[f[4]];
+  const int nu = n[f[2]], nv = n[f[4]];
+  const long base = (long)p * stride[f[0]];
+  orc_ellip_consts Kc;
+  const int per_node = theta != NULL;
+  if (!per_node) {
+    double N[6];
+    orc_sym3_inverse(M6, N);
+    orc_ellip_constants(dir, N, h, &Kc);
+  }
+  double maxd = 0.0;
+  long a;
+#pragma omp parallel for schedule(static) reduction(max : maxd) num_threads(g_ell_threads > 0 ? g_ell_threads : omp_get_max_threads())
+  for (a = 1; a <= nu; a++) {
+    long b;
+    for (b = 1; b <= nv; b++) {
+      const long c = base + a * stride[f[2]] + b * stride[f[4]];
+      const long t = c + sm;
+      double pv[9];
+      pv[0] = T[c];
+      int k;
+      for (k = 0; k < 8; k++) pv[1 + k] = T[c + RING[k][0] * su + RING[k][1] * sv];
+      const double v = speed_field ? speed_field[t] : speed;
+      const double s = 1.0 / v;
+      orc_ellip_consts Kn;
+      const orc_ellip_consts *K = &Kc;
+      if (per_node) {
+        double M[6], N[6];
+        orc_tilted_metric(theta[t], phi[t], 1.0, ra[t], rb[t], M);
+        orc_sym3_inverse(M, N);
+        orc_ellip_constants(dir, N, h, &Kn);
+        K = &Kn;
+      }
+      const double told = T[t];
+      const double tn = orc_ellip_node(told, pv, s, K);
+      T[t] = tn;
+      const double d = fabs(tn - told);
+      if (d > maxd) maxd = d;
+    }
+  }
+  return maxd;
+}
+
+int orc_ellip_sweeps(double *T, const int n[3], int max_iters, double tol,
+                     const double h[3], const double M6[6], double speed,
+                     const double *speed_field, const double *theta,
+                     const double *phi, const double *ra, const double *rb,
+                     double *max_dt)
+{
+  int it;
+  double last = -1.0;
+  for (it = 0; it < max_iters;) {
+    double m = 0.0;
+    int d;
+    for (d = 0; d < 6; d++) {
+      const int ax = FR[d][0];
+      int p;
+      if (FR[d][1] > 0) {
+        for (p = 1; p < n[ax]; p++) {
+          double x = orc_ellip_plane(d, T, n, p, h, M6, speed, speed_field, theta, phi, ra, rb);
+          if (x > m) m = x;
+        }
+      } else {
+        for (p = n[ax]; p > 1; p--) {
+          double x = orc_ellip_plane(d, T, n, p, h, M6, speed, speed_field, theta, phi, ra, rb);
+          if (x > m) m = x;
+        }
+      }
+    }
+    it++;
+    last = m;
+    if (tol > 0 && m < tol) break;
+  }
+  if (max_dt) *max_dt = last;
+  return it;
+}

@@ -57,6 +57,33 @@ void orc_sweeps(double *T, int n, int iterations, double F, const double a[3],
 /* OpenMP thread count used by the plane loops (0 = OpenMP default). */
 void orc_set_threads(int nthreads);
 
+/* ---- tilted elliptic model sqrt(grad T^T M grad T) = 1/v (no reference code) ---- */
+typedef struct {
+  double vert[9];     /* |X - Y|_N for the centre and the 8 ring nodes */
+  double edge[16][3]; /* P, l0, kappa: 8 spokes centre->ring k, 8 rims ring k->k+1 */
+  double tri[8][6];   /* P11, P12, P22, l1, l2, kappa: centre, ring k, ring k+1 */
+} orc_ellip_consts;
+
+void orc_tilted_metric(double theta, double phi, double a1, double a2,
+                       double a3, double M[6]);
+void orc_sym3_inverse(const double M[6], double N[6]);
+void orc_ellip_constants(int dir, const double N[6], const double h[3],
+                         orc_ellip_consts *K);
+double orc_ellip_edge(double t0, double t1, double s2, const double E[3]);
+double orc_ellip_tri(double t0, double t1, double t2, double s2, const double Tr[6]);
+double orc_ellip_node(double told, const double p[9], double s,
+                      const orc_ellip_consts *K);
+void orc_ellip_set_threads(int t);
+double orc_ellip_plane(int dir, double *T, const int n[3], int p,
+                       const double h[3], const double M6[6], double speed,
+                       const double *speed_field, const double *theta,
+                       const double *phi, const double *ra, const double *rb);
+int orc_ellip_sweeps(double *T, const int n[3], int max_iters, double tol,
+                     const double h[3], const double M6[6], double speed,
+                     const double *speed_field, const double *theta,
+                     const double *phi, const double *ra, const double *rb,
+                     double *max_dt);
+
 #ifdef __cplusplus
 }
 #endif

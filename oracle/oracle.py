@@ -103,6 +103,101 @@ class Oracle:
 
     def set_threads(self, k: int):
         self.lib.orc_set_threads(k)
+        self.lib.orc_ellip_set_threads(k)
+
+    # tilted elliptic model -------------------------------------------------
+    def _ellip_proto(self):
+        lib = self.lib
+        if getattr(self, "_ellip_ready", False):
+            return lib
+        lib.orc_tilted_metric.argtypes = [_d] * 5 + [_dp]
+        lib.orc_sym3_inverse.argtypes = [_dp, _dp]
+        lib.orc_ellip_constants.argtypes = [_i, _dp, _dp, C.c_void_p]
+        lib.orc_ellip_edge.restype = _d
+        lib.orc_ellip_edge.argtypes = [_d, _d, _d, _dp]
+        lib.orc_ellip_tri.restype = _d
+        lib.orc_ellip_tri.argtypes = [_d, _d, _d, _d, _dp]
+        lib.orc_ellip_node.restype = _d
+        lib.orc_ellip_node.argtypes = [_d, _dp, _d, C.c_void_p]
+        lib.orc_ellip_set_threads.argtypes = [_i]
+        ip = C.POINTER(C.c_int)
+        lib.orc_ellip_plane.restype = _d
+        lib.orc_ellip_plane.argtypes = [_i, _dp, ip, _i, _dp, _dp, _d] + [_dp] * 5
+        lib.orc_ellip_sweeps.restype = _i
+        lib.orc_ellip_sweeps.argtypes = [_dp, ip, _i, _d, _dp, _dp, _d] + [_dp] * 5 + [_dp]
+        self._ellip_ready = True
+        return lib
+
+    def tilted_metric(self, theta, phi, a1, a2, a3) -> np.ndarray:
+        M = np.zeros(6)
+        self._ellip_proto().orc_tilted_metric(theta, phi, a1, a2, a3, _ptr(M))
+        return M
+
+    def sym3_inverse(self, M) -> np.ndarray:
+        M = np.ascontiguousarray(M, dtype=np.float64)
+        N = np.zeros(6)
+        self._ellip_proto().orc_sym3_inverse(_ptr(M), _ptr(N))
+        return N
+
+    def ellip_constants(self, d: int, N, h) -> np.ndarray:
+        """(9 + 48 + 48) doubles: vert[9], edge[16][3], tri[8][6]."""
+        K = np.zeros(9 + 48 + 48)
+        N = np.ascontiguousarray(N, dtype=np.float64)
+        h = np.ascontiguousarray(h, dtype=np.float64)
+        self._ellip_proto().orc_ellip_constants(d, _ptr(N), _ptr(h), K.ctypes.data)
+        return K
+
+    def ellip_edge(self, t0, t1, s2, E):
+        E = np.ascontiguousarray(E, dtype=np.float64)
+        return self._ellip_proto().orc_ellip_edge(t0, t1, s2, _ptr(E))
+
+    def ellip_tri(self, t0, t1, t2, s2, Tr):
+        Tr = np.ascontiguousarray(Tr, dtype=np.float64)
+        return self._ellip_proto().orc_ellip_tri(t0, t1, t2, s2, _ptr(Tr))
+
+    def ellip_node(self, told, p9, s, K):
+        p9 = np.ascontiguousarray(p9, dtype=np.float64)
+        K = np.ascontiguousarray(K, dtype=np.float64)
+        return self._ellip_proto().orc_ellip_node(told, _ptr(p9), s, K.ctypes.data)
+
+    @staticmethod
+    def _fields(n, speed_field, theta, phi, ra, rb):
+        out = []
+        for f in (speed_field, theta, phi, ra, rb):
+            if f is None:
+                out.append(None)
+            else:
+                f = np.ascontiguousarray(f, dtype=np.float64).ravel()
+                assert f.size == (n[0] + 2) * (n[1] + 2) * (n[2] + 2)
+                out.append(f)
+        return out
+
+    def ellip_plane(self, d, T, n, p, h, M=None, speed=1.0, speed_field=None,
+                    theta=None, phi=None, ra=None, rb=None) -> float:
+        n3 = (C.c_int * 3)(*((n,) * 3 if np.isscalar(n) else n))
+        nn = tuple(n3)
+        M = np.ascontiguousarray(np.eye(3)[[0, 1, 2, 0, 0, 1], [0, 1, 2, 1, 2, 2]]
+                                 if M is None else M, dtype=np.float64)
+        h = np.ascontiguousarray((h,) * 3 if np.isscalar(h) else h, dtype=np.float64)
+        fs = self._fields(nn, speed_field, theta, phi, ra, rb)
+        ptrs = [None if f is None else _ptr(f) for f in fs]
+        return self._ellip_proto().orc_ellip_plane(d, _ptr(T), n3, p, _ptr(h), _ptr(M),
+                                                   speed, *ptrs)
+
+    def ellip_sweeps(self, T, n, max_iters, tol, h, M=None, speed=1.0, speed_field=None,
+                     theta=None, phi=None, ra=None, rb=None):
+        """Returns (iterations run, max|dT| of the last iteration)."""
+        n3 = (C.c_int * 3)(*((n,) * 3 if np.isscalar(n) else n))
+        nn = tuple(n3)
+        M = np.ascontiguousarray(np.eye(3)[[0, 1, 2, 0, 0, 1], [0, 1, 2, 1, 2, 2]]
+                                 if M is None else M, dtype=np.float64)
+        h = np.ascontiguousarray((h,) * 3 if np.isscalar(h) else h, dtype=np.float64)
+        fs = self._fields(nn, speed_field, theta, phi, ra, rb)
+        ptrs = [None if f is None else _ptr(f) for f in fs]
+        md = np.zeros(1)
+        it = self._ellip_proto().orc_ellip_sweeps(_ptr(T), n3, max_iters, tol, _ptr(h),
+                                                  _ptr(M), speed, *ptrs, _ptr(md))
+        return it, float(md[0])
 
 
 class RefBuild:

@@ -1,0 +1,143 @@
+"""Parity of the HIP elliptic-model solve with the elliptic oracle.
+
+FP64 with a constant metric is bit-for-bit equal to the oracle (same
+constants, same candidate arithmetic; the kernel prunes simplices that cannot
+win, the oracle evaluates all of them).  With a per-node metric the device
+evaluates sin/cos itself, so FP64 is held to 1e-12 relative.  FP32 is held to
+1e-5 relative of the FP64 oracle on nodes at least 8 cells from the source
+and to 1e-5 * max(T) absolute everywhere.
+"""
+import numpy as np
+import pytest
+
+from conftest import bits_equal
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+import fold3d  # noqa: E402
+from fold3d import workloads as wl  # noqa: E402
+
+
+def dev(a, dtype=torch.float64):
+    return torch.from_numpy(np.ascontiguousarray(a)).to("cuda", dtype)
+
+
+def oracle_fields(W):
+    out = {}
+    for k in ("speed_field", "theta", "phi", "ra", "rb"):
+        f = getattr(W, k)
+        out[k] = None if f is None else np.asarray(f, dtype=np.float64)
+    return out
+
+
+def to_device_workload(W, dtype):
+    Wd = wl.Workload(W.name, W.n, W.h, W.M, W.speed, W.source, W.v_source, meta=W.meta)
+    for k in ("speed_field", "theta", "phi", "ra", "rb"):
+        f = getattr(W, k)
+        setattr(Wd, k, None if f is None else dev(f, dtype))
+    return Wd
+
+
+@pytest.mark.parametrize("n", [16, 33])
+def test_homogeneous_bitwise(solver, oracle, n):
+    print(f"ellip homogeneous n={n}", flush=True)
+    W = wl.tilted_homogeneous(n)
+    T0 = wl.point_source_start(W)
+    want = T0.copy()
+    it_o, md_o = oracle.ellip_sweeps(want, n, 3, 0.0, W.h, M=W.M, speed=W.speed)
+    T = dev(T0)
+    st = solver.solve(T, n, spacing=W.h, model=W.model(), max_iters=3)
+    assert bits_equal(T.cpu().numpy(), want)
+
+
+def test_blob_speed_bitwise_and_converges(solver, oracle):
+    n = 28
+    W = wl.blob_speed(n)
+    T0 = wl.point_source_start(W)
+    want = T0.copy()
+    it_o, md_o = oracle.ellip_sweeps(want, n, 40, 1e-10, W.h, M=W.M, speed_field=W.speed_field)
+    Wd = to_device_workload(W, torch.float64)
+    T = dev(T0)
+    st = solver.solve(T, n, spacing=W.h, model=Wd.model(), max_iters=40, tol=1e-10)
+    assert st.iterations == it_o and st.max_dT == md_o
+    assert bits_equal(T.cpu().numpy(), want)
+
+
+def test_per_node_metric_fp64(solver, oracle):
+    n = 20
+    W = wl.tilted_field(n, dtype=np.float64)
+    T0 = wl.point_source_start(W)
+    want = T0.copy()
+    f = oracle_fields(W)
+    oracle.ellip_sweeps(want, n, 4, 0.0, W.h, M=W.M, **f)
+    Wd = to_device_workload(W, torch.float64)
+    T = dev(T0)
+    solver.solve(T, n, spacing=W.h, model=Wd.model(), max_iters=4)
+    got = T.cpu().numpy()
+    reached = want < 1e15
+    assert np.array_equal(reached, got < 1e15)
+    rel = np.abs(got[reached] - want[reached]) / want[reached].clip(min=1e-300)
+    print(f"per-node metric FP64: max rel {rel.max():.2e}")
+    assert rel.max() < 1e-12
+
+
+def fp32_error(got, want, n, h, vmax):
+    reached = want < 1e15
+    far = reached & (want > 8 * h / vmax)
+    rel = np.abs(got[far] - want[far]) / want[far]
+    absd = np.abs(got[reached] - want[reached]).max() / want[reached].max()
+    return rel.max(), absd
+
+
+@pytest.mark.parametrize("which", ["homogeneous", "blob", "field"])
+def test_fp32_vs_fp64_oracle(solver, oracle, which):
+    n = 40
+    if which == "homogeneous":
+        W = wl.tilted_homogeneous(n)
+        vmax = 1.0
+    elif which == "blob":
+        W = wl.blob_speed(n)
+        vmax = 2.3
+    else:
+        W = wl.tilted_field(n, dtype=np.float64)
+        vmax = 3.6
+    T0 = wl.point_source_start(W)
+    want = T0.copy()
+    it, md = oracle.ellip_sweeps(want, n, 60, 1e-10, W.h, M=W.M, **oracle_fields(W))
+    Wd = to_device_workload(W, torch.float32)
+    T = dev(T0, torch.float32)
+    st = solver.solve(T, n, spacing=W.h, model=Wd.model(), max_iters=60, tol=1e-6)
+    got = T.double().cpu().numpy()
+    rel, absd = fp32_error(got, want, n, W.h, vmax)
+    print(f"FP32 {which}: {st.iterations} iterations (FP64 {it}); max rel {rel:.2e}, max abs/maxT {absd:.2e}")
+    assert rel < 1e-5 and absd < 1e-5
+
+
+def test_closed_form_error_reported(solver):
+    """Homogeneous tilted medium: converged PMM field vs sqrt(x^T M^-1 x)/v."""
+    n = 64
+    W = wl.tilted_homogeneous(n)
+    T = dev(wl.point_source_start(W))
+    st = solver.solve(T, n, spacing=W.h, model=W.model(), max_iters=100, tol=1e-10)
+    got = T.cpu().numpy()
+    exact = wl.closed_form(W)
+    H = n + 2
+    inner = np.zeros((H, H, H), bool)
+    inner[1:-1, 1:-1, 1:-1] = True
+    inner = inner.reshape(-1)
+    err = np.abs(got[inner] - exact[inner])
+    print(f"closed form n={n}: {st.iterations} iterations, max err {err.max():.3e}, "
+          f"mean {err.mean():.3e}, max rel {(err / exact[inner].clip(min=W.h)).max():.3e}")
+    assert st.max_dT < 1e-10
+    assert np.all(got[inner] >= exact[inner] - 1e-12)
+    assert err.max() < 0.05
+
+
+def test_bad_metric_is_rejected(solver):
+    m = wl.tilted_homogeneous(16).model()
+    m.M[0] = -1.0
+    T = torch.zeros(18 ** 3, dtype=torch.float64, device="cuda")
+    with pytest.raises(fold3d.Fold3dError) as e:
+        solver.solve(T, 16, spacing=0.1, model=m)
+    assert e.value.code == fold3d.FOLD3D_E_INVALID

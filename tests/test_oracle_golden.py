@@ -65,12 +65,20 @@ def test_full_solves(oracle, golden, n):
     assert bits_equal(T, golden[f"solve1_{n}"])
     oracle.sweeps(T, n, 7)
     assert bits_equal(T, golden[f"solve8_{n}"])
+    # the first iteration that changes nothing, counted from the start
     its = int(golden[f"fixed_iters_{n}"][0])
-    oracle.sweeps(T, n, its - 8)
-    assert bits_equal(T, golden[f"fixed_{n}"])
-    U = T.copy()
-    oracle.sweeps(U, n, 1)
-    assert bits_equal(U, T)  # a fixed point
+    U = oracle.initialiser(n)
+    oracle.sweeps(U, n, its - 1)
+    V = U.copy()
+    oracle.sweeps(V, n, 1)
+    assert bits_equal(V, U) and bits_equal(U, golden[f"fixed_{n}"])
+    W = oracle.initialiser(n)
+    oracle.sweeps(W, n, its - 2)
+    X = W.copy()
+    oracle.sweeps(X, n, 1)
+    assert not bits_equal(X, W)  # one iteration earlier it still moved
+    if its <= 8:
+        assert bits_equal(T, golden[f"fixed_{n}"])
 
 
 @pytest.mark.slow

@@ -38,47 +38,41 @@ def cell_updates(n, iters):
     return iters * 6 * (n - 1) * n * n
 
 
-def time_solves(solver, T0d, n, steps, warmup, dtype, torch):
+def time_solves(solver, T0d, n, steps, warmup, dtype, torch, engine=0):
+    """Wall time of `steps` solves (reset copy included), plus the summed
+    HIP-event time of the sweep loops themselves."""
     T = torch.empty_like(T0d, dtype=dtype)
     for _ in range(warmup):
         T.copy_(T0d)
-        solver.solve(T, n, max_iters=8)
+        solver.solve(T, n, max_iters=8, engine=engine)
     torch.cuda.synchronize()
+    sweep_ms = 0.0
     t0 = time.perf_counter()
     for _ in range(steps):
         T.copy_(T0d)
-        solver.solve(T, n, max_iters=8)
+        st = solver.solve(T, n, max_iters=8, engine=engine)
+        sweep_ms += st.sweep_ms
     torch.cuda.synchronize()
-    return (time.perf_counter() - t0) * 1e3, T
+    return (time.perf_counter() - t0) * 1e3, T, sweep_ms
 
 
-def kernel_launch_us(solver, T, n, torch, reps=40):
-    """Average duration of single plane-step launches of the dominant kernel,
-    bracketed by HIP events on the stream the kernel runs on, over planes of
-    all six directions of a mid-solve field."""
-    import numpy as np
-    import fold3d
-    h = fold3d.reference_spacing(n)
-    dg = float(np.sqrt(2 * h * h))
-    A, B, Cc = fold3d.REF_A
-    F = fold3d.REF_F
-    args = {0: (F, B, Cc, A), 1: (F, -B, -Cc, -A), 2: (F, A, B, Cc),
-            3: (F, -A, -B, -Cc), 4: (F, -Cc, -A, -B), 5: (F, Cc, A, B)}
-    stream = torch.cuda.current_stream()
-    total, count = 0.0, 0
-    for d in range(6):
-        for p in (n // 4, n // 2, 3 * n // 4):
-            e0 = torch.cuda.Event(enable_timing=True)
-            e1 = torch.cuda.Event(enable_timing=True)
-            e0.record(stream)
-            for _ in range(reps):
-                solver.plane_step(d, T, n, p, h, h, h, dg, dg, dg, *args[d],
-                                  stream=stream.cuda_stream)
-            e1.record(stream)
-            e1.synchronize()
-            total += e0.elapsed_time(e1) * 1e3
-            count += reps
-    return total / count
+def ellip_run(solver, n, dtype, tol, torch):
+    """The 512^3 tilted-metric, Gaussian-blob speed workload, to tolerance."""
+    from fold3d import workloads as wl
+    W = wl.blob_speed(n, xp=torch, dtype=dtype, device="cuda")
+    T0 = wl.point_source_start(W, xp=torch, dtype=dtype, device="cuda")
+    T = T0.clone()
+    st = solver.solve(T, n, spacing=W.h, model=W.model(), max_iters=200, tol=tol)  # warm
+    T.copy_(T0)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    st = solver.solve(T, n, spacing=W.h, model=W.model(), max_iters=200, tol=tol)
+    torch.cuda.synchronize()
+    wall = (time.perf_counter() - t0) * 1e3
+    W_upd = st.cell_updates
+    return {"iterations": st.iterations, "max_dT": st.max_dT, "sweep_ms": st.sweep_ms,
+            "wall_ms": wall, "value": W_upd / (st.sweep_ms / 1e3) / 1e9,
+            "unit": "Gcell-updates/s", "cell_updates": W_upd}, T
 
 
 def cpu_baseline(T0_host, n, threads):
@@ -102,6 +96,8 @@ def main():
     ap.add_argument("--n", type=int, default=512)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp32", action="store_true")
+    ap.add_argument("--no-ellip", action="store_true")
+    ap.add_argument("--no-ablation", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
@@ -121,28 +117,59 @@ def main():
     solver = fold3d.Solver(0)
 
     W = cell_updates(n, 8)
-    log(f"bench: FP64 solve x{args.steps} (+{args.warmup} warmup)")
-    ms, T = time_solves(solver, T0d, n, args.steps, args.warmup, torch.float64, torch)
+    launches = 8 * 6 * (n - 1)
+    log(f"bench: FP64 solve x{args.steps} (+{args.warmup} warmup), tiled engine")
+    ms, T, sweep_ms = time_solves(solver, T0d, n, args.steps, args.warmup, torch.float64, torch)
     ms_step = ms / args.steps
     value = W * args.steps / (ms / 1e3) / 1e9
     log(f"bench: FP64 {ms_step:.1f} ms/solve, {value:.2f} Gcell-upd/s")
-
-    log("bench: per-launch kernel timing")
-    us = kernel_launch_us(solver, T, n, torch)
+    # average plane-kernel duration from the solve's own HIP events (the
+    # four layout changes per iteration and inter-launch gaps are included,
+    # so this is an upper bound on the kernel time)
+    us = sweep_ms * 1e3 / (args.steps * launches)
     bytes_per_launch = 16.0 * n * n
     achieved = bytes_per_launch / (us * 1e-6) / 1e9
+
+    engines = {}
+    if not args.no_ablation:
+        for name, eng in (("plane", fold3d.ENGINE_PLANE), ("hoisted", fold3d.ENGINE_HOISTED)):
+            log(f"bench: FP64 solve, {name} engine")
+            ms_e, _, sw_e = time_solves(solver, T0d, n, 1, 0, torch.float64, torch, engine=eng)
+            engines[name] = {"ms_per_step": ms_e, "value": W / (ms_e / 1e3) / 1e9}
+        engines["tiled"] = {"ms_per_step": ms_step, "value": value}
 
     fp32 = None
     if not args.no_fp32:
         log("bench: FP32 solve")
-        ms32, T32 = time_solves(solver, T0d, n, args.steps, args.warmup, torch.float32, torch)
+        ms32, T32, _ = time_solves(solver, T0d, n, args.steps, args.warmup, torch.float32, torch)
         ref = T.cpu().numpy()
         got = T32.double().cpu().numpy()
         reached = np.abs(ref) < 1e15
-        rel = float(np.max(np.abs(got[reached] - ref[reached]) / np.abs(ref[reached])))
+        rel = np.abs(got[reached] - ref[reached]) / np.abs(ref[reached])
         fp32 = {"value": W * args.steps / (ms32 / 1e3) / 1e9, "unit": "Gcell-updates/s",
-                "ms_per_step": ms32 / args.steps, "max_rel_err_vs_fp64": rel}
-        log(f"bench: FP32 {fp32['value']:.2f} Gcell-upd/s, max rel err {rel:.2e}")
+                "ms_per_step": ms32 / args.steps, "max_rel_err_vs_fp64": float(rel.max()),
+                "frac_rel_err_above_1e-5": float(np.mean(rel > 1e-5))}
+        log(f"bench: FP32 {fp32['value']:.2f} Gcell-upd/s, max rel err {rel.max():.2e}")
+        del T32
+
+    ellip = None
+    if not args.no_ellip:
+        ellip = {}
+        for name, dt, tol in (("fp64", torch.float64, 1e-10), ("fp32", torch.float32, 1e-6)):
+            log(f"bench: elliptic blob-speed workload {n}^3 {name}, tol {tol}")
+            r, Te = ellip_run(solver, n, dt, tol, torch)
+            ellip[name] = r
+            if name == "fp64":
+                ref64 = Te
+            else:
+                a = ref64.double()
+                b = Te.double()
+                reached = a < 1e15
+                far = reached & (a > 8 / (n - 1) / 2.3)
+                r["max_rel_err_vs_fp64"] = float(((b - a).abs()[far] / a[far]).max())
+            log(f"bench: elliptic {name}: {r['iterations']} iterations, {r['value']:.2f} Gcell-upd/s")
+            del Te
+        del ref64
 
     cpu = None
     if not args.no_cpu:
@@ -171,9 +198,11 @@ def main():
                    "n": n, "iterations": 8, "cell_updates_per_step": W},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "fold_plane_kernel<double>", "kernel_us": us,
+                     "kernel": "fold_tile_kernel<double>", "kernel_us": us,
                      "bytes_per_launch": bytes_per_launch},
+        "engines_fp64": engines,
         "fp32": fp32,
+        "elliptic_blob_speed": ellip,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

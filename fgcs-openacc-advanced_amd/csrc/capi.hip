@@ -18,7 +18,24 @@ struct fold3d_handle {
   void *d_consts;             // per-direction constants of the running solve
   void *h_consts;             // pinned staging for them
   hipEvent_t consts_copied;   // staging may be rewritten once this completes
+  void *d_scratch;            // a second field for the layout changes
+  size_t scratch_bytes;
 };
+
+// Grow the handle's scratch field to at least `bytes` (stream-ordered free of
+// the old one is not available, so the device is synchronised first).
+static int ensure_scratch(fold3d_handle *h, size_t bytes) {
+  if (h->scratch_bytes >= bytes) return FOLD3D_OK;
+  if (h->d_scratch) {
+    F3D_HIP(hipDeviceSynchronize());
+    F3D_HIP(hipFree(h->d_scratch));
+    h->d_scratch = nullptr;
+    h->scratch_bytes = 0;
+  }
+  F3D_HIP(hipMalloc(&h->d_scratch, bytes));
+  h->scratch_bytes = bytes;
+  return FOLD3D_OK;
+}
 
 static constexpr size_t kConstBytes = 6 * sizeof(FoldFast<double>);
 
@@ -76,6 +93,7 @@ int fold3d_destroy(fold3d_handle *h) {
   if (!h) return FOLD3D_OK;
   (void)hipSetDevice(h->device);
   (void)hipEventSynchronize(h->consts_copied);
+  if (h->d_scratch) (void)hipFree(h->d_scratch);
   (void)hipFree(h->d_consts);
   (void)hipHostFree(h->h_consts);
   (void)hipEventDestroy(h->consts_copied);
@@ -104,6 +122,8 @@ int fold3d_solve(fold3d_handle *h, void *T, int nx, int ny, int nz, double dx,
             "fold3d_solve: bad precision %d", o.precision);
   F3D_CHECK(o.memory == FOLD3D_DEVICE || o.memory == FOLD3D_HOST, FOLD3D_E_INVALID,
             "fold3d_solve: bad memory kind %d", o.memory);
+  F3D_CHECK(o.engine >= FOLD3D_ENGINE_AUTO && o.engine <= FOLD3D_ENGINE_TILED, FOLD3D_E_INVALID,
+            "fold3d_solve: bad engine %d", o.engine);
   F3D_CHECK(model->kind == FOLD3D_MODEL_FOLD || model->kind == FOLD3D_MODEL_ELLIP,
             FOLD3D_E_INVALID, "fold3d_solve: unknown model kind %d", model->kind);
   if (model->kind == FOLD3D_MODEL_FOLD) {
@@ -163,13 +183,25 @@ int fold3d_solve(fold3d_handle *h, void *T, int nx, int ny, int nz, double dx,
     }
     F3D_HIP(hipMemcpyAsync(h->d_consts, h->h_consts, kConstBytes, hipMemcpyHostToDevice, st));
     F3D_HIP(hipEventRecord(h->consts_copied, st));
-    rc = o.precision == FOLD3D_F64
-             ? fold_sweeps_fast_engine<double>(st, (double *)dT, G,
-                                               (const FoldFast<double> *)h->d_consts,
-                                               o.max_iters, o.tol, h->d_max, &iters, &maxdt)
-             : fold_sweeps_fast_engine<float>(st, (float *)dT, G,
-                                              (const FoldFast<float> *)h->d_consts,
-                                              o.max_iters, o.tol, h->d_max, &iters, &maxdt);
+    if (o.engine == FOLD3D_ENGINE_HOISTED) {
+      rc = o.precision == FOLD3D_F64
+               ? fold_sweeps_fast_engine<double>(st, (double *)dT, G,
+                                                 (const FoldFast<double> *)h->d_consts,
+                                                 o.max_iters, o.tol, h->d_max, &iters, &maxdt)
+               : fold_sweeps_fast_engine<float>(st, (float *)dT, G,
+                                                (const FoldFast<float> *)h->d_consts,
+                                                o.max_iters, o.tol, h->d_max, &iters, &maxdt);
+    } else {
+      rc = ensure_scratch(h, bytes);
+      if (rc != FOLD3D_OK) return rc;
+      rc = o.precision == FOLD3D_F64
+               ? fold_sweeps_tiled_engine<double>(st, (double *)dT, (double *)h->d_scratch, G,
+                                                  (const FoldFast<double> *)h->d_consts,
+                                                  o.max_iters, o.tol, h->d_max, &iters, &maxdt)
+               : fold_sweeps_tiled_engine<float>(st, (float *)dT, (float *)h->d_scratch, G,
+                                                 (const FoldFast<float> *)h->d_consts,
+                                                 o.max_iters, o.tol, h->d_max, &iters, &maxdt);
+    }
   } else
     rc = o.precision == FOLD3D_F64
              ? ellip_sweeps_plane_engine<double>(st, (double *)dT, G, *model, hsp, o.max_iters,

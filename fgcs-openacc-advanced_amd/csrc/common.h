@@ -50,20 +50,30 @@ constexpr Frame kFrames[6] = {
     {1, +1, 0, +1, 2, +1},  // y_down: j -> j+1, x^ = +x, y^ = +z
 };
 
+// Axis orders (fastest to slowest) of the layouts the engine uses.  kOrderZ
+// is the reference's T[i + H0*(j + H1*k)]; kOrderX / kOrderY put x / y
+// slowest so an x- / y-sweep plane is one contiguous slab.
+constexpr int kOrderZ[3] = {0, 1, 2};
+constexpr int kOrderX[3] = {1, 2, 0};
+constexpr int kOrderY[3] = {0, 2, 1};
+
 // Padded grid: nodes 1..n[a] on each axis, ghosts at 0 and n[a]+1.
 struct Grid {
   int n[3];
   long long stride[3];
   long long count;  // (nx+2)(ny+2)(nz+2)
-  Grid(int nx, int ny, int nz) {
+  Grid(int nx, int ny, int nz, const int order[3] = kOrderZ) {
     n[0] = nx;
     n[1] = ny;
     n[2] = nz;
-    stride[0] = 1;
-    stride[1] = (long long)nx + 2;
-    stride[2] = stride[1] * ((long long)ny + 2);
-    count = stride[2] * ((long long)nz + 2);
+    long long s = 1;
+    for (int q = 0; q < 3; ++q) {
+      stride[order[q]] = s;
+      s *= (long long)n[order[q]] + 2;
+    }
+    count = s;
   }
+  Grid with_order(const int order[3]) const { return Grid(n[0], n[1], n[2], order); }
   // Planes of a sweep along axis m: forward 1..n-1, backward n..2.
   int planes(int m) const { return n[m] - 1; }
 };

@@ -105,8 +105,9 @@ static FoldFrame<R> fold_frame(int dir, double F, const double a[3],
 PlaneGeom plane_geom(const Grid &G, int dir, int plane) {
   const Frame &fr = kFrames[dir];
   // lane axis: the in-plane axis with the smaller stride
-  int u = fr.ux, v = fr.uy;
-  int la = u < v ? u : v, lb = u < v ? v : u;
+  const int u = fr.ux, v = fr.uy;
+  const bool u_first = G.stride[u] < G.stride[v];
+  const int la = u_first ? u : v, lb = u_first ? v : u;
   PlaneGeom g;
   g.base = (long long)plane * G.stride[fr.m];
   g.Sa = G.stride[la];
@@ -116,6 +117,10 @@ PlaneGeom plane_geom(const Grid &G, int dir, int plane) {
   g.sv = fr.sy * G.stride[fr.uy];
   g.na = G.n[la];
   g.nb = G.n[lb];
+  g.ua = fr.ux == la ? fr.sx : 0;
+  g.ub = fr.ux == la ? 0 : fr.sx;
+  g.va = fr.uy == la ? fr.sy : 0;
+  g.vb = fr.uy == la ? 0 : fr.sy;
   return g;
 }
 
@@ -283,5 +288,124 @@ template int fold_sweeps_fast_engine<double>(hipStream_t, double *, const Grid &
 template int fold_sweeps_fast_engine<float>(hipStream_t, float *, const Grid &,
                                             const FoldFast<float> *, int, double,
                                             unsigned long long *, int *, double *);
+
+// ---------------------------------------------------------------------------
+// Tiled engine.  A block owns a kTileA x kTileB tile of the target plane; it
+// stages the source-plane tile plus a one-node halo, as magnitudes, in LDS
+// (each source node is read from HBM/L2 once per block instead of nine times
+// per node) and reads the ten-value patch of every node from there.  Rows of
+// the tile run along the plane's unit-stride axis in every layout.
+// ---------------------------------------------------------------------------
+constexpr int kTileA = 64;
+constexpr int kTileB = 4;
+
+template <typename R, bool kReduce>
+__global__ __launch_bounds__(kTileA *kTileB) void fold_tile_kernel(
+    R *__restrict__ T, PlaneGeom g, const FoldFast<R> *__restrict__ C,
+    unsigned long long *__restrict__ max_dt) {
+  __shared__ R s[kTileB + 2][kTileA + 2];
+  const int tx = threadIdx.x, ty = threadIdx.y;
+  const int a0 = blockIdx.x * kTileA, b0 = blockIdx.y * kTileB;
+  const int a = a0 + 1 + tx, b = b0 + 1 + ty;
+  const bool active = a <= g.na && b <= g.nb;
+  const long long c = g.base + (long long)a * g.Sa + (long long)b * g.Sb;
+  const R told = active ? T[c + g.sm] : R(0);
+  for (int idx = ty * kTileA + tx; idx < (kTileA + 2) * (kTileB + 2); idx += kTileA * kTileB) {
+    const int la = idx % (kTileA + 2), lb = idx / (kTileA + 2);
+    const int sa = a0 + la, sb = b0 + lb;
+    if (sa <= g.na + 1 && sb <= g.nb + 1)
+      s[lb][la] = fabs(T[g.base + (long long)sa * g.Sa + (long long)sb * g.Sb]);
+  }
+  __syncthreads();
+  R d = R(0);
+  if (active) {
+    const int ca = tx + 1, cb = ty + 1;
+    const int ua = g.ua, ub = g.ub, va = g.va, vb = g.vb;
+    Patch<R> p;
+    p.c = s[cb][ca];
+    p.px = s[cb + ub][ca + ua];
+    p.py = s[cb + vb][ca + va];
+    p.mx = s[cb - ub][ca - ua];
+    p.my = s[cb - vb][ca - va];
+    p.pxpy = s[cb + ub + vb][ca + ua + va];
+    p.pxmy = s[cb + ub - vb][ca + ua - va];
+    p.mxpy = s[cb - ub + vb][ca - ua + va];
+    p.mxmy = s[cb - ub - vb][ca - ua - va];
+    R tn;
+    if (told < R(0))
+      tn = fold_node_fast(told, p, C->sgn[1]);
+    else
+      tn = fold_node_fast(told, p, C->sgn[0]);
+    if (tn != told) T[c + g.sm] = tn;
+    if (kReduce) d = fabs(tn - told);
+  }
+  if (kReduce) {
+    double v = (double)d;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, 64));
+    if (tx == 0 && v > 0.0) atomicMax(max_dt, (unsigned long long)__double_as_longlong(v));
+  }
+}
+
+template <typename R>
+static void tiled_direction(hipStream_t st, R *T, const Grid &G, int d,
+                            const FoldFast<R> *C, unsigned long long *max_dt) {
+  const Frame &fr = kFrames[d];
+  const int n = G.n[fr.m];
+  const dim3 block(kTileA, kTileB);
+  for (int q = 0; q < n - 1; ++q) {
+    const int p = fr.s > 0 ? 1 + q : n - q;
+    const PlaneGeom g = plane_geom(G, d, p);
+    const dim3 grid((g.na + kTileA - 1) / kTileA, (g.nb + kTileB - 1) / kTileB);
+    if (max_dt)
+      fold_tile_kernel<R, true><<<grid, block, 0, st>>>(T, g, C, max_dt);
+    else
+      fold_tile_kernel<R, false><<<grid, block, 0, st>>>(T, g, C, nullptr);
+  }
+}
+
+template <typename R>
+int fold_sweeps_tiled_engine(hipStream_t st, R *T, R *Y, const Grid &G,
+                             const FoldFast<R> *d_consts, int max_iters, double tol,
+                             unsigned long long *d_max, int *iters_out, double *max_dt_out) {
+  const Grid GX = G.with_order(kOrderX), GZ = G.with_order(kOrderZ), GY = G.with_order(kOrderY);
+  int it = 0;
+  double last = -1.0;
+  for (; it < max_iters;) {
+    unsigned long long *md = tol > 0 ? d_max : nullptr;
+    if (md) zero_u64_kernel<<<1, 1, 0, st>>>(d_max);
+    // x_down, x_up with x slowest
+    permute_layout<R>(st, T, GZ, Y, GX);
+    tiled_direction<R>(st, Y, GX, FOLD3D_X_DOWN, d_consts + FOLD3D_X_DOWN, md);
+    tiled_direction<R>(st, Y, GX, FOLD3D_X_UP, d_consts + FOLD3D_X_UP, md);
+    permute_layout<R>(st, Y, GX, T, GZ);
+    // k_up, k_down in the reference layout
+    tiled_direction<R>(st, T, GZ, FOLD3D_K_UP, d_consts + FOLD3D_K_UP, md);
+    tiled_direction<R>(st, T, GZ, FOLD3D_K_DOWN, d_consts + FOLD3D_K_DOWN, md);
+    // y_up, y_down with y slowest
+    permute_layout<R>(st, T, GZ, Y, GY);
+    tiled_direction<R>(st, Y, GY, FOLD3D_Y_UP, d_consts + FOLD3D_Y_UP, md);
+    tiled_direction<R>(st, Y, GY, FOLD3D_Y_DOWN, d_consts + FOLD3D_Y_DOWN, md);
+    permute_layout<R>(st, Y, GY, T, GZ);
+    F3D_HIP(hipGetLastError());
+    ++it;
+    if (md) {
+      unsigned long long bits = 0;
+      F3D_HIP(hipMemcpyAsync(&bits, d_max, sizeof bits, hipMemcpyDeviceToHost, st));
+      F3D_HIP(hipStreamSynchronize(st));
+      memcpy(&last, &bits, sizeof last);
+      if (last < tol) break;
+    }
+  }
+  *iters_out = it;
+  *max_dt_out = last;
+  return FOLD3D_OK;
+}
+template int fold_sweeps_tiled_engine<double>(hipStream_t, double *, double *, const Grid &,
+                                              const FoldFast<double> *, int, double,
+                                              unsigned long long *, int *, double *);
+template int fold_sweeps_tiled_engine<float>(hipStream_t, float *, float *, const Grid &,
+                                             const FoldFast<float> *, int, double,
+                                             unsigned long long *, int *, double *);
 
 }  // namespace fold3d

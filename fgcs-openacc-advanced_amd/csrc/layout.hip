@@ -1,11 +1,15 @@
-// layout.hip — reorders a padded field between the reference layout
-//   A: T[i + H0*(j + H1*k)]          (i fastest; Fold3dPMM.h:27-30)
-// and the x-march layout
-//   B: T[j + H1*(k + H2*i)]          (j fastest, i slowest)
-// in which a plane of constant i is one contiguous slab whose rows run along
-// j, so the x-sweeps read and write whole rows like the y- and z-sweeps do in A.
-// Both directions stage a 32x32 tile through LDS so reads and writes are
-// coalesced.
+// layout.hip — moves a padded field between axis orders.
+//
+// The reference stores T[i + H0*(j + H1*k)]: x fastest, z slowest (layout Z,
+// Fold3dPMM.h:27-30).  A sweep reads plane p and writes plane p+-1 along its
+// march axis; when that axis is the slowest one, a plane is one contiguous
+// slab and the three source rows a node needs are neighbours in memory.  The
+// engine therefore runs each pair of sweeps in its own order:
+//   layout X = (y, z, x)  for the x-sweeps   T[j + H1*(k + H2*i)]
+//   layout Z = (x, y, z)  for the z-sweeps   (the reference layout)
+//   layout Y = (x, z, y)  for the y-sweeps   T[i + H0*(k + H2*j)]
+// and moves the field between them with the kernel below (a 32x32 tile of the
+// (x, y) plane per block, staged through LDS when the fastest axis changes).
 #include <hip/hip_runtime.h>
 
 #include "common.h"
@@ -16,65 +20,54 @@ namespace fold3d {
 constexpr int kTile = 32;
 constexpr int kTileRows = 8;
 
-// A -> B: for each k, B[i][k][j] = A[k][j][i].
-template <typename R>
-__global__ __launch_bounds__(kTile *kTileRows) void to_xmarch_kernel(
-    const R *__restrict__ A, R *__restrict__ B, int H0, int H1, int H2) {
-  __shared__ R tile[kTile][kTile + 1];
-  const int i0 = blockIdx.x * kTile, j0 = blockIdx.y * kTile, k = blockIdx.z;
-  const long long plane = (long long)H0 * H1 * k;
+struct Strides3 {
+  long long s[3];
+};
+
+template <typename R, bool kInFastX, bool kOutFastX>
+__global__ __launch_bounds__(kTile *kTileRows) void permute_kernel(
+    const R *__restrict__ in, R *__restrict__ out, Strides3 is, Strides3 os, int Hx, int Hy) {
+  __shared__ R tile[kTile][kTile + 1];  // [y - y0][x - x0]
+  const int x0 = blockIdx.x * kTile, y0 = blockIdx.y * kTile;
+  const long long z = blockIdx.z;
+  const long long ib = z * is.s[2], ob = z * os.s[2];
+  const int tx = threadIdx.x;
 #pragma unroll
   for (int r = threadIdx.y; r < kTile; r += kTileRows) {
-    const int i = i0 + threadIdx.x, j = j0 + r;
-    if (i < H0 && j < H1) tile[r][threadIdx.x] = A[plane + (long long)H0 * j + i];
+    const int x = x0 + (kInFastX ? tx : r), y = y0 + (kInFastX ? r : tx);
+    if (x < Hx && y < Hy)
+      tile[y - y0][x - x0] = in[ib + x * is.s[0] + y * is.s[1]];
   }
   __syncthreads();
 #pragma unroll
   for (int r = threadIdx.y; r < kTile; r += kTileRows) {
-    const int j = j0 + threadIdx.x, i = i0 + r;
-    if (i < H0 && j < H1) B[(long long)H1 * ((long long)H2 * i + k) + j] = tile[threadIdx.x][r];
+    const int x = x0 + (kOutFastX ? tx : r), y = y0 + (kOutFastX ? r : tx);
+    if (x < Hx && y < Hy) out[ob + x * os.s[0] + y * os.s[1]] = tile[y - y0][x - x0];
   }
 }
 
-// B -> A: for each k, A[k][j][i] = B[i][k][j].
-template <typename R>
-__global__ __launch_bounds__(kTile *kTileRows) void from_xmarch_kernel(
-    const R *__restrict__ B, R *__restrict__ A, int H0, int H1, int H2) {
-  __shared__ R tile[kTile][kTile + 1];
-  const int j0 = blockIdx.x * kTile, i0 = blockIdx.y * kTile, k = blockIdx.z;
-#pragma unroll
-  for (int r = threadIdx.y; r < kTile; r += kTileRows) {
-    const int j = j0 + threadIdx.x, i = i0 + r;
-    if (i < H0 && j < H1) tile[r][threadIdx.x] = B[(long long)H1 * ((long long)H2 * i + k) + j];
-  }
-  __syncthreads();
-  const long long plane = (long long)H0 * H1 * k;
-#pragma unroll
-  for (int r = threadIdx.y; r < kTile; r += kTileRows) {
-    const int i = i0 + threadIdx.x, j = j0 + r;
-    if (i < H0 && j < H1) A[plane + (long long)H0 * j + i] = tile[threadIdx.x][r];
-  }
+static Strides3 strides_of(const Grid &G) {
+  Strides3 s;
+  for (int a = 0; a < 3; ++a) s.s[a] = G.stride[a];
+  return s;
 }
 
 template <typename R>
-void to_xmarch(hipStream_t st, const R *A, R *B, const Grid &G) {
-  const int H0 = G.n[0] + 2, H1 = G.n[1] + 2, H2 = G.n[2] + 2;
-  dim3 block(kTile, kTileRows);
-  dim3 grid((H0 + kTile - 1) / kTile, (H1 + kTile - 1) / kTile, H2);
-  to_xmarch_kernel<R><<<grid, block, 0, st>>>(A, B, H0, H1, H2);
+void permute_layout(hipStream_t st, const R *in, const Grid &Gin, R *out, const Grid &Gout) {
+  const int Hx = Gin.n[0] + 2, Hy = Gin.n[1] + 2, Hz = Gin.n[2] + 2;
+  const bool in_x = Gin.stride[0] == 1, out_x = Gout.stride[0] == 1;
+  const dim3 block(kTile, kTileRows);
+  const dim3 grid((Hx + kTile - 1) / kTile, (Hy + kTile - 1) / kTile, Hz);
+  const Strides3 is = strides_of(Gin), os = strides_of(Gout);
+  if (in_x && out_x) permute_kernel<R, true, true><<<grid, block, 0, st>>>(in, out, is, os, Hx, Hy);
+  else if (in_x) permute_kernel<R, true, false><<<grid, block, 0, st>>>(in, out, is, os, Hx, Hy);
+  else if (out_x) permute_kernel<R, false, true><<<grid, block, 0, st>>>(in, out, is, os, Hx, Hy);
+  else permute_kernel<R, false, false><<<grid, block, 0, st>>>(in, out, is, os, Hx, Hy);
 }
 
-template <typename R>
-void from_xmarch(hipStream_t st, const R *B, R *A, const Grid &G) {
-  const int H0 = G.n[0] + 2, H1 = G.n[1] + 2, H2 = G.n[2] + 2;
-  dim3 block(kTile, kTileRows);
-  dim3 grid((H1 + kTile - 1) / kTile, (H0 + kTile - 1) / kTile, H2);
-  from_xmarch_kernel<R><<<grid, block, 0, st>>>(B, A, H0, H1, H2);
-}
-
-template void to_xmarch<double>(hipStream_t, const double *, double *, const Grid &);
-template void to_xmarch<float>(hipStream_t, const float *, float *, const Grid &);
-template void from_xmarch<double>(hipStream_t, const double *, double *, const Grid &);
-template void from_xmarch<float>(hipStream_t, const float *, float *, const Grid &);
+template void permute_layout<double>(hipStream_t, const double *, const Grid &, double *,
+                                     const Grid &);
+template void permute_layout<float>(hipStream_t, const float *, const Grid &, float *,
+                                    const Grid &);
 
 }  // namespace fold3d

@@ -14,6 +14,7 @@ struct PlaneGeom {
   long long base, Sa, Sb;
   long long sm, su, sv;
   int na, nb;
+  int ua, ub, va, vb;  // x^ and y^ as (a, b) offsets within a tile
 };
 
 PlaneGeom plane_geom(const Grid &G, int dir, int plane);
@@ -41,6 +42,17 @@ template <typename R>
 int fold_sweeps_fast_engine(hipStream_t st, R *T, const Grid &G, const FoldFast<R> *d_consts,
                             int max_iters, double tol, unsigned long long *d_max,
                             int *iters_out, double *max_dt_out);
+
+// Hoisted constants + LDS-staged tiles + per-direction layouts.  `Y` is a
+// scratch field of G.count elements.
+template <typename R>
+int fold_sweeps_tiled_engine(hipStream_t st, R *T, R *Y, const Grid &G,
+                             const FoldFast<R> *d_consts, int max_iters, double tol,
+                             unsigned long long *d_max, int *iters_out, double *max_dt_out);
+
+// Copy a padded field from one axis order to another (layout.hip).
+template <typename R>
+void permute_layout(hipStream_t st, const R *in, const Grid &Gin, R *out, const Grid &Gout);
 
 int fold_plane_step_f64(hipStream_t st, double *T, const Grid &G, int dir,
                         int plane, const FoldFrame<double> &f);

@@ -28,7 +28,7 @@ FOLD3D_E_NOT_CONVERGED = -5
 F64, F32 = 0, 1
 DEVICE, HOST = 0, 1
 MODEL_FOLD, MODEL_ELLIP = 0, 1
-ENGINE_AUTO, ENGINE_PLANE = 0, 1
+ENGINE_AUTO, ENGINE_PLANE, ENGINE_HOISTED, ENGINE_TILED = 0, 1, 2, 3
 
 MAXT = 10e15 * 1.0
 REF_F = 1.1                      # reference Fold3dPMM.h:52

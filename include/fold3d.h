@@ -94,8 +94,12 @@ enum fold3d_engine {
   FOLD3D_ENGINE_AUTO = 0,    /* the fastest engine available for the model */
   FOLD3D_ENGINE_PLANE = 1,   /* one launch per plane, node update written as the
                                 reference's local solvers (fold model) */
-  FOLD3D_ENGINE_HOISTED = 2  /* one launch per plane, per-direction invariants
+  FOLD3D_ENGINE_HOISTED = 2, /* one launch per plane, per-direction invariants
                                 hoisted out of the node update (same bits) */
+  FOLD3D_ENGINE_TILED = 3    /* hoisted node update, source tile staged in LDS,
+                                each sweep pair run in a layout whose march axis
+                                is slowest (needs a scratch copy of T, owned by
+                                the handle) */
 };
 
 typedef struct fold3d_stats {

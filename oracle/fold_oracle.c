@@ -137,12 +137,12 @@ static double diag(const double h[3], int a, int b)
   return sqrt(h[lo] * h[lo] + h[hi] * h[hi]);
 }
 
-static local_t make_local(int dir, int n, double F, const double a[3],
+static local_t make_local(int dir, const int n[3], double F, const double a[3],
                           const double h[3])
 {
   const frame_t *f = &FRAMES[dir];
-  long H = n + 2;
-  long stride[3] = {1, H, H * H};
+  const long H0 = n[0] + 2, H1 = n[1] + 2;
+  const long stride[3] = {1, H0, H0 * H1};
   local_t L;
   L.F = F;
   L.ax = f->sx * a[f->ux];
@@ -225,39 +225,63 @@ static void cell_update(double *T, long c, const local_t *L)
   T[c + L->sm] = rsign(told) * tnew; /* :326 */
 }
 
-void orc_plane(int dir, double *T, int n, int p, double F, const double a[3],
-               const double h[3])
+/* Box grids (nx, ny, nz); the reference itself is cubic (one HEIGHT). */
+void orc_plane3(int dir, double *T, const int n[3], int p, double F,
+                const double a[3], const double h[3])
 {
   const frame_t *f = &FRAMES[dir];
   const local_t L = make_local(dir, n, F, a, h);
-  const long H = n + 2;
-  const long stride[3] = {1, H, H * H};
+  const long H0 = n[0] + 2, H1 = n[1] + 2;
+  const long stride[3] = {1, H0, H0 * H1};
   const long base = (long)p * stride[f->m];
   const long s_u = stride[f->ux], s_v = stride[f->uy];
+  const long nu = n[f->ux], nv = n[f->uy];
   long u;
 #pragma omp parallel for schedule(static) num_threads(g_threads > 0 ? g_threads : omp_get_max_threads())
-  for (u = 1; u <= n; ++u) {
+  for (u = 1; u <= nu; ++u) {
     long v;
-    for (v = 1; v <= n; ++v) cell_update(T, base + u * s_u + v * s_v, &L);
+    for (v = 1; v <= nv; ++v) cell_update(T, base + u * s_u + v * s_v, &L);
   }
+}
+
+void orc_direction3(int dir, double *T, const int n[3], double F,
+                    const double a[3], const double h[3])
+{
+  const int nm = n[FRAMES[dir].m];
+  int p;
+  if (FRAMES[dir].s > 0)
+    for (p = 1; p < nm; ++p) orc_plane3(dir, T, n, p, F, a, h);
+  else
+    for (p = nm; p > 1; --p) orc_plane3(dir, T, n, p, F, a, h);
+}
+
+void orc_sweeps3(double *T, const int n[3], int iterations, double F,
+                 const double a[3], const double h[3])
+{
+  int it, d;
+  for (it = 0; it < iterations; ++it)
+    for (d = 0; d < 6; ++d) orc_direction3(d, T, n, F, a, h);
+}
+
+void orc_plane(int dir, double *T, int n, int p, double F, const double a[3],
+               const double h[3])
+{
+  const int n3[3] = {n, n, n};
+  orc_plane3(dir, T, n3, p, F, a, h);
 }
 
 void orc_direction(int dir, double *T, int n, double F, const double a[3],
                    const double h[3])
 {
-  int p;
-  if (FRAMES[dir].s > 0)
-    for (p = 1; p < n; ++p) orc_plane(dir, T, n, p, F, a, h);
-  else
-    for (p = n; p > 1; --p) orc_plane(dir, T, n, p, F, a, h);
+  const int n3[3] = {n, n, n};
+  orc_direction3(dir, T, n3, F, a, h);
 }
 
 void orc_sweeps(double *T, int n, int iterations, double F, const double a[3],
                 const double h[3])
 {
-  int it, d;
-  for (it = 0; it < iterations; ++it)
-    for (d = 0; d < 6; ++d) orc_direction(d, T, n, F, a, h);
+  const int n3[3] = {n, n, n};
+  orc_sweeps3(T, n3, iterations, F, a, h);
 }
 
 /* ---------------------------------------------------------------------- */

@@ -54,6 +54,14 @@ void orc_direction(int dir, double *T, int n, double F, const double a[3],
 void orc_sweeps(double *T, int n, int iterations, double F, const double a[3],
                 const double h[3]);
 
+/* The same on a box grid n = (nx, ny, nz) (the reference's grid is cubic). */
+void orc_plane3(int dir, double *T, const int n[3], int p, double F,
+                const double a[3], const double h[3]);
+void orc_direction3(int dir, double *T, const int n[3], double F,
+                    const double a[3], const double h[3]);
+void orc_sweeps3(double *T, const int n[3], int iterations, double F,
+                 const double a[3], const double h[3]);
+
 /* OpenMP thread count used by the plane loops (0 = OpenMP default). */
 void orc_set_threads(int nthreads);
 

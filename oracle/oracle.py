@@ -101,6 +101,16 @@ class Oracle:
         F, a, h = self._model(n, F, a, h)
         self.lib.orc_sweeps(_ptr(T), n, iterations, F, _ptr(a), _ptr(h))
 
+    def sweeps3(self, T: np.ndarray, n3, iterations: int, F, a, h):
+        """Fold-model sweeps on a box grid n3 = (nx, ny, nz)."""
+        lib = self.lib
+        ip = C.POINTER(C.c_int)
+        lib.orc_sweeps3.argtypes = [_dp, ip, _i, _d, _dp, _dp]
+        nn = (C.c_int * 3)(*n3)
+        a = np.ascontiguousarray(a, dtype=np.float64)
+        h = np.ascontiguousarray(h, dtype=np.float64)
+        lib.orc_sweeps3(_ptr(T), nn, iterations, F, _ptr(a), _ptr(h))
+
     def set_threads(self, k: int):
         self.lib.orc_set_threads(k)
         self.lib.orc_ellip_set_threads(k)

@@ -38,19 +38,19 @@ def cell_updates(n, iters):
     return iters * 6 * (n - 1) * n * n
 
 
-def time_solves(solver, T0d, n, steps, warmup, dtype, torch, engine=0):
+def time_solves(solver, T0d, n, steps, warmup, dtype, torch, engine=0, iters=8):
     """Wall time of `steps` solves (reset copy included), plus the summed
     HIP-event time of the sweep loops themselves."""
     T = torch.empty_like(T0d, dtype=dtype)
     for _ in range(warmup):
         T.copy_(T0d)
-        solver.solve(T, n, max_iters=8, engine=engine)
+        solver.solve(T, n, max_iters=iters, engine=engine)
     torch.cuda.synchronize()
     sweep_ms = 0.0
     t0 = time.perf_counter()
     for _ in range(steps):
         T.copy_(T0d)
-        st = solver.solve(T, n, max_iters=8, engine=engine)
+        st = solver.solve(T, n, max_iters=iters, engine=engine)
         sweep_ms += st.sweep_ms
     torch.cuda.synchronize()
     return (time.perf_counter() - t0) * 1e3, T, sweep_ms
@@ -98,6 +98,7 @@ def main():
     ap.add_argument("--no-fp32", action="store_true")
     ap.add_argument("--no-ellip", action="store_true")
     ap.add_argument("--no-ablation", action="store_true")
+    ap.add_argument("--iters", type=int, default=8, help="iterations per solve (the reference runs 8)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
@@ -116,10 +117,16 @@ def main():
     T0d = torch.from_numpy(T0).to("cuda")
     solver = fold3d.Solver(0)
 
-    W = cell_updates(n, 8)
-    launches = 8 * 6 * (n - 1)
-    log(f"bench: FP64 solve x{args.steps} (+{args.warmup} warmup), tiled engine")
-    ms, T, sweep_ms = time_solves(solver, T0d, n, args.steps, args.warmup, torch.float64, torch)
+    log("bench: autotuning waves per node row (one iteration each of 2, 4, 8)")
+    groups, tune_ms = solver.autotune(T0d, n, iterations=1)
+    log(f"bench: autotuner picked {groups} waves per node row ({tune_ms:.1f} ms/iteration)")
+
+    iters = args.iters
+    W = cell_updates(n, iters)
+    launches = iters * 6 * (n - 1)
+    log(f"bench: FP64 solve x{args.steps} (+{args.warmup} warmup), split engine (default)")
+    ms, T, sweep_ms = time_solves(solver, T0d, n, args.steps, args.warmup, torch.float64, torch,
+                                  iters=iters)
     ms_step = ms / args.steps
     value = W * args.steps / (ms / 1e3) / 1e9
     log(f"bench: FP64 {ms_step:.1f} ms/solve, {value:.2f} Gcell-upd/s")
@@ -132,16 +139,19 @@ def main():
 
     engines = {}
     if not args.no_ablation:
-        for name, eng in (("plane", fold3d.ENGINE_PLANE), ("hoisted", fold3d.ENGINE_HOISTED)):
+        for name, eng in (("plane", fold3d.ENGINE_PLANE), ("hoisted", fold3d.ENGINE_HOISTED),
+                          ("tiled", fold3d.ENGINE_TILED)):
             log(f"bench: FP64 solve, {name} engine")
-            ms_e, _, sw_e = time_solves(solver, T0d, n, 1, 0, torch.float64, torch, engine=eng)
+            ms_e, _, sw_e = time_solves(solver, T0d, n, 1, 0, torch.float64, torch, engine=eng,
+                                        iters=iters)
             engines[name] = {"ms_per_step": ms_e, "value": W / (ms_e / 1e3) / 1e9}
-        engines["tiled"] = {"ms_per_step": ms_step, "value": value}
+        engines["split"] = {"ms_per_step": ms_step, "value": value}
 
     fp32 = None
     if not args.no_fp32:
         log("bench: FP32 solve")
-        ms32, T32, _ = time_solves(solver, T0d, n, args.steps, args.warmup, torch.float32, torch)
+        ms32, T32, _ = time_solves(solver, T0d, n, args.steps, args.warmup, torch.float32, torch,
+                                   iters=iters)
         ref = T.cpu().numpy()
         got = T32.double().cpu().numpy()
         reached = np.abs(ref) < 1e15
@@ -194,11 +204,12 @@ def main():
         "vs_baseline": value / A100_V2_512_GCUPS if n == 512 else None,
         "dtype": "f64",
         "data": "synthetic: the reference's implicit-surface initial field (Fold3dPMM.c)",
-        "config": {"workload": f"fold-model PMM solve, {n}^3 nodes, FP64, 8 iterations x 6 sweeps",
-                   "n": n, "iterations": 8, "cell_updates_per_step": W},
+        "config": {"workload": f"fold-model PMM solve, {n}^3 nodes, FP64, {iters} iterations x 6 sweeps",
+                   "n": n, "iterations": iters, "cell_updates_per_step": W,
+                   "engine": "split", "split_groups": groups},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "fold_tile_kernel<double>", "kernel_us": us,
+                     "kernel": "fold_split_kernel<double>", "kernel_us": us,
                      "bytes_per_launch": bytes_per_launch},
         "engines_fp64": engines,
         "fp32": fp32,

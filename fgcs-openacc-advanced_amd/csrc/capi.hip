@@ -20,6 +20,7 @@ struct fold3d_handle {
   hipEvent_t consts_copied;   // staging may be rewritten once this completes
   void *d_scratch;            // a second field for the layout changes
   size_t scratch_bytes;
+  int split_groups;           // waves per node row of the split kernel (2, 4, 8)
 };
 
 // Grow the handle's scratch field to at least `bytes` (stream-ordered free of
@@ -74,6 +75,7 @@ int fold3d_create(fold3d_handle **out, int device) {
   F3D_HIP(hipSetDevice(device));
   fold3d_handle *h = new fold3d_handle();
   h->device = device;
+  h->split_groups = 4;
   hipError_t e = hipMalloc((void **)&h->d_max, sizeof(unsigned long long));
   if (e != hipSuccess) {
     delete h;
@@ -122,7 +124,7 @@ int fold3d_solve(fold3d_handle *h, void *T, int nx, int ny, int nz, double dx,
             "fold3d_solve: bad precision %d", o.precision);
   F3D_CHECK(o.memory == FOLD3D_DEVICE || o.memory == FOLD3D_HOST, FOLD3D_E_INVALID,
             "fold3d_solve: bad memory kind %d", o.memory);
-  F3D_CHECK(o.engine >= FOLD3D_ENGINE_AUTO && o.engine <= FOLD3D_ENGINE_TILED, FOLD3D_E_INVALID,
+  F3D_CHECK(o.engine >= FOLD3D_ENGINE_AUTO && o.engine <= FOLD3D_ENGINE_SPLIT, FOLD3D_E_INVALID,
             "fold3d_solve: bad engine %d", o.engine);
   F3D_CHECK(model->kind == FOLD3D_MODEL_FOLD || model->kind == FOLD3D_MODEL_ELLIP,
             FOLD3D_E_INVALID, "fold3d_solve: unknown model kind %d", model->kind);
@@ -194,13 +196,16 @@ int fold3d_solve(fold3d_handle *h, void *T, int nx, int ny, int nz, double dx,
     } else {
       rc = ensure_scratch(h, bytes);
       if (rc != FOLD3D_OK) return rc;
+      const int groups = o.engine == FOLD3D_ENGINE_TILED ? 0 : h->split_groups;
       rc = o.precision == FOLD3D_F64
                ? fold_sweeps_tiled_engine<double>(st, (double *)dT, (double *)h->d_scratch, G,
                                                   (const FoldFast<double> *)h->d_consts,
-                                                  o.max_iters, o.tol, h->d_max, &iters, &maxdt)
+                                                  o.max_iters, o.tol, groups, h->d_max, &iters,
+                                                  &maxdt)
                : fold_sweeps_tiled_engine<float>(st, (float *)dT, (float *)h->d_scratch, G,
                                                  (const FoldFast<float> *)h->d_consts,
-                                                 o.max_iters, o.tol, h->d_max, &iters, &maxdt);
+                                                 o.max_iters, o.tol, groups, h->d_max, &iters,
+                                                 &maxdt);
     }
   } else
     rc = o.precision == FOLD3D_F64
@@ -232,6 +237,65 @@ int fold3d_solve(fold3d_handle *h, void *T, int nx, int ny, int nz, double dx,
     return FOLD3D_E_NOT_CONVERGED;
   }
   return FOLD3D_OK;
+}
+
+int fold3d_set_tuning(fold3d_handle *h, int split_groups) {
+  F3D_CHECK(h != nullptr, FOLD3D_E_INVALID, "fold3d_set_tuning: NULL handle");
+  F3D_CHECK(split_groups == 0 || split_groups == 2 || split_groups == 4 || split_groups == 8,
+            FOLD3D_E_INVALID, "fold3d_set_tuning: split_groups must be 0, 2, 4 or 8");
+  h->split_groups = split_groups == 0 ? 4 : split_groups;
+  return FOLD3D_OK;
+}
+
+int fold3d_get_tuning(fold3d_handle *h, int *split_groups) {
+  F3D_CHECK(h && split_groups, FOLD3D_E_INVALID, "fold3d_get_tuning: NULL argument");
+  *split_groups = h->split_groups;
+  return FOLD3D_OK;
+}
+
+int fold3d_autotune(fold3d_handle *h, const void *T, int nx, int ny, int nz, double dx,
+                    double dy, double dz, const fold3d_model *model, const fold3d_opts *opts,
+                    int iterations, float *ms_out) {
+  F3D_CHECK(h && T && model, FOLD3D_E_INVALID, "fold3d_autotune: NULL handle, T or model");
+  F3D_CHECK(iterations >= 1, FOLD3D_E_INVALID, "fold3d_autotune: iterations must be >= 1");
+  fold3d_opts o;
+  fold3d_default_opts(&o);
+  if (opts) o = *opts;
+  F3D_CHECK(o.memory == FOLD3D_DEVICE, FOLD3D_E_INVALID, "fold3d_autotune: T must be on the device");
+  F3D_CHECK(nx >= 3 && ny >= 3 && nz >= 3, FOLD3D_E_INVALID, "fold3d_autotune: grid too small");
+  const Grid G(nx, ny, nz);
+  const size_t bytes = (size_t)G.count * (o.precision == FOLD3D_F64 ? 8 : 4);
+  F3D_HIP(hipSetDevice(h->device));
+  void *work = nullptr;
+  F3D_HIP(hipMalloc(&work, bytes));
+  hipStream_t st = (hipStream_t)o.stream;
+  o.max_iters = iterations;
+  o.tol = 0.0;
+  o.engine = FOLD3D_ENGINE_SPLIT;
+  static const int kCandidates[3] = {2, 4, 8};
+  int best = h->split_groups;
+  float best_ms = -1.0f;
+  int rc = FOLD3D_OK;
+  for (int r = 0; r < 2 && rc == FOLD3D_OK; ++r) {  // round 0 warms up
+    for (int c = 0; c < 3 && rc == FOLD3D_OK; ++c) {
+      h->split_groups = kCandidates[c];
+      if (hipMemcpyAsync(work, T, bytes, hipMemcpyDeviceToDevice, st) != hipSuccess) {
+        rc = FOLD3D_E_HIP;
+        break;
+      }
+      fold3d_stats s;
+      rc = fold3d_solve(h, work, nx, ny, nz, dx, dy, dz, model, &o, &s);
+      if (rc == FOLD3D_OK && r == 1 && (best_ms < 0 || s.sweep_ms < best_ms)) {
+        best_ms = (float)s.sweep_ms;
+        best = kCandidates[c];
+      }
+    }
+  }
+  h->split_groups = best;
+  (void)hipStreamSynchronize(st);
+  (void)hipFree(work);
+  if (ms_out) *ms_out = best_ms;
+  return rc;
 }
 
 int fold3d_plane_step(fold3d_handle *h, int direction, double *T, int nx, int ny,

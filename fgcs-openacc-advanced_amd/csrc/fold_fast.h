@@ -156,6 +156,68 @@ __device__ __forceinline__ R fold_node_fast(R told, const Patch<R> &p, const Fol
   return ((told > R(0)) ? R(1) : R(-1)) * tnew;
 }
 
+// The node update cut into eight units, each one triangle (with its sub-edge)
+// and one of the node's edges.  The minimum over the units' candidates and
+// the centre one-point value equals fold_node_fast() (the candidates may be
+// taken in any order), so a block can let different waves evaluate different
+// units of the same nodes and take the minimum at the end.
+template <typename R>
+__device__ __forceinline__ R fold_start(R told, R st, const FoldSign<R> &S) {
+  R tnew = fabs(told);
+  if (st < tnew && S.k0_ok) tnew = rmin(tnew, st + S.k0);
+  return tnew;
+}
+
+// The reference's gate (AN_ISO_1_OpenACC.c:231): with it closed the node
+// keeps its one-point value.
+template <typename R>
+__device__ __forceinline__ bool fold_gate_open(R tnew, const Patch<R> &p) {
+  return !(tnew < p.c && tnew < rmin(p.px, p.py) && tnew < rmin(p.pxpy, p.pxmy) &&
+           tnew < rmin(p.mxmy, p.mxpy));
+}
+
+// Unit u (uniform across the wave): triangle u with sub-edge 8+u, and edge E(u)
+// with E = {px, pxmy, mx, mxpy, mxmy, my, pxpy, py}.
+template <typename R>
+__device__ __forceinline__ void fold_unit(R &tnew, const Patch<R> &p, const FoldSign<R> &S,
+                                          int u) {
+  const R st = p.c;
+  switch (u) {
+    case 0:
+      fast_tri(tnew, st, p.px, p.pxpy, S, S.tri[0], S.edge[8]);
+      fast_edge(tnew, st, p.px, S.F, S.edge[0]);
+      break;
+    case 1:
+      fast_tri(tnew, st, p.px, p.pxmy, S, S.tri[1], S.edge[9]);
+      fast_edge(tnew, st, p.pxmy, S.F, S.edge[7]);
+      break;
+    case 2:
+      fast_tri(tnew, st, p.mx, p.mxmy, S, S.tri[2], S.edge[10]);
+      fast_edge(tnew, st, p.mx, S.F, S.edge[4]);
+      break;
+    case 3:
+      fast_tri(tnew, st, p.mx, p.mxpy, S, S.tri[3], S.edge[11]);
+      fast_edge(tnew, st, p.mxpy, S.F, S.edge[3]);
+      break;
+    case 4:
+      fast_tri(tnew, st, p.my, p.mxmy, S, S.tri[4], S.edge[12]);
+      fast_edge(tnew, st, p.mxmy, S.F, S.edge[5]);
+      break;
+    case 5:
+      fast_tri(tnew, st, p.my, p.pxmy, S, S.tri[5], S.edge[13]);
+      fast_edge(tnew, st, p.my, S.F, S.edge[6]);
+      break;
+    case 6:
+      fast_tri(tnew, st, p.py, p.pxpy, S, S.tri[6], S.edge[14]);
+      fast_edge(tnew, st, p.pxpy, S.F, S.edge[1]);
+      break;
+    default:
+      fast_tri(tnew, st, p.py, p.mxpy, S, S.tri[7], S.edge[15]);
+      fast_edge(tnew, st, p.py, S.F, S.edge[2]);
+      break;
+  }
+}
+
 // ---------------------------------------------------------------------------
 // Host-side construction (plain double arithmetic, same expressions as the
 // reference, so the hoisted values carry the bits the node loop would compute).

@@ -347,26 +347,160 @@ __global__ __launch_bounds__(kTileA *kTileB) void fold_tile_kernel(
   }
 }
 
+// ---------------------------------------------------------------------------
+// Split kernel.  A block owns one row of 64 target nodes and has kGroups
+// waves; wave w evaluates units w, w + kGroups, ... (fold_unit) of all 64
+// nodes with wave-uniform constants, and the partial minima meet in LDS.  A
+// plane thus runs kGroups times as many waves as nodes/64, which is what the
+// latency-bound node update needs to keep the SIMDs issuing.
+// ---------------------------------------------------------------------------
+// Units G0, G0 + kStride, ... (< 8) of one node, starting from the centre
+// one-point value; unit numbers are compile-time constants.
+template <typename R, int G0, int kStride>
+__device__ __forceinline__ void units_of(R &tn, const Patch<R> &p, const FoldSign<R> &S) {
+#pragma unroll
+  for (int u = G0; u < 8; u += kStride) fold_unit(tn, p, S, u);
+}
+
+template <typename R, int G0, int kStride>
+__device__ __forceinline__ R split_units(R told, const Patch<R> &p, const FoldFast<R> *C) {
+  R tn;
+  if (told < R(0)) {
+    const FoldSign<R> &S = C->sgn[1];
+    tn = fold_start(told, p.c, S);
+    if (fold_gate_open(tn, p)) units_of<R, G0, kStride>(tn, p, S);
+  } else {
+    const FoldSign<R> &S = C->sgn[0];
+    tn = fold_start(told, p.c, S);
+    if (fold_gate_open(tn, p)) units_of<R, G0, kStride>(tn, p, S);
+  }
+  return tn;
+}
+
+template <typename R, int kGroups>
+__device__ __forceinline__ R split_dispatch(int grp, R told, const Patch<R> &p,
+                                            const FoldFast<R> *C) {
+  if constexpr (kGroups == 2) {
+    return grp == 0 ? split_units<R, 0, 2>(told, p, C) : split_units<R, 1, 2>(told, p, C);
+  } else if constexpr (kGroups == 4) {
+    switch (grp) {
+      case 0: return split_units<R, 0, 4>(told, p, C);
+      case 1: return split_units<R, 1, 4>(told, p, C);
+      case 2: return split_units<R, 2, 4>(told, p, C);
+      default: return split_units<R, 3, 4>(told, p, C);
+    }
+  } else {
+    static_assert(kGroups == 8, "two, four or eight waves per node row");
+    switch (grp) {
+      case 0: return split_units<R, 0, 8>(told, p, C);
+      case 1: return split_units<R, 1, 8>(told, p, C);
+      case 2: return split_units<R, 2, 8>(told, p, C);
+      case 3: return split_units<R, 3, 8>(told, p, C);
+      case 4: return split_units<R, 4, 8>(told, p, C);
+      case 5: return split_units<R, 5, 8>(told, p, C);
+      case 6: return split_units<R, 6, 8>(told, p, C);
+      default: return split_units<R, 7, 8>(told, p, C);
+    }
+  }
+}
+
+template <typename R, bool kReduce, int kGroups>
+__global__ __launch_bounds__(64 * kGroups) void fold_split_kernel(
+    R *__restrict__ T, PlaneGeom g, const FoldFast<R> *__restrict__ C,
+    unsigned long long *__restrict__ max_dt) {
+  __shared__ R s[3][64 + 2];
+  __shared__ R part[kGroups][64];
+  const int tx = threadIdx.x;
+  const int grp = __builtin_amdgcn_readfirstlane(threadIdx.y);
+  const int a0 = blockIdx.x * 64, b = 1 + blockIdx.y;
+  for (int idx = grp * 64 + tx; idx < 3 * 66; idx += 64 * kGroups) {
+    const int la = idx % 66, lb = idx / 66;
+    const int sa = a0 + la;
+    if (sa <= g.na + 1)
+      s[lb][la] = fabs(T[g.base + (long long)sa * g.Sa + (long long)(b - 1 + lb) * g.Sb]);
+  }
+  const int a = a0 + 1 + tx;
+  const bool active = a <= g.na;
+  const long long c = g.base + (long long)a * g.Sa + (long long)b * g.Sb;
+  const R told = active ? T[c + g.sm] : R(0);
+  __syncthreads();
+  R tn = fabs(told);
+  if (active) {
+    const int ca = tx + 1;
+    const int ua = g.ua, ub = g.ub, va = g.va, vb = g.vb;
+    Patch<R> p;
+    p.c = s[1][ca];
+    p.px = s[1 + ub][ca + ua];
+    p.py = s[1 + vb][ca + va];
+    p.mx = s[1 - ub][ca - ua];
+    p.my = s[1 - vb][ca - va];
+    p.pxpy = s[1 + ub + vb][ca + ua + va];
+    p.pxmy = s[1 + ub - vb][ca + ua - va];
+    p.mxpy = s[1 - ub + vb][ca - ua + va];
+    p.mxmy = s[1 - ub - vb][ca - ua - va];
+    tn = split_dispatch<R, kGroups>(grp, told, p, C);  // grp is wave-uniform
+  }
+  part[grp][tx] = tn;
+  __syncthreads();
+  if (grp == 0) {
+    R d = R(0);
+    if (active) {
+      R m = part[0][tx];
+#pragma unroll
+      for (int k = 1; k < kGroups; ++k) m = rmin(part[k][tx], m);
+      const R res = ((told > R(0)) ? R(1) : R(-1)) * m;
+      if (res != told) T[c + g.sm] = res;
+      if (kReduce) d = fabs(res - told);
+    }
+    if (kReduce) {
+      double v = (double)d;
+#pragma unroll
+      for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, 64));
+      if (tx == 0 && v > 0.0) atomicMax(max_dt, (unsigned long long)__double_as_longlong(v));
+    }
+  }
+}
+
+template <typename R, int kGroups>
+static void launch_split(hipStream_t st, R *T, const PlaneGeom &g, const FoldFast<R> *C,
+                         unsigned long long *max_dt) {
+  const dim3 block(64, kGroups);
+  const dim3 grid((g.na + 63) / 64, g.nb);
+  if (max_dt)
+    fold_split_kernel<R, true, kGroups><<<grid, block, 0, st>>>(T, g, C, max_dt);
+  else
+    fold_split_kernel<R, false, kGroups><<<grid, block, 0, st>>>(T, g, C, nullptr);
+}
+
+// groups: 0 = one thread per node (fold_tile_kernel), 2/4/8 = waves per node row.
 template <typename R>
 static void tiled_direction(hipStream_t st, R *T, const Grid &G, int d,
-                            const FoldFast<R> *C, unsigned long long *max_dt) {
+                            const FoldFast<R> *C, unsigned long long *max_dt, int groups) {
   const Frame &fr = kFrames[d];
   const int n = G.n[fr.m];
-  const dim3 block(kTileA, kTileB);
   for (int q = 0; q < n - 1; ++q) {
     const int p = fr.s > 0 ? 1 + q : n - q;
     const PlaneGeom g = plane_geom(G, d, p);
-    const dim3 grid((g.na + kTileA - 1) / kTileA, (g.nb + kTileB - 1) / kTileB);
-    if (max_dt)
-      fold_tile_kernel<R, true><<<grid, block, 0, st>>>(T, g, C, max_dt);
-    else
-      fold_tile_kernel<R, false><<<grid, block, 0, st>>>(T, g, C, nullptr);
+    if (groups == 2) {
+      launch_split<R, 2>(st, T, g, C, max_dt);
+    } else if (groups == 4) {
+      launch_split<R, 4>(st, T, g, C, max_dt);
+    } else if (groups == 8) {
+      launch_split<R, 8>(st, T, g, C, max_dt);
+    } else {
+      const dim3 block(kTileA, kTileB);
+      const dim3 grid((g.na + kTileA - 1) / kTileA, (g.nb + kTileB - 1) / kTileB);
+      if (max_dt)
+        fold_tile_kernel<R, true><<<grid, block, 0, st>>>(T, g, C, max_dt);
+      else
+        fold_tile_kernel<R, false><<<grid, block, 0, st>>>(T, g, C, nullptr);
+    }
   }
 }
 
 template <typename R>
 int fold_sweeps_tiled_engine(hipStream_t st, R *T, R *Y, const Grid &G,
-                             const FoldFast<R> *d_consts, int max_iters, double tol,
+                             const FoldFast<R> *d_consts, int max_iters, double tol, int groups,
                              unsigned long long *d_max, int *iters_out, double *max_dt_out) {
   const Grid GX = G.with_order(kOrderX), GZ = G.with_order(kOrderZ), GY = G.with_order(kOrderY);
   int it = 0;
@@ -376,16 +510,16 @@ int fold_sweeps_tiled_engine(hipStream_t st, R *T, R *Y, const Grid &G,
     if (md) zero_u64_kernel<<<1, 1, 0, st>>>(d_max);
     // x_down, x_up with x slowest
     permute_layout<R>(st, T, GZ, Y, GX);
-    tiled_direction<R>(st, Y, GX, FOLD3D_X_DOWN, d_consts + FOLD3D_X_DOWN, md);
-    tiled_direction<R>(st, Y, GX, FOLD3D_X_UP, d_consts + FOLD3D_X_UP, md);
+    tiled_direction<R>(st, Y, GX, FOLD3D_X_DOWN, d_consts + FOLD3D_X_DOWN, md, groups);
+    tiled_direction<R>(st, Y, GX, FOLD3D_X_UP, d_consts + FOLD3D_X_UP, md, groups);
     permute_layout<R>(st, Y, GX, T, GZ);
     // k_up, k_down in the reference layout
-    tiled_direction<R>(st, T, GZ, FOLD3D_K_UP, d_consts + FOLD3D_K_UP, md);
-    tiled_direction<R>(st, T, GZ, FOLD3D_K_DOWN, d_consts + FOLD3D_K_DOWN, md);
+    tiled_direction<R>(st, T, GZ, FOLD3D_K_UP, d_consts + FOLD3D_K_UP, md, groups);
+    tiled_direction<R>(st, T, GZ, FOLD3D_K_DOWN, d_consts + FOLD3D_K_DOWN, md, groups);
     // y_up, y_down with y slowest
     permute_layout<R>(st, T, GZ, Y, GY);
-    tiled_direction<R>(st, Y, GY, FOLD3D_Y_UP, d_consts + FOLD3D_Y_UP, md);
-    tiled_direction<R>(st, Y, GY, FOLD3D_Y_DOWN, d_consts + FOLD3D_Y_DOWN, md);
+    tiled_direction<R>(st, Y, GY, FOLD3D_Y_UP, d_consts + FOLD3D_Y_UP, md, groups);
+    tiled_direction<R>(st, Y, GY, FOLD3D_Y_DOWN, d_consts + FOLD3D_Y_DOWN, md, groups);
     permute_layout<R>(st, Y, GY, T, GZ);
     F3D_HIP(hipGetLastError());
     ++it;
@@ -402,10 +536,10 @@ int fold_sweeps_tiled_engine(hipStream_t st, R *T, R *Y, const Grid &G,
   return FOLD3D_OK;
 }
 template int fold_sweeps_tiled_engine<double>(hipStream_t, double *, double *, const Grid &,
-                                              const FoldFast<double> *, int, double,
+                                              const FoldFast<double> *, int, double, int,
                                               unsigned long long *, int *, double *);
 template int fold_sweeps_tiled_engine<float>(hipStream_t, float *, float *, const Grid &,
-                                             const FoldFast<float> *, int, double,
+                                             const FoldFast<float> *, int, double, int,
                                              unsigned long long *, int *, double *);
 
 }  // namespace fold3d

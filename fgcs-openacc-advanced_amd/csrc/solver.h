@@ -45,9 +45,11 @@ int fold_sweeps_fast_engine(hipStream_t st, R *T, const Grid &G, const FoldFast<
 
 // Hoisted constants + LDS-staged tiles + per-direction layouts.  `Y` is a
 // scratch field of G.count elements.
+// `groups`: 0 = one thread per node (fold_tile_kernel); 2, 4 or 8 = the
+// node's eight candidate units spread over that many waves (fold_split_kernel).
 template <typename R>
 int fold_sweeps_tiled_engine(hipStream_t st, R *T, R *Y, const Grid &G,
-                             const FoldFast<R> *d_consts, int max_iters, double tol,
+                             const FoldFast<R> *d_consts, int max_iters, double tol, int groups,
                              unsigned long long *d_max, int *iters_out, double *max_dt_out);
 
 // Copy a padded field from one axis order to another (layout.hip).

@@ -28,7 +28,7 @@ FOLD3D_E_NOT_CONVERGED = -5
 F64, F32 = 0, 1
 DEVICE, HOST = 0, 1
 MODEL_FOLD, MODEL_ELLIP = 0, 1
-ENGINE_AUTO, ENGINE_PLANE, ENGINE_HOISTED, ENGINE_TILED = 0, 1, 2, 3
+ENGINE_AUTO, ENGINE_PLANE, ENGINE_HOISTED, ENGINE_TILED, ENGINE_SPLIT = 0, 1, 2, 3, 4
 
 MAXT = 10e15 * 1.0
 REF_F = 1.1                      # reference Fold3dPMM.h:52
@@ -100,6 +100,10 @@ def lib() -> C.CDLL:
     L.fold3d_solve.argtypes = [vp, vp, i, i, i, d, d, d, C.POINTER(Model),
                                C.POINTER(Opts), C.POINTER(Stats)]
     L.fold3d_plane_step.argtypes = [vp, i, vp, i, i, i, i] + [d] * 10 + [vp]
+    L.fold3d_set_tuning.argtypes = [vp, i]
+    L.fold3d_get_tuning.argtypes = [vp, C.POINTER(i)]
+    L.fold3d_autotune.argtypes = [vp, vp, i, i, i, d, d, d, C.POINTER(Model),
+                                  C.POINTER(Opts), i, C.POINTER(C.c_float)]
     L.fold3d_synchronize.argtypes = [vp]
     L.ImplicitInitialiser.argtypes = [dp, i, i, i, d, d, d]
     L.InitFoldFromPoint.restype = d
@@ -202,6 +206,33 @@ class Solver:
             return st
         check(rc)
         return st
+
+    def set_tuning(self, split_groups: int):
+        check(lib().fold3d_set_tuning(self.h, split_groups))
+
+    def tuning(self) -> int:
+        g = C.c_int()
+        check(lib().fold3d_get_tuning(self.h, C.byref(g)))
+        return g.value
+
+    def autotune(self, T, n, spacing=None, model: Model | None = None,
+                 iterations: int = 1, stream=None):
+        """Time 2/4/8 waves per node row on a copy of T; keep the fastest.
+        Returns (groups, sweep ms of the winner)."""
+        import torch
+        nx, ny, nz = (n, n, n) if np.isscalar(n) else n
+        if spacing is None:
+            spacing = reference_spacing(nx)
+        dx, dy, dz = (spacing,) * 3 if np.isscalar(spacing) else spacing
+        o = Opts()
+        lib().fold3d_default_opts(C.byref(o))
+        o.precision = F64 if T.dtype == torch.float64 else F32
+        o.stream = stream if stream is not None else torch.cuda.current_stream(T.device).cuda_stream
+        m = model if model is not None else reference_model()
+        ms = C.c_float()
+        check(lib().fold3d_autotune(self.h, C.c_void_p(T.data_ptr()), nx, ny, nz, dx, dy, dz,
+                                    C.byref(m), C.byref(o), iterations, C.byref(ms)))
+        return self.tuning(), ms.value
 
     def plane_step(self, direction: int, T, n, plane: int, dzz, dxx, dyy, dxy,
                    dxz, dyz, F, ay, ax, az, stream=None):

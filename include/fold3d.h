@@ -96,10 +96,13 @@ enum fold3d_engine {
                                 reference's local solvers (fold model) */
   FOLD3D_ENGINE_HOISTED = 2, /* one launch per plane, per-direction invariants
                                 hoisted out of the node update (same bits) */
-  FOLD3D_ENGINE_TILED = 3    /* hoisted node update, source tile staged in LDS,
+  FOLD3D_ENGINE_TILED = 3,   /* hoisted node update, source tile staged in LDS,
                                 each sweep pair run in a layout whose march axis
                                 is slowest (needs a scratch copy of T, owned by
                                 the handle) */
+  FOLD3D_ENGINE_SPLIT = 4    /* as TILED, with each node's candidate units spread
+                                over four wavefronts whose minima meet in LDS
+                                (same bits; the default for the fold model) */
 };
 
 typedef struct fold3d_stats {
@@ -127,6 +130,16 @@ void fold3d_reference_model(fold3d_model *m);       /* F = 1.1, a = (-0.35, 0.4,
 int fold3d_solve(fold3d_handle *h, void *T, int nx, int ny, int nz,
                  double dx, double dy, double dz, const fold3d_model *model,
                  const fold3d_opts *opts, fold3d_stats *stats);
+
+/* Tuning of the split engine: how many wavefronts share one row of 64 nodes
+ * (2, 4 or 8; 0 restores the default 4).  fold3d_autotune() times every
+ * choice on `iterations` iterations of a scratch copy of T (T itself is not
+ * modified), keeps the fastest on the handle and returns its sweep time. */
+int fold3d_set_tuning(fold3d_handle *h, int split_groups);
+int fold3d_get_tuning(fold3d_handle *h, int *split_groups);
+int fold3d_autotune(fold3d_handle *h, const void *T, int nx, int ny, int nz,
+                    double dx, double dy, double dz, const fold3d_model *model,
+                    const fold3d_opts *opts, int iterations, float *ms_out);
 
 /* One plane step of one direction on a device FP64 field, fold model, with
  * explicit local-frame arguments exactly as the reference kernels take them

@@ -61,6 +61,7 @@ def ellip_run(solver, n, dtype, tol, torch):
     from fold3d import workloads as wl
     W = wl.blob_speed(n, xp=torch, dtype=dtype, device="cuda")
     T0 = wl.point_source_start(W, xp=torch, dtype=dtype, device="cuda")
+    groups, _ = solver.autotune(T0, n, spacing=W.h, model=W.model(), iterations=1)
     T = T0.clone()
     st = solver.solve(T, n, spacing=W.h, model=W.model(), max_iters=200, tol=tol)  # warm
     T.copy_(T0)
@@ -72,7 +73,7 @@ def ellip_run(solver, n, dtype, tol, torch):
     W_upd = st.cell_updates
     return {"iterations": st.iterations, "max_dT": st.max_dT, "sweep_ms": st.sweep_ms,
             "wall_ms": wall, "value": W_upd / (st.sweep_ms / 1e3) / 1e9,
-            "unit": "Gcell-updates/s", "cell_updates": W_upd}, T
+            "unit": "Gcell-updates/s", "cell_updates": W_upd, "split_groups": groups}, T
 
 
 def cpu_baseline(T0_host, n, threads):
@@ -191,6 +192,14 @@ def main():
                          f"{cell_updates(n, 1):.3e} cell-updates in {dt:.1f} s"}
         log(f"bench: CPU {v:.3f} Gcell-upd/s")
 
+    log("bench: measured ceilings of this GPU (FMA rate, device copy)")
+    peaks = fold3d.measure_peaks()
+    log(f"bench: {peaks}")
+    # VALU view of the same kernel: wave-instructions per plane launch over the
+    # measured FP64 FMA issue rate is what binds (see roofline.valu)
+    valu = {"measured_fp64_tflops": peaks["fp64_tflops"], "published_fp64_tflops": 78.6,
+            "measured_copy_gbs": peaks["copy_gbs"]}
+
     out = {
         "metric": "Gcell-updates/s at 512^3 (fold-model PMM solve, 8 iterations)",
         "value": value,
@@ -215,6 +224,7 @@ def main():
         "fp32": fp32,
         "elliptic_blob_speed": ellip,
         "cpu_baseline": cpu,
+        "device_ceilings": valu,
     }
     print(json.dumps(out), flush=True)
 

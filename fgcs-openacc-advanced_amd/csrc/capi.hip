@@ -20,7 +20,8 @@ struct fold3d_handle {
   hipEvent_t consts_copied;   // staging may be rewritten once this completes
   void *d_scratch;            // a second field for the layout changes
   size_t scratch_bytes;
-  int split_groups;           // waves per node row of the split kernel (2, 4, 8)
+  int split_groups;           // waves per node row of the fold split kernel (2, 4, 8)
+  int ellip_groups;           // waves per node row of the elliptic split kernel (1, 2, 4)
 };
 
 // Grow the handle's scratch field to at least `bytes` (stream-ordered free of
@@ -76,6 +77,7 @@ int fold3d_create(fold3d_handle **out, int device) {
   fold3d_handle *h = new fold3d_handle();
   h->device = device;
   h->split_groups = 4;
+  h->ellip_groups = 2;
   hipError_t e = hipMalloc((void **)&h->d_max, sizeof(unsigned long long));
   if (e != hipSuccess) {
     delete h;
@@ -207,6 +209,23 @@ int fold3d_solve(fold3d_handle *h, void *T, int nx, int ny, int nz, double dx,
                                                  o.max_iters, o.tol, groups, h->d_max, &iters,
                                                  &maxdt);
     }
+  } else if (model->theta == nullptr && o.engine != FOLD3D_ENGINE_PLANE &&
+             o.engine != FOLD3D_ENGINE_HOISTED) {
+    // constant metric: split engine; scratch = T copy (+ speed field in two layouts)
+    const int nscr = model->speed_field ? 3 : 1;
+    rc = ensure_scratch(h, nscr * bytes);
+    if (rc != FOLD3D_OK) return rc;
+    char *scr = (char *)h->d_scratch;
+    const int groups = h->ellip_groups;
+    rc = o.precision == FOLD3D_F64
+             ? ellip_sweeps_split_engine<double>(
+                   st, (double *)dT, (double *)scr, (double *)(scr + bytes),
+                   (double *)(scr + 2 * bytes), G, *model, hsp, o.max_iters, o.tol, groups,
+                   h->d_max, &iters, &maxdt)
+             : ellip_sweeps_split_engine<float>(
+                   st, (float *)dT, (float *)scr, (float *)(scr + bytes),
+                   (float *)(scr + 2 * bytes), G, *model, hsp, o.max_iters, o.tol, groups,
+                   h->d_max, &iters, &maxdt);
   } else
     rc = o.precision == FOLD3D_F64
              ? ellip_sweeps_plane_engine<double>(st, (double *)dT, G, *model, hsp, o.max_iters,
@@ -239,17 +258,73 @@ int fold3d_solve(fold3d_handle *h, void *T, int nx, int ny, int nz, double dx,
   return FOLD3D_OK;
 }
 
-int fold3d_set_tuning(fold3d_handle *h, int split_groups) {
+int fold3d_solve_batched(fold3d_handle *h, int count, const fold3d_instance *inst,
+                         const fold3d_opts *opts, int *iterations_out, double *max_dt_out,
+                         double *ms_out) {
+  F3D_CHECK(h && inst, FOLD3D_E_INVALID, "fold3d_solve_batched: NULL handle or instances");
+  F3D_CHECK(count >= 1, FOLD3D_E_INVALID, "fold3d_solve_batched: count must be >= 1");
+  fold3d_opts o;
+  fold3d_default_opts(&o);
+  if (opts) o = *opts;
+  F3D_CHECK(o.memory == FOLD3D_DEVICE, FOLD3D_E_INVALID,
+            "fold3d_solve_batched: instances must be device-resident");
+  F3D_CHECK(o.precision == FOLD3D_F64 || o.precision == FOLD3D_F32, FOLD3D_E_INVALID,
+            "fold3d_solve_batched: bad precision %d", o.precision);
+  F3D_CHECK(o.max_iters >= 1 && o.tol >= 0 && std::isfinite(o.tol), FOLD3D_E_INVALID,
+            "fold3d_solve_batched: bad max_iters or tol");
+  for (int i = 0; i < count; ++i) {
+    const fold3d_instance &x = inst[i];
+    F3D_CHECK(x.T != nullptr, FOLD3D_E_INVALID, "fold3d_solve_batched: instance %d has no T", i);
+    F3D_CHECK(x.n[0] >= 3 && x.n[1] >= 3 && x.n[2] >= 3, FOLD3D_E_INVALID,
+              "fold3d_solve_batched: instance %d grid too small", i);
+    F3D_CHECK(x.h[0] > 0 && x.h[1] > 0 && x.h[2] > 0, FOLD3D_E_INVALID,
+              "fold3d_solve_batched: instance %d spacings must be positive", i);
+    const int nf = (x.theta != nullptr) + (x.phi != nullptr) + (x.ra != nullptr) + (x.rb != nullptr);
+    F3D_CHECK(nf == 0 || nf == 4, FOLD3D_E_INVALID,
+              "fold3d_solve_batched: instance %d: give all of theta, phi, ra, rb or none", i);
+    F3D_CHECK(x.speed_field != nullptr || x.speed > 0, FOLD3D_E_INVALID,
+              "fold3d_solve_batched: instance %d speed must be positive", i);
+    if (nf == 0) {
+      const double *M = x.M;
+      const double m2 = M[0] * M[1] - M[3] * M[3];
+      const double m3 = M[0] * (M[1] * M[2] - M[5] * M[5]) - M[3] * (M[3] * M[2] - M[5] * M[4]) +
+                        M[4] * (M[3] * M[5] - M[1] * M[4]);
+      F3D_CHECK(M[0] > 0 && m2 > 0 && m3 > 0, FOLD3D_E_INVALID,
+                "fold3d_solve_batched: instance %d metric is not positive definite", i);
+    }
+  }
+  F3D_HIP(hipSetDevice(h->device));
+  int cus = 0;
+  F3D_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, h->device));
+  const int blocks = cus * 16;
+  const int use_graph = o.engine != FOLD3D_ENGINE_PLANE;
+  hipStream_t st = (hipStream_t)o.stream;
+  return o.precision == FOLD3D_F64
+             ? batch_solve<double>(st, count, inst, o.max_iters, o.tol, blocks, use_graph,
+                                   iterations_out, max_dt_out, ms_out)
+             : batch_solve<float>(st, count, inst, o.max_iters, o.tol, blocks, use_graph,
+                                  iterations_out, max_dt_out, ms_out);
+}
+
+int fold3d_set_tuning(fold3d_handle *h, int model_kind, int groups) {
   F3D_CHECK(h != nullptr, FOLD3D_E_INVALID, "fold3d_set_tuning: NULL handle");
-  F3D_CHECK(split_groups == 0 || split_groups == 2 || split_groups == 4 || split_groups == 8,
-            FOLD3D_E_INVALID, "fold3d_set_tuning: split_groups must be 0, 2, 4 or 8");
-  h->split_groups = split_groups == 0 ? 4 : split_groups;
+  if (model_kind == FOLD3D_MODEL_FOLD) {
+    F3D_CHECK(groups == 0 || groups == 2 || groups == 4 || groups == 8, FOLD3D_E_INVALID,
+              "fold3d_set_tuning: fold-model groups must be 0, 2, 4 or 8");
+    h->split_groups = groups == 0 ? 4 : groups;
+  } else {
+    F3D_CHECK(model_kind == FOLD3D_MODEL_ELLIP, FOLD3D_E_INVALID,
+              "fold3d_set_tuning: unknown model kind %d", model_kind);
+    F3D_CHECK(groups == 0 || groups == 1 || groups == 2 || groups == 4, FOLD3D_E_INVALID,
+              "fold3d_set_tuning: elliptic-model groups must be 0, 1, 2 or 4");
+    h->ellip_groups = groups == 0 ? 2 : groups;
+  }
   return FOLD3D_OK;
 }
 
-int fold3d_get_tuning(fold3d_handle *h, int *split_groups) {
-  F3D_CHECK(h && split_groups, FOLD3D_E_INVALID, "fold3d_get_tuning: NULL argument");
-  *split_groups = h->split_groups;
+int fold3d_get_tuning(fold3d_handle *h, int model_kind, int *groups) {
+  F3D_CHECK(h && groups, FOLD3D_E_INVALID, "fold3d_get_tuning: NULL argument");
+  *groups = model_kind == FOLD3D_MODEL_FOLD ? h->split_groups : h->ellip_groups;
   return FOLD3D_OK;
 }
 
@@ -272,13 +347,17 @@ int fold3d_autotune(fold3d_handle *h, const void *T, int nx, int ny, int nz, dou
   o.max_iters = iterations;
   o.tol = 0.0;
   o.engine = FOLD3D_ENGINE_SPLIT;
-  static const int kCandidates[3] = {2, 4, 8};
-  int best = h->split_groups;
+  const bool fold = model->kind == FOLD3D_MODEL_FOLD;
+  static const int kFoldCandidates[3] = {2, 4, 8};
+  static const int kEllipCandidates[3] = {1, 2, 4};
+  const int *cand = fold ? kFoldCandidates : kEllipCandidates;
+  int &knob = fold ? h->split_groups : h->ellip_groups;
+  int best = knob;
   float best_ms = -1.0f;
   int rc = FOLD3D_OK;
   for (int r = 0; r < 2 && rc == FOLD3D_OK; ++r) {  // round 0 warms up
     for (int c = 0; c < 3 && rc == FOLD3D_OK; ++c) {
-      h->split_groups = kCandidates[c];
+      knob = cand[c];
       if (hipMemcpyAsync(work, T, bytes, hipMemcpyDeviceToDevice, st) != hipSuccess) {
         rc = FOLD3D_E_HIP;
         break;
@@ -287,11 +366,11 @@ int fold3d_autotune(fold3d_handle *h, const void *T, int nx, int ny, int nz, dou
       rc = fold3d_solve(h, work, nx, ny, nz, dx, dy, dz, model, &o, &s);
       if (rc == FOLD3D_OK && r == 1 && (best_ms < 0 || s.sweep_ms < best_ms)) {
         best_ms = (float)s.sweep_ms;
-        best = kCandidates[c];
+        best = cand[c];
       }
     }
   }
-  h->split_groups = best;
+  knob = best;
   (void)hipStreamSynchronize(st);
   (void)hipFree(work);
   if (ms_out) *ms_out = best_ms;

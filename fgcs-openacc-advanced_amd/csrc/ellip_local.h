@@ -160,6 +160,100 @@ __host__ __device__ inline void ellip_constants(const R N[6], const int u[3],
   }
 }
 
+// ---------------------------------------------------------------------------
+// Per-node metric.  Every patch vector is a lattice vector w = (a dxx, b dyy,
+// c dzz) with small integers a, b, c, so w^T N w and v^T N w are integer
+// combinations of six products of the local inverse metric with the spacings.
+// A unit's constants are then a handful of FMAs and one division, and they are
+// computed only for units that survive the pruning.
+template <typename R>
+struct LatticeN {
+  R A, B, C, E, Fx, Fy;  // n11 dxx^2, n22 dyy^2, n33 dzz^2, n12 dxx dyy, n13 dxx dzz, n23 dyy dzz
+};
+
+template <typename R>
+__device__ __forceinline__ R lat_dot(const LatticeN<R> &L, int a1, int b1, int c1, int a2,
+                                     int b2, int c2) {
+  return R(a1 * a2) * L.A + R(b1 * b2) * L.B + R(c1 * c2) * L.C + R(a1 * b2 + a2 * b1) * L.E +
+         R(a1 * c2 + a2 * c1) * L.Fx + R(b1 * c2 + b2 * c1) * L.Fy;
+}
+
+template <typename R>
+__device__ __forceinline__ LatticeN<R> lattice_of(const R N[6], const int u[3], const R sg[3],
+                                                  const R hl[3]) {
+  const R Ng[3][3] = {{N[0], N[3], N[4]}, {N[3], N[1], N[5]}, {N[4], N[5], N[2]}};
+  auto nl = [&](int i, int j) { return sg[i] * sg[j] * Ng[u[i]][u[j]]; };
+  LatticeN<R> L;
+  L.A = nl(0, 0) * hl[0] * hl[0];
+  L.B = nl(1, 1) * hl[1] * hl[1];
+  L.C = nl(2, 2) * hl[2] * hl[2];
+  L.E = nl(0, 1) * hl[0] * hl[1];
+  L.Fx = nl(0, 2) * hl[0] * hl[2];
+  L.Fy = nl(1, 2) * hl[1] * hl[2];
+  return L;
+}
+
+// Unit U of the node (triangle U, spoke U, rim U) with constants from L.
+template <typename R, int U>
+__device__ __forceinline__ void ellip_unit_lattice(R &best, const R p[9], R s2,
+                                                   const LatticeN<R> &L) {
+  constexpr int k1 = 1 + U, k2 = 1 + (U + 1) % 8;
+  constexpr int a1 = kRing[U][0], b1 = kRing[U][1];
+  constexpr int a2 = kRing[(U + 1) % 8][0], b2 = kRing[(U + 1) % 8][1];
+  const R t0 = p[0], t1 = p[k1], t2 = p[k2];
+  if (!(rmin(t0, rmin(t1, t2)) < best)) return;  // nothing of this unit can win
+  {  // spoke centre -> ring U: E = (a1, b1, 0), D = (0, 0, 1)
+    const R g = lat_dot(L, a1, b1, 0, a1, b1, 0);
+    const R b = lat_dot(L, a1, b1, 0, 0, 0, 1);
+    const R P = R(1) / g, l0 = P * b;
+    const R K[3] = {P, l0, L.C - b * l0};
+    ellip_edge(best, t0, t1, s2, K);
+  }
+  {  // rim ring U -> ring U+1: E = (a2-a1, b2-b1, 0), D = (-a1, -b1, 1)
+    constexpr int ea = a2 - a1, eb = b2 - b1;
+    const R g = lat_dot(L, ea, eb, 0, ea, eb, 0);
+    const R b = lat_dot(L, ea, eb, 0, -a1, -b1, 1);
+    const R c = lat_dot(L, -a1, -b1, 1, -a1, -b1, 1);
+    const R P = R(1) / g, l0 = P * b;
+    const R K[3] = {P, l0, c - b * l0};
+    ellip_edge(best, t1, t2, s2, K);
+  }
+  {  // triangle centre, ring U, ring U+1
+    const R g11 = lat_dot(L, a1, b1, 0, a1, b1, 0), g22 = lat_dot(L, a2, b2, 0, a2, b2, 0);
+    const R g12 = lat_dot(L, a1, b1, 0, a2, b2, 0);
+    const R q1 = lat_dot(L, a1, b1, 0, 0, 0, 1), q2 = lat_dot(L, a2, b2, 0, 0, 0, 1);
+    const R rdet = R(1) / (g11 * g22 - g12 * g12);
+    const R P11 = g22 * rdet, P12 = -g12 * rdet, P22 = g11 * rdet;
+    const R l1 = P11 * q1 + P12 * q2, l2 = P12 * q1 + P22 * q2;
+    const R K[6] = {P11, P12, P22, l1, l2, L.C - (q1 * l1 + q2 * l2)};
+    ellip_tri(best, t0, t1, t2, s2, K);
+  }
+}
+
+// Whole node with a per-node metric (all eight units; the nine one-point
+// candidates first so the pruning starts low).
+template <typename R>
+__device__ __forceinline__ R ellip_node_lattice(R told, const R p[9], R s, const LatticeN<R> &L) {
+  R best = told;
+#pragma unroll
+  for (int k = 0; k < 9; ++k) {
+    if (reached(p[k])) {
+      const int a = k == 0 ? 0 : kRing[k - 1][0], b = k == 0 ? 0 : kRing[k - 1][1];
+      best = rmin(p[k] + s * sqrt(lat_dot(L, -a, -b, 1, -a, -b, 1)), best);
+    }
+  }
+  const R s2 = s * s;
+  ellip_unit_lattice<R, 0>(best, p, s2, L);
+  ellip_unit_lattice<R, 1>(best, p, s2, L);
+  ellip_unit_lattice<R, 2>(best, p, s2, L);
+  ellip_unit_lattice<R, 3>(best, p, s2, L);
+  ellip_unit_lattice<R, 4>(best, p, s2, L);
+  ellip_unit_lattice<R, 5>(best, p, s2, L);
+  ellip_unit_lattice<R, 6>(best, p, s2, L);
+  ellip_unit_lattice<R, 7>(best, p, s2, L);
+  return best;
+}
+
 // M = R diag(1, ra, rb)^2 R^T with R = Rz(phi) Ry(theta); returns N = M^-1 =
 // R diag(1, 1/ra^2, 1/rb^2) R^T directly (no 3x3 inverse needed).
 template <typename R>

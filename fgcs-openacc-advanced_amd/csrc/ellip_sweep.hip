@@ -47,9 +47,8 @@ __global__ __launch_bounds__(kLaneE *kRowsE) void ellip_plane_kernel(
     if (kMetric) {
       R N[6];
       tilted_inverse_metric<R>(A.theta[t], A.phi[t], A.ra[t], A.rb[t], N);
-      EllipConsts<R> K;
-      ellip_constants<R>(N, A.u, A.sg, A.hl, K);
-      tn = ellip_node_update<R>(told, p, s, K);
+      const LatticeN<R> L = lattice_of<R>(N, A.u, A.sg, A.hl);
+      tn = ellip_node_lattice<R>(told, p, s, L);
     } else {
       tn = ellip_node_update<R>(told, p, s, A.K);
     }
@@ -181,5 +180,188 @@ template int ellip_sweeps_plane_engine<float>(hipStream_t, float *, const Grid &
                                               const fold3d_model &, const double *,
                                               int, double, unsigned long long *,
                                               int *, double *);
+
+// ---------------------------------------------------------------------------
+// Split engine for a constant metric.  As for the fold model: a block owns a
+// row of 64 target nodes, stages the three source rows in LDS, and kGroups
+// waves share the node's eight units (triangle k with spoke k and rim k); each
+// wave also takes the nine one-point candidates so its pruning starts low.
+// The partial minima meet in LDS.  Same bits as ellip_plane_kernel.
+// ---------------------------------------------------------------------------
+template <typename R, int U>
+__device__ __forceinline__ void ellip_unit(R &best, const R p[9], R s2, const EllipConsts<R> &K) {
+  constexpr int k1 = 1 + U, k2 = 1 + (U + 1) % 8;
+  ellip_edge(best, p[0], p[k1], s2, K.edge[U]);
+  ellip_edge(best, p[k1], p[k2], s2, K.edge[8 + U]);
+  ellip_tri(best, p[0], p[k1], p[k2], s2, K.tri[U]);
+}
+
+template <typename R, int G0, int kStride>
+__device__ __forceinline__ void ellip_units(R &best, const R p[9], R s2, const EllipConsts<R> &K) {
+  if constexpr (G0 < 8) {
+    ellip_unit<R, G0>(best, p, s2, K);
+    ellip_units<R, G0 + kStride, kStride>(best, p, s2, K);
+  }
+}
+
+template <typename R, int kGroups>
+__device__ __forceinline__ void ellip_dispatch_units(int grp, R &best, const R p[9], R s2,
+                                                     const EllipConsts<R> &K) {
+  if constexpr (kGroups == 1) {
+    ellip_units<R, 0, 1>(best, p, s2, K);
+  } else if constexpr (kGroups == 2) {
+    if (grp == 0) ellip_units<R, 0, 2>(best, p, s2, K);
+    else ellip_units<R, 1, 2>(best, p, s2, K);
+  } else {
+    static_assert(kGroups == 4, "one, two or four waves per node row");
+    switch (grp) {
+      case 0: ellip_units<R, 0, 4>(best, p, s2, K); break;
+      case 1: ellip_units<R, 1, 4>(best, p, s2, K); break;
+      case 2: ellip_units<R, 2, 4>(best, p, s2, K); break;
+      default: ellip_units<R, 3, 4>(best, p, s2, K); break;
+    }
+  }
+}
+
+template <typename R, bool kReduce, bool kSpeed, int kGroups>
+__global__ __launch_bounds__(64 * kGroups) void ellip_split_kernel(
+    R *__restrict__ T, const R *__restrict__ V, PlaneGeom g, EllipArgs<R> A,
+    unsigned long long *__restrict__ max_dt) {
+  __shared__ R s[3][64 + 2];
+  __shared__ R part[kGroups][64];
+  const int tx = threadIdx.x;
+  const int grp = __builtin_amdgcn_readfirstlane(threadIdx.y);
+  const int a0 = blockIdx.x * 64, b = 1 + blockIdx.y;
+  for (int idx = grp * 64 + tx; idx < 3 * 66; idx += 64 * kGroups) {
+    const int la = idx % 66, lb = idx / 66;
+    const int sa = a0 + la;
+    if (sa <= g.na + 1) s[lb][la] = T[g.base + (long long)sa * g.Sa + (long long)(b - 1 + lb) * g.Sb];
+  }
+  const int a = a0 + 1 + tx;
+  const bool active = a <= g.na;
+  const long long c = g.base + (long long)a * g.Sa + (long long)b * g.Sb;
+  const long long t = c + g.sm;
+  const R told = active ? T[t] : R(0);
+  const R sl = kSpeed ? (active ? R(1) / V[t] : R(1)) : R(1) / A.speed;
+  __syncthreads();
+  R best = told;
+  if (active) {
+    const int ca = tx + 1;
+    const int ua = g.ua, ub = g.ub, va = g.va, vb = g.vb;
+    R p[9];
+    p[0] = s[1][ca];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      const int i = kRing[k][0], j = kRing[k][1];
+      p[1 + k] = s[1 + i * ub + j * vb][ca + i * ua + j * va];
+    }
+#pragma unroll
+    for (int k = 0; k < 9; ++k)
+      if (reached(p[k])) best = rmin(p[k] + sl * A.K.vert[k], best);
+    ellip_dispatch_units<R, kGroups>(grp, best, p, sl * sl, A.K);
+  }
+  part[grp][tx] = best;
+  __syncthreads();
+  if (grp == 0) {
+    R d = R(0);
+    if (active) {
+      R m = part[0][tx];
+#pragma unroll
+      for (int k = 1; k < kGroups; ++k) m = rmin(part[k][tx], m);
+      if (m != told) T[t] = m;
+      if (kReduce) d = told - m;
+    }
+    if (kReduce) {
+      double v = (double)d;
+#pragma unroll
+      for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, 64));
+      if (tx == 0 && v > 0.0) atomicMax(max_dt, (unsigned long long)__double_as_longlong(v));
+    }
+  }
+}
+
+template <typename R, bool kSpeed, int kGroups>
+static void ellip_split_launch(hipStream_t st, R *T, const R *V, const PlaneGeom &g,
+                               const EllipArgs<R> &A, unsigned long long *max_dt) {
+  const dim3 block(64, kGroups);
+  const dim3 grid((g.na + 63) / 64, g.nb);
+  if (max_dt)
+    ellip_split_kernel<R, true, kSpeed, kGroups><<<grid, block, 0, st>>>(T, V, g, A, max_dt);
+  else
+    ellip_split_kernel<R, false, kSpeed, kGroups><<<grid, block, 0, st>>>(T, V, g, A, nullptr);
+}
+
+template <typename R>
+static void ellip_split_direction(hipStream_t st, R *T, const R *V, const Grid &G, int d,
+                                  const EllipArgs<R> &A, unsigned long long *max_dt,
+                                  int groups) {
+  const Frame &fr = kFrames[d];
+  const int n = G.n[fr.m];
+  for (int q = 0; q < n - 1; ++q) {
+    const int p = fr.s > 0 ? 1 + q : n - q;
+    const PlaneGeom g = plane_geom(G, d, p);
+    if (V) {
+      if (groups == 1) ellip_split_launch<R, true, 1>(st, T, V, g, A, max_dt);
+      else if (groups == 2) ellip_split_launch<R, true, 2>(st, T, V, g, A, max_dt);
+      else ellip_split_launch<R, true, 4>(st, T, V, g, A, max_dt);
+    } else {
+      if (groups == 1) ellip_split_launch<R, false, 1>(st, T, V, g, A, max_dt);
+      else if (groups == 2) ellip_split_launch<R, false, 2>(st, T, V, g, A, max_dt);
+      else ellip_split_launch<R, false, 4>(st, T, V, g, A, max_dt);
+    }
+  }
+}
+
+template <typename R>
+int ellip_sweeps_split_engine(hipStream_t st, R *T, R *Y, R *VX, R *VY, const Grid &G,
+                              const fold3d_model &m, const double h[3], int max_iters,
+                              double tol, int groups, unsigned long long *d_max,
+                              int *iters_out, double *max_dt_out) {
+  EllipArgs<R> args[6];
+  for (int d = 0; d < 6; ++d) args[d] = ellip_args<R>(d, m, h);
+  const Grid GX = G.with_order(kOrderX), GZ = G.with_order(kOrderZ), GY = G.with_order(kOrderY);
+  const R *V = (const R *)m.speed_field;
+  if (V) {  // the speed field in the x- and y-sweep layouts, once per solve
+    permute_layout<R>(st, V, GZ, VX, GX);
+    permute_layout<R>(st, V, GZ, VY, GY);
+  }
+  int it = 0;
+  double last = -1.0;
+  for (; it < max_iters;) {
+    unsigned long long *md = tol > 0 ? d_max : nullptr;
+    if (md) ellip_zero_u64<<<1, 1, 0, st>>>(d_max);
+    permute_layout<R>(st, T, GZ, Y, GX);
+    ellip_split_direction<R>(st, Y, V ? VX : nullptr, GX, FOLD3D_X_DOWN, args[FOLD3D_X_DOWN], md, groups);
+    ellip_split_direction<R>(st, Y, V ? VX : nullptr, GX, FOLD3D_X_UP, args[FOLD3D_X_UP], md, groups);
+    permute_layout<R>(st, Y, GX, T, GZ);
+    ellip_split_direction<R>(st, T, V, GZ, FOLD3D_K_UP, args[FOLD3D_K_UP], md, groups);
+    ellip_split_direction<R>(st, T, V, GZ, FOLD3D_K_DOWN, args[FOLD3D_K_DOWN], md, groups);
+    permute_layout<R>(st, T, GZ, Y, GY);
+    ellip_split_direction<R>(st, Y, V ? VY : nullptr, GY, FOLD3D_Y_UP, args[FOLD3D_Y_UP], md, groups);
+    ellip_split_direction<R>(st, Y, V ? VY : nullptr, GY, FOLD3D_Y_DOWN, args[FOLD3D_Y_DOWN], md, groups);
+    permute_layout<R>(st, Y, GY, T, GZ);
+    F3D_HIP(hipGetLastError());
+    ++it;
+    if (md) {
+      unsigned long long bits = 0;
+      F3D_HIP(hipMemcpyAsync(&bits, d_max, sizeof bits, hipMemcpyDeviceToHost, st));
+      F3D_HIP(hipStreamSynchronize(st));
+      memcpy(&last, &bits, sizeof last);
+      if (last < tol) break;
+    }
+  }
+  *iters_out = it;
+  *max_dt_out = last;
+  return FOLD3D_OK;
+}
+
+template int ellip_sweeps_split_engine<double>(hipStream_t, double *, double *, double *,
+                                               double *, const Grid &, const fold3d_model &,
+                                               const double *, int, double, int,
+                                               unsigned long long *, int *, double *);
+template int ellip_sweeps_split_engine<float>(hipStream_t, float *, float *, float *, float *,
+                                              const Grid &, const fold3d_model &,
+                                              const double *, int, double, int,
+                                              unsigned long long *, int *, double *);
 
 }  // namespace fold3d

@@ -33,6 +33,15 @@ int ellip_sweeps_plane_engine(hipStream_t st, R *T, const Grid &G,
                               unsigned long long *d_max, int *iters_out,
                               double *max_dt_out);
 
+// Constant metric: layouts + LDS rows + units spread over `groups` (1, 2, 4)
+// waves.  Y = a T-sized scratch field; VX, VY = the speed field in the x- and
+// y-sweep layouts (unused without a speed field).
+template <typename R>
+int ellip_sweeps_split_engine(hipStream_t st, R *T, R *Y, R *VX, R *VY, const Grid &G,
+                              const fold3d_model &m, const double h[3], int max_iters,
+                              double tol, int groups, unsigned long long *d_max,
+                              int *iters_out, double *max_dt_out);
+
 void sym3_inverse(const double M[6], double N[6]);
 
 template <typename R>
@@ -51,6 +60,12 @@ template <typename R>
 int fold_sweeps_tiled_engine(hipStream_t st, R *T, R *Y, const Grid &G,
                              const FoldFast<R> *d_consts, int max_iters, double tol, int groups,
                              unsigned long long *d_max, int *iters_out, double *max_dt_out);
+
+// Batched elliptic solves (batch.hip).  `blocks` = persistent grid per launch.
+template <typename R>
+int batch_solve(hipStream_t st, int count, const fold3d_instance *inst, int max_iters,
+                double tol, int blocks, int use_graph, int *iters_out, double *max_dt_out,
+                double *ms_out);
 
 // Copy a padded field from one axis order to another (layout.hip).
 template <typename R>

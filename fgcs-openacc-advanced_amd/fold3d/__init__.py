@@ -77,6 +77,21 @@ class Stats(C.Structure):
     ]
 
 
+class Instance(C.Structure):
+    _fields_ = [
+        ("T", C.c_void_p),
+        ("n", C.c_int * 3),
+        ("h", C.c_double * 3),
+        ("M", C.c_double * 6),
+        ("speed", C.c_double),
+        ("speed_field", C.c_void_p),
+        ("theta", C.c_void_p),
+        ("phi", C.c_void_p),
+        ("ra", C.c_void_p),
+        ("rb", C.c_void_p),
+    ]
+
+
 _lib = None
 
 
@@ -100,8 +115,11 @@ def lib() -> C.CDLL:
     L.fold3d_solve.argtypes = [vp, vp, i, i, i, d, d, d, C.POINTER(Model),
                                C.POINTER(Opts), C.POINTER(Stats)]
     L.fold3d_plane_step.argtypes = [vp, i, vp, i, i, i, i] + [d] * 10 + [vp]
-    L.fold3d_set_tuning.argtypes = [vp, i]
-    L.fold3d_get_tuning.argtypes = [vp, C.POINTER(i)]
+    L.fold3d_measure_peaks.argtypes = [vp, dp, dp, dp]
+    L.fold3d_solve_batched.argtypes = [vp, i, C.POINTER(Instance), C.POINTER(Opts),
+                                       C.POINTER(i), dp, dp]
+    L.fold3d_set_tuning.argtypes = [vp, i, i]
+    L.fold3d_get_tuning.argtypes = [vp, i, C.POINTER(i)]
     L.fold3d_autotune.argtypes = [vp, vp, i, i, i, d, d, d, C.POINTER(Model),
                                   C.POINTER(Opts), i, C.POINTER(C.c_float)]
     L.fold3d_synchronize.argtypes = [vp]
@@ -146,6 +164,14 @@ def fold_model(F: float = REF_F, a=REF_A) -> Model:
     for k in range(3):
         m.a[k] = a[k]
     return m
+
+
+def measure_peaks(stream=None) -> dict:
+    """Sustained FP64/FP32 FMA rate and device copy bandwidth of this GPU."""
+    f64, f32, cp = C.c_double(), C.c_double(), C.c_double()
+    check(lib().fold3d_measure_peaks(stream, C.byref(f64), C.byref(f32), C.byref(cp)))
+    return {"fp64_tflops": f64.value / 1e12, "fp32_tflops": f32.value / 1e12,
+            "copy_gbs": cp.value / 1e9}
 
 
 def implicit_initialiser(n: int) -> np.ndarray:
@@ -207,12 +233,48 @@ class Solver:
         check(rc)
         return st
 
-    def set_tuning(self, split_groups: int):
-        check(lib().fold3d_set_tuning(self.h, split_groups))
+    def solve_batched(self, workloads, Ts, max_iters=200, tol=1e-6, use_graph=True,
+                      stream=None, raise_not_converged=True):
+        """Solve many elliptic-model instances (fold3d.workloads.Workload with
+        torch fields on the device, and their torch T tensors) in one batch.
+        Returns (iterations per instance, max|dT| per instance, device ms)."""
+        import torch
+        count = len(workloads)
+        arr = (Instance * count)()
+        for k, (W, T) in enumerate(zip(workloads, Ts)):
+            x = arr[k]
+            x.T = T.data_ptr()
+            nn = (W.n,) * 3 if np.isscalar(W.n) else W.n
+            hh = (W.h,) * 3 if np.isscalar(W.h) else W.h
+            for a in range(3):
+                x.n[a] = int(nn[a])
+                x.h[a] = float(hh[a])
+            for a in range(6):
+                x.M[a] = float(W.M[a])
+            x.speed = float(W.speed)
+            for name in ("speed_field", "theta", "phi", "ra", "rb"):
+                f = getattr(W, name)
+                setattr(x, name, None if f is None else f.data_ptr())
+        o = Opts()
+        lib().fold3d_default_opts(C.byref(o))
+        o.precision = F64 if Ts[0].dtype == torch.float64 else F32
+        o.max_iters, o.tol = max_iters, tol
+        o.engine = ENGINE_AUTO if use_graph else ENGINE_PLANE
+        o.stream = stream if stream is not None else torch.cuda.current_stream().cuda_stream
+        its = (C.c_int * count)()
+        mds = (C.c_double * count)()
+        ms = C.c_double()
+        rc = lib().fold3d_solve_batched(self.h, count, arr, C.byref(o), its, mds, C.byref(ms))
+        if not (rc == FOLD3D_E_NOT_CONVERGED and not raise_not_converged):
+            check(rc)
+        return list(its), list(mds), ms.value
 
-    def tuning(self) -> int:
+    def set_tuning(self, groups: int, kind: int = MODEL_FOLD):
+        check(lib().fold3d_set_tuning(self.h, kind, groups))
+
+    def tuning(self, kind: int = MODEL_FOLD) -> int:
         g = C.c_int()
-        check(lib().fold3d_get_tuning(self.h, C.byref(g)))
+        check(lib().fold3d_get_tuning(self.h, kind, C.byref(g)))
         return g.value
 
     def autotune(self, T, n, spacing=None, model: Model | None = None,
@@ -232,7 +294,7 @@ class Solver:
         ms = C.c_float()
         check(lib().fold3d_autotune(self.h, C.c_void_p(T.data_ptr()), nx, ny, nz, dx, dy, dz,
                                     C.byref(m), C.byref(o), iterations, C.byref(ms)))
-        return self.tuning(), ms.value
+        return self.tuning(m.kind), ms.value
 
     def plane_step(self, direction: int, T, n, plane: int, dzz, dxx, dyy, dxy,
                    dxz, dyz, F, ay, ax, az, stream=None):

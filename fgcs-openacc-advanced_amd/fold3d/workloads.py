@@ -157,6 +157,40 @@ def tilted_field(n=1280, seed=20261204, xp=np, dtype=np.float32, device=None) ->
     return W
 
 
+def batch_instance(n, seed, xp=np, dtype=np.float32, device=None) -> Workload:
+    """One instance of the batched workload: blob speed field (as blob_speed),
+    tilt and axis-ratio fields (as tilted_field) and a seeded interior source."""
+    rng = np.random.default_rng(seed)
+    h = 1.0 / (n - 1)
+    amps = rng.uniform(-0.3, 0.3, 8)
+    centres = rng.uniform(0.0, 1.0, (8, 3))
+    v = 2.0 + _gauss_sum(xp, n, h, centres, [0.1] * 8, amps, dtype, device)
+    theta = _rescaled_modes(xp, n, h, rng, 0.0, math.radians(60), dtype, device)
+    phi = _rescaled_modes(xp, n, h, rng, 0.0, 2 * math.pi * (1 - 1e-6), dtype, device)
+    ra = _rescaled_modes(xp, n, h, rng, 0.6, 1.0, dtype, device)
+    rb = _rescaled_modes(xp, n, h, rng, 0.6, 1.0, dtype, device)
+    src = tuple(float(c) for c in rng.uniform(0.2, 0.8, 3))
+    vs = 2.0 + sum(a * math.exp(-sum((s - c) ** 2 for s, c in zip(src, cc)) / (2 * 0.01))
+                   for a, cc in zip(amps, centres))
+    W = Workload("batch_instance", n, h, CONFIG_METRIC, 2.0, src, vs,
+                 speed_field=_cast(xp, v, dtype).reshape(-1))
+    W.theta = _cast(xp, theta, dtype).reshape(-1)
+    W.phi = _cast(xp, phi, dtype).reshape(-1)
+    W.ra = _cast(xp, ra, dtype).reshape(-1)
+    W.rb = _cast(xp, rb, dtype).reshape(-1)
+    ijk = [int(round(c / h)) + 1 for c in src]
+    flat = ijk[0] + (n + 2) * (ijk[1] + (n + 2) * ijk[2])
+    tv = [float(f[flat]) for f in (W.theta, W.phi, W.ra, W.rb)]
+    W.meta = {"seed": seed, "metric_at_source": tilted_metric(tv[0], tv[1], 1.0, tv[2], tv[3]).tolist()}
+    return W
+
+
+def batch_sides(count=96, seed=20261205, choices=tuple(range(64, 321, 32))):
+    """Instance sides drawn uniformly from {64, 96, ..., 320}."""
+    rng = np.random.default_rng(seed)
+    return [int(x) for x in rng.choice(choices, count)]
+
+
 def point_source_start(W: Workload, xp=np, dtype=np.float64, device=None, radius=1.6):
     """T = MAXT except the nodes within `radius` cells (max-norm) of the source,
     which get the closed form sqrt(d^T M^-1 d)/v(source).  Ghosts stay MAXT."""

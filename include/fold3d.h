@@ -131,12 +131,39 @@ int fold3d_solve(fold3d_handle *h, void *T, int nx, int ny, int nz,
                  double dx, double dy, double dz, const fold3d_model *model,
                  const fold3d_opts *opts, fold3d_stats *stats);
 
-/* Tuning of the split engine: how many wavefronts share one row of 64 nodes
- * (2, 4 or 8; 0 restores the default 4).  fold3d_autotune() times every
- * choice on `iterations` iterations of a scratch copy of T (T itself is not
- * modified), keeps the fastest on the handle and returns its sweep time. */
-int fold3d_set_tuning(fold3d_handle *h, int split_groups);
-int fold3d_get_tuning(fold3d_handle *h, int *split_groups);
+/* Batched elliptic-model solves: many independent instances, each with its
+ * own grid, spacings, metric and speed (constant or per node), in one stream
+ * of launches (one launch per sweep direction and plane step serves every
+ * instance that has that plane; persistent blocks pull rows of 64 nodes from a
+ * device queue).  The launch sequence of one iteration is captured in a
+ * hipGraph and replayed (opts->engine == FOLD3D_ENGINE_PLANE launches it
+ * directly instead).  All arrays are device arrays in opts->precision, on the
+ * padded layout of their own grid.  An instance stops being updated once an
+ * iteration changes none of its nodes by opts->tol or more.  Outputs (each
+ * may be NULL, arrays of length `count`): iterations run and last max|dT| per
+ * instance, and the device time of the whole batch. */
+typedef struct fold3d_instance {
+  void *T;
+  int n[3];
+  double h[3];
+  double M[6];                        /* used when theta..rb are NULL */
+  double speed;                       /* used when speed_field is NULL */
+  const void *speed_field;
+  const void *theta, *phi, *ra, *rb;  /* all four or none */
+} fold3d_instance;
+
+int fold3d_solve_batched(fold3d_handle *h, int count, const fold3d_instance *inst,
+                         const fold3d_opts *opts, int *iterations_out,
+                         double *max_dt_out, double *ms_out);
+
+/* Tuning of the split engines, per model kind: how many wavefronts share one
+ * row of 64 nodes (fold model: 2, 4 or 8, default 4; elliptic model with a
+ * constant metric: 1, 2 or 4, default 2; 0 restores the default).
+ * fold3d_autotune() times every choice on `iterations` iterations of a
+ * scratch copy of T (T itself is not modified), keeps the fastest on the
+ * handle for that model kind and returns its sweep time. */
+int fold3d_set_tuning(fold3d_handle *h, int model_kind, int groups);
+int fold3d_get_tuning(fold3d_handle *h, int model_kind, int *groups);
 int fold3d_autotune(fold3d_handle *h, const void *T, int nx, int ny, int nz,
                     double dx, double dy, double dz, const fold3d_model *model,
                     const fold3d_opts *opts, int iterations, float *ms_out);
@@ -149,6 +176,13 @@ int fold3d_plane_step(fold3d_handle *h, int direction, double *T, int nx,
                       int ny, int nz, int plane, double dzz, double dxx,
                       double dyy, double dxy, double dxz, double dyz, double F,
                       double ay, double ax, double az, void *stream);
+
+/* Measured ceilings of the current device, for reporting next to the
+ * published peaks: sustained FP64 and FP32 FMA rate (FLOP/s) and the
+ * bandwidth of a plain device-to-device copy (bytes read + written per s).
+ * Any output pointer may be NULL.  Allocates 4 GiB briefly for the copy. */
+int fold3d_measure_peaks(void *stream, double *fp64_flops, double *fp32_flops,
+                         double *copy_bytes_per_s);
 
 /* Device memory helpers for C callers that have no HIP headers
  * (stand in for acc_malloc / acc_memcpy_* at AN_ISO_1_OpenACC.c:1185-1186,1241). */

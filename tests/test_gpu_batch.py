@@ -1,0 +1,132 @@
+"""Batched elliptic solves (many instances of uneven size in one launch
+sequence) against the elliptic oracle, instance by instance.
+
+The batched kernel builds the simplex constants per node from the lattice
+form of the metric, so FP64 agrees with the oracle to rounding (held to
+1e-12 relative), and the replayed hipGraph gives the same bits as direct
+launches.  FP32 is held to 1e-5 relative of the FP64 oracle (nodes at least
+8 cells from the source) and 1e-5 * max(T) absolute.
+"""
+import numpy as np
+import pytest
+
+from conftest import bits_equal
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+import fold3d  # noqa: E402
+from fold3d import workloads as wl  # noqa: E402
+
+
+def dev(a, dtype):
+    return torch.from_numpy(np.ascontiguousarray(a)).to("cuda", dtype)
+
+
+def make_batch(sides, dtype_np=np.float64):
+    Ws = []
+    for k, n in enumerate(sides):
+        if k % 3 == 0:
+            W = wl.tilted_homogeneous(n)
+            W.source = (0.37, 0.52, 0.61)
+        elif k % 3 == 1:
+            W = wl.blob_speed(n, seed=100 + k)
+        else:
+            W = wl.batch_instance(n, seed=200 + k, dtype=dtype_np)
+        Ws.append(W)
+    return Ws
+
+
+def to_dev_workload(W, dtype):
+    Wd = wl.Workload(W.name, W.n, W.h, W.M, W.speed, W.source, W.v_source, meta=W.meta)
+    for k in ("speed_field", "theta", "phi", "ra", "rb"):
+        f = getattr(W, k)
+        setattr(Wd, k, None if f is None else dev(f, dtype))
+    return Wd
+
+
+def oracle_solve(oracle, W, T0, iters):
+    T = T0.copy()
+    f = {k: (None if getattr(W, k) is None else np.asarray(getattr(W, k), np.float64))
+         for k in ("speed_field", "theta", "phi", "ra", "rb")}
+    oracle.ellip_sweeps(T, W.n, iters, 0.0, W.h, M=W.M, speed=W.speed, **f)
+    return T
+
+
+SIDES = [17, 24, 9, 30, 12, 21, 26]
+
+
+def test_batched_fp64_matches_oracle_per_instance(solver, oracle):
+    print(f"batched FP64, sides {SIDES}", flush=True)
+    Ws = make_batch(SIDES)
+    T0s = [wl.point_source_start(W) for W in Ws]
+    Wd = [to_dev_workload(W, torch.float64) for W in Ws]
+    Ts = [dev(T0, torch.float64) for T0 in T0s]
+    its, mds, ms = solver.solve_batched(Wd, Ts, max_iters=3, tol=0.0)
+    assert its == [3] * len(SIDES)
+    for W, T0, T in zip(Ws, T0s, Ts):
+        want = oracle_solve(oracle, W, T0, 3)
+        got = T.cpu().numpy()
+        reached = want < 1e15
+        assert np.array_equal(reached, got < 1e15)
+        rel = np.abs(got[reached] - want[reached]) / want[reached]
+        assert rel.max() < 1e-12, (W.name, W.n, rel.max())
+
+
+def test_graph_and_direct_launches_agree(solver):
+    Ws = make_batch(SIDES)
+    T0s = [wl.point_source_start(W) for W in Ws]
+    Wd = [to_dev_workload(W, torch.float64) for W in Ws]
+    A = [dev(T0, torch.float64) for T0 in T0s]
+    B = [dev(T0, torch.float64) for T0 in T0s]
+    ia, _, _ = solver.solve_batched(Wd, A, max_iters=50, tol=1e-12, use_graph=True)
+    ib, _, _ = solver.solve_batched(Wd, B, max_iters=50, tol=1e-12, use_graph=False)
+    assert ia == ib
+    for a, b in zip(A, B):
+        assert torch.equal(a.view(torch.int64), b.view(torch.int64))
+
+
+def test_converged_instances_stop(solver, oracle):
+    """Per-instance convergence: iteration counts differ and each instance's
+    field is the one its own tolerance stop gives."""
+    Ws = make_batch([10, 28, 16])
+    T0s = [wl.point_source_start(W) for W in Ws]
+    Wd = [to_dev_workload(W, torch.float64) for W in Ws]
+    Ts = [dev(T0, torch.float64) for T0 in T0s]
+    its, mds, _ = solver.solve_batched(Wd, Ts, max_iters=60, tol=1e-10)
+    assert all(m < 1e-10 for m in mds)
+    for W, T0, T, k in zip(Ws, T0s, Ts, its):
+        want = oracle_solve(oracle, W, T0, k)
+        got = T.cpu().numpy()
+        r = want < 1e15
+        assert np.abs(got[r] - want[r]).max() < 1e-12 * want[r].max()
+
+
+def test_batched_fp32_within_tolerance(solver, oracle):
+    Ws = make_batch([34, 40, 28])
+    T0s = [wl.point_source_start(W) for W in Ws]
+    Wd = [to_dev_workload(W, torch.float32) for W in Ws]
+    Ts = [dev(T0, torch.float32) for T0 in T0s]
+    its, mds, _ = solver.solve_batched(Wd, Ts, max_iters=80, tol=1e-6)
+    for W, T0, T, k in zip(Ws, T0s, Ts, its):
+        want = T0.copy()
+        f = {kk: (None if getattr(W, kk) is None else np.asarray(getattr(W, kk), np.float64))
+             for kk in ("speed_field", "theta", "phi", "ra", "rb")}
+        oracle.ellip_sweeps(want, W.n, 80, 1e-10, W.h, M=W.M, speed=W.speed, **f)
+        got = T.double().cpu().numpy()
+        r = want < 1e15
+        far = r & (want > 8 * W.h / 3.6)
+        rel = np.abs(got[far] - want[far]) / want[far]
+        absd = np.abs(got[r] - want[r]).max() / want[r].max()
+        print(f"batched FP32 {W.name} n={W.n}: {k} iterations, max rel {rel.max():.2e}, abs {absd:.2e}")
+        assert rel.max() < 1e-5 and absd < 1e-5
+
+
+def test_batch_rejects_bad_instances(solver):
+    W = wl.tilted_homogeneous(12)
+    Wd = to_dev_workload(W, torch.float64)
+    T = dev(wl.point_source_start(W), torch.float64)
+    Wd.M = np.array([1.0, 1.0, -1.0, 0, 0, 0])
+    with pytest.raises(fold3d.Fold3dError) as e:
+        solver.solve_batched([Wd], [T])
+    assert e.value.code == fold3d.FOLD3D_E_INVALID

@@ -134,6 +134,43 @@ def test_closed_form_error_reported(solver):
     assert err.max() < 0.05
 
 
+@pytest.mark.parametrize("groups", [1, 2, 4])
+@pytest.mark.parametrize("with_speed", [False, True])
+def test_split_engine_bitwise_on_box_grid(solver, oracle, groups, with_speed):
+    """Layout changes + LDS rows + units over waves: same bits as the oracle,
+    on a box grid with partial tiles, with and without a speed field."""
+    n3 = (45, 70, 38)
+    h = (1.0 / 44, 1.0 / 50, 1.0 / 41)
+    W = wl.tilted_homogeneous(20)
+    rng = np.random.default_rng(groups)
+    size = (n3[0] + 2) * (n3[1] + 2) * (n3[2] + 2)
+    # a smooth start: closed form from an interior point with a few nodes set
+    k, j, i = np.indices((n3[2] + 2, n3[1] + 2, n3[0] + 2)).reshape(3, -1)
+    X = np.stack([(i - 1) * h[0], (j - 1) * h[1], (k - 1) * h[2]]) - np.array([[0.4], [0.7], [0.3]])
+    N = np.linalg.inv(wl.sym(W.M))
+    r = np.sqrt(np.einsum("in,ij,jn->n", X, N, X))
+    T0 = np.where(r < 0.06, r, 1e16)
+    ghost = (i == 0) | (j == 0) | (k == 0) | (i == n3[0] + 1) | (j == n3[1] + 1) | (k == n3[2] + 1)
+    T0[ghost] = 1e16
+    v = 1.0 + 0.5 * rng.random(size) if with_speed else None
+    want = T0.copy()
+    oracle.ellip_sweeps(want, n3, 3, 0.0, h, M=W.M, speed=1.3, speed_field=v)
+    m = W.model()
+    m.speed = 1.3
+    vd = dev(v) if with_speed else None
+    m.speed_field = vd.data_ptr() if with_speed else None
+    solver.set_tuning(groups, fold3d.MODEL_ELLIP)
+    try:
+        T = dev(T0)
+        solver.solve(T, n3, spacing=h, model=m, max_iters=3, engine=fold3d.ENGINE_SPLIT)
+        assert bits_equal(T.cpu().numpy(), want)
+        T = dev(T0)
+        solver.solve(T, n3, spacing=h, model=m, max_iters=3, engine=fold3d.ENGINE_PLANE)
+        assert bits_equal(T.cpu().numpy(), want)
+    finally:
+        solver.set_tuning(0, fold3d.MODEL_ELLIP)
+
+
 def test_bad_metric_is_rejected(solver):
     m = wl.tilted_homogeneous(16).model()
     m.M[0] = -1.0

@@ -103,14 +103,14 @@ def test_split_groups_are_bitwise_equal(solver, oracle, groups):
     h, a = (0.3, 0.3, 0.3), (-0.35, 0.4, 0.7)
     want = T0.copy()
     oracle.sweeps3(want, n3, 2, 1.1, a, h)
-    solver.set_tuning(groups)
+    solver.set_tuning(groups, fold3d.MODEL_FOLD)
     try:
         T = to_dev(T0)
         solver.solve(T, n3, spacing=h, model=fold3d.fold_model(1.1, a), max_iters=2,
                      engine=fold3d.ENGINE_SPLIT)
         assert bits_equal(T.cpu().numpy(), want)
     finally:
-        solver.set_tuning(0)
+        solver.set_tuning(0, fold3d.MODEL_FOLD)
 
 
 def test_autotune_keeps_input_and_picks_a_choice(solver):
@@ -120,7 +120,7 @@ def test_autotune_keeps_input_and_picks_a_choice(solver):
     groups, ms = solver.autotune(T0, n, iterations=1)
     assert groups in (2, 4, 8) and ms > 0
     assert torch.equal(T0.view(torch.int64), before.view(torch.int64))
-    solver.set_tuning(0)
+    solver.set_tuning(0, fold3d.MODEL_FOLD)
     assert solver.tuning() == 4
 
 

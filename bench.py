@@ -76,6 +76,63 @@ def ellip_run(solver, n, dtype, tol, torch):
             "unit": "Gcell-updates/s", "cell_updates": W_upd, "split_groups": groups}, T
 
 
+def batch_run(solver, torch, count=96, check=6):
+    """96 instances, sides uniform in {64, ..., 320}: per-instance blob speed,
+    tilt/axis-ratio fields and interior source, FP32 to max|dT| < 1e-6 —
+    solved as one batch and as a loop of plain solves."""
+    from fold3d import workloads as wl
+    sides = wl.batch_sides(count)
+    Ws, T0s = [], []
+    for k, n in enumerate(sides):
+        W = wl.batch_instance(n, 1000 + k, xp=torch, dtype=torch.float32, device="cuda")
+        Ws.append(W)
+        T0s.append(wl.point_source_start(W, xp=torch, dtype=torch.float32, device="cuda"))
+    cells = sum(n ** 3 for n in sides)
+    log(f"bench: batch of {count} instances, {cells / 1e9:.2f} G nodes")
+    # warm-up on three instances (code load, graph path)
+    warm = [t.clone() for t in T0s[:3]]
+    solver.solve_batched(Ws[:3], warm, max_iters=2, tol=0.0)
+    Ts = [t.clone() for t in T0s]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    its, mds, dev_ms = solver.solve_batched(Ws, Ts, max_iters=200, tol=1e-6)
+    torch.cuda.synchronize()
+    batch_s = time.perf_counter() - t0
+    upd = sum(6 * (n - 1) * n * n * k for n, k in zip(sides, its))
+    log(f"bench: batched {batch_s:.2f} s ({count / batch_s:.1f} solves/s), iterations "
+        f"{min(its)}..{max(its)}")
+    # looped plain solves of the same instances
+    Ls = [t.clone() for t in T0s]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    lits = []
+    for W, T in zip(Ws, Ls):
+        st = solver.solve(T, W.n, spacing=W.h, model=W.model(), max_iters=200, tol=1e-6)
+        lits.append(st.iterations)
+    torch.cuda.synchronize()
+    loop_s = time.perf_counter() - t0
+    log(f"bench: looped {loop_s:.2f} s ({count / loop_s:.1f} solves/s)")
+    # FP32 batch vs FP64 batch on the `check` smallest instances
+    idx = sorted(range(count), key=lambda k: sides[k])[:check]
+    W64 = [wl.batch_instance(sides[k], 1000 + k, xp=torch, dtype=torch.float64, device="cuda")
+           for k in idx]
+    T64 = [wl.point_source_start(W, xp=torch, dtype=torch.float64, device="cuda") for W in W64]
+    solver.solve_batched(W64, T64, max_iters=200, tol=1e-10)
+    worst = 0.0
+    for k, T in zip(idx, T64):
+        a, b = T, Ts[k].double()
+        far = (a < 1e15) & (a > 8 * Ws[k].h / 3.6)
+        worst = max(worst, float(((b - a).abs()[far] / a[far]).max()))
+    del Ws, T0s, Ts, Ls, W64, T64
+    torch.cuda.empty_cache()
+    return {"instances": count, "nodes": cells, "batched_s": batch_s, "batched_device_ms": dev_ms,
+            "batched_solves_per_s": count / batch_s, "looped_s": loop_s,
+            "looped_solves_per_s": count / loop_s, "speedup": loop_s / batch_s,
+            "iterations_min": min(its), "iterations_max": max(its),
+            "batched_cell_updates": upd, "batched_gcups": upd / batch_s / 1e9,
+            "fp32_max_rel_err_vs_fp64_on_smallest": {"instances": check, "value": worst}}
+
+
 def cpu_baseline(T0_host, n, threads):
     """The oracle (FP64 OpenMP restatement, bit-identical to the reference)
     on a bounded sample: one of the 8 iterations of the same 512^3 solve."""
@@ -99,6 +156,7 @@ def main():
     ap.add_argument("--no-fp32", action="store_true")
     ap.add_argument("--no-ellip", action="store_true")
     ap.add_argument("--no-ablation", action="store_true")
+    ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--iters", type=int, default=8, help="iterations per solve (the reference runs 8)")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -192,6 +250,10 @@ def main():
                          f"{cell_updates(n, 1):.3e} cell-updates in {dt:.1f} s"}
         log(f"bench: CPU {v:.3f} Gcell-upd/s")
 
+    batch = None
+    if not args.no_batch:
+        batch = batch_run(solver, torch)
+
     log("bench: measured ceilings of this GPU (FMA rate, device copy)")
     peaks = fold3d.measure_peaks()
     log(f"bench: {peaks}")
@@ -224,6 +286,7 @@ def main():
         "fp32": fp32,
         "elliptic_blob_speed": ellip,
         "cpu_baseline": cpu,
+        "batched": batch,
         "device_ceilings": valu,
     }
     print(json.dumps(out), flush=True)

@@ -169,11 +169,20 @@ static long long rows_of(const fold3d_instance &x, int d) {
 }
 
 template <typename R>
-int batch_solve(hipStream_t st, int count, const fold3d_instance *inst, int max_iters,
+int batch_solve(hipStream_t user_st, int count, const fold3d_instance *inst, int max_iters,
                 double tol, int blocks, int use_graph, int *iters_out, double *max_dt_out,
                 double *ms_out) {
   int dev = 0;
   F3D_HIP(hipGetDevice(&dev));
+  // The batch runs on a stream of its own (the legacy default stream cannot be
+  // captured), ordered after the caller's stream and before its later work.
+  hipStream_t st = nullptr;
+  hipEvent_t order_ev = nullptr;
+  F3D_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+  F3D_HIP(hipEventCreateWithFlags(&order_ev, hipEventDisableTiming));
+  F3D_HIP(hipEventRecord(order_ev, user_st));
+  F3D_HIP(hipStreamWaitEvent(st, order_ev, 0));
+  bool capturing = false;
   // host tables
   std::vector<BatchInst<R>> hinst(count);
   for (int i = 0; i < count; ++i) hinst[i] = make_inst<R>(inst[i]);
@@ -254,7 +263,9 @@ int batch_solve(hipStream_t st, int count, const fold3d_instance *inst, int max_
 
     if (use_graph) {
       BATCH_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+      capturing = true;
       enqueue_iteration(st);
+      capturing = false;
       BATCH_HIP(hipStreamEndCapture(st, &graph));
       BATCH_HIP(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
     }
@@ -306,6 +317,11 @@ int batch_solve(hipStream_t st, int count, const fold3d_instance *inst, int max_
   }
 cleanup:
 #undef BATCH_HIP
+  if (capturing) {  // a failure inside the capture: end it before anything else
+    hipGraph_t g2 = nullptr;
+    (void)hipStreamEndCapture(st, &g2);
+    if (g2) (void)hipGraphDestroy(g2);
+  }
   if (exec) (void)hipGraphExecDestroy(exec);
   if (graph) (void)hipGraphDestroy(graph);
   if (e0) (void)hipEventDestroy(e0);
@@ -317,6 +333,12 @@ cleanup:
   (void)hipFree(d_done);
   (void)hipFree(d_counters);
   (void)hipFree(d_max);
+  // later work on the caller's stream sees the solved fields
+  (void)hipEventRecord(order_ev, st);
+  (void)hipStreamWaitEvent(user_st, order_ev, 0);
+  (void)hipStreamSynchronize(st);
+  (void)hipEventDestroy(order_ev);
+  (void)hipStreamDestroy(st);
   return rc;
 }
 

@@ -1,0 +1,404 @@
+// adjoint.hip — gradient of a receiver traveltime misfit with respect to the
+// per-node slowness s = 1/v and the tilt angle theta of the elliptic model.
+//
+// At the converged forward field every node X is the minimum of its candidates
+// over the six sweep directions, so T_X = F_X(T at its winning simplex, s_X,
+// N_X).  By the envelope theorem the derivatives of that candidate are
+//   dT_X/dT_Y   = barycentric weight of Y at the minimising foot point,
+//   dT_X/ds_X   = |w|_N               (w = X - foot point),
+//   dT_X/dtheta = s (w^T dN/dtheta w) / (2 |w|_N).
+// The adjoint state mu solves mu_X = r_X + sum_Z (dT_Z/dT_X) mu_Z (Z downstream
+// of X), a triangular system in the order of T, solved here by reverse-ordered
+// plane sweeps (Gauss-Seidel) until mu stops changing; then
+//   dJ/ds_X = mu_X dT_X/ds_X,  dJ/dtheta_X = mu_X dT_X/dtheta_X.
+// Nodes no candidate reproduces (the closed-form start around a source) are
+// fixed: they pass nothing upstream and contribute no gradient.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstring>
+
+#include "common.h"
+#include "ellip_local.h"
+#include "solver.h"
+
+namespace fold3d {
+
+// Winning stencil of every node (structure of arrays, reference layout).
+template <typename R>
+struct Stencil {
+  int *off0, *off1, *off2;  // linear offsets of the simplex vertices from the node
+  R *w0, *w1, *w2;          // their barycentric weights (0 for unused slots)
+  R *dTds, *dTdth;          // local partial derivatives
+};
+
+template <typename R>
+struct AdjArgs {
+  int n[3];
+  long long stride[3];
+  R h[3];
+  R speed;
+  const R *V;                      // speed field (required for a slowness gradient)
+  const R *th, *ph, *ra, *rb;      // per-node metric or null
+  R Nc[6];                         // constant inverse metric
+  R rel_tol;                       // a candidate within rel_tol of T counts as the winner
+};
+
+// dN/dtheta of N = Q diag(1, 1/ra^2, 1/rb^2) Q^T, Q = Rz(phi) Ry(theta).
+template <typename R>
+__device__ __forceinline__ void dN_dtheta(R theta, R phi, R ra, R rb, R dN[6]) {
+  const R ct = cos(theta), st = sin(theta), cp = cos(phi), sp = sin(phi);
+  const R Q[3][3] = {{cp * ct, -sp, cp * st}, {sp * ct, cp, sp * st}, {-st, R(0), ct}};
+  const R dQ[3][3] = {{-cp * st, R(0), cp * ct}, {-sp * st, R(0), sp * ct}, {-ct, R(0), -st}};
+  const R wgt[3] = {R(1), R(1) / (ra * ra), R(1) / (rb * rb)};
+  R F[3][3];
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) {
+      R acc = R(0);
+      for (int k = 0; k < 3; ++k) acc += (dQ[i][k] * Q[j][k] + Q[i][k] * dQ[j][k]) * wgt[k];
+      F[i][j] = acc;
+    }
+  dN[0] = F[0][0];
+  dN[1] = F[1][1];
+  dN[2] = F[2][2];
+  dN[3] = F[0][1];
+  dN[4] = F[0][2];
+  dN[5] = F[1][2];
+}
+
+template <typename R>
+__device__ __forceinline__ R quad6(const R M[6], const R w[3]) {
+  return M[0] * w[0] * w[0] + M[1] * w[1] * w[1] + M[2] * w[2] * w[2] +
+         R(2) * (M[3] * w[0] * w[1] + M[4] * w[0] * w[2] + M[5] * w[1] * w[2]);
+}
+
+// Best candidate so far, with what the adjoint needs from it.
+template <typename R>
+struct Winner {
+  R val;
+  int nv;                // vertices used (1, 2 or 3)
+  long long off[3];      // linear offsets from the node
+  R wgt[3];
+  R ell;                 // |w|_N
+  R wg[3];               // w in global axes
+};
+
+template <typename R>
+__global__ __launch_bounds__(256) void stencil_kernel(const R *__restrict__ T, AdjArgs<R> A,
+                                                      Stencil<R> S) {
+  const long long total = (long long)A.n[0] * A.n[1] * A.n[2];
+  const long long tid = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (tid >= total) return;
+  const int i = 1 + (int)(tid % A.n[0]);
+  const int j = 1 + (int)((tid / A.n[0]) % A.n[1]);
+  const int k = 1 + (int)(tid / ((long long)A.n[0] * A.n[1]));
+  const int ijk[3] = {i, j, k};
+  const long long X = i * A.stride[0] + j * A.stride[1] + k * A.stride[2];
+  const R tX = T[X];
+  const R s = A.V ? R(1) / A.V[X] : R(1) / A.speed;
+  R N[6], dN[6];
+  if (A.th) {
+    tilted_inverse_metric<R>(A.th[X], A.ph[X], A.ra[X], A.rb[X], N);
+    dN_dtheta<R>(A.th[X], A.ph[X], A.ra[X], A.rb[X], dN);
+  } else {
+    for (int q = 0; q < 6; ++q) {
+      N[q] = A.Nc[q];
+      dN[q] = R(0);
+    }
+  }
+  Winner<R> best;
+  best.val = R(1e30);
+  best.nv = 0;
+  for (int d = 0; d < 6; ++d) {
+    const Frame fr = kFrames[d];
+    const int cm = ijk[fr.m] - fr.s;  // upwind centre along the march axis
+    if (cm < 1 || cm > A.n[fr.m]) continue;
+    const int u[3] = {fr.ux, fr.uy, fr.m};
+    const R sg[3] = {R(fr.sx), R(fr.sy), R(fr.s)};
+    const R hl[3] = {A.h[fr.ux], A.h[fr.uy], A.h[fr.m]};
+    const LatticeN<R> L = lattice_of<R>(N, u, sg, hl);
+    const long long sm = fr.s * A.stride[fr.m];
+    const long long su = fr.sx * A.stride[fr.ux], sv = fr.sy * A.stride[fr.uy];
+    const long long c0 = X - sm;
+    R p[9];
+    long long po[9];
+    po[0] = -sm;
+    for (int q = 0; q < 8; ++q) po[1 + q] = -sm + kRing[q][0] * su + kRing[q][1] * sv;
+    for (int q = 0; q < 9; ++q) p[q] = T[X + po[q]];
+    (void)c0;
+    // local lattice coordinates of the patch nodes relative to the target
+    auto lat = [&](int q, int &a, int &b) {
+      a = q == 0 ? 0 : kRing[q - 1][0];
+      b = q == 0 ? 0 : kRing[q - 1][1];
+    };
+    auto to_global = [&](const R wl[3], R wg[3]) {
+      for (int q = 0; q < 3; ++q) wg[u[q]] = sg[q] * wl[q];
+    };
+    // vertices
+    for (int q = 0; q < 9; ++q) {
+      if (!reached(p[q])) continue;
+      int a, b;
+      lat(q, a, b);
+      const R len = sqrt(lat_dot(L, -a, -b, 1, -a, -b, 1));
+      const R c = p[q] + s * len;
+      if (c < best.val) {
+        best.val = c;
+        best.nv = 1;
+        best.off[0] = po[q];
+        best.wgt[0] = R(1);
+        best.ell = len;
+        const R wl[3] = {-a * hl[0], -b * hl[1], hl[2]};
+        to_global(wl, best.wg);
+      }
+    }
+    const R s2 = s * s;
+    // edges: spokes 0..7 (centre -> ring q), rims 8..15 (ring q -> ring q+1)
+    for (int e = 0; e < 16; ++e) {
+      const int q0 = e < 8 ? 0 : 1 + (e - 8), q1 = e < 8 ? 1 + e : 1 + ((e - 8 + 1) % 8);
+      const R t0 = p[q0], t1 = p[q1];
+      if (!(reached(t0) && reached(t1))) continue;
+      int a0, b0, a1, b1;
+      lat(q0, a0, b0);
+      lat(q1, a1, b1);
+      const int ea = a1 - a0, eb = b1 - b0;
+      const R g = lat_dot(L, ea, eb, 0, ea, eb, 0);
+      const R bb = lat_dot(L, ea, eb, 0, -a0, -b0, 1);
+      const R cc = lat_dot(L, -a0, -b0, 1, -a0, -b0, 1);
+      const R P = R(1) / g, l0 = P * bb, kap = cc - bb * l0;
+      const R dd = t1 - t0, mu = P * dd, r = s2 - dd * mu;
+      if (!(r > R(0))) continue;
+      const R root = sqrt(kap * r);
+      const R lam = l0 - (root / r) * mu;
+      if (!(lam >= R(0) && lam <= R(1))) continue;
+      const R c = t0 + dd * l0 + root;
+      if (c < best.val) {
+        best.val = c;
+        best.nv = 2;
+        best.off[0] = po[q0];
+        best.off[1] = po[q1];
+        best.wgt[0] = R(1) - lam;
+        best.wgt[1] = lam;
+        best.ell = s * root / r;
+        const R wl[3] = {(-a0 - lam * ea) * hl[0], (-b0 - lam * eb) * hl[1], hl[2]};
+        to_global(wl, best.wg);
+      }
+    }
+    // triangles: centre, ring q, ring q+1
+    for (int t = 0; t < 8; ++t) {
+      const int q1 = 1 + t, q2 = 1 + (t + 1) % 8;
+      const R t0 = p[0], t1 = p[q1], t2 = p[q2];
+      if (!(reached(t0) && reached(t1) && reached(t2))) continue;
+      int a1, b1, a2, b2;
+      lat(q1, a1, b1);
+      lat(q2, a2, b2);
+      const R g11 = lat_dot(L, a1, b1, 0, a1, b1, 0), g22 = lat_dot(L, a2, b2, 0, a2, b2, 0);
+      const R g12 = lat_dot(L, a1, b1, 0, a2, b2, 0);
+      const R r1 = lat_dot(L, a1, b1, 0, 0, 0, 1), r2 = lat_dot(L, a2, b2, 0, 0, 0, 1);
+      const R rdet = R(1) / (g11 * g22 - g12 * g12);
+      const R P11 = g22 * rdet, P12 = -g12 * rdet, P22 = g11 * rdet;
+      const R l1 = P11 * r1 + P12 * r2, l2 = P12 * r1 + P22 * r2;
+      const R kap = L.C - (r1 * l1 + r2 * l2);
+      const R d1 = t1 - t0, d2 = t2 - t0;
+      const R m1 = P11 * d1 + P12 * d2, m2 = P12 * d1 + P22 * d2;
+      const R r = s2 - (d1 * m1 + d2 * m2);
+      if (!(r > R(0))) continue;
+      const R root = sqrt(kap * r);
+      const R x1 = l1 - (root / r) * m1, x2 = l2 - (root / r) * m2;
+      if (!(x1 >= R(0) && x2 >= R(0) && x1 + x2 <= R(1))) continue;
+      const R c = t0 + (d1 * l1 + d2 * l2) + root;
+      if (c < best.val) {
+        best.val = c;
+        best.nv = 3;
+        best.off[0] = po[0];
+        best.off[1] = po[q1];
+        best.off[2] = po[q2];
+        best.wgt[0] = R(1) - x1 - x2;
+        best.wgt[1] = x1;
+        best.wgt[2] = x2;
+        best.ell = s * root / r;
+        const R wl[3] = {(-x1 * a1 - x2 * a2) * hl[0], (-x1 * b1 - x2 * b2) * hl[1], hl[2]};
+        to_global(wl, best.wg);
+      }
+    }
+  }
+  // a node is a stencil node if its best candidate reproduces its value
+  const bool follows = best.nv > 0 && reached(tX) && best.val <= tX * (R(1) + A.rel_tol);
+  int o[3] = {0, 0, 0};
+  R w[3] = {R(0), R(0), R(0)};
+  R gs = R(0), gth = R(0);
+  if (follows) {
+    for (int q = 0; q < best.nv; ++q) {
+      o[q] = (int)best.off[q];
+      w[q] = best.wgt[q];
+    }
+    gs = best.ell;
+    gth = A.th ? s * quad6(dN, best.wg) / (R(2) * best.ell) : R(0);
+  }
+  S.off0[X] = o[0];
+  S.off1[X] = o[1];
+  S.off2[X] = o[2];
+  S.w0[X] = w[0];
+  S.w1[X] = w[1];
+  S.w2[X] = w[2];
+  S.dTds[X] = gs;
+  S.dTdth[X] = gth;
+}
+
+// One plane of an adjoint sweep: mu_X = r_X + sum over the 26 neighbours Z
+// that list X in their stencil of weight * mu_Z.
+template <typename R>
+__global__ __launch_bounds__(256) void adjoint_plane_kernel(R *__restrict__ mu,
+                                                            const R *__restrict__ res,
+                                                            Stencil<R> S, PlaneGeom g,
+                                                            long long sx, long long sy,
+                                                            long long sz,
+                                                            unsigned long long *__restrict__ maxd) {
+  const int a = 1 + blockIdx.x * 64 + threadIdx.x;
+  const int b = 1 + blockIdx.y * 4 + threadIdx.y;
+  R d = R(0);
+  if (a <= g.na && b <= g.nb) {
+    const long long X = g.base + a * g.Sa + b * g.Sb + g.sm;  // the plane being swept
+    R acc = res[X];
+    for (int dz = -1; dz <= 1; ++dz)
+      for (int dy = -1; dy <= 1; ++dy)
+        for (int dx = -1; dx <= 1; ++dx) {
+          if (dx == 0 && dy == 0 && dz == 0) continue;
+          const long long o = dx * sx + dy * sy + dz * sz;
+          const long long Z = X + o;  // neighbour; X sits at offset -o from it
+          const int back = (int)(-o);
+          R wz = R(0);
+          if (S.off0[Z] == back) wz += S.w0[Z];
+          if (S.off1[Z] == back) wz += S.w1[Z];
+          if (S.off2[Z] == back) wz += S.w2[Z];
+          if (wz != R(0)) acc += wz * mu[Z];
+        }
+    const R old = mu[X];
+    mu[X] = acc;
+    d = fabs(acc - old);
+  }
+  double v = (double)d;
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, 64));
+  if ((threadIdx.x & 63) == 0 && v > 0.0) atomicMax(maxd, (unsigned long long)__double_as_longlong(v));
+}
+
+template <typename R>
+__global__ void adjoint_gradient_kernel(const R *__restrict__ mu, Stencil<R> S, long long count,
+                                        R *__restrict__ gs, R *__restrict__ gth) {
+  for (long long x = blockIdx.x * (long long)blockDim.x + threadIdx.x; x < count;
+       x += (long long)gridDim.x * blockDim.x) {
+    const R m = mu[x];
+    if (m != R(0)) {
+      gs[x] += m * S.dTds[x];
+      if (gth) gth[x] += m * S.dTdth[x];
+    }
+  }
+}
+
+template <typename R>
+__global__ void scatter_residual_kernel(R *__restrict__ res, const int *__restrict__ idx,
+                                        const R *__restrict__ val, int count) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < count) res[idx[k]] += val[k];
+}
+
+__global__ void adj_zero_u64(unsigned long long *p) { *p = 0ull; }
+
+template <typename R>
+int adjoint_solve(hipStream_t st, const R *T, const Grid &G, const fold3d_model &m,
+                  const double h[3], int nrec, const int *rec_idx, const R *rec_res,
+                  int max_iters, double tol, R *grad_s, R *grad_th, void *scratch,
+                  unsigned long long *d_max, int *iters_out, double *maxd_out) {
+  const long long count = G.count;
+  // scratch: mu, res, 3 offsets (int), 3 weights, 2 partials
+  char *p = (char *)scratch;
+  R *mu = (R *)p;
+  p += count * sizeof(R);
+  R *res = (R *)p;
+  p += count * sizeof(R);
+  Stencil<R> S;
+  S.off0 = (int *)p;
+  p += count * sizeof(int);
+  S.off1 = (int *)p;
+  p += count * sizeof(int);
+  S.off2 = (int *)p;
+  p += count * sizeof(int);
+  S.w0 = (R *)p;
+  p += count * sizeof(R);
+  S.w1 = (R *)p;
+  p += count * sizeof(R);
+  S.w2 = (R *)p;
+  p += count * sizeof(R);
+  S.dTds = (R *)p;
+  p += count * sizeof(R);
+  S.dTdth = (R *)p;
+  F3D_HIP(hipMemsetAsync(scratch, 0, (size_t)count * (7 * sizeof(R) + 3 * sizeof(int)), st));
+  AdjArgs<R> A;
+  memset(&A, 0, sizeof A);
+  for (int a = 0; a < 3; ++a) {
+    A.n[a] = G.n[a];
+    A.stride[a] = G.stride[a];
+    A.h[a] = (R)h[a];
+  }
+  A.speed = (R)m.speed;
+  A.V = (const R *)m.speed_field;
+  A.th = (const R *)m.theta;
+  A.ph = (const R *)m.phi;
+  A.ra = (const R *)m.ra;
+  A.rb = (const R *)m.rb;
+  double N[6];
+  sym3_inverse(m.M, N);
+  for (int q = 0; q < 6; ++q) A.Nc[q] = (R)N[q];
+  A.rel_tol = sizeof(R) == 8 ? (R)1e-12 : (R)1e-6;
+  const long long interior = (long long)G.n[0] * G.n[1] * G.n[2];
+  stencil_kernel<R><<<(unsigned)((interior + 255) / 256), 256, 0, st>>>(T, A, S);
+  if (nrec > 0) scatter_residual_kernel<R><<<(nrec + 255) / 256, 256, 0, st>>>(res, rec_idx, rec_res, nrec);
+  F3D_HIP(hipGetLastError());
+  int it = 0;
+  double last = -1.0;
+  const dim3 block(64, 4);
+  for (; it < max_iters;) {
+    adj_zero_u64<<<1, 1, 0, st>>>(d_max);
+    // reverse of the forward order: each direction's planes from last to first
+    for (int dd = 5; dd >= 0; --dd) {
+      const Frame &fr = kFrames[dd];
+      const int n = G.n[fr.m];
+      for (int q = 0; q < n; ++q) {
+        // sweep plane index along the march axis, against the characteristics
+        const int pl = fr.s > 0 ? n - q : 1 + q;
+        PlaneGeom g = plane_geom(G, dd, pl);
+        g.sm = 0;  // the plane itself (plane_geom addresses a source plane)
+        const dim3 grid((g.na + 63) / 64, (g.nb + 3) / 4);
+        adjoint_plane_kernel<R><<<grid, block, 0, st>>>(mu, res, S, g, G.stride[0], G.stride[1],
+                                                        G.stride[2], d_max);
+      }
+    }
+    F3D_HIP(hipGetLastError());
+    ++it;
+    unsigned long long bits = 0;
+    F3D_HIP(hipMemcpyAsync(&bits, d_max, sizeof bits, hipMemcpyDeviceToHost, st));
+    F3D_HIP(hipStreamSynchronize(st));
+    memcpy(&last, &bits, sizeof last);
+    if (last <= tol) break;
+  }
+  adjoint_gradient_kernel<R><<<1024, 256, 0, st>>>(mu, S, count, grad_s, grad_th);
+  F3D_HIP(hipGetLastError());
+  *iters_out = it;
+  *maxd_out = last;
+  return FOLD3D_OK;
+}
+
+size_t adjoint_scratch_bytes(const Grid &G, int precision) {
+  const size_t r = precision == FOLD3D_F64 ? 8 : 4;
+  return (size_t)G.count * (7 * r + 3 * sizeof(int));
+}
+
+template int adjoint_solve<double>(hipStream_t, const double *, const Grid &, const fold3d_model &,
+                                   const double *, int, const int *, const double *, int, double,
+                                   double *, double *, void *, unsigned long long *, int *,
+                                   double *);
+template int adjoint_solve<float>(hipStream_t, const float *, const Grid &, const fold3d_model &,
+                                  const double *, int, const int *, const float *, int, double,
+                                  float *, float *, void *, unsigned long long *, int *, double *);
+
+}  // namespace fold3d

@@ -2,6 +2,8 @@
 // device half of include/Fold3dPMM.h).
 #include <hip/hip_runtime.h>
 
+#include <vector>
+
 #include <cmath>
 #include <cstring>
 
@@ -304,6 +306,78 @@ int fold3d_solve_batched(fold3d_handle *h, int count, const fold3d_instance *ins
                                    iterations_out, max_dt_out, ms_out)
              : batch_solve<float>(st, count, inst, o.max_iters, o.tol, blocks, use_graph,
                                   iterations_out, max_dt_out, ms_out);
+}
+
+int fold3d_adjoint(fold3d_handle *h, const void *T, int nx, int ny, int nz, double dx,
+                   double dy, double dz, const fold3d_model *model, int nrec,
+                   const int *rec_idx, const void *rec_res, const fold3d_opts *opts,
+                   void *grad_slowness, void *grad_theta, fold3d_stats *stats) {
+  F3D_CHECK(h && T && model && grad_slowness, FOLD3D_E_INVALID,
+            "fold3d_adjoint: NULL handle, T, model or grad_slowness");
+  F3D_CHECK(model->kind == FOLD3D_MODEL_ELLIP, FOLD3D_E_INVALID,
+            "fold3d_adjoint: the adjoint is for the elliptic model");
+  F3D_CHECK(nx >= 3 && ny >= 3 && nz >= 3, FOLD3D_E_INVALID, "fold3d_adjoint: grid too small");
+  F3D_CHECK(dx > 0 && dy > 0 && dz > 0, FOLD3D_E_INVALID, "fold3d_adjoint: bad spacings");
+  F3D_CHECK(nrec >= 0 && (nrec == 0 || (rec_idx && rec_res)), FOLD3D_E_INVALID,
+            "fold3d_adjoint: bad receiver arrays");
+  F3D_CHECK(grad_theta == nullptr || model->theta != nullptr, FOLD3D_E_INVALID,
+            "fold3d_adjoint: a tilt gradient needs per-node tilt fields");
+  F3D_CHECK(model->speed_field != nullptr || model->speed > 0, FOLD3D_E_INVALID,
+            "fold3d_adjoint: speed must be positive");
+  fold3d_opts o;
+  fold3d_default_opts(&o);
+  if (opts) o = *opts;
+  F3D_CHECK(o.memory == FOLD3D_DEVICE, FOLD3D_E_INVALID, "fold3d_adjoint: T must be on the device");
+  F3D_CHECK(o.max_iters >= 1 && o.tol >= 0, FOLD3D_E_INVALID, "fold3d_adjoint: bad max_iters/tol");
+  F3D_HIP(hipSetDevice(h->device));
+  const Grid G(nx, ny, nz);
+  hipStream_t st = (hipStream_t)o.stream;
+  if (nrec > 0) {
+    // receivers must be interior nodes: their residuals are scattered by index
+    std::vector<int> idx(nrec);
+    F3D_HIP(hipMemcpyAsync(idx.data(), rec_idx, sizeof(int) * nrec, hipMemcpyDeviceToHost, st));
+    F3D_HIP(hipStreamSynchronize(st));
+    for (int r = 0; r < nrec; ++r) {
+      const long long x = idx[r];
+      const long long i = x % G.stride[1], j = (x / G.stride[1]) % (ny + 2), k = x / G.stride[2];
+      F3D_CHECK(x >= 0 && x < G.count && i >= 1 && i <= nx && j >= 1 && j <= ny && k >= 1 &&
+                    k <= nz,
+                FOLD3D_E_INVALID, "fold3d_adjoint: receiver %d index %lld is not an interior node",
+                r, x);
+    }
+  }
+  const int rc0 = ensure_scratch(h, adjoint_scratch_bytes(G, o.precision));
+  if (rc0 != FOLD3D_OK) return rc0;
+  const double hsp[3] = {dx, dy, dz};
+  int iters = 0;
+  double maxd = -1;
+  F3D_HIP(hipEventRecord(h->ev0, st));
+  const int rc =
+      o.precision == FOLD3D_F64
+          ? adjoint_solve<double>(st, (const double *)T, G, *model, hsp, nrec, rec_idx,
+                                  (const double *)rec_res, o.max_iters, o.tol,
+                                  (double *)grad_slowness, (double *)grad_theta, h->d_scratch,
+                                  h->d_max, &iters, &maxd)
+          : adjoint_solve<float>(st, (const float *)T, G, *model, hsp, nrec, rec_idx,
+                                 (const float *)rec_res, o.max_iters, o.tol,
+                                 (float *)grad_slowness, (float *)grad_theta, h->d_scratch,
+                                 h->d_max, &iters, &maxd);
+  F3D_HIP(hipEventRecord(h->ev1, st));
+  if (rc != FOLD3D_OK) return rc;
+  F3D_HIP(hipEventSynchronize(h->ev1));
+  float ms = 0;
+  F3D_HIP(hipEventElapsedTime(&ms, h->ev0, h->ev1));
+  if (stats) {
+    stats->iterations = iters;
+    stats->max_dT = maxd;
+    stats->cell_updates = 0;
+    stats->sweep_ms = ms;
+  }
+  if (!(maxd <= o.tol)) {
+    set_error("fold3d_adjoint: adjoint change %g > tol %g after %d sweeps", maxd, o.tol, iters);
+    return FOLD3D_E_NOT_CONVERGED;
+  }
+  return FOLD3D_OK;
 }
 
 int fold3d_set_tuning(fold3d_handle *h, int model_kind, int groups) {

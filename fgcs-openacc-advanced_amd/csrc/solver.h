@@ -67,6 +67,15 @@ int batch_solve(hipStream_t st, int count, const fold3d_instance *inst, int max_
                 double tol, int blocks, int use_graph, int *iters_out, double *max_dt_out,
                 double *ms_out);
 
+// Adjoint state and gradients (adjoint.hip); `scratch` holds
+// adjoint_scratch_bytes() bytes.
+template <typename R>
+int adjoint_solve(hipStream_t st, const R *T, const Grid &G, const fold3d_model &m,
+                  const double h[3], int nrec, const int *rec_idx, const R *rec_res,
+                  int max_iters, double tol, R *grad_s, R *grad_th, void *scratch,
+                  unsigned long long *d_max, int *iters_out, double *maxd_out);
+size_t adjoint_scratch_bytes(const Grid &G, int precision);
+
 // Copy a padded field from one axis order to another (layout.hip).
 template <typename R>
 void permute_layout(hipStream_t st, const R *in, const Grid &Gin, R *out, const Grid &Gout);

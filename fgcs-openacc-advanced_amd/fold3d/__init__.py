@@ -122,6 +122,8 @@ def lib() -> C.CDLL:
     L.fold3d_get_tuning.argtypes = [vp, i, C.POINTER(i)]
     L.fold3d_autotune.argtypes = [vp, vp, i, i, i, d, d, d, C.POINTER(Model),
                                   C.POINTER(Opts), i, C.POINTER(C.c_float)]
+    L.fold3d_adjoint.argtypes = [vp, vp, i, i, i, d, d, d, C.POINTER(Model), i, vp, vp,
+                                 C.POINTER(Opts), vp, vp, C.POINTER(Stats)]
     L.fold3d_synchronize.argtypes = [vp]
     L.ImplicitInitialiser.argtypes = [dp, i, i, i, d, d, d]
     L.InitFoldFromPoint.restype = d
@@ -268,6 +270,33 @@ class Solver:
         if not (rc == FOLD3D_E_NOT_CONVERGED and not raise_not_converged):
             check(rc)
         return list(its), list(mds), ms.value
+
+    def adjoint(self, T, n, spacing, model: Model, rec_idx, rec_res, grad_slowness,
+                grad_theta=None, max_iters: int = 200, tol: float = 0.0, stream=None,
+                raise_not_converged: bool = True) -> Stats:
+        """Adds dJ/d(slowness) (and dJ/d(theta) with per-node tilt fields) of
+        J = 1/2 sum (T[rec_idx] - T_obs)^2 to the gradient tensors, given the
+        converged field T and the residuals rec_res = T[rec_idx] - T_obs (torch
+        device tensors; rec_idx int32 linear indices on the padded layout)."""
+        import torch
+        nx, ny, nz = (n, n, n) if np.isscalar(n) else n
+        dx, dy, dz = (spacing,) * 3 if np.isscalar(spacing) else spacing
+        if rec_idx.dtype != torch.int32 or rec_res.dtype != T.dtype:
+            raise TypeError("rec_idx must be int32 and rec_res of T's dtype")
+        o = Opts()
+        lib().fold3d_default_opts(C.byref(o))
+        o.precision = F64 if T.dtype == torch.float64 else F32
+        o.max_iters, o.tol = max_iters, tol
+        o.stream = stream if stream is not None else torch.cuda.current_stream(T.device).cuda_stream
+        st = Stats()
+        rc = lib().fold3d_adjoint(
+            self.h, C.c_void_p(T.data_ptr()), nx, ny, nz, dx, dy, dz, C.byref(model),
+            int(rec_idx.numel()), C.c_void_p(rec_idx.data_ptr()), C.c_void_p(rec_res.data_ptr()),
+            C.byref(o), C.c_void_p(grad_slowness.data_ptr()),
+            None if grad_theta is None else C.c_void_p(grad_theta.data_ptr()), C.byref(st))
+        if not (rc == FOLD3D_E_NOT_CONVERGED and not raise_not_converged):
+            check(rc)
+        return st
 
     def set_tuning(self, groups: int, kind: int = MODEL_FOLD):
         check(lib().fold3d_set_tuning(self.h, kind, groups))

@@ -156,6 +156,22 @@ int fold3d_solve_batched(fold3d_handle *h, int count, const fold3d_instance *ins
                          const fold3d_opts *opts, int *iterations_out,
                          double *max_dt_out, double *ms_out);
 
+/* Adjoint state of a converged elliptic-model field T (device, opts->precision)
+ * for the misfit J = 1/2 sum_r (T[rec_idx[r]] - T_obs[r])^2: `rec_res` holds
+ * the residuals T[rec] - T_obs at the `nrec` receiver nodes (device arrays of
+ * linear node indices and values).  Adds dJ/d(slowness) per node to
+ * grad_slowness and, with per-node tilt fields, dJ/d(theta) to grad_theta
+ * (may be NULL); both device arrays on the padded layout.  The linear upwind
+ * system of T is solved by reverse-ordered sweeps until no adjoint value
+ * changes by more than opts->tol (opts->max_iters at most).  Slowness = 1/v,
+ * so the model needs a speed field or a constant speed; nodes no candidate
+ * reproduces (the closed-form start around a source) are held fixed. */
+int fold3d_adjoint(fold3d_handle *h, const void *T, int nx, int ny, int nz,
+                   double dx, double dy, double dz, const fold3d_model *model,
+                   int nrec, const int *rec_idx, const void *rec_res,
+                   const fold3d_opts *opts, void *grad_slowness, void *grad_theta,
+                   fold3d_stats *stats);
+
 /* Tuning of the split engines, per model kind: how many wavefronts share one
  * row of 64 nodes (fold model: 2, 4 or 8, default 4; elliptic model with a
  * constant metric: 1, 2 or 4, default 2; 0 restores the default).

@@ -291,3 +291,206 @@ int orc_ellip_sweeps(double *T, const int n[3], int max_iters, double tol,
   if (max_dt) *max_dt = last;
   return it;
 }
+
+/* ---------------------------------------------------------------------- */
+/* Adjoint (FP64, independent of the GPU path): winning stencil per node by
+ * exhaustive search over the six directions' candidates, local derivatives by
+ * central differences of that candidate, the transposed upwind system solved
+ * by a push in order of decreasing T followed by Jacobi passes to a fixed
+ * point. */
+
+static const double *g_sort_key;
+static int cmp_desc(const void *a, const void *b)
+{
+  const double x = g_sort_key[*(const long *)a], y = g_sort_key[*(const long *)b];
+  return x < y ? 1 : x > y ? -1 : 0;
+}
+static void sort_by_value_desc(long *idx, long m, const double *key)
+{
+  g_sort_key = key;
+  qsort(idx, m, sizeof(long), cmp_desc);
+}
+
+typedef struct {
+  int dir, type, id;  /* type 1 vertex, 2 edge, 3 triangle; id as in the kernels */
+  long v[3];
+  double w[3];
+  double val;
+} orc_stencil;
+
+static void node_metric(const double *M6, const double *theta, const double *phi,
+                        const double *ra, const double *rb, long t, double dth,
+                        double N[6])
+{
+  if (theta) {
+    double M[6];
+    orc_tilted_metric(theta[t] + dth, phi[t], 1.0, ra[t], rb[t], M);
+    orc_sym3_inverse(M, N);
+  } else {
+    orc_sym3_inverse(M6, N);
+  }
+}
+
+/* Value of one candidate (type/id of direction d) at node X with inverse
+ * metric N and slowness s; weights of its vertices in w (if non-NULL).
+ * Returns +inf when the candidate is not admissible. */
+static double eval_candidate(const double *T, const int n[3], const double h[3], long X,
+                             int d, int type, int id, const double N[6], double s,
+                             long v[3], double w[3])
+{
+  const int *f = FR[d];
+  const long H0 = n[0] + 2, H1 = n[1] + 2;
+  const long stride[3] = {1, H0, H0 * H1};
+  const long sm = f[1] * stride[f[0]], su = f[3] * stride[f[2]], sv = f[5] * stride[f[4]];
+  const long c = X - sm;
+  orc_ellip_consts K;
+  orc_ellip_constants(d, N, h, &K);
+  long pidx[9];
+  int k;
+  pidx[0] = c;
+  for (k = 0; k < 8; k++) pidx[1 + k] = c + RING[k][0] * su + RING[k][1] * sv;
+  const double s2 = s * s;
+  if (type == 1) {
+    const double t0 = T[pidx[id]];
+    if (!(t0 < BIG)) return INFINITY;
+    if (v) { v[0] = pidx[id]; w[0] = 1.0; }
+    return t0 + s * K.vert[id];
+  }
+  if (type == 2) {
+    const int q0 = id < 8 ? 0 : 1 + (id - 8), q1 = id < 8 ? 1 + id : 1 + ((id - 8 + 1) % 8);
+    const double t0 = T[pidx[q0]], t1 = T[pidx[q1]];
+    const double val = orc_ellip_edge(t0, t1, s2, K.edge[id]);
+    if (v && isfinite(val)) {
+      const double P = K.edge[id][0], l0 = K.edge[id][1], kap = K.edge[id][2];
+      const double dd = t1 - t0, mu = P * dd, r = s2 - dd * mu;
+      const double lam = l0 - sqrt(kap * r) / r * mu;
+      v[0] = pidx[q0]; v[1] = pidx[q1];
+      w[0] = 1.0 - lam; w[1] = lam;
+    }
+    return val;
+  }
+  {
+    const int q1 = 1 + id, q2 = 1 + (id + 1) % 8;
+    const double t0 = T[pidx[0]], t1 = T[pidx[q1]], t2 = T[pidx[q2]];
+    const double *Tr = K.tri[id];
+    const double val = orc_ellip_tri(t0, t1, t2, s2, Tr);
+    if (v && isfinite(val)) {
+      const double d1 = t1 - t0, d2 = t2 - t0;
+      const double m1 = Tr[0] * d1 + Tr[1] * d2, m2 = Tr[1] * d1 + Tr[2] * d2;
+      const double r = s2 - (d1 * m1 + d2 * m2);
+      const double q = sqrt(Tr[5] * r) / r;
+      const double x1 = Tr[3] - q * m1, x2 = Tr[4] - q * m2;
+      v[0] = pidx[0]; v[1] = pidx[q1]; v[2] = pidx[q2];
+      w[0] = 1.0 - x1 - x2; w[1] = x1; w[2] = x2;
+    }
+    return val;
+  }
+}
+
+int orc_ellip_adjoint(const double *T, const int n[3], const double h[3], const double M6[6],
+                      double speed, const double *speed_field, const double *theta,
+                      const double *phi, const double *ra, const double *rb, int nrec,
+                      const long *rec_idx, const double *rec_res, double *grad_s,
+                      double *grad_th, double rel_tol, int max_passes, double *mu_out,
+                      unsigned char *fixed_out)
+{
+  const long H0 = n[0] + 2, H1 = n[1] + 2, H2 = n[2] + 2;
+  const long count = H0 * H1 * H2;
+  const long stride[3] = {1, H0, H0 * H1};
+  orc_stencil *S = (orc_stencil *)calloc(count, sizeof(orc_stencil));
+  double *dTds = (double *)calloc(count, sizeof(double));
+  double *dTdth = (double *)calloc(count, sizeof(double));
+  double *mu = (double *)calloc(count, sizeof(double));
+  double *nxt = (double *)calloc(count, sizeof(double));
+  double *res = (double *)calloc(count, sizeof(double));
+  long *order = (long *)malloc(sizeof(long) * count);
+  long i, nord = 0;
+  int r;
+  for (r = 0; r < nrec; r++) res[rec_idx[r]] += rec_res[r];
+
+#pragma omp parallel for schedule(dynamic, 64)
+  for (i = 0; i < count; i++) {
+    const long x = i % H0, y = (i / H0) % H1, z = i / (H0 * H1);
+    const long ijk[3] = {x, y, z};
+    S[i].type = 0;
+    if (x < 1 || x > n[0] || y < 1 || y > n[1] || z < 1 || z > n[2]) continue;
+    if (!(T[i] < BIG)) continue;
+    const double s = 1.0 / (speed_field ? speed_field[i] : speed);
+    double N[6];
+    node_metric(M6, theta, phi, ra, rb, i, 0.0, N);
+    double best = INFINITY;
+    int d, type, id;
+    for (d = 0; d < 6; d++) {
+      const long cm = ijk[FR[d][0]] - FR[d][1];
+      if (cm < 1 || cm > n[FR[d][0]]) continue;
+      for (type = 1; type <= 3; type++) {
+        const int nid = type == 1 ? 9 : type == 2 ? 16 : 8;
+        for (id = 0; id < nid; id++) {
+          const double val = eval_candidate(T, n, h, i, d, type, id, N, s, NULL, NULL);
+          if (val < best) {
+            best = val;
+            S[i].dir = d; S[i].type = type; S[i].id = id;
+          }
+        }
+      }
+    }
+    if (!(best <= T[i] * (1.0 + rel_tol))) { S[i].type = 0; continue; }
+    S[i].val = best;
+    eval_candidate(T, n, h, i, S[i].dir, S[i].type, S[i].id, N, s, S[i].v, S[i].w);
+    /* local derivatives by central differences of the winning candidate */
+    const double es = 1e-6 * s;
+    dTds[i] = (eval_candidate(T, n, h, i, S[i].dir, S[i].type, S[i].id, N, s + es, NULL, NULL) -
+               eval_candidate(T, n, h, i, S[i].dir, S[i].type, S[i].id, N, s - es, NULL, NULL)) /
+              (2 * es);
+    if (theta) {
+      const double et = 1e-6;
+      double Np[6], Nm[6];
+      node_metric(M6, theta, phi, ra, rb, i, et, Np);
+      node_metric(M6, theta, phi, ra, rb, i, -et, Nm);
+      dTdth[i] = (eval_candidate(T, n, h, i, S[i].dir, S[i].type, S[i].id, Np, s, NULL, NULL) -
+                  eval_candidate(T, n, h, i, S[i].dir, S[i].type, S[i].id, Nm, s, NULL, NULL)) /
+                 (2 * et);
+    }
+  }
+  (void)stride;
+  /* push in order of decreasing T */
+  for (i = 0; i < count; i++)
+    if (S[i].type) order[nord++] = i;
+  sort_by_value_desc(order, nord, T);
+  for (i = 0; i < count; i++) mu[i] = res[i];
+  for (i = 0; i < nord; i++) {
+    const orc_stencil *st = &S[order[i]];
+    const int nv = st->type == 1 ? 1 : st->type == 2 ? 2 : 3;
+    int q;
+    for (q = 0; q < nv; q++) mu[st->v[q]] += st->w[q] * mu[order[i]];
+  }
+  /* Jacobi passes: mu = res + A^T mu, until a pass changes nothing */
+  int pass;
+  for (pass = 0; pass < max_passes; pass++) {
+    for (i = 0; i < count; i++) nxt[i] = res[i];
+    for (i = 0; i < count; i++) {
+      if (!S[i].type) continue;
+      const int nv = S[i].type == 1 ? 1 : S[i].type == 2 ? 2 : 3;
+      int q;
+      for (q = 0; q < nv; q++) nxt[S[i].v[q]] += S[i].w[q] * mu[i];
+    }
+    double dmax = 0.0;
+    for (i = 0; i < count; i++) {
+      const double dd = fabs(nxt[i] - mu[i]);
+      if (dd > dmax) dmax = dd;
+      mu[i] = nxt[i];
+    }
+    if (dmax == 0.0) break;
+  }
+  if (mu_out) memcpy(mu_out, mu, sizeof(double) * count);
+  if (fixed_out)
+    for (i = 0; i < count; i++) fixed_out[i] = !S[i].type && T[i] < BIG;
+  for (i = 0; i < count; i++) {
+    if (!S[i].type) continue;
+    grad_s[i] += mu[i] * dTds[i];
+    if (grad_th && theta) grad_th[i] += mu[i] * dTdth[i];
+  }
+  free(S); free(dTds); free(dTdth); free(mu); free(nxt); free(res); free(order);
+  return pass;
+}
+

@@ -92,6 +92,20 @@ int orc_ellip_sweeps(double *T, const int n[3], int max_iters, double tol,
                      const double *phi, const double *ra, const double *rb,
                      double *max_dt);
 
+/* Adjoint of a converged elliptic field T (FP64, independent of the GPU path).
+ * Adds dJ/d(slowness) to grad_s and, with theta..rb given, dJ/d(theta) to
+ * grad_th for J = 1/2 sum_r res_r^2 with res_r the residual at node rec_idx[r].
+ * A node's stencil is its best candidate over all six directions if that
+ * reproduces T within rel_tol, else the node is held fixed.  Optional outputs:
+ * the adjoint state mu and a flag per node that is reached but held fixed.
+ * Returns the Jacobi passes used after the ordered push. */
+int orc_ellip_adjoint(const double *T, const int n[3], const double h[3], const double M6[6],
+                      double speed, const double *speed_field, const double *theta,
+                      const double *phi, const double *ra, const double *rb, int nrec,
+                      const long *rec_idx, const double *rec_res, double *grad_s,
+                      double *grad_th, double rel_tol, int max_passes, double *mu_out,
+                      unsigned char *fixed_out);
+
 #ifdef __cplusplus
 }
 #endif

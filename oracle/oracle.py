@@ -135,6 +135,10 @@ class Oracle:
         lib.orc_ellip_plane.argtypes = [_i, _dp, ip, _i, _dp, _dp, _d] + [_dp] * 5
         lib.orc_ellip_sweeps.restype = _i
         lib.orc_ellip_sweeps.argtypes = [_dp, ip, _i, _d, _dp, _dp, _d] + [_dp] * 5 + [_dp]
+        lp = C.POINTER(C.c_long)
+        lib.orc_ellip_adjoint.restype = _i
+        lib.orc_ellip_adjoint.argtypes = ([_dp, ip, _dp, _dp, _d] + [_dp] * 5 +
+                                          [_i, lp, _dp, _dp, _dp, _d, _i, _dp, C.c_void_p])
         self._ellip_ready = True
         return lib
 
@@ -208,6 +212,35 @@ class Oracle:
         it = self._ellip_proto().orc_ellip_sweeps(_ptr(T), n3, max_iters, tol, _ptr(h),
                                                   _ptr(M), speed, *ptrs, _ptr(md))
         return it, float(md[0])
+
+    def ellip_adjoint(self, T, n, h, rec_idx, rec_res, M=None, speed=1.0, speed_field=None,
+                      theta=None, phi=None, ra=None, rb=None, rel_tol=1e-12, max_passes=100000,
+                      state=False):
+        """FP64 adjoint of the converged field T: returns (dJ/ds, dJ/dtheta or
+        None, Jacobi passes) for J = 1/2 sum rec_res^2 at nodes rec_idx; with
+        `state`, also the adjoint state mu and the held-fixed flags."""
+        n3 = (C.c_int * 3)(*((n,) * 3 if np.isscalar(n) else n))
+        nn = tuple(n3)
+        M = np.ascontiguousarray(np.eye(3)[[0, 1, 2, 0, 0, 1], [0, 1, 2, 1, 2, 2]]
+                                 if M is None else M, dtype=np.float64)
+        h = np.ascontiguousarray((h,) * 3 if np.isscalar(h) else h, dtype=np.float64)
+        T = np.ascontiguousarray(T, dtype=np.float64).ravel()
+        fs = self._fields(nn, speed_field, theta, phi, ra, rb)
+        ptrs = [None if f is None else _ptr(f) for f in fs]
+        idx = np.ascontiguousarray(rec_idx, dtype=np.int64)
+        res = np.ascontiguousarray(rec_res, dtype=np.float64)
+        gs = np.zeros(T.size)
+        gt = np.zeros(T.size) if theta is not None else None
+        mu = np.zeros(T.size) if state else None
+        fixed = np.zeros(T.size, dtype=np.uint8) if state else None
+        passes = self._ellip_proto().orc_ellip_adjoint(
+            _ptr(T), n3, _ptr(h), _ptr(M), speed, *ptrs, int(idx.size),
+            idx.ctypes.data_as(C.POINTER(C.c_long)), _ptr(res), _ptr(gs),
+            None if gt is None else _ptr(gt), rel_tol, max_passes,
+            None if mu is None else _ptr(mu), None if fixed is None else fixed.ctypes.data)
+        if state:
+            return gs, gt, passes, mu, fixed.astype(bool)
+        return gs, gt, passes
 
 
 class RefBuild:

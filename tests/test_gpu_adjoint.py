@@ -1,0 +1,176 @@
+"""Adjoint-state gradients on the GPU (fold3d_adjoint through the C-ABI and
+the torch.autograd harness), against
+  * the FP64 CPU adjoint of the oracle on the same converged field;
+  * central finite differences of the GPU forward solve (FP64, relative
+    steps 1e-3 and 5e-4, Richardson-extrapolated) at 32 seeded cells;
+  * itself under checkpoint budgets and batch sizes, and in FP32."""
+import numpy as np
+import pytest
+
+import fold3d
+from fold3d.tomography import Tomography, adjoint_tol, c4_fields, top_face_survey
+
+pytestmark = pytest.mark.gpu
+
+
+def _fields(n, dtype, anomalies=False):
+    import torch
+    return c4_fields(n, xp=torch, dtype=dtype, device="cuda", anomalies=anomalies)
+
+
+def _tomo(solver, n, dtype, src_side, rec_side, **kw):
+    f = _fields(n, dtype)
+    ft = _fields(n, dtype, anomalies=True)
+    S = top_face_survey(n, src_side=src_side, rec_side=rec_side)
+    tomo = Tomography(solver, S, f["phi"], f["ra"], f["rb"], **kw)
+    tomo.observe(ft["slowness"], ft["theta"])
+    return tomo, f
+
+
+def _host(f):
+    return {k: v.double().cpu().numpy() for k, v in f.items()}
+
+
+def test_adjoint_matches_cpu_fp64(solver, oracle):
+    import torch
+    n = 20
+    tomo, f = _tomo(solver, n, torch.float64, 1, n)
+    S = tomo.survey
+    T = tomo.solve_group([0], f["slowness"], f["theta"])[0]
+    res = (tomo.receiver_times(T) - tomo.T_obs[0]).contiguous()
+    gs = torch.zeros_like(T)
+    gt = torch.zeros_like(T)
+    m = tomo.model((1.0 / f["slowness"]).contiguous(), f["theta"])
+    idx = torch.as_tensor(S.rec_idx, device="cuda")
+    st = solver.adjoint(T, n, S.h, m, idx, res, gs, gt, max_iters=200, tol=adjoint_tol(res))
+    assert 1 <= st.iterations < 200
+    h = _host(f)
+    cs, ct, _ = oracle.ellip_adjoint(T.cpu().numpy(), n, S.h, S.rec_idx.astype(np.int64),
+                                     res.cpu().numpy(), speed_field=1.0 / h["slowness"],
+                                     theta=h["theta"], phi=h["phi"], ra=h["ra"], rb=h["rb"])
+    g_s, g_t = gs.cpu().numpy(), gt.cpu().numpy()
+    assert np.abs(cs).max() > 0 and np.abs(ct).max() > 0
+    # 1e-8: the oracle's local derivatives are central differences (step 1e-6)
+    assert np.abs(g_s - cs).max() <= 1e-8 * np.abs(cs).max()
+    assert np.abs(g_t - ct).max() <= 1e-8 * np.abs(ct).max()
+    assert (np.abs(g_s - cs) <= 1e-8 * np.abs(cs).max()).all()
+
+
+def _seeded_cells(g, srcs, H, count, seed):
+    idx = np.flatnonzero(np.abs(g) > 1e-3 * np.abs(g).max())
+    i, j, k = idx % H, (idx // H) % H, idx // (H * H)
+    far = np.ones(idx.size, bool)
+    for s in srcs:
+        si, sj, sk = s % H, (s // H) % H, s // (H * H)
+        far &= np.maximum(np.maximum(abs(i - si), abs(j - sj)), abs(k - sk)) >= 3
+    return np.random.default_rng(seed).choice(idx[far], count, replace=False)
+
+
+@pytest.mark.parametrize("name", ["slowness", "theta"])
+def test_gradient_vs_finite_differences(solver, name):
+    """32 seeded cells with a non-negligible gradient, 3+ cells from every
+    source.  Central differences at relative step 1e-3 carry an O(step^2)
+    error of order 1e-2 here (the discrete traveltime map is strongly curved
+    at grazing simplices), so the comparison uses the Richardson combination
+    of steps 1e-3 and 5e-4; at least 80 % of the cells must agree to 1e-4
+    relative (a cell whose perturbation moves a foot point across a simplex
+    boundary sees a kink) and every cell to 1e-3 of the largest gradient
+    checked."""
+    import torch
+    n = 32
+    tomo, f = _tomo(solver, n, torch.float64, 2, 16, batch=4)
+    S = tomo.survey
+    s0 = f["slowness"].clone().requires_grad_(True)
+    t0 = f["theta"].clone().requires_grad_(True)
+    J = tomo.misfit(s0, t0)
+    J.backward()
+    g = (s0.grad if name == "slowness" else t0.grad).cpu().numpy()
+    H = n + 2
+    srcs = [min(n, max(1, int(round(c / S.h)) + 1)) for src in S.sources for c in src]
+    srcs = [srcs[3 * q] + H * (srcs[3 * q + 1] + H * srcs[3 * q + 2]) for q in range(len(S.sources))]
+    cells = _seeded_cells(g, srcs, H, 32, seed=20261207)
+
+    def Jof(field, c, d):
+        s, t = f["slowness"].clone(), f["theta"].clone()
+        (s if field == "slowness" else t)[c] += d
+        with torch.no_grad():
+            return float(tomo.forward_pass(s, t)[0])
+
+    rows = []
+    for c in cells:
+        base = float(f[name][c])
+        d = 1e-3 * abs(base)
+        d1 = (Jof(name, c, d) - Jof(name, c, -d)) / (2 * d)
+        d2 = (Jof(name, c, d / 2) - Jof(name, c, -d / 2)) / d
+        rows.append((g[c], d1, (4 * d2 - d1) / 3))
+    R = np.array(rows)
+    rel = np.abs(R[:, 0] - R[:, 2]) / np.abs(R[:, 2])
+    print(f"{name}: raw central 1e-3 median rel {np.median(np.abs(R[:, 0] - R[:, 1]) / np.abs(R[:, 1])):.2e}; "
+          f"Richardson median {np.median(rel):.2e} max {rel.max():.2e}")
+    assert (rel < 1e-4).mean() >= 0.8
+    assert np.abs(R[:, 0] - R[:, 2]).max() <= 1e-3 * np.abs(R[:, 2]).max()
+
+
+def test_checkpoint_budget_and_batching_give_the_same_gradient(solver):
+    import torch
+    n = 24
+    grads = {}
+    for key, kw in (("all", dict(batch=4)), ("none", dict(batch=4, budget_bytes=0)),
+                    ("half", dict(batch=1, budget_bytes=2 * 26 ** 3 * 8))):
+        tomo, f = _tomo(solver, n, torch.float64, 2, 12, **kw)
+        s0 = f["slowness"].clone().requires_grad_(True)
+        t0 = f["theta"].clone().requires_grad_(True)
+        tomo.misfit(s0, t0).backward()
+        grads[key] = (s0.grad.cpu().numpy(), t0.grad.cpu().numpy(), tomo.stats["recomputed"])
+    assert grads["all"][2] == 0 and grads["none"][2] == 4 and grads["half"][2] == 2
+    for q in (0, 1):
+        a, b = grads["all"][q], grads["none"][q]
+        assert np.abs(a - b).max() <= 1e-12 * np.abs(a).max()
+        a, b = grads["all"][q], grads["half"][q]
+        assert np.abs(a - b).max() <= 1e-10 * np.abs(a).max()
+
+
+def test_fp32_gradient_close_to_fp64(solver):
+    import torch
+    n = 32
+    out = {}
+    for dt in (torch.float64, torch.float32):
+        tomo, f = _tomo(solver, n, dt, 2, 16, batch=4)
+        s0 = f["slowness"].clone().requires_grad_(True)
+        t0 = f["theta"].clone().requires_grad_(True)
+        J = tomo.misfit(s0, t0)
+        J.backward()
+        out[dt] = (float(J), s0.grad.double().cpu().numpy(), t0.grad.double().cpu().numpy())
+    J64, s64, t64 = out[torch.float64]
+    J32, s32, t32 = out[torch.float32]
+    assert abs(J32 - J64) <= 1e-2 * J64
+    for a, b in ((s64, s32), (t64, t32)):
+        err = np.linalg.norm(a - b) / np.linalg.norm(a)
+        print("fp32 gradient relative L2 error", err)
+        assert err < 1e-2
+
+
+def test_zero_residual_and_invalid_arguments(solver):
+    import torch
+    n = 16
+    tomo, f = _tomo(solver, n, torch.float64, 1, 8)
+    S = tomo.survey
+    T = tomo.solve_group([0], f["slowness"], f["theta"])[0]
+    m = tomo.model((1.0 / f["slowness"]).contiguous(), f["theta"])
+    idx = torch.as_tensor(S.rec_idx, device="cuda")
+    gs = torch.zeros_like(T)
+    gt = torch.zeros_like(T)
+    solver.adjoint(T, n, S.h, m, idx, torch.zeros(S.nrec, dtype=T.dtype, device="cuda"), gs, gt)
+    assert not gs.any() and not gt.any()
+    bad = idx.clone()
+    bad[3] = 0  # a ghost node
+    with pytest.raises(fold3d.Fold3dError):
+        solver.adjoint(T, n, S.h, m, bad, torch.ones(S.nrec, dtype=T.dtype, device="cuda"), gs, gt)
+    m2 = tomo.model((1.0 / f["slowness"]).contiguous(), f["theta"])
+    m2.theta = m2.phi = m2.ra = m2.rb = None
+    with pytest.raises(fold3d.Fold3dError):
+        solver.adjoint(T, n, S.h, m2, idx, torch.ones(S.nrec, dtype=T.dtype, device="cuda"), gs, gt)
+    with pytest.raises(fold3d.Fold3dError):
+        solver.adjoint(T, n, S.h, fold3d.reference_model(), idx,
+                       torch.ones(S.nrec, dtype=T.dtype, device="cuda"), gs, None)
+    assert not gs.any()

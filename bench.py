@@ -133,6 +133,107 @@ def batch_run(solver, torch, count=96, check=6):
             "fp32_max_rel_err_vs_fp64_on_smallest": {"instances": check, "value": worst}}
 
 
+def c0_run(solver, torch, n=128):
+    """Homogeneous tilted medium, 128^3 FP64 on [0,1]^3, source at the centre
+    (between nodes), to max|dT| < 1e-10; error against sqrt(x^T M^-1 x)/v."""
+    from fold3d import workloads as wl
+    W = wl.tilted_homogeneous(n)
+    T0 = wl.point_source_start(W, xp=torch, dtype=torch.float64, device="cuda")
+    T = T0.clone()
+    solver.solve(T, n, spacing=W.h, model=W.model(), max_iters=200, tol=1e-10)
+    T.copy_(T0)
+    torch.cuda.synchronize()
+    st = solver.solve(T, n, spacing=W.h, model=W.model(), max_iters=200, tol=1e-10)
+    exact = wl.closed_form(W, xp=torch, device="cuda")
+    reached = T < 1e15
+    err = (T - exact).abs()[reached]
+    far = reached & (exact > 0.25)
+    return {"n": n, "dtype": "f64", "iterations": st.iterations, "max_dT": st.max_dT,
+            "sweep_ms": st.sweep_ms, "value": st.cell_updates / (st.sweep_ms / 1e3) / 1e9,
+            "unit": "Gcell-updates/s", "max_abs_err_vs_closed_form": float(err.max()),
+            "mean_abs_err_vs_closed_form": float(err.mean()),
+            "max_rel_err_beyond_r0.25": float(((T - exact).abs()[far] / exact[far]).max())}
+
+
+def c2_run(torch, n=1280):
+    """Per-node tilt and axis-ratio fields with v = 1.5 + 2z + 5 % blobs,
+    FP32, source near the top face, to max|dT| < 1e-6 (metric engine)."""
+    import fold3d
+    from fold3d import workloads as wl
+    solver = fold3d.Solver(0)
+    try:
+        W = wl.tilted_field(n, xp=torch, dtype=torch.float32, device="cuda")
+        T0 = wl.point_source_start(W, xp=torch, dtype=torch.float32, device="cuda")
+        torch.cuda.empty_cache()
+        groups, tune_ms = solver.autotune(T0, n, spacing=W.h, model=W.model(), iterations=1)
+        log(f"bench: C2 autotuner picked {groups} waves per node row ({tune_ms:.0f} ms/iteration)")
+        T = T0
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        st = solver.solve(T, n, spacing=W.h, model=W.model(), max_iters=200, tol=1e-6)
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        out = {"n": n, "dtype": "f32", "iterations": st.iterations, "max_dT": st.max_dT,
+               "sweep_ms": st.sweep_ms, "wall_s": wall, "groups": groups,
+               "value": st.cell_updates / (st.sweep_ms / 1e3) / 1e9, "unit": "Gcell-updates/s",
+               "cell_updates": st.cell_updates,
+               "torch_device_bytes": torch.cuda.max_memory_allocated(),
+               "solver_scratch_bytes": 15 * (n + 2) ** 3 * 4}
+        del W, T0, T
+    finally:
+        solver.close()
+        torch.cuda.empty_cache()
+    return out
+
+
+def adjoint_run(solver, torch, n=256):
+    """Tomography gradient on the 256^3 FP32 model: 16 sources on a 4x4 grid
+    below the top face, 64x64 top-face receivers, observed times from the
+    background plus 3 anomalies of +-5 %.  One gradient evaluation = the 16
+    forward solves (one batch, checkpointed) + 16 adjoint solves."""
+    from fold3d.tomography import Tomography, c4_fields, top_face_survey
+    f = c4_fields(n, xp=torch, dtype=torch.float32, device="cuda")
+    ft = c4_fields(n, xp=torch, dtype=torch.float32, device="cuda", anomalies=True)
+    S = top_face_survey(n, 4, 64)
+    tomo = Tomography(solver, S, f["phi"], f["ra"], f["rb"], batch=16)
+    tomo.observe(ft["slowness"], ft["theta"])       # also the warm-up
+    del ft
+    torch.cuda.synchronize()
+    # looped plain forward solves of the 16 sources
+    t0 = time.perf_counter()
+    for k in range(len(S.sources)):
+        tomo.solve_group([k], f["slowness"], f["theta"])
+    torch.cuda.synchronize()
+    loop_s = time.perf_counter() - t0
+    tomo.stats.clear()
+    s0 = f["slowness"].clone().requires_grad_(True)
+    t0_ = f["theta"].clone().requires_grad_(True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    J = tomo.misfit(s0, t0_)
+    torch.cuda.synchronize()
+    fwd_s = time.perf_counter() - t0
+    J.backward()
+    torch.cuda.synchronize()
+    grad_s = time.perf_counter() - t0
+    st = tomo.stats
+    nsrc = len(S.sources)
+    adj_ms = sum(st["adjoint_ms"])
+    out = {"n": n, "dtype": "f32", "sources": nsrc, "receivers": S.nrec,
+           "misfit": float(J.detach()), "gradient_s": grad_s,
+           "gradient_evals_per_s": 1.0 / grad_s,
+           "forward_batched_s": fwd_s, "forward_solves_per_s": nsrc / fwd_s,
+           "forward_looped_solves_per_s": nsrc / loop_s,
+           "adjoint_device_ms_total": adj_ms, "adjoint_solves_per_s": nsrc / (adj_ms / 1e3),
+           "adjoint_iterations": [min(st["adjoint_iters"]), max(st["adjoint_iters"])],
+           "forward_iterations": [min(st["forward_iters"]), max(st["forward_iters"])],
+           "checkpoint_bytes": st["checkpoint_bytes"], "recomputed": st["recomputed"],
+           "grad_norm_slowness": float(s0.grad.norm()), "grad_norm_theta": float(t0_.grad.norm())}
+    del tomo, f, s0, t0_
+    torch.cuda.empty_cache()
+    return out
+
+
 def cpu_baseline(T0_host, n, threads):
     """The oracle (FP64 OpenMP restatement, bit-identical to the reference)
     on a bounded sample: one of the 8 iterations of the same 512^3 solve."""
@@ -157,6 +258,9 @@ def main():
     ap.add_argument("--no-ellip", action="store_true")
     ap.add_argument("--no-ablation", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--no-c0", action="store_true")
+    ap.add_argument("--c2-n", type=int, default=1280, help="side of the per-node metric run (0: skip)")
     ap.add_argument("--iters", type=int, default=8, help="iterations per solve (the reference runs 8)")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -240,6 +344,12 @@ def main():
             del Te
         del ref64
 
+    c0 = None
+    if not args.no_c0:
+        log("bench: homogeneous tilted medium 128^3 FP64 vs closed form")
+        c0 = c0_run(solver, torch)
+        log(f"bench: C0 {c0['iterations']} iterations, max abs err {c0['max_abs_err_vs_closed_form']:.3e}")
+
     cpu = None
     if not args.no_cpu:
         threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or min(16, os.cpu_count() or 1)
@@ -253,6 +363,21 @@ def main():
     batch = None
     if not args.no_batch:
         batch = batch_run(solver, torch)
+
+    adjoint = None
+    if not args.no_adjoint:
+        log("bench: tomography gradient, 256^3 FP32, 16 sources, 64x64 receivers")
+        adjoint = adjoint_run(solver, torch)
+        log(f"bench: gradient {adjoint['gradient_s']:.2f} s (forward {adjoint['forward_batched_s']:.2f} s, "
+            f"adjoint {adjoint['adjoint_device_ms_total'] / 1e3:.2f} s)")
+
+    c2 = None
+    if args.c2_n:
+        del solver
+        torch.cuda.empty_cache()
+        log(f"bench: per-node metric workload {args.c2_n}^3 FP32")
+        c2 = c2_run(torch, args.c2_n)
+        log(f"bench: C2 {c2['iterations']} iterations, {c2['value']:.2f} Gcell-upd/s")
 
     log("bench: measured ceilings of this GPU (FMA rate, device copy)")
     peaks = fold3d.measure_peaks()
@@ -287,6 +412,9 @@ def main():
         "elliptic_blob_speed": ellip,
         "cpu_baseline": cpu,
         "batched": batch,
+        "adjoint": adjoint,
+        "c0_closed_form": c0,
+        "c2_tilted_field": c2,
         "device_ceilings": valu,
     }
     print(json.dumps(out), flush=True)

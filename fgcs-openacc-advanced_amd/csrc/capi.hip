@@ -228,6 +228,19 @@ int fold3d_solve(fold3d_handle *h, void *T, int nx, int ny, int nz, double dx,
                    st, (float *)dT, (float *)scr, (float *)(scr + bytes),
                    (float *)(scr + 2 * bytes), G, *model, hsp, o.max_iters, o.tol, groups,
                    h->d_max, &iters, &maxdt);
+  } else if (model->theta != nullptr && o.engine != FOLD3D_ENGINE_PLANE &&
+             o.engine != FOLD3D_ENGINE_HOISTED) {
+    // per-node metric: metric engine; scratch = T copy + two sets of metric fields
+    rc = ensure_scratch(h, ellip_metric_scratch_fields() * bytes);
+    if (rc != FOLD3D_OK) return rc;
+    const int groups = h->ellip_groups;
+    rc = o.precision == FOLD3D_F64
+             ? ellip_sweeps_metric_engine<double>(st, (double *)dT, (double *)h->d_scratch, G,
+                                                  *model, hsp, o.max_iters, o.tol, groups,
+                                                  h->d_max, &iters, &maxdt)
+             : ellip_sweeps_metric_engine<float>(st, (float *)dT, (float *)h->d_scratch, G,
+                                                 *model, hsp, o.max_iters, o.tol, groups,
+                                                 h->d_max, &iters, &maxdt);
   } else
     rc = o.precision == FOLD3D_F64
              ? ellip_sweeps_plane_engine<double>(st, (double *)dT, G, *model, hsp, o.max_iters,

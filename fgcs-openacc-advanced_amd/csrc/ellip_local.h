@@ -230,11 +230,10 @@ __device__ __forceinline__ void ellip_unit_lattice(R &best, const R p[9], R s2,
   }
 }
 
-// Whole node with a per-node metric (all eight units; the nine one-point
-// candidates first so the pruning starts low).
+// The nine one-point candidates of a node with a per-node metric.
 template <typename R>
-__device__ __forceinline__ R ellip_node_lattice(R told, const R p[9], R s, const LatticeN<R> &L) {
-  R best = told;
+__device__ __forceinline__ void ellip_vertices_lattice(R &best, const R p[9], R s,
+                                                       const LatticeN<R> &L) {
 #pragma unroll
   for (int k = 0; k < 9; ++k) {
     if (reached(p[k])) {
@@ -242,6 +241,14 @@ __device__ __forceinline__ R ellip_node_lattice(R told, const R p[9], R s, const
       best = rmin(p[k] + s * sqrt(lat_dot(L, -a, -b, 1, -a, -b, 1)), best);
     }
   }
+}
+
+// Whole node with a per-node metric (all eight units; the nine one-point
+// candidates first so the pruning starts low).
+template <typename R>
+__device__ __forceinline__ R ellip_node_lattice(R told, const R p[9], R s, const LatticeN<R> &L) {
+  R best = told;
+  ellip_vertices_lattice<R>(best, p, s, L);
   const R s2 = s * s;
   ellip_unit_lattice<R, 0>(best, p, s2, L);
   ellip_unit_lattice<R, 1>(best, p, s2, L);

@@ -355,6 +355,236 @@ int ellip_sweeps_split_engine(hipStream_t st, R *T, R *Y, R *VX, R *VY, const Gr
   return FOLD3D_OK;
 }
 
+// ---------------------------------------------------------------------------
+// Metric engine: per-node metric (tilt and axis-ratio fields).  The slowness
+// and the six entries of N = M^-1 are computed once per solve into seven
+// fields of the reference layout (metric_prep_kernel) and copied, with T,
+// into the x- and y-sweep layouts before those sweeps; the plane steps are
+// the split kernel's (LDS rows, units over kGroups waves) with the node's
+// lattice products read from those fields.  Same bits as ellip_plane_kernel
+// with kMetric.
+// ---------------------------------------------------------------------------
+template <typename R>
+__global__ void metric_prep_kernel(long long count, const R *__restrict__ V, R speed,
+                                   const R *__restrict__ th, const R *__restrict__ ph,
+                                   const R *__restrict__ ra, const R *__restrict__ rb,
+                                   R *__restrict__ out) {
+  for (long long x = blockIdx.x * (long long)blockDim.x + threadIdx.x; x < count;
+       x += (long long)gridDim.x * blockDim.x) {
+    R N[6];
+    tilted_inverse_metric<R>(th[x], ph[x], ra[x], rb[x], N);
+    out[x] = V ? R(1) / V[x] : R(1) / speed;
+#pragma unroll
+    for (int q = 0; q < 6; ++q) out[(q + 1) * count + x] = N[q];
+  }
+}
+
+// Where each lattice product of one direction comes from: L.X = (sg * N_q) * h1 * h2.
+template <typename R>
+struct LatticeMap {
+  const R *src[6];  // N_q field in the current layout, for A, B, C, E, Fx, Fy
+  R sg[6], h1[6], h2[6];
+};
+
+template <typename R>
+static LatticeMap<R> lattice_map(int d, const R *fields, long long count, const double h[3]) {
+  const Frame &fr = kFrames[d];
+  const int u[3] = {fr.ux, fr.uy, fr.m};
+  const int sgn[3] = {fr.sx, fr.sy, fr.s};
+  const int pairs[6][2] = {{0, 0}, {1, 1}, {2, 2}, {0, 1}, {0, 2}, {1, 2}};
+  LatticeMap<R> M;
+  for (int e = 0; e < 6; ++e) {
+    const int i = pairs[e][0], j = pairs[e][1];
+    const int gi = u[i], gj = u[j];
+    int q;  // symmetric storage (xx, yy, zz, xy, xz, yz)
+    if (gi == gj) q = gi;
+    else q = (gi + gj == 1) ? 3 : (gi + gj == 2) ? 4 : 5;
+    M.src[e] = fields + (1 + q) * count;
+    M.sg[e] = R(sgn[i] * sgn[j]);
+    M.h1[e] = (R)h[gi];
+    M.h2[e] = (R)h[gj];
+  }
+  return M;
+}
+
+template <typename R, int G0, int kStride>
+__device__ __forceinline__ void lattice_units(R &best, const R p[9], R s2, const LatticeN<R> &L) {
+  if constexpr (G0 < 8) {
+    ellip_unit_lattice<R, G0>(best, p, s2, L);
+    lattice_units<R, G0 + kStride, kStride>(best, p, s2, L);
+  }
+}
+
+template <typename R, int kGroups>
+__device__ __forceinline__ void lattice_dispatch_units(int grp, R &best, const R p[9], R s2,
+                                                       const LatticeN<R> &L) {
+  if constexpr (kGroups == 1) {
+    lattice_units<R, 0, 1>(best, p, s2, L);
+  } else if constexpr (kGroups == 2) {
+    if (grp == 0) lattice_units<R, 0, 2>(best, p, s2, L);
+    else lattice_units<R, 1, 2>(best, p, s2, L);
+  } else {
+    static_assert(kGroups == 4, "one, two or four waves per node row");
+    switch (grp) {
+      case 0: lattice_units<R, 0, 4>(best, p, s2, L); break;
+      case 1: lattice_units<R, 1, 4>(best, p, s2, L); break;
+      case 2: lattice_units<R, 2, 4>(best, p, s2, L); break;
+      default: lattice_units<R, 3, 4>(best, p, s2, L); break;
+    }
+  }
+}
+
+template <typename R, bool kReduce, int kGroups>
+__global__ __launch_bounds__(64 * kGroups) void ellip_metric_kernel(
+    R *__restrict__ T, const R *__restrict__ S, LatticeMap<R> Lm, PlaneGeom g,
+    unsigned long long *__restrict__ max_dt) {
+  __shared__ R s[3][64 + 2];
+  __shared__ R part[kGroups][64];
+  const int tx = threadIdx.x;
+  const int grp = __builtin_amdgcn_readfirstlane(threadIdx.y);
+  const int a0 = blockIdx.x * 64, b = 1 + blockIdx.y;
+  for (int idx = grp * 64 + tx; idx < 3 * 66; idx += 64 * kGroups) {
+    const int la = idx % 66, lb = idx / 66;
+    const int sa = a0 + la;
+    if (sa <= g.na + 1) s[lb][la] = T[g.base + (long long)sa * g.Sa + (long long)(b - 1 + lb) * g.Sb];
+  }
+  const int a = a0 + 1 + tx;
+  const bool active = a <= g.na;
+  const long long c = g.base + (long long)a * g.Sa + (long long)b * g.Sb;
+  const long long t = c + g.sm;
+  R told = R(0), sl = R(1);
+  LatticeN<R> L;
+  if (active) {
+    told = T[t];
+    sl = S[t];
+    L.A = (Lm.sg[0] * Lm.src[0][t]) * Lm.h1[0] * Lm.h2[0];
+    L.B = (Lm.sg[1] * Lm.src[1][t]) * Lm.h1[1] * Lm.h2[1];
+    L.C = (Lm.sg[2] * Lm.src[2][t]) * Lm.h1[2] * Lm.h2[2];
+    L.E = (Lm.sg[3] * Lm.src[3][t]) * Lm.h1[3] * Lm.h2[3];
+    L.Fx = (Lm.sg[4] * Lm.src[4][t]) * Lm.h1[4] * Lm.h2[4];
+    L.Fy = (Lm.sg[5] * Lm.src[5][t]) * Lm.h1[5] * Lm.h2[5];
+  }
+  __syncthreads();
+  R best = told;
+  if (active) {
+    const int ca = tx + 1;
+    const int ua = g.ua, ub = g.ub, va = g.va, vb = g.vb;
+    R p[9];
+    p[0] = s[1][ca];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      const int i = kRing[k][0], j = kRing[k][1];
+      p[1 + k] = s[1 + i * ub + j * vb][ca + i * ua + j * va];
+    }
+    ellip_vertices_lattice<R>(best, p, sl, L);
+    lattice_dispatch_units<R, kGroups>(grp, best, p, sl * sl, L);
+  }
+  part[grp][tx] = best;
+  __syncthreads();
+  if (grp == 0) {
+    R d = R(0);
+    if (active) {
+      R m = part[0][tx];
+#pragma unroll
+      for (int k = 1; k < kGroups; ++k) m = rmin(part[k][tx], m);
+      if (m != told) T[t] = m;
+      if (kReduce) d = told - m;
+    }
+    if (kReduce) {
+      double v = (double)d;
+#pragma unroll
+      for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, 64));
+      if (tx == 0 && v > 0.0) atomicMax(max_dt, (unsigned long long)__double_as_longlong(v));
+    }
+  }
+}
+
+template <typename R, int kGroups>
+static void ellip_metric_launch(hipStream_t st, R *T, const R *S, const LatticeMap<R> &Lm,
+                                const PlaneGeom &g, unsigned long long *max_dt) {
+  const dim3 block(64, kGroups);
+  const dim3 grid((g.na + 63) / 64, g.nb);
+  if (max_dt)
+    ellip_metric_kernel<R, true, kGroups><<<grid, block, 0, st>>>(T, S, Lm, g, max_dt);
+  else
+    ellip_metric_kernel<R, false, kGroups><<<grid, block, 0, st>>>(T, S, Lm, g, nullptr);
+}
+
+template <typename R>
+static void ellip_metric_direction(hipStream_t st, R *T, const R *fields, const Grid &G, int d,
+                                   const double h[3], unsigned long long *max_dt, int groups) {
+  const Frame &fr = kFrames[d];
+  const int n = G.n[fr.m];
+  const LatticeMap<R> Lm = lattice_map<R>(d, fields, G.count, h);
+  for (int q = 0; q < n - 1; ++q) {
+    const int p = fr.s > 0 ? 1 + q : n - q;
+    const PlaneGeom g = plane_geom(G, d, p);
+    if (groups == 1) ellip_metric_launch<R, 1>(st, T, fields, Lm, g, max_dt);
+    else if (groups == 2) ellip_metric_launch<R, 2>(st, T, fields, Lm, g, max_dt);
+    else ellip_metric_launch<R, 4>(st, T, fields, Lm, g, max_dt);
+  }
+}
+
+size_t ellip_metric_scratch_fields() { return 1 + 2 * kMetricFields; }
+
+template <typename R>
+int ellip_sweeps_metric_engine(hipStream_t st, R *T, R *scratch, const Grid &G,
+                               const fold3d_model &m, const double h[3], int max_iters,
+                               double tol, int groups, unsigned long long *d_max,
+                               int *iters_out, double *max_dt_out) {
+  const long long count = G.count;
+  R *Y = scratch;                        // T in the x- or y-sweep layout
+  R *MZ = scratch + count;               // metric fields, reference layout
+  R *MX = MZ + kMetricFields * count;    // metric fields, x- or y-sweep layout
+  metric_prep_kernel<R><<<2048, 256, 0, st>>>(count, (const R *)m.speed_field, (R)m.speed,
+                                              (const R *)m.theta, (const R *)m.phi,
+                                              (const R *)m.ra, (const R *)m.rb, MZ);
+  F3D_HIP(hipGetLastError());
+  const Grid GX = G.with_order(kOrderX), GZ = G.with_order(kOrderZ), GY = G.with_order(kOrderY);
+  auto permute_metric = [&](const Grid &Gout) {
+    for (int q = 0; q < kMetricFields; ++q)
+      permute_layout<R>(st, MZ + q * count, GZ, MX + q * count, Gout);
+  };
+  int it = 0;
+  double last = -1.0;
+  for (; it < max_iters;) {
+    unsigned long long *md = tol > 0 ? d_max : nullptr;
+    if (md) ellip_zero_u64<<<1, 1, 0, st>>>(d_max);
+    permute_layout<R>(st, T, GZ, Y, GX);
+    permute_metric(GX);
+    ellip_metric_direction<R>(st, Y, MX, GX, FOLD3D_X_DOWN, h, md, groups);
+    ellip_metric_direction<R>(st, Y, MX, GX, FOLD3D_X_UP, h, md, groups);
+    permute_layout<R>(st, Y, GX, T, GZ);
+    ellip_metric_direction<R>(st, T, MZ, GZ, FOLD3D_K_UP, h, md, groups);
+    ellip_metric_direction<R>(st, T, MZ, GZ, FOLD3D_K_DOWN, h, md, groups);
+    permute_layout<R>(st, T, GZ, Y, GY);
+    permute_metric(GY);
+    ellip_metric_direction<R>(st, Y, MX, GY, FOLD3D_Y_UP, h, md, groups);
+    ellip_metric_direction<R>(st, Y, MX, GY, FOLD3D_Y_DOWN, h, md, groups);
+    permute_layout<R>(st, Y, GY, T, GZ);
+    F3D_HIP(hipGetLastError());
+    ++it;
+    if (md) {
+      unsigned long long bits = 0;
+      F3D_HIP(hipMemcpyAsync(&bits, d_max, sizeof bits, hipMemcpyDeviceToHost, st));
+      F3D_HIP(hipStreamSynchronize(st));
+      memcpy(&last, &bits, sizeof last);
+      if (last < tol) break;
+    }
+  }
+  *iters_out = it;
+  *max_dt_out = last;
+  return FOLD3D_OK;
+}
+
+template int ellip_sweeps_metric_engine<double>(hipStream_t, double *, double *, const Grid &,
+                                                const fold3d_model &, const double *, int,
+                                                double, int, unsigned long long *, int *,
+                                                double *);
+template int ellip_sweeps_metric_engine<float>(hipStream_t, float *, float *, const Grid &,
+                                               const fold3d_model &, const double *, int, double,
+                                               int, unsigned long long *, int *, double *);
+
 template int ellip_sweeps_split_engine<double>(hipStream_t, double *, double *, double *,
                                                double *, const Grid &, const fold3d_model &,
                                                const double *, int, double, int,

@@ -42,6 +42,18 @@ int ellip_sweeps_split_engine(hipStream_t st, R *T, R *Y, R *VX, R *VY, const Gr
                               double tol, int groups, unsigned long long *d_max,
                               int *iters_out, double *max_dt_out);
 
+// Per-node metric (tilt and axis-ratio fields): slowness and N = M^-1 hoisted
+// into kMetricFields fields once per solve, layouts + LDS rows + units over
+// `groups` (1, 2, 4) waves.  `scratch` holds ellip_metric_scratch_fields()
+// fields of G.count elements.
+constexpr int kMetricFields = 7;
+size_t ellip_metric_scratch_fields();
+template <typename R>
+int ellip_sweeps_metric_engine(hipStream_t st, R *T, R *scratch, const Grid &G,
+                               const fold3d_model &m, const double h[3], int max_iters,
+                               double tol, int groups, unsigned long long *d_max,
+                               int *iters_out, double *max_dt_out);
+
 void sym3_inverse(const double M[6], double N[6]);
 
 template <typename R>

@@ -56,19 +56,26 @@ def top_face_survey(n: int, src_side: int = 4, rec_side: int = 64) -> Survey:
 
 def c4_fields(n: int = 256, seed: int = 20261206, xp=np, dtype=np.float32, device=None,
               anomalies: bool = False):
-    """Fields of the tomography workload on the padded grid: slowness of the
-    blob-speed background v = 2 + sum_8 A_b exp(-|x-c_b|^2/(2*0.1^2)), A_b ~ U(+-.3)
-    (with `anomalies`, times 3 seeded Gaussian anomalies of +-5 %, width 0.08),
-    and the tilt fields theta = 30 deg, phi = 45 deg, ra = 0.8, rb = 0.6 with
-    theta varying by +-10 deg over 4 seeded Gaussian modes.
+    """Fields of the tomography workload on the padded grid.
+
+    Speed: v = 1 + 6 d + sum_8 A_b exp(-|x-c_b|^2/(2*0.1^2)), A_b ~ U(+-.3),
+    c_b ~ U[0,1]^3, with d = 1 - z the depth below the top face (the vertical
+    gradient turns the first arrivals between top-face points back up through
+    the volume, down to d ~ 0.35 at unit offset; without it they run along
+    the face).  With `anomalies`, v is multiplied by 1 + 3 seeded Gaussian
+    anomalies of +-5 %, width 0.08, centred at x, y ~ U(.25, .75), d ~ U(.1, .3).
+    Tilt: theta = 30 deg varying by up to +-10 deg over 4 seeded Gaussian
+    modes, phi = 45 deg, ra = 0.8, rb = 0.6.
     Returns dict(slowness, theta, phi, ra, rb)."""
     rng = np.random.default_rng(seed)
     h = 1.0 / (n - 1)
     amps = rng.uniform(-0.3, 0.3, 8)
     centres = rng.uniform(0.0, 1.0, (8, 3))
-    v = 2.0 + _gauss_sum(xp, n, h, centres, [0.1] * 8, amps, dtype, device)
+    z = _axis(xp, n, dtype, device) * h
+    v = (1.0 + 6.0 * (1.0 - z))[:, None, None] + _gauss_sum(xp, n, h, centres, [0.1] * 8, amps,
+                                                             dtype, device)
     an_amp = rng.choice([-0.05, 0.05], 3)
-    an_c = rng.uniform(0.25, 0.75, (3, 3))
+    an_c = np.column_stack([rng.uniform(0.25, 0.75, (3, 2)), 1.0 - rng.uniform(0.1, 0.3, 3)])
     if anomalies:
         v = v * (1.0 + _gauss_sum(xp, n, h, an_c, [0.08] * 3, an_amp, dtype, device))
     th_amp = rng.uniform(-1.0, 1.0, 4)
@@ -221,13 +228,14 @@ class Tomography:
         nsrc = len(S.sources)
         missing = [k for k in range(nsrc) if ckpt[k] is None]
         fields = dict((k, ckpt[k]) for k in range(nsrc) if ckpt[k] is not None)
-        adj_iters = []
+        adj_iters, adj_ms = [], []
         order = [k for k in range(nsrc) if ckpt[k] is not None]
         for k in order:
             r = (res[k] * scale).contiguous()
             st = self.solver.adjoint(fields[k], S.n, S.h, m, idx, r, gs, gt,
                                      max_iters=self.adj_iters, tol=adjoint_tol(r))
             adj_iters.append(st.iterations)
+            adj_ms.append(st.sweep_ms)
         for g0 in range(0, len(missing), self.batch):
             ks = missing[g0:g0 + self.batch]
             Ts = self.solve_group(ks, slowness, theta)
@@ -236,8 +244,10 @@ class Tomography:
                 st = self.solver.adjoint(T, S.n, S.h, m, idx, r, gs, gt,
                                          max_iters=self.adj_iters, tol=adjoint_tol(r))
                 adj_iters.append(st.iterations)
+                adj_ms.append(st.sweep_ms)
             del Ts
         self.stats["adjoint_iters"] = adj_iters
+        self.stats["adjoint_ms"] = adj_ms
         return gs, gt
 
     def misfit(self, slowness, theta):

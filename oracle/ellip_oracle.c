@@ -387,6 +387,33 @@ static double eval_candidate(const double *T, const int n[3], const double h[3],
   }
 }
 
+/* Minimum over every candidate of the six patches of node X (the node's local
+ * solver without its current value), with the winning candidate. */
+static double node_min(const double *T, const int n[3], const double h[3], long X,
+                       const double N[6], double s, int *dir, int *type, int *id)
+{
+  const long H0 = n[0] + 2, H1 = n[1] + 2;
+  const long ijk[3] = {X % H0, (X / H0) % H1, X / (H0 * H1)};
+  double best = INFINITY;
+  int d, t, k;
+  *type = 0;
+  for (d = 0; d < 6; d++) {
+    const long cm = ijk[FR[d][0]] - FR[d][1];
+    if (cm < 1 || cm > n[FR[d][0]]) continue;
+    for (t = 1; t <= 3; t++) {
+      const int nid = t == 1 ? 9 : t == 2 ? 16 : 8;
+      for (k = 0; k < nid; k++) {
+        const double val = eval_candidate(T, n, h, X, d, t, k, N, s, NULL, NULL);
+        if (val < best) {
+          best = val;
+          *dir = d; *type = t; *id = k;
+        }
+      }
+    }
+  }
+  return best;
+}
+
 int orc_ellip_adjoint(const double *T, const int n[3], const double h[3], const double M6[6],
                       double speed, const double *speed_field, const double *theta,
                       const double *phi, const double *ra, const double *rb, int nrec,
@@ -411,45 +438,30 @@ int orc_ellip_adjoint(const double *T, const int n[3], const double h[3], const 
 #pragma omp parallel for schedule(dynamic, 64)
   for (i = 0; i < count; i++) {
     const long x = i % H0, y = (i / H0) % H1, z = i / (H0 * H1);
-    const long ijk[3] = {x, y, z};
     S[i].type = 0;
     if (x < 1 || x > n[0] || y < 1 || y > n[1] || z < 1 || z > n[2]) continue;
     if (!(T[i] < BIG)) continue;
     const double s = 1.0 / (speed_field ? speed_field[i] : speed);
     double N[6];
     node_metric(M6, theta, phi, ra, rb, i, 0.0, N);
-    double best = INFINITY;
-    int d, type, id;
-    for (d = 0; d < 6; d++) {
-      const long cm = ijk[FR[d][0]] - FR[d][1];
-      if (cm < 1 || cm > n[FR[d][0]]) continue;
-      for (type = 1; type <= 3; type++) {
-        const int nid = type == 1 ? 9 : type == 2 ? 16 : 8;
-        for (id = 0; id < nid; id++) {
-          const double val = eval_candidate(T, n, h, i, d, type, id, N, s, NULL, NULL);
-          if (val < best) {
-            best = val;
-            S[i].dir = d; S[i].type = type; S[i].id = id;
-          }
-        }
-      }
-    }
+    const double best = node_min(T, n, h, i, N, s, &S[i].dir, &S[i].type, &S[i].id);
     if (!(best <= T[i] * (1.0 + rel_tol))) { S[i].type = 0; continue; }
     S[i].val = best;
     eval_candidate(T, n, h, i, S[i].dir, S[i].type, S[i].id, N, s, S[i].v, S[i].w);
-    /* local derivatives by central differences of the winning candidate */
+    /* local derivatives: central differences of the node's whole local
+     * solver (the minimum over every candidate of the six patches), which is
+     * smooth across the boundary between a simplex and its faces */
+    int dd, tt, ii;
     const double es = 1e-6 * s;
-    dTds[i] = (eval_candidate(T, n, h, i, S[i].dir, S[i].type, S[i].id, N, s + es, NULL, NULL) -
-               eval_candidate(T, n, h, i, S[i].dir, S[i].type, S[i].id, N, s - es, NULL, NULL)) /
-              (2 * es);
+    dTds[i] = (node_min(T, n, h, i, N, s + es, &dd, &tt, &ii) -
+               node_min(T, n, h, i, N, s - es, &dd, &tt, &ii)) / (2 * es);
     if (theta) {
       const double et = 1e-6;
       double Np[6], Nm[6];
       node_metric(M6, theta, phi, ra, rb, i, et, Np);
       node_metric(M6, theta, phi, ra, rb, i, -et, Nm);
-      dTdth[i] = (eval_candidate(T, n, h, i, S[i].dir, S[i].type, S[i].id, Np, s, NULL, NULL) -
-                  eval_candidate(T, n, h, i, S[i].dir, S[i].type, S[i].id, Nm, s, NULL, NULL)) /
-                 (2 * et);
+      dTdth[i] = (node_min(T, n, h, i, Np, s, &dd, &tt, &ii) -
+                  node_min(T, n, h, i, Nm, s, &dd, &tt, &ii)) / (2 * et);
     }
   }
   (void)stride;

@@ -1,16 +1,24 @@
 // batch.hip — many independent elliptic-model solves of uneven size in one
 // stream of launches.
 //
-// One launch per (sweep direction d, plane step q) serves every instance that
-// still has a plane q in direction d; instances are sorted by their plane count
-// along d, so those are a prefix of the sorted list.  A launch runs a fixed
-// grid of persistent blocks (one wave each) that pull work items — one row of
-// 64 target nodes of one instance — from a per-launch device counter until the
+// Once per batch the slowness and the inverse metric N = M^-1 of every
+// instance are written to seven fields of its own (tilt fields or a constant
+// metric alike).  An iteration then runs, for all instances together, the
+// single-instance metric engine's schedule: move T and the metric fields into
+// the x-sweep layout (one launch per field set for the whole batch), run the
+// x-sweeps, move T back, run the z-sweeps in the reference layout, and the
+// same for y.  One launch per (sweep direction d, plane step q) serves every
+// instance that still has a plane q in direction d; instances are sorted by
+// their plane count along d, so those are a prefix of the sorted list.  A
+// launch runs a fixed grid of persistent blocks (`groups` waves each, which
+// share the candidate units of a row) that pull work items — one row of 64
+// target nodes of one instance — from a per-launch device counter until the
 // launch's items run out.  The launches of one iteration (plus the counter and
 // maximum resets) are captured once in a hipGraph and replayed; between
 // replays the host reads each instance's max|dT| and retires converged
 // instances through a device flag, so they cost only a flag test afterwards.
-// Plane order and node update are those of the single-instance solve.
+// Plane order and node update are those of the single-instance metric engine
+// (same bits for per-node fields).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -25,76 +33,156 @@
 
 namespace fold3d {
 
-// Device view of one instance (layout Z, the reference layout).
+// Layouts of a batch: every instance's fields move between the reference
+// layout Z and the layout of the sweep pair being run (X or Y), as in the
+// single-instance engines.
+enum BatchLayout { kLayZ = 0, kLayX = 1, kLayY = 2 };
+
+// Device view of one instance.  Scratch per instance: Y (T in layout X or Y)
+// and two sets of kMetricFields fields (slowness and N = M^-1): MZ in the
+// reference layout, MX in the layout of the current sweep pair.
 template <typename R>
 struct BatchInst {
-  R *T;
-  const R *V;                      // speed field or null
-  const R *th, *ph, *ra, *rb;      // per-node metric or null
-  long long stride[3];
+  R *T, *Y, *MZ, *MX;
+  long long count;
+  long long stride[3][3];  // [layout][axis]
   int n[3];
   R h[3];
-  R speed;
-  R Nc[6];                         // constant inverse metric (no per-node fields)
 };
 
-// Per (direction): instances sorted by plane count, prefix sums of their rows.
+// Per (direction): instances sorted by plane count, prefix sums of their rows
+// and, for every row of one plane step, the sorted position of its instance.
 struct BatchDir {
-  const int *order;       // [count]
+  const int *order;         // [count]
   const long long *prefix;  // [count + 1], rows of order[0..k)
+  const int *owner;         // [prefix[count]]
 };
 
-template <typename R>
-__global__ __launch_bounds__(64) void batch_plane_kernel(
-    const BatchInst<R> *__restrict__ inst, BatchDir bd, int dir, int q, long long total,
-    int active, unsigned int *__restrict__ counter, const int *__restrict__ done,
-    unsigned long long *__restrict__ max_dt) {
+// Lattice products of direction d from the metric fields (as lattice_map in
+// the single-instance engine): L.X = (sg * N_q) * h_i * h_j.
+struct LatticeIdx {
+  int q[6];      // index of N_q (0..5) for A, B, C, E, Fx, Fy
+  int ai[6], aj[6];  // global axes of the spacings
+  int sg[6];
+};
+
+static LatticeIdx lattice_idx(int d) {
+  const Frame &fr = kFrames[d];
+  const int u[3] = {fr.ux, fr.uy, fr.m};
+  const int sgn[3] = {fr.sx, fr.sy, fr.s};
+  const int pairs[6][2] = {{0, 0}, {1, 1}, {2, 2}, {0, 1}, {0, 2}, {1, 2}};
+  LatticeIdx L;
+  for (int e = 0; e < 6; ++e) {
+    const int gi = u[pairs[e][0]], gj = u[pairs[e][1]];
+    L.q[e] = gi == gj ? gi : (gi + gj == 1) ? 3 : (gi + gj == 2) ? 4 : 5;
+    L.ai[e] = gi;
+    L.aj[e] = gj;
+    L.sg[e] = sgn[pairs[e][0]] * sgn[pairs[e][1]];
+  }
+  return L;
+}
+
+template <typename R, int G0, int kStride>
+__device__ __forceinline__ void batch_units(R &best, const R p[9], R s2, const LatticeN<R> &L) {
+  if constexpr (G0 < 8) {
+    ellip_unit_lattice<R, G0>(best, p, s2, L);
+    batch_units<R, G0 + kStride, kStride>(best, p, s2, L);
+  }
+}
+
+template <typename R, int kGroups>
+__device__ __forceinline__ void batch_dispatch_units(int grp, R &best, const R p[9], R s2,
+                                                     const LatticeN<R> &L) {
+  if constexpr (kGroups == 1) {
+    batch_units<R, 0, 1>(best, p, s2, L);
+  } else if constexpr (kGroups == 2) {
+    if (grp == 0) batch_units<R, 0, 2>(best, p, s2, L);
+    else batch_units<R, 1, 2>(best, p, s2, L);
+  } else {
+    static_assert(kGroups == 4, "one, two or four waves per node row");
+    switch (grp) {
+      case 0: batch_units<R, 0, 4>(best, p, s2, L); break;
+      case 1: batch_units<R, 1, 4>(best, p, s2, L); break;
+      case 2: batch_units<R, 2, 4>(best, p, s2, L); break;
+      default: batch_units<R, 3, 4>(best, p, s2, L); break;
+    }
+  }
+}
+
+// One plane step q of direction `dir` for every instance that has it:
+// persistent blocks of kGroups waves pull rows of 64 target nodes from the
+// launch's counter, `chunk` consecutive rows per pull.
+template <typename R, int kGroups>
+__global__ __launch_bounds__(64 * kGroups) void batch_plane_kernel(
+    const BatchInst<R> *__restrict__ inst, BatchDir bd, int dir, int lay, LatticeIdx LI, int q,
+    long long total, int chunk, unsigned int *__restrict__ counter,
+    const int *__restrict__ done, unsigned long long *__restrict__ max_dt) {
   __shared__ R s[3][64 + 2];
+  __shared__ R part[kGroups][64];
   __shared__ long long item_s;
   const Frame fr = kFrames[dir];
   const int tx = threadIdx.x;
+  const int grp = __builtin_amdgcn_readfirstlane(threadIdx.y);
+  long long item = 0, end = 0;
   for (;;) {
-    if (tx == 0) item_s = (long long)atomicAdd(counter, 1u);
-    __syncthreads();
-    const long long item = item_s;
-    __syncthreads();
-    if (item >= total) break;
-    // instance: largest k with prefix[k] <= item (k < active)
-    int lo = 0, hi = active - 1;
-    while (lo < hi) {
-      const int mid = (lo + hi + 1) >> 1;
-      if (bd.prefix[mid] <= item) lo = mid;
-      else hi = mid - 1;
+    if (item == end) {
+      if (tx == 0 && grp == 0) item_s = (long long)atomicAdd(counter, (unsigned)chunk);
+      __syncthreads();
+      item = item_s;
+      __syncthreads();
+      if (item >= total) break;
+      end = item + chunk < total ? item + chunk : total;
     }
+    const long long cur = item++;
+    const int lo = bd.owner[cur];
     const int i = bd.order[lo];
-    if (done[i]) continue;
+    if (done[i]) continue;  // uniform over the block
     const BatchInst<R> &I = inst[i];
-    // plane geometry in the reference layout (lanes along the smaller stride)
+    const long long *st = I.stride[lay];
+    // plane geometry (lanes along the in-plane axis of smaller stride)
     const int u = fr.ux, v = fr.uy;
-    const bool u_first = I.stride[u] < I.stride[v];
+    const bool u_first = st[u] < st[v];
     const int la = u_first ? u : v, lb = u_first ? v : u;
-    const int na = I.n[la], nb = I.n[lb];
+    const int na = I.n[la];
     const int nm = I.n[fr.m];
     const int p = fr.s > 0 ? 1 + q : nm - q;
-    const long long base = (long long)p * I.stride[fr.m];
-    const long long Sa = I.stride[la], Sb = I.stride[lb];
-    const long long sm = fr.s * I.stride[fr.m];
+    const long long base = (long long)p * st[fr.m];
+    const long long Sa = st[la], Sb = st[lb];
+    const long long sm = fr.s * st[fr.m];
     const int ua = fr.ux == la ? fr.sx : 0, ub = fr.ux == la ? 0 : fr.sx;
     const int va = fr.uy == la ? fr.sy : 0, vb = fr.uy == la ? 0 : fr.sy;
-    const long long r = item - bd.prefix[lo];
+    const long long r = cur - bd.prefix[lo];
     const int tiles_a = (na + 63) / 64;
     const int a0 = (int)(r % tiles_a) * 64, b = 1 + (int)(r / tiles_a);
-    R *T = I.T;
-    for (int idx = tx; idx < 3 * 66; idx += 64) {
+    R *T = lay == kLayZ ? I.T : I.Y;
+    const R *M = lay == kLayZ ? I.MZ : I.MX;
+    for (int idx = grp * 64 + tx; idx < 3 * 66; idx += 64 * kGroups) {
       const int l_a = idx % 66, l_b = idx / 66;
       const int sa = a0 + l_a;
       if (sa <= na + 1) s[l_b][l_a] = T[base + (long long)sa * Sa + (long long)(b - 1 + l_b) * Sb];
     }
-    __syncthreads();
     const int a = a0 + 1 + tx;
-    R d = R(0);
-    if (a <= na) {
-      const long long t = base + (long long)a * Sa + (long long)b * Sb + sm;
+    const bool act = a <= na;
+    const long long t = base + (long long)a * Sa + (long long)b * Sb + sm;
+    R told = R(0), sl = R(1);
+    LatticeN<R> L;
+    if (act) {
+      told = T[t];
+      sl = M[t];
+      R Lv[6];
+#pragma unroll
+      for (int e = 0; e < 6; ++e)
+        Lv[e] = (R(LI.sg[e]) * M[(1 + LI.q[e]) * I.count + t]) * I.h[LI.ai[e]] * I.h[LI.aj[e]];
+      L.A = Lv[0];
+      L.B = Lv[1];
+      L.C = Lv[2];
+      L.E = Lv[3];
+      L.Fx = Lv[4];
+      L.Fy = Lv[5];
+    }
+    __syncthreads();
+    R best = told;
+    if (act) {
       const int ca = tx + 1;
       R pv[9];
       pv[0] = s[1][ca];
@@ -103,28 +191,99 @@ __global__ __launch_bounds__(64) void batch_plane_kernel(
         const int ii = kRing[k][0], jj = kRing[k][1];
         pv[1 + k] = s[1 + ii * ub + jj * vb][ca + ii * ua + jj * va];
       }
-      const R told = T[t];
-      const R sl = I.V ? R(1) / I.V[t] : R(1) / I.speed;
-      const int uu[3] = {fr.ux, fr.uy, fr.m};
-      const R sg[3] = {R(fr.sx), R(fr.sy), R(fr.s)};
-      const R hl[3] = {I.h[fr.ux], I.h[fr.uy], I.h[fr.m]};
-      R N[6];
-      if (I.th) {
-        tilted_inverse_metric<R>(I.th[t], I.ph[t], I.ra[t], I.rb[t], N);
-      } else {
-#pragma unroll
-        for (int k = 0; k < 6; ++k) N[k] = I.Nc[k];
-      }
-      const LatticeN<R> L = lattice_of<R>(N, uu, sg, hl);
-      const R tn = ellip_node_lattice<R>(told, pv, sl, L);
-      if (tn != told) T[t] = tn;
-      d = told - tn;
+      ellip_vertices_lattice<R>(best, pv, sl, L);
+      batch_dispatch_units<R, kGroups>(grp, best, pv, sl * sl, L);
     }
-    double vmax = (double)d;
+    part[grp][tx] = best;
+    __syncthreads();
+    if (grp == 0) {
+      R d = R(0);
+      if (act) {
+        R mn = part[0][tx];
 #pragma unroll
-    for (int off = 32; off > 0; off >>= 1) vmax = fmax(vmax, __shfl_xor(vmax, off, 64));
-    if (tx == 0 && vmax > 0.0) atomicMax(&max_dt[i], (unsigned long long)__double_as_longlong(vmax));
-    __syncthreads();  // s[][] is reused by the next item
+        for (int k = 1; k < kGroups; ++k) mn = rmin(part[k][tx], mn);
+        if (mn != told) T[t] = mn;
+        d = told - mn;
+      }
+      double vmax = (double)d;
+#pragma unroll
+      for (int off = 32; off > 0; off >>= 1) vmax = fmax(vmax, __shfl_xor(vmax, off, 64));
+      if (tx == 0 && vmax > 0.0)
+        atomicMax(&max_dt[i], (unsigned long long)__double_as_longlong(vmax));
+    }
+    __syncthreads();  // s[][] and part[][] are reused by the next item
+  }
+}
+
+// Moves `fields` consecutive fields of every active instance between two
+// layouts: block = one 32x32 tile of one (x, y) plane of one instance.
+constexpr int kBTile = 32, kBTileRows = 8;
+
+template <typename R>
+struct PermuteJob {
+  int which;          // 0: T -> Y, 1: Y -> T, 2: MZ -> MX
+  int lin, lout;      // layouts
+  int fields;
+};
+
+template <typename R>
+__global__ __launch_bounds__(kBTile *kBTileRows) void batch_permute_kernel(
+    const BatchInst<R> *__restrict__ inst, const long long *__restrict__ tiles_prefix, int count,
+    PermuteJob<R> job, const int *__restrict__ done) {
+  __shared__ R tile[kBTile][kBTile + 1];  // [y - y0][x - x0]
+  const long long item = blockIdx.x;
+  int lo = 0, hi = count - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (tiles_prefix[mid] <= item) lo = mid;
+    else hi = mid - 1;
+  }
+  if (done[lo]) return;
+  const BatchInst<R> &I = inst[lo];
+  const int Hx = I.n[0] + 2, Hy = I.n[1] + 2;
+  const int tx_n = (Hx + kBTile - 1) / kBTile, ty_n = (Hy + kBTile - 1) / kBTile;
+  long long r = item - tiles_prefix[lo];
+  const int bx = (int)(r % tx_n);
+  r /= tx_n;
+  const int by = (int)(r % ty_n);
+  const long long z = r / ty_n;
+  const long long *is = I.stride[job.lin], *os = I.stride[job.lout];
+  const bool in_x = is[0] == 1, out_x = os[0] == 1;
+  const R *src = job.which == 0 ? I.T : job.which == 1 ? I.Y : I.MZ;
+  R *dst = job.which == 0 ? I.Y : job.which == 1 ? I.T : I.MX;
+  const int x0 = bx * kBTile, y0 = by * kBTile;
+  const int tx = threadIdx.x;
+  for (int f = 0; f < job.fields; ++f) {
+    const R *in = src + f * I.count + z * is[2];
+    R *out = dst + f * I.count + z * os[2];
+#pragma unroll
+    for (int rr = threadIdx.y; rr < kBTile; rr += kBTileRows) {
+      const int x = x0 + (in_x ? tx : rr), y = y0 + (in_x ? rr : tx);
+      if (x < Hx && y < Hy) tile[y - y0][x - x0] = in[x * is[0] + y * is[1]];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int rr = threadIdx.y; rr < kBTile; rr += kBTileRows) {
+      const int x = x0 + (out_x ? tx : rr), y = y0 + (out_x ? rr : tx);
+      if (x < Hx && y < Hy) out[x * os[0] + y * os[1]] = tile[y - y0][x - x0];
+    }
+    __syncthreads();
+  }
+}
+
+// Slowness and N = M^-1 of one instance in the reference layout.
+template <typename R>
+__global__ void batch_prep_kernel(long long count, const R *__restrict__ V, R speed,
+                                  const R *__restrict__ th, const R *__restrict__ ph,
+                                  const R *__restrict__ ra, const R *__restrict__ rb, R Nc0,
+                                  R Nc1, R Nc2, R Nc3, R Nc4, R Nc5, R *__restrict__ out) {
+  for (long long x = blockIdx.x * (long long)blockDim.x + threadIdx.x; x < count;
+       x += (long long)gridDim.x * blockDim.x) {
+    R N[6] = {Nc0, Nc1, Nc2, Nc3, Nc4, Nc5};
+    if (th) tilted_inverse_metric<R>(th[x], ph[x], ra[x], rb[x], N);
+    out[x] = V ? R(1) / V[x] : R(1) / speed;
+#pragma unroll
+    for (int k = 0; k < 6; ++k) out[(k + 1) * count + x] = N[k];
   }
 }
 
@@ -136,42 +295,32 @@ __global__ void batch_reset_kernel(unsigned int *counters, int nc, unsigned long
   }
 }
 
-template <typename R>
-static BatchInst<R> make_inst(const fold3d_instance &x) {
-  BatchInst<R> b;
-  memset(&b, 0, sizeof b);
-  b.T = (R *)x.T;
-  b.V = (const R *)x.speed_field;
-  b.th = (const R *)x.theta;
-  b.ph = (const R *)x.phi;
-  b.ra = (const R *)x.ra;
-  b.rb = (const R *)x.rb;
-  const Grid G(x.n[0], x.n[1], x.n[2]);
-  for (int a = 0; a < 3; ++a) {
-    b.stride[a] = G.stride[a];
-    b.n[a] = x.n[a];
-    b.h[a] = (R)x.h[a];
-  }
-  b.speed = (R)x.speed;
-  double N[6];
-  sym3_inverse(x.M, N);
-  for (int k = 0; k < 6; ++k) b.Nc[k] = (R)N[k];
-  return b;
-}
-
-// rows of 64 target nodes in one plane of direction d of instance x
+// rows of 64 target nodes in one plane of direction d of instance x, in the
+// layout that direction runs in
 static long long rows_of(const fold3d_instance &x, int d) {
   const Frame &fr = kFrames[d];
-  const Grid G(x.n[0], x.n[1], x.n[2]);
+  const int *order = fr.m == 0 ? kOrderX : fr.m == 2 ? kOrderZ : kOrderY;
+  const Grid G(x.n[0], x.n[1], x.n[2], order);
   const bool u_first = G.stride[fr.ux] < G.stride[fr.uy];
   const int la = u_first ? fr.ux : fr.uy, lb = u_first ? fr.uy : fr.ux;
   return (long long)((x.n[la] + 63) / 64) * x.n[lb];
 }
 
+size_t batch_scratch_bytes(int count, const fold3d_instance *inst, int precision) {
+  const size_t r = precision == FOLD3D_F64 ? 8 : 4;
+  size_t total = 0;
+  for (int i = 0; i < count; ++i) {
+    const Grid G(inst[i].n[0], inst[i].n[1], inst[i].n[2]);
+    total += (size_t)G.count * r * (1 + 2 * kMetricFields);
+    total = (total + 255) & ~(size_t)255;
+  }
+  return total;
+}
+
 template <typename R>
 int batch_solve(hipStream_t user_st, int count, const fold3d_instance *inst, int max_iters,
-                double tol, int blocks, int use_graph, int *iters_out, double *max_dt_out,
-                double *ms_out) {
+                double tol, int blocks, int groups, int use_graph, void *scratch,
+                int *iters_out, double *max_dt_out, double *ms_out) {
   int dev = 0;
   F3D_HIP(hipGetDevice(&dev));
   // The batch runs on a stream of its own (the legacy default stream cannot be
@@ -185,10 +334,39 @@ int batch_solve(hipStream_t user_st, int count, const fold3d_instance *inst, int
   bool capturing = false;
   // host tables
   std::vector<BatchInst<R>> hinst(count);
-  for (int i = 0; i < count; ++i) hinst[i] = make_inst<R>(inst[i]);
+  std::vector<long long> tiles_prefix(count + 1, 0);
+  {
+    char *p = (char *)scratch;
+    for (int i = 0; i < count; ++i) {
+      const fold3d_instance &x = inst[i];
+      BatchInst<R> &b = hinst[i];
+      memset(&b, 0, sizeof b);
+      const Grid GZ(x.n[0], x.n[1], x.n[2], kOrderZ), GX = GZ.with_order(kOrderX),
+          GY = GZ.with_order(kOrderY);
+      b.T = (R *)x.T;
+      b.count = GZ.count;
+      b.Y = (R *)p;
+      b.MZ = b.Y + GZ.count;
+      b.MX = b.MZ + kMetricFields * GZ.count;
+      p += (size_t)GZ.count * sizeof(R) * (1 + 2 * kMetricFields);
+      p = (char *)(((uintptr_t)p + 255) & ~(uintptr_t)255);
+      for (int a = 0; a < 3; ++a) {
+        b.stride[kLayZ][a] = GZ.stride[a];
+        b.stride[kLayX][a] = GX.stride[a];
+        b.stride[kLayY][a] = GY.stride[a];
+        b.n[a] = x.n[a];
+        b.h[a] = (R)x.h[a];
+      }
+      const long long tiles = (long long)((x.n[0] + 2 + kBTile - 1) / kBTile) *
+                              ((x.n[1] + 2 + kBTile - 1) / kBTile) * (x.n[2] + 2);
+      tiles_prefix[i + 1] = tiles_prefix[i] + tiles;
+    }
+  }
+  const long long total_tiles = tiles_prefix[count];
   std::vector<int> order_all(6 * count);
   std::vector<long long> prefix_all(6 * (count + 1));
   std::vector<int> steps(6);
+  std::vector<long long> owner_off(7, 0);
   for (int d = 0; d < 6; ++d) {
     const int m = kFrames[d].m;
     int *ord = &order_all[d * count];
@@ -198,13 +376,21 @@ int batch_solve(hipStream_t user_st, int count, const fold3d_instance *inst, int
     pre[0] = 0;
     for (int k = 0; k < count; ++k) pre[k + 1] = pre[k] + rows_of(inst[ord[k]], d);
     steps[d] = inst[ord[0]].n[m] - 1;
+    owner_off[d + 1] = owner_off[d] + pre[count];
+  }
+  std::vector<int> owner_all(owner_off[6]);
+  for (int d = 0; d < 6; ++d) {
+    const long long *pre = &prefix_all[d * (count + 1)];
+    int *own = &owner_all[owner_off[d]];
+    for (int k = 0; k < count; ++k)
+      for (long long r = pre[k]; r < pre[k + 1]; ++r) own[r] = k;
   }
   int launches = 0;
   for (int d = 0; d < 6; ++d) launches += steps[d];
   // device tables (freed at the end of the solve)
   BatchInst<R> *d_inst = nullptr;
-  int *d_order = nullptr, *d_done = nullptr;
-  long long *d_prefix = nullptr;
+  int *d_order = nullptr, *d_done = nullptr, *d_owner = nullptr;
+  long long *d_prefix = nullptr, *d_tiles = nullptr;
   unsigned int *d_counters = nullptr;
   unsigned long long *d_max = nullptr;
   hipGraph_t graph = nullptr;
@@ -230,6 +416,12 @@ int batch_solve(hipStream_t user_st, int count, const fold3d_instance *inst, int
     BATCH_HIP(hipMalloc(&d_done, sizeof(int) * count));
     BATCH_HIP(hipMalloc(&d_counters, sizeof(unsigned int) * launches));
     BATCH_HIP(hipMalloc(&d_max, sizeof(unsigned long long) * count));
+    BATCH_HIP(hipMalloc(&d_tiles, sizeof(long long) * (count + 1)));
+    BATCH_HIP(hipMalloc(&d_owner, sizeof(int) * owner_all.size()));
+    BATCH_HIP(hipMemcpyAsync(d_owner, owner_all.data(), sizeof(int) * owner_all.size(),
+                             hipMemcpyHostToDevice, st));
+    BATCH_HIP(hipMemcpyAsync(d_tiles, tiles_prefix.data(), sizeof(long long) * (count + 1),
+                             hipMemcpyHostToDevice, st));
     BATCH_HIP(hipMemcpyAsync(d_inst, hinst.data(), sizeof(BatchInst<R>) * count,
                              hipMemcpyHostToDevice, st));
     BATCH_HIP(hipMemcpyAsync(d_order, order_all.data(), sizeof(int) * order_all.size(),
@@ -238,6 +430,17 @@ int batch_solve(hipStream_t user_st, int count, const fold3d_instance *inst, int
                              hipMemcpyHostToDevice, st));
     std::vector<int> done(count, 0);
     BATCH_HIP(hipMemcpyAsync(d_done, done.data(), sizeof(int) * count, hipMemcpyHostToDevice, st));
+    // slowness and inverse metric of every instance, once per batch
+    for (int i = 0; i < count; ++i) {
+      const fold3d_instance &x = inst[i];
+      double N[6];
+      sym3_inverse(x.M, N);
+      batch_prep_kernel<R><<<1024, 256, 0, st>>>(
+          hinst[i].count, (const R *)x.speed_field, (R)x.speed, (const R *)x.theta,
+          (const R *)x.phi, (const R *)x.ra, (const R *)x.rb, (R)N[0], (R)N[1], (R)N[2],
+          (R)N[3], (R)N[4], (R)N[5], hinst[i].MZ);
+    }
+    BATCH_HIP(hipGetLastError());
     BATCH_HIP(hipStreamSynchronize(st));
 
     // one iteration as a launch sequence
@@ -245,20 +448,47 @@ int batch_solve(hipStream_t user_st, int count, const fold3d_instance *inst, int
       batch_reset_kernel<<<(launches + count + 255) / 256, 256, 0, s>>>(d_counters, launches,
                                                                       d_max, count);
       int slot = 0;
-      for (int d = 0; d < 6; ++d) {
+      auto permute = [&](int which, int lin, int lout, int fields) {
+        PermuteJob<R> job{which, lin, lout, fields};
+        batch_permute_kernel<R><<<(unsigned)total_tiles, dim3(kBTile, kBTileRows), 0, s>>>(
+            d_inst, d_tiles, count, job, d_done);
+      };
+      auto direction = [&](int d, int lay) {
         const int m = kFrames[d].m;
         const int *ord = &order_all[d * count];
         const long long *pre = &prefix_all[d * (count + 1)];
-        BatchDir bd{d_order + d * count, d_prefix + d * (count + 1)};
+        BatchDir bd{d_order + d * count, d_prefix + d * (count + 1), d_owner + owner_off[d]};
+        const LatticeIdx LI = lattice_idx(d);
         int active = count;
         for (int q = 0; q < steps[d]; ++q, ++slot) {
           while (active > 0 && inst[ord[active - 1]].n[m] - 1 <= q) --active;
           const long long total = pre[active];
-          const int grid = (int)std::min<long long>(total, blocks);
-          batch_plane_kernel<R><<<grid, 64, 0, s>>>(d_inst, bd, d, q, total, active,
-                                                    d_counters + slot, d_done, d_max);
+          // rows per pull: enough pulls per block to balance, few enough atomics
+          const int chunk = (int)std::max<long long>(1, std::min<long long>(8, total / (4 * blocks)));
+          const int grid = (int)std::min<long long>((total + chunk - 1) / chunk, blocks);
+          if (groups == 1)
+            batch_plane_kernel<R, 1><<<grid, dim3(64, 1), 0, s>>>(
+                d_inst, bd, d, lay, LI, q, total, chunk, d_counters + slot, d_done, d_max);
+          else if (groups == 2)
+            batch_plane_kernel<R, 2><<<grid, dim3(64, 2), 0, s>>>(
+                d_inst, bd, d, lay, LI, q, total, chunk, d_counters + slot, d_done, d_max);
+          else
+            batch_plane_kernel<R, 4><<<grid, dim3(64, 4), 0, s>>>(
+                d_inst, bd, d, lay, LI, q, total, chunk, d_counters + slot, d_done, d_max);
         }
-      }
+      };
+      permute(0, kLayZ, kLayX, 1);
+      permute(2, kLayZ, kLayX, kMetricFields);
+      direction(FOLD3D_X_DOWN, kLayX);
+      direction(FOLD3D_X_UP, kLayX);
+      permute(1, kLayX, kLayZ, 1);
+      direction(FOLD3D_K_UP, kLayZ);
+      direction(FOLD3D_K_DOWN, kLayZ);
+      permute(0, kLayZ, kLayY, 1);
+      permute(2, kLayZ, kLayY, kMetricFields);
+      direction(FOLD3D_Y_UP, kLayY);
+      direction(FOLD3D_Y_DOWN, kLayY);
+      permute(1, kLayY, kLayZ, 1);
     };
 
     if (use_graph) {
@@ -333,6 +563,8 @@ cleanup:
   (void)hipFree(d_done);
   (void)hipFree(d_counters);
   (void)hipFree(d_max);
+  (void)hipFree(d_tiles);
+  (void)hipFree(d_owner);
   // later work on the caller's stream sees the solved fields
   (void)hipEventRecord(order_ev, st);
   (void)hipStreamWaitEvent(user_st, order_ev, 0);
@@ -343,8 +575,8 @@ cleanup:
 }
 
 template int batch_solve<double>(hipStream_t, int, const fold3d_instance *, int, double, int, int,
-                                 int *, double *, double *);
+                                 int, void *, int *, double *, double *);
 template int batch_solve<float>(hipStream_t, int, const fold3d_instance *, int, double, int, int,
-                                int *, double *, double *);
+                                int, void *, int *, double *, double *);
 
 }  // namespace fold3d

@@ -311,14 +311,17 @@ int fold3d_solve_batched(fold3d_handle *h, int count, const fold3d_instance *ins
   F3D_HIP(hipSetDevice(h->device));
   int cus = 0;
   F3D_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, h->device));
-  const int blocks = cus * 16;
   const int use_graph = o.engine != FOLD3D_ENGINE_PLANE;
+  const int groups = h->ellip_groups;
+  const int blocks = cus * 16 / groups;
+  const int rc = ensure_scratch(h, batch_scratch_bytes(count, inst, o.precision));
+  if (rc != FOLD3D_OK) return rc;
   hipStream_t st = (hipStream_t)o.stream;
   return o.precision == FOLD3D_F64
-             ? batch_solve<double>(st, count, inst, o.max_iters, o.tol, blocks, use_graph,
-                                   iterations_out, max_dt_out, ms_out)
-             : batch_solve<float>(st, count, inst, o.max_iters, o.tol, blocks, use_graph,
-                                  iterations_out, max_dt_out, ms_out);
+             ? batch_solve<double>(st, count, inst, o.max_iters, o.tol, blocks, groups,
+                                   use_graph, h->d_scratch, iterations_out, max_dt_out, ms_out)
+             : batch_solve<float>(st, count, inst, o.max_iters, o.tol, blocks, groups,
+                                  use_graph, h->d_scratch, iterations_out, max_dt_out, ms_out);
 }
 
 int fold3d_adjoint(fold3d_handle *h, const void *T, int nx, int ny, int nz, double dx,

@@ -89,9 +89,12 @@ def batch_run(solver, torch, count=96, check=6):
         T0s.append(wl.point_source_start(W, xp=torch, dtype=torch.float32, device="cuda"))
     cells = sum(n ** 3 for n in sides)
     log(f"bench: batch of {count} instances, {cells / 1e9:.2f} G nodes")
-    # warm-up on three instances (code load, graph path)
+    # warm-up on three instances (code load, graph path), then the schedule autotuner
     warm = [t.clone() for t in T0s[:3]]
     solver.solve_batched(Ws[:3], warm, max_iters=2, tol=0.0)
+    tuned, tune_ms = solver.autotune_batched(Ws, T0s, iterations=1)
+    log(f"bench: batch autotuner picked (waves per row, blocks per CU, rows per pull) = {tuned} "
+        f"({tune_ms:.0f} ms/iteration)")
     Ts = [t.clone() for t in T0s]
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -130,6 +133,8 @@ def batch_run(solver, torch, count=96, check=6):
             "looped_solves_per_s": count / loop_s, "speedup": loop_s / batch_s,
             "iterations_min": min(its), "iterations_max": max(its),
             "batched_cell_updates": upd, "batched_gcups": upd / batch_s / 1e9,
+            "batch_tuning": {"waves_per_row": tuned[0], "blocks_per_cu": tuned[1],
+                             "rows_per_pull": tuned[2]},
             "fp32_max_rel_err_vs_fp64_on_smallest": {"instances": check, "value": worst}}
 
 

@@ -123,17 +123,26 @@ __global__ __launch_bounds__(64 * kGroups) void batch_plane_kernel(
   const Frame fr = kFrames[dir];
   const int tx = threadIdx.x;
   const int grp = __builtin_amdgcn_readfirstlane(threadIdx.y);
-  long long item = 0, end = 0;
+  // chunk > 0: pull `chunk` rows at a time from the launch's counter;
+  // chunk == 0: static schedule, rows blockIdx.x, blockIdx.x + gridDim.x, ...
+  long long item = chunk > 0 ? 0 : blockIdx.x, end = 0;
   for (;;) {
-    if (item == end) {
-      if (tx == 0 && grp == 0) item_s = (long long)atomicAdd(counter, (unsigned)chunk);
-      __syncthreads();
-      item = item_s;
-      __syncthreads();
+    long long cur;
+    if (chunk > 0) {
+      if (item == end) {
+        if (tx == 0 && grp == 0) item_s = (long long)atomicAdd(counter, (unsigned)chunk);
+        __syncthreads();
+        item = item_s;
+        __syncthreads();
+        if (item >= total) break;
+        end = item + chunk < total ? item + chunk : total;
+      }
+      cur = item++;
+    } else {
       if (item >= total) break;
-      end = item + chunk < total ? item + chunk : total;
+      cur = item;
+      item += gridDim.x;
     }
-    const long long cur = item++;
     const int lo = bd.owner[cur];
     const int i = bd.order[lo];
     if (done[i]) continue;  // uniform over the block
@@ -319,8 +328,8 @@ size_t batch_scratch_bytes(int count, const fold3d_instance *inst, int precision
 
 template <typename R>
 int batch_solve(hipStream_t user_st, int count, const fold3d_instance *inst, int max_iters,
-                double tol, int blocks, int groups, int use_graph, void *scratch,
-                int *iters_out, double *max_dt_out, double *ms_out) {
+                double tol, int blocks, int groups, int rows_per_pull, int use_graph,
+                void *scratch, int *iters_out, double *max_dt_out, double *ms_out) {
   int dev = 0;
   F3D_HIP(hipGetDevice(&dev));
   // The batch runs on a stream of its own (the legacy default stream cannot be
@@ -464,8 +473,14 @@ int batch_solve(hipStream_t user_st, int count, const fold3d_instance *inst, int
           while (active > 0 && inst[ord[active - 1]].n[m] - 1 <= q) --active;
           const long long total = pre[active];
           // rows per pull: enough pulls per block to balance, few enough atomics
-          const int chunk = (int)std::max<long long>(1, std::min<long long>(8, total / (4 * blocks)));
-          const int grid = (int)std::min<long long>((total + chunk - 1) / chunk, blocks);
+          // rows_per_pull < 0: static schedule; 0: automatic pull size
+          const int chunk =
+              rows_per_pull < 0   ? 0
+              : rows_per_pull > 0 ? rows_per_pull
+                                  : (int)std::max<long long>(
+                                        1, std::min<long long>(8, total / (4 * blocks)));
+          const int grid =
+              (int)std::min<long long>(chunk > 0 ? (total + chunk - 1) / chunk : total, blocks);
           if (groups == 1)
             batch_plane_kernel<R, 1><<<grid, dim3(64, 1), 0, s>>>(
                 d_inst, bd, d, lay, LI, q, total, chunk, d_counters + slot, d_done, d_max);
@@ -575,8 +590,8 @@ cleanup:
 }
 
 template int batch_solve<double>(hipStream_t, int, const fold3d_instance *, int, double, int, int,
-                                 int, void *, int *, double *, double *);
+                                 int, int, void *, int *, double *, double *);
 template int batch_solve<float>(hipStream_t, int, const fold3d_instance *, int, double, int, int,
-                                int, void *, int *, double *, double *);
+                                int, int, void *, int *, double *, double *);
 
 }  // namespace fold3d

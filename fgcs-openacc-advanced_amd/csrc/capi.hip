@@ -24,6 +24,9 @@ struct fold3d_handle {
   size_t scratch_bytes;
   int split_groups;           // waves per node row of the fold split kernel (2, 4, 8)
   int ellip_groups;           // waves per node row of the elliptic split kernel (1, 2, 4)
+  int batch_groups;           // batched solve: waves per row (1, 2, 4)
+  int batch_blocks_per_cu;    // batched solve: persistent blocks per CU per launch
+  int batch_rows_per_pull;    // batched solve: rows per queue pull (0 = automatic)
 };
 
 // Grow the handle's scratch field to at least `bytes` (stream-ordered free of
@@ -80,6 +83,9 @@ int fold3d_create(fold3d_handle **out, int device) {
   h->device = device;
   h->split_groups = 4;
   h->ellip_groups = 2;
+  h->batch_groups = 1;
+  h->batch_blocks_per_cu = 32;
+  h->batch_rows_per_pull = -1;
   hipError_t e = hipMalloc((void **)&h->d_max, sizeof(unsigned long long));
   if (e != hipSuccess) {
     delete h;
@@ -273,14 +279,10 @@ int fold3d_solve(fold3d_handle *h, void *T, int nx, int ny, int nz, double dx,
   return FOLD3D_OK;
 }
 
-int fold3d_solve_batched(fold3d_handle *h, int count, const fold3d_instance *inst,
-                         const fold3d_opts *opts, int *iterations_out, double *max_dt_out,
-                         double *ms_out) {
+static int check_batch(fold3d_handle *h, int count, const fold3d_instance *inst,
+                       const fold3d_opts &o) {
   F3D_CHECK(h && inst, FOLD3D_E_INVALID, "fold3d_solve_batched: NULL handle or instances");
   F3D_CHECK(count >= 1, FOLD3D_E_INVALID, "fold3d_solve_batched: count must be >= 1");
-  fold3d_opts o;
-  fold3d_default_opts(&o);
-  if (opts) o = *opts;
   F3D_CHECK(o.memory == FOLD3D_DEVICE, FOLD3D_E_INVALID,
             "fold3d_solve_batched: instances must be device-resident");
   F3D_CHECK(o.precision == FOLD3D_F64 || o.precision == FOLD3D_F32, FOLD3D_E_INVALID,
@@ -308,20 +310,97 @@ int fold3d_solve_batched(fold3d_handle *h, int count, const fold3d_instance *ins
                 "fold3d_solve_batched: instance %d metric is not positive definite", i);
     }
   }
-  F3D_HIP(hipSetDevice(h->device));
+  return FOLD3D_OK;
+}
+
+static int run_batch(fold3d_handle *h, int count, const fold3d_instance *inst,
+                     const fold3d_opts &o, int groups, int blocks_per_cu, int rows_per_pull,
+                     int *iterations_out, double *max_dt_out, double *ms_out) {
   int cus = 0;
   F3D_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, h->device));
   const int use_graph = o.engine != FOLD3D_ENGINE_PLANE;
-  const int groups = h->ellip_groups;
-  const int blocks = cus * 16 / groups;
+  const int blocks = cus * blocks_per_cu;
   const int rc = ensure_scratch(h, batch_scratch_bytes(count, inst, o.precision));
   if (rc != FOLD3D_OK) return rc;
   hipStream_t st = (hipStream_t)o.stream;
   return o.precision == FOLD3D_F64
              ? batch_solve<double>(st, count, inst, o.max_iters, o.tol, blocks, groups,
-                                   use_graph, h->d_scratch, iterations_out, max_dt_out, ms_out)
+                                   rows_per_pull, use_graph, h->d_scratch, iterations_out,
+                                   max_dt_out, ms_out)
              : batch_solve<float>(st, count, inst, o.max_iters, o.tol, blocks, groups,
-                                  use_graph, h->d_scratch, iterations_out, max_dt_out, ms_out);
+                                  rows_per_pull, use_graph, h->d_scratch, iterations_out,
+                                  max_dt_out, ms_out);
+}
+
+int fold3d_solve_batched(fold3d_handle *h, int count, const fold3d_instance *inst,
+                         const fold3d_opts *opts, int *iterations_out, double *max_dt_out,
+                         double *ms_out) {
+  fold3d_opts o;
+  fold3d_default_opts(&o);
+  if (opts) o = *opts;
+  const int rc = check_batch(h, count, inst, o);
+  if (rc != FOLD3D_OK) return rc;
+  F3D_HIP(hipSetDevice(h->device));
+  return run_batch(h, count, inst, o, h->batch_groups, h->batch_blocks_per_cu,
+                   h->batch_rows_per_pull, iterations_out, max_dt_out, ms_out);
+}
+
+int fold3d_autotune_batched(fold3d_handle *h, int count, const fold3d_instance *inst,
+                            const fold3d_opts *opts, int iterations, float *ms_out) {
+  fold3d_opts o;
+  fold3d_default_opts(&o);
+  if (opts) o = *opts;
+  int rc = check_batch(h, count, inst, o);
+  if (rc != FOLD3D_OK) return rc;
+  F3D_CHECK(iterations >= 1, FOLD3D_E_INVALID, "fold3d_autotune_batched: iterations must be >= 1");
+  F3D_HIP(hipSetDevice(h->device));
+  // scratch copies of every T (the caller's fields are not modified)
+  const size_t r = o.precision == FOLD3D_F64 ? 8 : 4;
+  std::vector<size_t> off(count + 1, 0);
+  for (int i = 0; i < count; ++i) {
+    const Grid G(inst[i].n[0], inst[i].n[1], inst[i].n[2]);
+    off[i + 1] = (off[i] + (size_t)G.count * r + 255) & ~(size_t)255;
+  }
+  char *work = nullptr;
+  F3D_HIP(hipMalloc((void **)&work, off[count]));
+  std::vector<fold3d_instance> copy(inst, inst + count);
+  for (int i = 0; i < count; ++i) copy[i].T = work + off[i];
+  hipStream_t st = (hipStream_t)o.stream;
+  o.max_iters = iterations;
+  o.tol = 0.0;
+  // (groups, blocks per CU, rows per pull): static schedules and queue pulls
+  static const int kConfigs[][3] = {{1, 16, -1}, {1, 32, -1}, {1, 64, -1}, {2, 16, -1},
+                                    {2, 32, -1}, {2, 64, -1}, {4, 16, -1}, {1, 8, 8},
+                                    {2, 8, 8}};
+  float best_ms = 0;
+  int best = -1;
+  for (int round = 0; round < 2 && rc == FOLD3D_OK; ++round) {  // the first round warms up
+    for (int c = 0; c < (int)(sizeof kConfigs / sizeof kConfigs[0]); ++c) {
+      for (int i = 0; i < count && rc == FOLD3D_OK; ++i) {
+        const Grid G(inst[i].n[0], inst[i].n[1], inst[i].n[2]);
+        if (hipMemcpyAsync(copy[i].T, inst[i].T, (size_t)G.count * r, hipMemcpyDeviceToDevice,
+                           st) != hipSuccess)
+          rc = FOLD3D_E_HIP;
+      }
+      double ms = 0;
+      if (rc == FOLD3D_OK)
+        rc = run_batch(h, count, copy.data(), o, kConfigs[c][0], kConfigs[c][1], kConfigs[c][2],
+                       nullptr, nullptr, &ms);
+      if (rc != FOLD3D_OK) break;
+      if (round == 1 && (best < 0 || ms < best_ms)) {
+        best = c;
+        best_ms = (float)ms;
+      }
+    }
+  }
+  (void)hipStreamSynchronize(st);
+  (void)hipFree(work);
+  if (rc != FOLD3D_OK) return rc;
+  h->batch_groups = kConfigs[best][0];
+  h->batch_blocks_per_cu = kConfigs[best][1];
+  h->batch_rows_per_pull = kConfigs[best][2];
+  if (ms_out) *ms_out = best_ms;
+  return FOLD3D_OK;
 }
 
 int fold3d_adjoint(fold3d_handle *h, const void *T, int nx, int ny, int nz, double dx,
@@ -415,6 +494,30 @@ int fold3d_set_tuning(fold3d_handle *h, int model_kind, int groups) {
 int fold3d_get_tuning(fold3d_handle *h, int model_kind, int *groups) {
   F3D_CHECK(h && groups, FOLD3D_E_INVALID, "fold3d_get_tuning: NULL argument");
   *groups = model_kind == FOLD3D_MODEL_FOLD ? h->split_groups : h->ellip_groups;
+  return FOLD3D_OK;
+}
+
+int fold3d_set_batch_tuning(fold3d_handle *h, int groups, int blocks_per_cu, int rows_per_pull) {
+  F3D_CHECK(h != nullptr, FOLD3D_E_INVALID, "fold3d_set_batch_tuning: NULL handle");
+  F3D_CHECK(groups == 0 || groups == 1 || groups == 2 || groups == 4, FOLD3D_E_INVALID,
+            "fold3d_set_batch_tuning: groups must be 0, 1, 2 or 4");
+  F3D_CHECK(blocks_per_cu >= 0 && blocks_per_cu <= 64, FOLD3D_E_INVALID,
+            "fold3d_set_batch_tuning: blocks_per_cu must be 0..64");
+  F3D_CHECK(rows_per_pull >= -1 && rows_per_pull <= 64, FOLD3D_E_INVALID,
+            "fold3d_set_batch_tuning: rows_per_pull must be -1..64");
+  h->batch_groups = groups == 0 ? 1 : groups;
+  h->batch_blocks_per_cu = blocks_per_cu == 0 ? 32 : blocks_per_cu;
+  h->batch_rows_per_pull = rows_per_pull;
+  return FOLD3D_OK;
+}
+
+int fold3d_get_batch_tuning(fold3d_handle *h, int *groups, int *blocks_per_cu,
+                            int *rows_per_pull) {
+  F3D_CHECK(h && groups && blocks_per_cu && rows_per_pull, FOLD3D_E_INVALID,
+            "fold3d_get_batch_tuning: NULL argument");
+  *groups = h->batch_groups;
+  *blocks_per_cu = h->batch_blocks_per_cu;
+  *rows_per_pull = h->batch_rows_per_pull;
   return FOLD3D_OK;
 }
 

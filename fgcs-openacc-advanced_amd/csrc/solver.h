@@ -74,11 +74,12 @@ int fold_sweeps_tiled_engine(hipStream_t st, R *T, R *Y, const Grid &G,
                              unsigned long long *d_max, int *iters_out, double *max_dt_out);
 
 // Batched elliptic solves (batch.hip).  `blocks` = persistent grid per launch,
-// `groups` = waves per row (1, 2, 4); `scratch` holds batch_scratch_bytes().
+// `groups` = waves per row (1, 2, 4), `rows_per_pull` = rows taken from the
+// queue at a time (0 = automatic); `scratch` holds batch_scratch_bytes().
 template <typename R>
 int batch_solve(hipStream_t st, int count, const fold3d_instance *inst, int max_iters,
-                double tol, int blocks, int groups, int use_graph, void *scratch,
-                int *iters_out, double *max_dt_out, double *ms_out);
+                double tol, int blocks, int groups, int rows_per_pull, int use_graph,
+                void *scratch, int *iters_out, double *max_dt_out, double *ms_out);
 size_t batch_scratch_bytes(int count, const fold3d_instance *inst, int precision);
 
 // Adjoint state and gradients (adjoint.hip); `scratch` holds

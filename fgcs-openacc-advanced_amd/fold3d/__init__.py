@@ -120,6 +120,10 @@ def lib() -> C.CDLL:
                                        C.POINTER(i), dp, dp]
     L.fold3d_set_tuning.argtypes = [vp, i, i]
     L.fold3d_get_tuning.argtypes = [vp, i, C.POINTER(i)]
+    L.fold3d_set_batch_tuning.argtypes = [vp, i, i, i]
+    L.fold3d_get_batch_tuning.argtypes = [vp, C.POINTER(i), C.POINTER(i), C.POINTER(i)]
+    L.fold3d_autotune_batched.argtypes = [vp, i, C.POINTER(Instance), C.POINTER(Opts), i,
+                                          C.POINTER(C.c_float)]
     L.fold3d_autotune.argtypes = [vp, vp, i, i, i, d, d, d, C.POINTER(Model),
                                   C.POINTER(Opts), i, C.POINTER(C.c_float)]
     L.fold3d_adjoint.argtypes = [vp, vp, i, i, i, d, d, d, C.POINTER(Model), i, vp, vp,
@@ -235,12 +239,8 @@ class Solver:
         check(rc)
         return st
 
-    def solve_batched(self, workloads, Ts, max_iters=200, tol=1e-6, use_graph=True,
-                      stream=None, raise_not_converged=True):
-        """Solve many elliptic-model instances (fold3d.workloads.Workload with
-        torch fields on the device, and their torch T tensors) in one batch.
-        Returns (iterations per instance, max|dT| per instance, device ms)."""
-        import torch
+    @staticmethod
+    def _instances(workloads, Ts):
         count = len(workloads)
         arr = (Instance * count)()
         for k, (W, T) in enumerate(zip(workloads, Ts)):
@@ -257,12 +257,26 @@ class Solver:
             for name in ("speed_field", "theta", "phi", "ra", "rb"):
                 f = getattr(W, name)
                 setattr(x, name, None if f is None else f.data_ptr())
+        return arr
+
+    def _batch_opts(self, Ts, max_iters, tol, use_graph, stream):
+        import torch
         o = Opts()
         lib().fold3d_default_opts(C.byref(o))
         o.precision = F64 if Ts[0].dtype == torch.float64 else F32
         o.max_iters, o.tol = max_iters, tol
         o.engine = ENGINE_AUTO if use_graph else ENGINE_PLANE
         o.stream = stream if stream is not None else torch.cuda.current_stream().cuda_stream
+        return o
+
+    def solve_batched(self, workloads, Ts, max_iters=200, tol=1e-6, use_graph=True,
+                      stream=None, raise_not_converged=True):
+        """Solve many elliptic-model instances (fold3d.workloads.Workload with
+        torch fields on the device, and their torch T tensors) in one batch.
+        Returns (iterations per instance, max|dT| per instance, device ms)."""
+        count = len(workloads)
+        arr = self._instances(workloads, Ts)
+        o = self._batch_opts(Ts, max_iters, tol, use_graph, stream)
         its = (C.c_int * count)()
         mds = (C.c_double * count)()
         ms = C.c_double()
@@ -270,6 +284,16 @@ class Solver:
         if not (rc == FOLD3D_E_NOT_CONVERGED and not raise_not_converged):
             check(rc)
         return list(its), list(mds), ms.value
+
+    def autotune_batched(self, workloads, Ts, iterations=1, stream=None):
+        """Time the batched solve's schedules on copies of the fields; keep the
+        fastest.  Returns ((groups, blocks per CU, rows per pull), device ms)."""
+        arr = self._instances(workloads, Ts)
+        o = self._batch_opts(Ts, 1, 0.0, True, stream)
+        ms = C.c_float()
+        check(lib().fold3d_autotune_batched(self.h, len(workloads), arr, C.byref(o),
+                                            iterations, C.byref(ms)))
+        return self.batch_tuning(), ms.value
 
     def adjoint(self, T, n, spacing, model: Model, rec_idx, rec_res, grad_slowness,
                 grad_theta=None, max_iters: int = 200, tol: float = 0.0, stream=None,
@@ -300,6 +324,14 @@ class Solver:
 
     def set_tuning(self, groups: int, kind: int = MODEL_FOLD):
         check(lib().fold3d_set_tuning(self.h, kind, groups))
+
+    def set_batch_tuning(self, groups: int = 0, blocks_per_cu: int = 0, rows_per_pull: int = 0):
+        check(lib().fold3d_set_batch_tuning(self.h, groups, blocks_per_cu, rows_per_pull))
+
+    def batch_tuning(self):
+        g, b, r = C.c_int(), C.c_int(), C.c_int()
+        check(lib().fold3d_get_batch_tuning(self.h, C.byref(g), C.byref(b), C.byref(r)))
+        return g.value, b.value, r.value
 
     def tuning(self, kind: int = MODEL_FOLD) -> int:
         g = C.c_int()

@@ -184,6 +184,21 @@ int fold3d_autotune(fold3d_handle *h, const void *T, int nx, int ny, int nz,
                     double dx, double dy, double dz, const fold3d_model *model,
                     const fold3d_opts *opts, int iterations, float *ms_out);
 
+/* Tuning of the batched solve: waves per row of 64 nodes (1, 2, 4; default
+ * 1), persistent blocks per compute unit in each launch (1..64; default 32)
+ * and how a block finds its rows: 1..64 = that many rows per pull from the
+ * launch's device queue, 0 = queue with an automatic pull size, -1 = static
+ * (block b takes rows b, b + grid, ...; the default).  0 arguments of
+ * set restore the defaults of groups and blocks.  fold3d_autotune_batched()
+ * times a set of these on `iterations` iterations of scratch copies of the
+ * batch's fields (which are not modified), keeps the fastest on the handle
+ * and returns its device time. */
+int fold3d_set_batch_tuning(fold3d_handle *h, int groups, int blocks_per_cu, int rows_per_pull);
+int fold3d_get_batch_tuning(fold3d_handle *h, int *groups, int *blocks_per_cu,
+                            int *rows_per_pull);
+int fold3d_autotune_batched(fold3d_handle *h, int count, const fold3d_instance *inst,
+                            const fold3d_opts *opts, int iterations, float *ms_out);
+
 /* One plane step of one direction on a device FP64 field, fold model, with
  * explicit local-frame arguments exactly as the reference kernels take them
  * (AN_ISO_1_OpenACC.c:180-183; the CUDA twin Stencil-CUDA/AN_ISO_2_GPU.cu:151-155

@@ -159,6 +159,7 @@ def test_batched_per_node_metric_equals_single_solves(solver, groups, dtype):
         Ws.append(W)
         T0s.append(T0)
     solver.set_tuning(groups, fold3d.MODEL_ELLIP)
+    solver.set_batch_tuning(groups, 4, 2)
     try:
         Ts = [dev(T0, dtype) for T0 in T0s]
         its, _, _ = solver.solve_batched(Ws, Ts, max_iters=3, tol=0.0)
@@ -170,3 +171,25 @@ def test_batched_per_node_metric_equals_single_solves(solver, groups, dtype):
             assert torch.equal(T.double(), S.double()), W.n
     finally:
         solver.set_tuning(0, fold3d.MODEL_ELLIP)
+        solver.set_batch_tuning(0, 0, 0)
+
+
+def test_autotune_batched_keeps_fields_and_picks_a_schedule(solver):
+    Ws = make_batch([17, 24, 30], dtype_np=np.float32)
+    T0s = [wl.point_source_start(W) for W in Ws]
+    Wd = [to_dev_workload(W, torch.float32) for W in Ws]
+    Ts = [dev(T0, torch.float32) for T0 in T0s]
+    before = [t.clone() for t in Ts]
+    try:
+        (g, b, r), ms = solver.autotune_batched(Wd, Ts, iterations=1)
+        assert g in (1, 2, 4) and 1 <= b <= 64 and -1 <= r <= 64 and ms > 0
+        for a, c in zip(Ts, before):
+            assert torch.equal(a, c)
+        its, _, _ = solver.solve_batched(Wd, Ts, max_iters=3, tol=0.0)
+        ref = [c.clone() for c in before]
+        solver.set_batch_tuning(2, 8, 8)  # the device queue
+        solver.solve_batched(Wd, ref, max_iters=3, tol=0.0)
+        for a, c in zip(Ts, ref):
+            assert torch.equal(a, c)
+    finally:
+        solver.set_batch_tuning(0, 0, 0)

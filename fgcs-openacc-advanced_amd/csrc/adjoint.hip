@@ -9,7 +9,10 @@
 //   dT_X/dtheta = s (w^T dN/dtheta w) / (2 |w|_N).
 // The adjoint state mu solves mu_X = r_X + sum_Z (dT_Z/dT_X) mu_Z (Z downstream
 // of X), a triangular system in the order of T, solved here by reverse-ordered
-// plane sweeps (Gauss-Seidel) until mu stops changing; then
+// plane sweeps (Gauss-Seidel) until mu stops changing.  Before the sweeps each
+// node's in-edges are collected once into a 64-bit mask (two bits per
+// neighbour direction: which stencil slot of that neighbour names the node),
+// so a sweep visits only the neighbours that feed it.  Then
 //   dJ/ds_X = mu_X dT_X/ds_X,  dJ/dtheta_X = mu_X dT_X/dtheta_X.
 // Nodes no candidate reproduces (the closed-form start around a source) are
 // fixed: they pass nothing upstream and contribute no gradient.
@@ -244,37 +247,73 @@ __global__ __launch_bounds__(256) void stencil_kernel(const R *__restrict__ T, A
   S.dTdth[X] = gth;
 }
 
-// One plane of an adjoint sweep: mu_X = r_X + sum over the 26 neighbours Z
-// that list X in their stencil of weight * mu_Z.
+// Neighbour code c = (dx+1) + 3 (dy+1) + 9 (dz+1), skipping c = 13 (the node
+// itself): 26 codes.  in_mask[X] holds two bits per code: 0 if the neighbour Z
+// in that direction does not list X in its stencil, else 1 + the slot q
+// (0..2) at which it does.
+__device__ __forceinline__ int code_slot(int c) { return c < 13 ? c : c - 1; }
+
+template <typename R>
+__global__ void in_mask_kernel(Stencil<R> S, long long count, long long sx, long long sy,
+                               long long sz, unsigned long long *__restrict__ in_mask) {
+  for (long long z = blockIdx.x * (long long)blockDim.x + threadIdx.x; z < count;
+       z += (long long)gridDim.x * blockDim.x) {
+    const int offs[3] = {S.off0[z], S.off1[z], S.off2[z]};
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+      const int o = offs[q];
+      if (o == 0) continue;
+      // X = z + o: Z sits at offset -o from X; decode -o into (dx, dy, dz)
+      const long long back = -(long long)o;
+      int dzv = 0, dyv = 0, dxv = 0;
+      long long rem = back;
+      for (int k = -1; k <= 1; ++k)
+        for (int j = -1; j <= 1; ++j)
+          for (int i = -1; i <= 1; ++i)
+            if (i * sx + j * sy + k * sz == rem) {
+              dxv = i;
+              dyv = j;
+              dzv = k;
+            }
+      const int c = code_slot((dxv + 1) + 3 * (dyv + 1) + 9 * (dzv + 1));
+      atomicOr(&in_mask[z + o], (unsigned long long)(q + 1) << (2 * c));
+    }
+  }
+}
+
+// One plane of an adjoint sweep: mu_X = r_X + sum over the neighbours Z that
+// list X in their stencil (in_mask[X]) of weight * mu_Z.
 template <typename R>
 __global__ __launch_bounds__(256) void adjoint_plane_kernel(R *__restrict__ mu,
                                                             const R *__restrict__ res,
                                                             Stencil<R> S, PlaneGeom g,
                                                             long long sx, long long sy,
                                                             long long sz,
+                                                            const unsigned long long *__restrict__ in_mask,
                                                             unsigned long long *__restrict__ maxd) {
   const int a = 1 + blockIdx.x * 64 + threadIdx.x;
   const int b = 1 + blockIdx.y * 4 + threadIdx.y;
   R d = R(0);
   if (a <= g.na && b <= g.nb) {
     const long long X = g.base + a * g.Sa + b * g.Sb + g.sm;  // the plane being swept
-    R acc = res[X];
-    for (int dz = -1; dz <= 1; ++dz)
-      for (int dy = -1; dy <= 1; ++dy)
-        for (int dx = -1; dx <= 1; ++dx) {
-          if (dx == 0 && dy == 0 && dz == 0) continue;
-          const long long o = dx * sx + dy * sy + dz * sz;
-          const long long Z = X + o;  // neighbour; X sits at offset -o from it
-          const int back = (int)(-o);
-          R wz = R(0);
-          if (S.off0[Z] == back) wz += S.w0[Z];
-          if (S.off1[Z] == back) wz += S.w1[Z];
-          if (S.off2[Z] == back) wz += S.w2[Z];
-          if (wz != R(0)) acc += wz * mu[Z];
-        }
-    const R old = mu[X];
-    mu[X] = acc;
-    d = fabs(acc - old);
+    unsigned long long m = in_mask[X];
+    if (m != 0ull) {
+      R acc = res[X];
+      while (m) {
+        const int bit = __builtin_ctzll(m);
+        const int c = bit >> 1;
+        const int q = (int)((m >> (2 * c)) & 3ull) - 1;
+        m &= ~(3ull << (2 * c));
+        const int cc = c < 13 ? c : c + 1;
+        const int dx = cc % 3 - 1, dy = (cc / 3) % 3 - 1, dz = cc / 9 - 1;
+        const long long Z = X + dx * sx + dy * sy + dz * sz;
+        const R w = q == 0 ? S.w0[Z] : q == 1 ? S.w1[Z] : S.w2[Z];
+        acc += w * mu[Z];
+      }
+      const R old = mu[X];
+      mu[X] = acc;
+      d = fabs(acc - old);
+    }
   }
   double v = (double)d;
 #pragma unroll
@@ -310,19 +349,15 @@ int adjoint_solve(hipStream_t st, const R *T, const Grid &G, const fold3d_model 
                   int max_iters, double tol, R *grad_s, R *grad_th, void *scratch,
                   unsigned long long *d_max, int *iters_out, double *maxd_out) {
   const long long count = G.count;
-  // scratch: mu, res, 3 offsets (int), 3 weights, 2 partials
+  // scratch: in-edge masks (u64), mu, res, 3 weights, 2 partials (R), 3 offsets (int)
   char *p = (char *)scratch;
+  unsigned long long *in_mask = (unsigned long long *)p;
+  p += count * sizeof(unsigned long long);
   R *mu = (R *)p;
   p += count * sizeof(R);
   R *res = (R *)p;
   p += count * sizeof(R);
   Stencil<R> S;
-  S.off0 = (int *)p;
-  p += count * sizeof(int);
-  S.off1 = (int *)p;
-  p += count * sizeof(int);
-  S.off2 = (int *)p;
-  p += count * sizeof(int);
   S.w0 = (R *)p;
   p += count * sizeof(R);
   S.w1 = (R *)p;
@@ -332,7 +367,13 @@ int adjoint_solve(hipStream_t st, const R *T, const Grid &G, const fold3d_model 
   S.dTds = (R *)p;
   p += count * sizeof(R);
   S.dTdth = (R *)p;
-  F3D_HIP(hipMemsetAsync(scratch, 0, (size_t)count * (7 * sizeof(R) + 3 * sizeof(int)), st));
+  p += count * sizeof(R);
+  S.off0 = (int *)p;
+  p += count * sizeof(int);
+  S.off1 = (int *)p;
+  p += count * sizeof(int);
+  S.off2 = (int *)p;
+  F3D_HIP(hipMemsetAsync(scratch, 0, adjoint_scratch_bytes(G, sizeof(R) == 8 ? FOLD3D_F64 : FOLD3D_F32), st));
   AdjArgs<R> A;
   memset(&A, 0, sizeof A);
   for (int a = 0; a < 3; ++a) {
@@ -352,8 +393,12 @@ int adjoint_solve(hipStream_t st, const R *T, const Grid &G, const fold3d_model 
   A.rel_tol = sizeof(R) == 8 ? (R)1e-12 : (R)1e-6;
   const long long interior = (long long)G.n[0] * G.n[1] * G.n[2];
   stencil_kernel<R><<<(unsigned)((interior + 255) / 256), 256, 0, st>>>(T, A, S);
+  in_mask_kernel<R><<<2048, 256, 0, st>>>(S, count, G.stride[0], G.stride[1], G.stride[2],
+                                          in_mask);
   if (nrec > 0) scatter_residual_kernel<R><<<(nrec + 255) / 256, 256, 0, st>>>(res, rec_idx, rec_res, nrec);
   F3D_HIP(hipGetLastError());
+  // nodes no other node lists keep mu = res; the sweeps update the rest
+  F3D_HIP(hipMemcpyAsync(mu, res, count * sizeof(R), hipMemcpyDeviceToDevice, st));
   int it = 0;
   double last = -1.0;
   const dim3 block(64, 4);
@@ -370,7 +415,7 @@ int adjoint_solve(hipStream_t st, const R *T, const Grid &G, const fold3d_model 
         g.sm = 0;  // the plane itself (plane_geom addresses a source plane)
         const dim3 grid((g.na + 63) / 64, (g.nb + 3) / 4);
         adjoint_plane_kernel<R><<<grid, block, 0, st>>>(mu, res, S, g, G.stride[0], G.stride[1],
-                                                        G.stride[2], d_max);
+                                                        G.stride[2], in_mask, d_max);
       }
     }
     F3D_HIP(hipGetLastError());
@@ -390,7 +435,7 @@ int adjoint_solve(hipStream_t st, const R *T, const Grid &G, const fold3d_model 
 
 size_t adjoint_scratch_bytes(const Grid &G, int precision) {
   const size_t r = precision == FOLD3D_F64 ? 8 : 4;
-  return (size_t)G.count * (7 * r + 3 * sizeof(int));
+  return (size_t)G.count * (sizeof(unsigned long long) + 7 * r + 3 * sizeof(int));
 }
 
 template int adjoint_solve<double>(hipStream_t, const double *, const Grid &, const fold3d_model &,

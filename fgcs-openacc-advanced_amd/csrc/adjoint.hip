@@ -399,38 +399,82 @@ int adjoint_solve(hipStream_t st, const R *T, const Grid &G, const fold3d_model 
   F3D_HIP(hipGetLastError());
   // nodes no other node lists keep mu = res; the sweeps update the rest
   F3D_HIP(hipMemcpyAsync(mu, res, count * sizeof(R), hipMemcpyDeviceToDevice, st));
+  // One sweep pass (all six directions, reverse order) is captured once in a
+  // hipGraph on a stream of its own (the legacy default stream cannot be
+  // captured) and replayed until no adjoint value changes by more than tol.
+  hipStream_t ws = nullptr;
+  hipEvent_t ev = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  bool capturing = false;
+  int rc = FOLD3D_OK;
   int it = 0;
   double last = -1.0;
   const dim3 block(64, 4);
-  for (; it < max_iters;) {
-    adj_zero_u64<<<1, 1, 0, st>>>(d_max);
-    // reverse of the forward order: each direction's planes from last to first
-    for (int dd = 5; dd >= 0; --dd) {
-      const Frame &fr = kFrames[dd];
-      const int n = G.n[fr.m];
-      for (int q = 0; q < n; ++q) {
-        // sweep plane index along the march axis, against the characteristics
-        const int pl = fr.s > 0 ? n - q : 1 + q;
-        PlaneGeom g = plane_geom(G, dd, pl);
-        g.sm = 0;  // the plane itself (plane_geom addresses a source plane)
-        const dim3 grid((g.na + 63) / 64, (g.nb + 3) / 4);
-        adjoint_plane_kernel<R><<<grid, block, 0, st>>>(mu, res, S, g, G.stride[0], G.stride[1],
-                                                        G.stride[2], in_mask, d_max);
-      }
+#define ADJ_HIP(call)                                                                  \
+  do {                                                                                 \
+    hipError_t e_ = (call);                                                            \
+    if (e_ != hipSuccess) {                                                            \
+      set_error("fold3d_adjoint: %s failed: %s", #call, hipGetErrorString(e_));        \
+      rc = e_ == hipErrorOutOfMemory ? FOLD3D_E_NOMEM : FOLD3D_E_HIP;                  \
+      goto done;                                                                       \
+    }                                                                                  \
+  } while (0)
+  ADJ_HIP(hipStreamCreateWithFlags(&ws, hipStreamNonBlocking));
+  ADJ_HIP(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+  ADJ_HIP(hipEventRecord(ev, st));
+  ADJ_HIP(hipStreamWaitEvent(ws, ev, 0));
+  ADJ_HIP(hipStreamBeginCapture(ws, hipStreamCaptureModeThreadLocal));
+  capturing = true;
+  adj_zero_u64<<<1, 1, 0, ws>>>(d_max);
+  // reverse of the forward order: each direction's planes from last to first
+  for (int dd = 5; dd >= 0; --dd) {
+    const Frame &fr = kFrames[dd];
+    const int n = G.n[fr.m];
+    for (int q = 0; q < n; ++q) {
+      // sweep plane index along the march axis, against the characteristics
+      const int pl = fr.s > 0 ? n - q : 1 + q;
+      PlaneGeom g = plane_geom(G, dd, pl);
+      g.sm = 0;  // the plane itself (plane_geom addresses a source plane)
+      const dim3 grid((g.na + 63) / 64, (g.nb + 3) / 4);
+      adjoint_plane_kernel<R><<<grid, block, 0, ws>>>(mu, res, S, g, G.stride[0], G.stride[1],
+                                                      G.stride[2], in_mask, d_max);
     }
-    F3D_HIP(hipGetLastError());
+  }
+  capturing = false;
+  ADJ_HIP(hipStreamEndCapture(ws, &graph));
+  ADJ_HIP(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+  for (; it < max_iters;) {
+    ADJ_HIP(hipGraphLaunch(exec, ws));
     ++it;
     unsigned long long bits = 0;
-    F3D_HIP(hipMemcpyAsync(&bits, d_max, sizeof bits, hipMemcpyDeviceToHost, st));
-    F3D_HIP(hipStreamSynchronize(st));
+    ADJ_HIP(hipMemcpyAsync(&bits, d_max, sizeof bits, hipMemcpyDeviceToHost, ws));
+    ADJ_HIP(hipStreamSynchronize(ws));
     memcpy(&last, &bits, sizeof last);
     if (last <= tol) break;
   }
-  adjoint_gradient_kernel<R><<<1024, 256, 0, st>>>(mu, S, count, grad_s, grad_th);
-  F3D_HIP(hipGetLastError());
+  adjoint_gradient_kernel<R><<<1024, 256, 0, ws>>>(mu, S, count, grad_s, grad_th);
+  ADJ_HIP(hipGetLastError());
+  // later work on the caller's stream sees the gradients
+  ADJ_HIP(hipEventRecord(ev, ws));
+  ADJ_HIP(hipStreamWaitEvent(st, ev, 0));
+done:
+#undef ADJ_HIP
+  if (capturing) {
+    hipGraph_t g2 = nullptr;
+    (void)hipStreamEndCapture(ws, &g2);
+    if (g2) (void)hipGraphDestroy(g2);
+  }
+  if (exec) (void)hipGraphExecDestroy(exec);
+  if (graph) (void)hipGraphDestroy(graph);
+  if (ws) {
+    (void)hipStreamSynchronize(ws);
+    (void)hipStreamDestroy(ws);
+  }
+  if (ev) (void)hipEventDestroy(ev);
   *iters_out = it;
   *maxd_out = last;
-  return FOLD3D_OK;
+  return rc;
 }
 
 size_t adjoint_scratch_bytes(const Grid &G, int precision) {

@@ -27,6 +27,11 @@ for p in (REPO, os.path.join(REPO, "fgcs-openacc-advanced_amd")):
         sys.path.insert(0, p)
 
 HBM_PEAK_GBS = 8000.0          # MI355X HBM3E
+# HBM bytes per plane launch of the split kernel (2 waves) at 512^3 FP64, from
+# two rocprofv3 --pmc passes over the first iteration: FETCH_SIZE x 2.0 plus
+# WRITE_SIZE x 1.0 KiB, the factors read off a plain device copy of known
+# size (profiles/fold_split_f64_512_pmc_traffic.md)
+PMC_TRAFFIC_512_F64 = (2901.2 * 2.0 + 849.6 * 1.0) * 1024
 A100_V2_512_GCUPS = 8 * 6 * 511 * 512 ** 2 / 3.038 / 1e9  # plots/perfect-2-gpu.py:20,24
 
 
@@ -409,7 +414,8 @@ def main():
                    "n": n, "iterations": iters, "cell_updates_per_step": W,
                    "engine": "split", "split_groups": groups},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": PMC_TRAFFIC_512_F64 if (n == 512 and groups == 2) else None,
                      "kernel": "fold_split_kernel<double>", "kernel_us": us,
                      "bytes_per_launch": bytes_per_launch},
         "engines_fp64": engines,

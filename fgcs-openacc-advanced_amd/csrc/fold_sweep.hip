@@ -30,6 +30,9 @@ void set_error(const char *fmt, ...) {
 
 extern "C" const char *fold3d_last_error(void) { return fold3d::g_err; }
 
+// for the host C parts of the library (io.c)
+extern "C" void fold3d_set_error_text(const char *msg) { fold3d::set_error("%s", msg); }
+
 namespace fold3d {
 
 // ---------------------------------------------------------------------------

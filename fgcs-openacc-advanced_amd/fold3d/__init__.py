@@ -24,6 +24,7 @@ FOLD3D_E_HIP = -2
 FOLD3D_E_NOMEM = -3
 FOLD3D_E_UNSUPPORTED = -4
 FOLD3D_E_NOT_CONVERGED = -5
+FOLD3D_E_IO = -6
 
 F64, F32 = 0, 1
 DEVICE, HOST = 0, 1
@@ -129,6 +130,10 @@ def lib() -> C.CDLL:
     L.fold3d_adjoint.argtypes = [vp, vp, i, i, i, d, d, d, C.POINTER(Model), i, vp, vp,
                                  C.POINTER(Opts), vp, vp, C.POINTER(Stats)]
     L.fold3d_synchronize.argtypes = [vp]
+    cp = C.c_char_p
+    L.fold3d_write_text.argtypes = [cp, dp, i, i, i]
+    L.fold3d_read_text.argtypes = [cp, dp, i, i, i]
+    L.fold3d_write_vtk.argtypes = [cp, dp, i, i, i, d, d, d, i]
     L.ImplicitInitialiser.argtypes = [dp, i, i, i, d, d, d]
     L.InitFoldFromPoint.restype = d
     L.InitFoldFromPoint.argtypes = [d] * 7
@@ -178,6 +183,34 @@ def measure_peaks(stream=None) -> dict:
     check(lib().fold3d_measure_peaks(stream, C.byref(f64), C.byref(f32), C.byref(cp)))
     return {"fp64_tflops": f64.value / 1e12, "fp32_tflops": f32.value / 1e12,
             "copy_gbs": cp.value / 1e9}
+
+
+def _dims(n):
+    return (n, n, n) if np.isscalar(n) else tuple(n)
+
+
+def write_text(path: str, T: np.ndarray, n) -> None:
+    """The reference's "%f" solution dump of a host FP64 field."""
+    nx, ny, nz = _dims(n)
+    T = np.ascontiguousarray(T, dtype=np.float64)
+    check(lib().fold3d_write_text(os.fsencode(path), T.ctypes.data_as(C.POINTER(C.c_double)),
+                                  nx, ny, nz))
+
+
+def read_text(path: str, n) -> np.ndarray:
+    nx, ny, nz = _dims(n)
+    T = np.empty((nx + 2) * (ny + 2) * (nz + 2))
+    check(lib().fold3d_read_text(os.fsencode(path), T.ctypes.data_as(C.POINTER(C.c_double)),
+                                 nx, ny, nz))
+    return T
+
+
+def write_vtk(path: str, T: np.ndarray, n, spacing, binary: bool = True) -> None:
+    nx, ny, nz = _dims(n)
+    dx, dy, dz = (spacing,) * 3 if np.isscalar(spacing) else spacing
+    T = np.ascontiguousarray(T, dtype=np.float64)
+    check(lib().fold3d_write_vtk(os.fsencode(path), T.ctypes.data_as(C.POINTER(C.c_double)),
+                                 nx, ny, nz, dx, dy, dz, int(binary)))
 
 
 def implicit_initialiser(n: int) -> np.ndarray:

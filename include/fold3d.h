@@ -37,7 +37,8 @@ enum fold3d_status {
   FOLD3D_E_HIP = -2,         /* a HIP runtime call failed (see fold3d_last_error) */
   FOLD3D_E_NOMEM = -3,       /* device allocation failed */
   FOLD3D_E_UNSUPPORTED = -4, /* valid request this build does not implement */
-  FOLD3D_E_NOT_CONVERGED = -5 /* max_iters reached before tol (result still written) */
+  FOLD3D_E_NOT_CONVERGED = -5, /* max_iters reached before tol (result still written) */
+  FOLD3D_E_IO = -6             /* a file could not be opened, read or written */
 };
 
 enum fold3d_precision { FOLD3D_F64 = 0, FOLD3D_F32 = 1 };
@@ -214,6 +215,20 @@ int fold3d_plane_step(fold3d_handle *h, int direction, double *T, int nx,
  * Any output pointer may be NULL.  Allocates 4 GiB briefly for the copy. */
 int fold3d_measure_peaks(void *stream, double *fp64_flops, double *fp32_flops,
                          double *copy_bytes_per_s);
+
+/* Host files of a padded FP64 field (host pointers).
+ * fold3d_write_text / fold3d_read_text: the reference's solution dump
+ *   (AN_ISO_1_OpenACC.c:1244-1260): "%f\n" per node, ghosts included, i
+ *   outermost and k innermost; the reader takes any whitespace-separated
+ *   numbers in that order and fails on too few or too many.
+ * fold3d_write_vtk: legacy VTK STRUCTURED_POINTS of the interior nodes, origin
+ *   0 at node (1,1,1), spacing dx, dy, dz; binary != 0 writes big-endian
+ *   doubles (replaces the header-only Write_VTK_Structured_Grid,
+ *   AN_ISO_1_OpenACC.c:1098-1135). */
+int fold3d_write_text(const char *path, const double *T, int nx, int ny, int nz);
+int fold3d_read_text(const char *path, double *T, int nx, int ny, int nz);
+int fold3d_write_vtk(const char *path, const double *T, int nx, int ny, int nz, double dx,
+                     double dy, double dz, int binary);
 
 /* Device memory helpers for C callers that have no HIP headers
  * (stand in for acc_malloc / acc_memcpy_* at AN_ISO_1_OpenACC.c:1185-1186,1241). */

@@ -43,7 +43,7 @@ def cell_updates(n, iters):
     return iters * 6 * (n - 1) * n * n
 
 
-def time_solves(solver, T0d, n, steps, warmup, dtype, torch, engine=0, iters=8):
+def time_solves(solver, T0d, n, steps, warmup, dtype, torch, engine=0, iters=8, stats=None):
     """Wall time of `steps` solves (reset copy included), plus the summed
     HIP-event time of the sweep loops themselves."""
     T = torch.empty_like(T0d, dtype=dtype)
@@ -57,6 +57,8 @@ def time_solves(solver, T0d, n, steps, warmup, dtype, torch, engine=0, iters=8):
         T.copy_(T0d)
         st = solver.solve(T, n, max_iters=iters, engine=engine)
         sweep_ms += st.sweep_ms
+        if stats is not None:
+            stats.append(st)
     torch.cuda.synchronize()
     return (time.perf_counter() - t0) * 1e3, T, sweep_ms
 
@@ -310,6 +312,22 @@ def main():
     bytes_per_launch = 16.0 * n * n
     achieved = bytes_per_launch / (us * 1e-6) / 1e9
 
+    # exact locking (a separate engine, same bits): time to the same result
+    log("bench: FP64 solve, locked engine (exact locking of unchanged rows)")
+    lst = []
+    ms_l, T_l, _ = time_solves(solver, T0d, n, args.steps, args.warmup, torch.float64, torch,
+                               engine=fold3d.ENGINE_LOCKED, iters=iters, stats=lst)
+    locked = {"ms_per_step": ms_l / args.steps,
+              "reference_equivalent_gcups": W * args.steps / (ms_l / 1e3) / 1e9,
+              "computed_fraction": lst[-1].updates_computed / lst[-1].cell_updates,
+              "same_bits_as_split": bool(torch.equal(T_l.view(torch.int64), T.view(torch.int64))),
+              "note": "time to the same 8-iteration result; rows whose source rows did not "
+                      "change since their last update in the same direction are not recomputed"}
+    log(f"bench: locked {locked['ms_per_step']:.1f} ms/solve, "
+        f"{locked['computed_fraction']:.3f} of the updates computed, same bits: "
+        f"{locked['same_bits_as_split']}")
+    del T_l
+
     engines = {}
     if not args.no_ablation:
         for name, eng in (("plane", fold3d.ENGINE_PLANE), ("hoisted", fold3d.ENGINE_HOISTED),
@@ -419,6 +437,7 @@ def main():
                      "kernel": "fold_split_kernel<double>", "kernel_us": us,
                      "bytes_per_launch": bytes_per_launch},
         "engines_fp64": engines,
+        "locked_engine_fp64": locked,
         "fp32": fp32,
         "elliptic_blob_speed": ellip,
         "cpu_baseline": cpu,

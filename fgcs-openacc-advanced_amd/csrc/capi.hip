@@ -134,8 +134,10 @@ int fold3d_solve(fold3d_handle *h, void *T, int nx, int ny, int nz, double dx,
             "fold3d_solve: bad precision %d", o.precision);
   F3D_CHECK(o.memory == FOLD3D_DEVICE || o.memory == FOLD3D_HOST, FOLD3D_E_INVALID,
             "fold3d_solve: bad memory kind %d", o.memory);
-  F3D_CHECK(o.engine >= FOLD3D_ENGINE_AUTO && o.engine <= FOLD3D_ENGINE_SPLIT, FOLD3D_E_INVALID,
+  F3D_CHECK(o.engine >= FOLD3D_ENGINE_AUTO && o.engine <= FOLD3D_ENGINE_LOCKED, FOLD3D_E_INVALID,
             "fold3d_solve: bad engine %d", o.engine);
+  F3D_CHECK(o.engine != FOLD3D_ENGINE_LOCKED || model->kind == FOLD3D_MODEL_FOLD,
+            FOLD3D_E_UNSUPPORTED, "fold3d_solve: locking is implemented for the fold model only");
   F3D_CHECK(model->kind == FOLD3D_MODEL_FOLD || model->kind == FOLD3D_MODEL_ELLIP,
             FOLD3D_E_INVALID, "fold3d_solve: unknown model kind %d", model->kind);
   if (model->kind == FOLD3D_MODEL_FOLD) {
@@ -177,6 +179,7 @@ int fold3d_solve(fold3d_handle *h, void *T, int nx, int ny, int nz, double dx,
   double maxdt = -1;
   F3D_HIP(hipEventRecord(h->ev0, st));
   int rc;
+  unsigned long long *done_ctr = nullptr;  // cell-updates computed (locking only)
   if (model->kind == FOLD3D_MODEL_FOLD && o.engine == FOLD3D_ENGINE_PLANE) {
     rc = o.precision == FOLD3D_F64
              ? fold_sweeps_plane_engine<double>(st, (double *)dT, G, *model, hsp, o.max_iters,
@@ -204,18 +207,27 @@ int fold3d_solve(fold3d_handle *h, void *T, int nx, int ny, int nz, double dx,
                                                 (const FoldFast<float> *)h->d_consts,
                                                 o.max_iters, o.tol, h->d_max, &iters, &maxdt);
     } else {
-      rc = ensure_scratch(h, bytes);
+      const bool lock = o.engine == FOLD3D_ENGINE_LOCKED;
+      // scratch: T copy [, stamps in two layouts, computed-updates counter]
+      const size_t tbytes = (bytes + 255) & ~(size_t)255;
+      const size_t sbytes = ((size_t)G.count * sizeof(int) + 255) & ~(size_t)255;
+      rc = ensure_scratch(h, lock ? tbytes + 2 * sbytes + 256 * 8 : bytes);
       if (rc != FOLD3D_OK) return rc;
+      char *scr = (char *)h->d_scratch;
+      int *stamp = lock ? (int *)(scr + tbytes) : nullptr;
+      int *stampY = lock ? (int *)(scr + tbytes + sbytes) : nullptr;
+      done_ctr = lock ? (unsigned long long *)(scr + tbytes + 2 * sbytes) : nullptr;
+      if (lock) F3D_HIP(hipMemsetAsync(done_ctr, 0, 256 * sizeof(unsigned long long), st));
       const int groups = o.engine == FOLD3D_ENGINE_TILED ? 0 : h->split_groups;
       rc = o.precision == FOLD3D_F64
                ? fold_sweeps_tiled_engine<double>(st, (double *)dT, (double *)h->d_scratch, G,
                                                   (const FoldFast<double> *)h->d_consts,
                                                   o.max_iters, o.tol, groups, h->d_max, &iters,
-                                                  &maxdt)
+                                                  &maxdt, stamp, stampY, done_ctr)
                : fold_sweeps_tiled_engine<float>(st, (float *)dT, (float *)h->d_scratch, G,
                                                  (const FoldFast<float> *)h->d_consts,
                                                  o.max_iters, o.tol, groups, h->d_max, &iters,
-                                                 &maxdt);
+                                                 &maxdt, stamp, stampY, done_ctr);
     }
   } else if (model->theta == nullptr && o.engine != FOLD3D_ENGINE_PLANE &&
              o.engine != FOLD3D_ENGINE_HOISTED) {
@@ -271,6 +283,14 @@ int fold3d_solve(fold3d_handle *h, void *T, int nx, int ny, int nz, double dx,
       w += 2.0 * (G.n[d] - 1) * (double)G.n[(d + 1) % 3] * G.n[(d + 2) % 3];
     stats->cell_updates = w * iters;
     stats->sweep_ms = ms;
+    stats->updates_computed = stats->cell_updates;
+    if (done_ctr) {
+      unsigned long long done[256];
+      F3D_HIP(hipMemcpy(done, done_ctr, sizeof done, hipMemcpyDeviceToHost));
+      double sum = 0;
+      for (int k = 0; k < 256; ++k) sum += (double)done[k];
+      stats->updates_computed = sum;
+    }
   }
   if (o.tol > 0 && !(maxdt < o.tol)) {
     set_error("fold3d_solve: max|dT| = %g >= tol %g after %d iterations", maxdt, o.tol, iters);

@@ -407,20 +407,40 @@ __device__ __forceinline__ R split_dispatch(int grp, R told, const Patch<R> &p,
   }
 }
 
-template <typename R, bool kReduce, int kGroups>
+constexpr int kLockCounters = 256;
+
+// Locking (kLock): stamp[x] is the sweep index sigma = 6*iteration + direction
+// at which node x last changed (0 before any change).  A row of targets was
+// last updated in this direction at sweep sigma - 6, after that sweep had
+// finished the source plane; if no node of its three source rows has changed
+// since (every stamp <= sigma - 6), its candidates are the ones it saw then,
+// all >= its value, so the update would return the value unchanged and the
+// row is skipped.  Same bits as without locking.
+template <typename R, bool kReduce, int kGroups, bool kLock>
 __global__ __launch_bounds__(64 * kGroups) void fold_split_kernel(
     R *__restrict__ T, PlaneGeom g, const FoldFast<R> *__restrict__ C,
-    unsigned long long *__restrict__ max_dt) {
+    unsigned long long *__restrict__ max_dt, int *__restrict__ stamp, int sigma,
+    unsigned long long *__restrict__ updates_done) {
   __shared__ R s[3][64 + 2];
   __shared__ R part[kGroups][64];
   const int tx = threadIdx.x;
   const int grp = __builtin_amdgcn_readfirstlane(threadIdx.y);
   const int a0 = blockIdx.x * 64, b = 1 + blockIdx.y;
+  int changed = 0;
   for (int idx = grp * 64 + tx; idx < 3 * 66; idx += 64 * kGroups) {
     const int la = idx % 66, lb = idx / 66;
     const int sa = a0 + la;
-    if (sa <= g.na + 1)
-      s[lb][la] = fabs(T[g.base + (long long)sa * g.Sa + (long long)(b - 1 + lb) * g.Sb]);
+    if (sa <= g.na + 1) {
+      const long long x = g.base + (long long)sa * g.Sa + (long long)(b - 1 + lb) * g.Sb;
+      s[lb][la] = fabs(T[x]);
+      if (kLock) changed |= stamp[x] > sigma - 6;
+    }
+  }
+  if (kLock) {
+    if (sigma >= 6 && !__syncthreads_or(changed)) return;  // uniform over the block
+    if (tx == 0 && grp == 0)  // spread over kLockCounters addresses
+      atomicAdd(&updates_done[(blockIdx.x + 7u * blockIdx.y) % kLockCounters],
+                (unsigned long long)(g.na - a0 < 64 ? g.na - a0 : 64));
   }
   const int a = a0 + 1 + tx;
   const bool active = a <= g.na;
@@ -452,7 +472,10 @@ __global__ __launch_bounds__(64 * kGroups) void fold_split_kernel(
 #pragma unroll
       for (int k = 1; k < kGroups; ++k) m = rmin(part[k][tx], m);
       const R res = ((told > R(0)) ? R(1) : R(-1)) * m;
-      if (res != told) T[c + g.sm] = res;
+      if (res != told) {
+        T[c + g.sm] = res;
+        if (kLock) stamp[c + g.sm] = sigma;
+      }
       if (kReduce) d = fabs(res - told);
     }
     if (kReduce) {
@@ -464,32 +487,50 @@ __global__ __launch_bounds__(64 * kGroups) void fold_split_kernel(
   }
 }
 
+// Locking state of one solve (null stamps: no locking).
+struct LockState {
+  int *stamp;                          // current layout
+  int sigma;                           // sweep index of the launch
+  unsigned long long *updates_done;    // cell-updates computed (device counter)
+};
+
 template <typename R, int kGroups>
 static void launch_split(hipStream_t st, R *T, const PlaneGeom &g, const FoldFast<R> *C,
-                         unsigned long long *max_dt) {
+                         unsigned long long *max_dt, const LockState &L) {
   const dim3 block(64, kGroups);
   const dim3 grid((g.na + 63) / 64, g.nb);
-  if (max_dt)
-    fold_split_kernel<R, true, kGroups><<<grid, block, 0, st>>>(T, g, C, max_dt);
-  else
-    fold_split_kernel<R, false, kGroups><<<grid, block, 0, st>>>(T, g, C, nullptr);
+  if (L.stamp) {
+    if (max_dt)
+      fold_split_kernel<R, true, kGroups, true><<<grid, block, 0, st>>>(T, g, C, max_dt, L.stamp,
+                                                                         L.sigma, L.updates_done);
+    else
+      fold_split_kernel<R, false, kGroups, true><<<grid, block, 0, st>>>(T, g, C, nullptr, L.stamp,
+                                                                          L.sigma, L.updates_done);
+  } else if (max_dt) {
+    fold_split_kernel<R, true, kGroups, false><<<grid, block, 0, st>>>(T, g, C, max_dt, nullptr, 0,
+                                                                        nullptr);
+  } else {
+    fold_split_kernel<R, false, kGroups, false><<<grid, block, 0, st>>>(T, g, C, nullptr, nullptr,
+                                                                         0, nullptr);
+  }
 }
 
 // groups: 0 = one thread per node (fold_tile_kernel), 2/4/8 = waves per node row.
 template <typename R>
 static void tiled_direction(hipStream_t st, R *T, const Grid &G, int d,
-                            const FoldFast<R> *C, unsigned long long *max_dt, int groups) {
+                            const FoldFast<R> *C, unsigned long long *max_dt, int groups,
+                            const LockState &L) {
   const Frame &fr = kFrames[d];
   const int n = G.n[fr.m];
   for (int q = 0; q < n - 1; ++q) {
     const int p = fr.s > 0 ? 1 + q : n - q;
     const PlaneGeom g = plane_geom(G, d, p);
     if (groups == 2) {
-      launch_split<R, 2>(st, T, g, C, max_dt);
+      launch_split<R, 2>(st, T, g, C, max_dt, L);
     } else if (groups == 4) {
-      launch_split<R, 4>(st, T, g, C, max_dt);
+      launch_split<R, 4>(st, T, g, C, max_dt, L);
     } else if (groups == 8) {
-      launch_split<R, 8>(st, T, g, C, max_dt);
+      launch_split<R, 8>(st, T, g, C, max_dt, L);
     } else {
       const dim3 block(kTileA, kTileB);
       const dim3 grid((g.na + kTileA - 1) / kTileA, (g.nb + kTileB - 1) / kTileB);
@@ -504,8 +545,18 @@ static void tiled_direction(hipStream_t st, R *T, const Grid &G, int d,
 template <typename R>
 int fold_sweeps_tiled_engine(hipStream_t st, R *T, R *Y, const Grid &G,
                              const FoldFast<R> *d_consts, int max_iters, double tol, int groups,
-                             unsigned long long *d_max, int *iters_out, double *max_dt_out) {
+                             unsigned long long *d_max, int *iters_out, double *max_dt_out,
+                             int *stamp, int *stampY, unsigned long long *updates_done) {
   const Grid GX = G.with_order(kOrderX), GZ = G.with_order(kOrderZ), GY = G.with_order(kOrderY);
+  const bool lock = stamp != nullptr && groups > 0;
+  if (lock) F3D_HIP(hipMemsetAsync(stamp, 0, sizeof(int) * G.count, st));
+  // stamps move with T (4-byte fields: the float permute copies the bits)
+  auto permute_stamp = [&](const int *in, const Grid &Gi, int *out, const Grid &Go) {
+    if (lock) permute_layout<float>(st, (const float *)in, Gi, (float *)out, Go);
+  };
+  auto lk = [&](int *sp, int it, int d) {
+    return LockState{lock ? sp : nullptr, 6 * it + d, updates_done};
+  };
   int it = 0;
   double last = -1.0;
   for (; it < max_iters;) {
@@ -513,17 +564,27 @@ int fold_sweeps_tiled_engine(hipStream_t st, R *T, R *Y, const Grid &G,
     if (md) zero_u64_kernel<<<1, 1, 0, st>>>(d_max);
     // x_down, x_up with x slowest
     permute_layout<R>(st, T, GZ, Y, GX);
-    tiled_direction<R>(st, Y, GX, FOLD3D_X_DOWN, d_consts + FOLD3D_X_DOWN, md, groups);
-    tiled_direction<R>(st, Y, GX, FOLD3D_X_UP, d_consts + FOLD3D_X_UP, md, groups);
+    permute_stamp(stamp, GZ, stampY, GX);
+    tiled_direction<R>(st, Y, GX, FOLD3D_X_DOWN, d_consts + FOLD3D_X_DOWN, md, groups,
+                       lk(stampY, it, FOLD3D_X_DOWN));
+    tiled_direction<R>(st, Y, GX, FOLD3D_X_UP, d_consts + FOLD3D_X_UP, md, groups,
+                       lk(stampY, it, FOLD3D_X_UP));
     permute_layout<R>(st, Y, GX, T, GZ);
+    permute_stamp(stampY, GX, stamp, GZ);
     // k_up, k_down in the reference layout
-    tiled_direction<R>(st, T, GZ, FOLD3D_K_UP, d_consts + FOLD3D_K_UP, md, groups);
-    tiled_direction<R>(st, T, GZ, FOLD3D_K_DOWN, d_consts + FOLD3D_K_DOWN, md, groups);
+    tiled_direction<R>(st, T, GZ, FOLD3D_K_UP, d_consts + FOLD3D_K_UP, md, groups,
+                       lk(stamp, it, FOLD3D_K_UP));
+    tiled_direction<R>(st, T, GZ, FOLD3D_K_DOWN, d_consts + FOLD3D_K_DOWN, md, groups,
+                       lk(stamp, it, FOLD3D_K_DOWN));
     // y_up, y_down with y slowest
     permute_layout<R>(st, T, GZ, Y, GY);
-    tiled_direction<R>(st, Y, GY, FOLD3D_Y_UP, d_consts + FOLD3D_Y_UP, md, groups);
-    tiled_direction<R>(st, Y, GY, FOLD3D_Y_DOWN, d_consts + FOLD3D_Y_DOWN, md, groups);
+    permute_stamp(stamp, GZ, stampY, GY);
+    tiled_direction<R>(st, Y, GY, FOLD3D_Y_UP, d_consts + FOLD3D_Y_UP, md, groups,
+                       lk(stampY, it, FOLD3D_Y_UP));
+    tiled_direction<R>(st, Y, GY, FOLD3D_Y_DOWN, d_consts + FOLD3D_Y_DOWN, md, groups,
+                       lk(stampY, it, FOLD3D_Y_DOWN));
     permute_layout<R>(st, Y, GY, T, GZ);
+    permute_stamp(stampY, GY, stamp, GZ);
     F3D_HIP(hipGetLastError());
     ++it;
     if (md) {
@@ -540,9 +601,11 @@ int fold_sweeps_tiled_engine(hipStream_t st, R *T, R *Y, const Grid &G,
 }
 template int fold_sweeps_tiled_engine<double>(hipStream_t, double *, double *, const Grid &,
                                               const FoldFast<double> *, int, double, int,
-                                              unsigned long long *, int *, double *);
+                                              unsigned long long *, int *, double *, int *, int *,
+                                              unsigned long long *);
 template int fold_sweeps_tiled_engine<float>(hipStream_t, float *, float *, const Grid &,
                                              const FoldFast<float> *, int, double, int,
-                                             unsigned long long *, int *, double *);
+                                             unsigned long long *, int *, double *, int *, int *,
+                                             unsigned long long *);
 
 }  // namespace fold3d

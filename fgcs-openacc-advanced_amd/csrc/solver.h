@@ -68,10 +68,16 @@ int fold_sweeps_fast_engine(hipStream_t st, R *T, const Grid &G, const FoldFast<
 // scratch field of G.count elements.
 // `groups`: 0 = one thread per node (fold_tile_kernel); 2, 4 or 8 = the
 // node's eight candidate units spread over that many waves (fold_split_kernel).
+// `stamp`, `stampY` (G.count ints each, or null): exact locking of rows whose
+// source rows have not changed since their last update in the same direction;
+// `updates_done` (device, 256 counters zeroed by the caller) counts the
+// cell-updates computed.
 template <typename R>
 int fold_sweeps_tiled_engine(hipStream_t st, R *T, R *Y, const Grid &G,
                              const FoldFast<R> *d_consts, int max_iters, double tol, int groups,
-                             unsigned long long *d_max, int *iters_out, double *max_dt_out);
+                             unsigned long long *d_max, int *iters_out, double *max_dt_out,
+                             int *stamp = nullptr, int *stampY = nullptr,
+                             unsigned long long *updates_done = nullptr);
 
 // Batched elliptic solves (batch.hip).  `blocks` = persistent grid per launch,
 // `groups` = waves per row (1, 2, 4), `rows_per_pull` = rows taken from the

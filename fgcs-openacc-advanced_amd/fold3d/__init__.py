@@ -30,6 +30,7 @@ F64, F32 = 0, 1
 DEVICE, HOST = 0, 1
 MODEL_FOLD, MODEL_ELLIP = 0, 1
 ENGINE_AUTO, ENGINE_PLANE, ENGINE_HOISTED, ENGINE_TILED, ENGINE_SPLIT = 0, 1, 2, 3, 4
+ENGINE_LOCKED = 5
 
 MAXT = 10e15 * 1.0
 REF_F = 1.1                      # reference Fold3dPMM.h:52
@@ -75,6 +76,7 @@ class Stats(C.Structure):
         ("max_dT", C.c_double),
         ("cell_updates", C.c_double),
         ("sweep_ms", C.c_double),
+        ("updates_computed", C.c_double),
     ]
 
 

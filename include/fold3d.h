@@ -101,9 +101,16 @@ enum fold3d_engine {
                                 each sweep pair run in a layout whose march axis
                                 is slowest (needs a scratch copy of T, owned by
                                 the handle) */
-  FOLD3D_ENGINE_SPLIT = 4    /* as TILED, with each node's candidate units spread
-                                over four wavefronts whose minima meet in LDS
-                                (same bits; the default for the fold model) */
+  FOLD3D_ENGINE_SPLIT = 4,   /* as TILED, with each node's candidate units spread
+                                over several wavefronts whose minima meet in LDS
+                                (same bits; the default for the fold model; for the
+                                elliptic model the split or metric engine) */
+  FOLD3D_ENGINE_LOCKED = 5   /* fold model: SPLIT plus exact locking — a row of 64
+                                targets whose source rows have not changed since
+                                its previous update in the same direction is not
+                                recomputed (its update provably returns the value
+                                it has).  Same bits as SPLIT; stats->updates_computed
+                                tells how many cell-updates were evaluated */
 };
 
 typedef struct fold3d_stats {
@@ -111,6 +118,8 @@ typedef struct fold3d_stats {
   double max_dT;         /* max |T_new - T_old| over the last iteration (tol > 0 only) */
   double cell_updates;   /* iterations * 6 * (n-1) * n^2 for cubes */
   double sweep_ms;       /* device time of the sweep loop (HIP events) */
+  double updates_computed; /* cell-updates evaluated: cell_updates, or fewer with
+                              FOLD3D_ENGINE_LOCKED (rows skipped by locking) */
 } fold3d_stats;
 
 typedef struct fold3d_handle fold3d_handle;

@@ -62,7 +62,8 @@ def test_direction_on_initial_field(solver, golden, d):
     assert bits_equal(T.cpu().numpy(), golden[f"direction_init_{d}_16"])
 
 
-ENGINES = [fold3d.ENGINE_PLANE, fold3d.ENGINE_HOISTED, fold3d.ENGINE_TILED, fold3d.ENGINE_SPLIT]
+ENGINES = [fold3d.ENGINE_PLANE, fold3d.ENGINE_HOISTED, fold3d.ENGINE_TILED, fold3d.ENGINE_SPLIT,
+           fold3d.ENGINE_LOCKED]
 
 
 @pytest.mark.parametrize("engine", ENGINES)
@@ -235,3 +236,49 @@ def test_fp32_within_tolerance_of_fp64(solver, oracle, n):
     print(f"FP32 n={n}: max rel err {rel.max():.3e}, median {np.median(rel):.3e}, "
           f"fraction above 1e-5 {frac:.2e}")
     assert frac < 1e-2 and rel.max() < 1e-2
+
+
+@pytest.mark.parametrize("groups", [2, 4, 8])
+@pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
+def test_locking_gives_the_same_bits(solver, groups, dtype):
+    """Exact locking: the reference workload at 64^3 for 8 iterations and a
+    box grid with a random field, locked and unlocked, same bits, with fewer
+    updates computed once the field settles."""
+    solver.set_tuning(groups, fold3d.MODEL_FOLD)
+    try:
+        n = 64
+        T0 = to_dev(fold3d.implicit_initialiser(n), dtype)
+        A, B = T0.clone(), T0.clone()
+        sa = solver.solve(A, n, max_iters=8, engine=fold3d.ENGINE_SPLIT)
+        sb = solver.solve(B, n, max_iters=8, engine=fold3d.ENGINE_LOCKED)
+        assert torch.equal(A.view(torch.int64 if dtype == torch.float64 else torch.int32),
+                           B.view(torch.int64 if dtype == torch.float64 else torch.int32))
+        assert sa.updates_computed == sa.cell_updates
+        assert sb.updates_computed < 0.8 * sb.cell_updates
+        print(f"locked {dtype} G={groups}: {sb.updates_computed / sb.cell_updates:.3f} of the updates computed")
+        n3 = (37, 70, 29)
+        rng = np.random.default_rng(groups)
+        size = (n3[0] + 2) * (n3[1] + 2) * (n3[2] + 2)
+        R0 = np.where(rng.random(size) < 0.3, -1.0, 1.0) * rng.uniform(1.0, 5.0, size)
+        h, a = (0.3, 0.25, 0.35), (-0.35, 0.4, 0.7)
+        A, B = to_dev(R0, dtype), to_dev(R0, dtype)
+        m = fold3d.fold_model(1.1, a)
+        solver.solve(A, n3, spacing=h, model=m, max_iters=6, engine=fold3d.ENGINE_SPLIT)
+        sb = solver.solve(B, n3, spacing=h, model=m, max_iters=60, tol=1e-300,
+                          engine=fold3d.ENGINE_LOCKED, raise_not_converged=False)
+        C = to_dev(R0, dtype)
+        sc = solver.solve(C, n3, spacing=h, model=m, max_iters=60, tol=1e-300,
+                          engine=fold3d.ENGINE_SPLIT, raise_not_converged=False)
+        assert sb.iterations == sc.iterations
+        assert torch.equal(B.double(), C.double())
+    finally:
+        solver.set_tuning(0, fold3d.MODEL_FOLD)
+
+
+def test_locking_is_rejected_for_the_elliptic_model(solver):
+    from fold3d import workloads as wl
+    W = wl.tilted_homogeneous(16)
+    T = torch.zeros(18 ** 3, dtype=torch.float64, device="cuda")
+    with pytest.raises(fold3d.Fold3dError) as e:
+        solver.solve(T, 16, spacing=W.h, model=W.model(), engine=fold3d.ENGINE_LOCKED)
+    assert e.value.code == fold3d.FOLD3D_E_UNSUPPORTED

@@ -291,6 +291,20 @@ def main():
     log(f"bench: initializer took {time.perf_counter() - t0:.1f} s")
     T0d = torch.from_numpy(T0).to("cuda")
     solver = fold3d.Solver(0)
+    host_init_s = time.perf_counter() - t0
+    # the same field built on the device (same bits as the host initializer)
+    Td = torch.empty_like(T0d)
+    solver.initialiser_device(Td, n)  # warm-up (code load)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    n_exact = solver.initialiser_device(Td, n)
+    torch.cuda.synchronize()
+    dev_init_s = time.perf_counter() - t1
+    init = {"host_s": host_init_s, "device_s": dev_init_s, "host_evaluated_nodes": n_exact,
+            "same_bits": bool(torch.equal(Td.view(torch.int64), T0d.view(torch.int64)))}
+    log(f"bench: device initializer {dev_init_s:.3f} s (host {host_init_s:.1f} s), "
+        f"same bits: {init['same_bits']}")
+    del Td
 
     log("bench: autotuning waves per node row (one iteration each of 2, 4, 8)")
     groups, tune_ms = solver.autotune(T0d, n, iterations=1)
@@ -438,6 +452,7 @@ def main():
                      "bytes_per_launch": bytes_per_launch},
         "engines_fp64": engines,
         "locked_engine_fp64": locked,
+        "initializer": init,
         "fp32": fp32,
         "elliptic_blob_speed": ellip,
         "cpu_baseline": cpu,

@@ -517,6 +517,19 @@ int fold3d_get_tuning(fold3d_handle *h, int model_kind, int *groups) {
   return FOLD3D_OK;
 }
 
+int fold3d_initialiser_device(fold3d_handle *h, double *T, int nx, int ny, int nz, double dx,
+                              double dy, double dz, void *stream, long long *n_exact) {
+  F3D_CHECK(h && T, FOLD3D_E_INVALID, "fold3d_initialiser_device: NULL handle or T");
+  F3D_CHECK(nx >= 1 && ny >= 1 && nz >= 1, FOLD3D_E_INVALID,
+            "fold3d_initialiser_device: bad grid %d x %d x %d", nx, ny, nz);
+  F3D_CHECK(dx > 0 && dy > 0 && dz > 0 && std::isfinite(dx) && std::isfinite(dy) &&
+                std::isfinite(dz),
+            FOLD3D_E_INVALID, "fold3d_initialiser_device: spacings must be positive");
+  F3D_HIP(hipSetDevice(h->device));
+  fold3d_set_grid(nx, ny, nz);
+  return device_initialiser((hipStream_t)stream, T, nx, ny, nz, dx, dy, dz, n_exact);
+}
+
 int fold3d_set_batch_tuning(fold3d_handle *h, int groups, int blocks_per_cu, int rows_per_pull) {
   F3D_CHECK(h != nullptr, FOLD3D_E_INVALID, "fold3d_set_batch_tuning: NULL handle");
   F3D_CHECK(groups == 0 || groups == 1 || groups == 2 || groups == 4, FOLD3D_E_INVALID,

@@ -97,6 +97,11 @@ int adjoint_solve(hipStream_t st, const R *T, const Grid &G, const fold3d_model 
                   unsigned long long *d_max, int *iters_out, double *maxd_out);
 size_t adjoint_scratch_bytes(const Grid &G, int precision);
 
+// The reference's input field built on the device, same bits as the host
+// initializer (gpu_init.hip); n_exact_out = nodes whose IF the host evaluated.
+int device_initialiser(hipStream_t st, double *T, int nx, int ny, int nz, double dx, double dy,
+                       double dz, long long *n_exact_out);
+
 // Copy a padded field from one axis order to another (layout.hip).
 template <typename R>
 void permute_layout(hipStream_t st, const R *in, const Grid &Gin, R *out, const Grid &Gout);

@@ -132,6 +132,7 @@ def lib() -> C.CDLL:
     L.fold3d_adjoint.argtypes = [vp, vp, i, i, i, d, d, d, C.POINTER(Model), i, vp, vp,
                                  C.POINTER(Opts), vp, vp, C.POINTER(Stats)]
     L.fold3d_synchronize.argtypes = [vp]
+    L.fold3d_initialiser_device.argtypes = [vp, vp, i, i, i, d, d, d, vp, C.POINTER(C.c_longlong)]
     cp = C.c_char_p
     L.fold3d_write_text.argtypes = [cp, dp, i, i, i]
     L.fold3d_read_text.argtypes = [cp, dp, i, i, i]
@@ -359,6 +360,24 @@ class Solver:
 
     def set_tuning(self, groups: int, kind: int = MODEL_FOLD):
         check(lib().fold3d_set_tuning(self.h, kind, groups))
+
+    def initialiser_device(self, T, n, spacing=None, stream=None) -> int:
+        """The reference's input field built on the device into the FP64 torch
+        tensor T (same bits as implicit_initialiser); returns the number of
+        nodes whose surface function the host evaluated."""
+        import torch
+        nx, ny, nz = _dims(n)
+        if spacing is None:
+            spacing = reference_spacing(nx)
+        dx, dy, dz = (spacing,) * 3 if np.isscalar(spacing) else spacing
+        if T.dtype != torch.float64 or not T.is_cuda:
+            raise TypeError("T must be a float64 device tensor")
+        if stream is None:
+            stream = torch.cuda.current_stream(T.device).cuda_stream
+        ne = C.c_longlong()
+        check(lib().fold3d_initialiser_device(self.h, C.c_void_p(T.data_ptr()), nx, ny, nz,
+                                              dx, dy, dz, stream, C.byref(ne)))
+        return ne.value
 
     def set_batch_tuning(self, groups: int = 0, blocks_per_cu: int = 0, rows_per_pull: int = 0):
         check(lib().fold3d_set_batch_tuning(self.h, groups, blocks_per_cu, rows_per_pull))

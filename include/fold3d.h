@@ -194,6 +194,17 @@ int fold3d_autotune(fold3d_handle *h, const void *T, int nx, int ny, int nz,
                     double dx, double dy, double dz, const fold3d_model *model,
                     const fold3d_opts *opts, int iterations, float *ms_out);
 
+/* The reference's input field (ImplicitInitialiser, Fold3dPMM.c:129-389)
+ * built on the device into a device FP64 array T on the padded grid, with the
+ * same bits as the host initializer: the device evaluates the implicit
+ * surface everywhere, and the host's libm evaluates it again at the nodes
+ * whose exact value matters (near the surface: ends of edges where the sign
+ * changes, and nodes where the sign is in doubt); *n_exact (may be NULL)
+ * receives their number.  Records the grid for the reference-signature plane
+ * steps like ImplicitInitialiser.  Synchronous on `stream`. */
+int fold3d_initialiser_device(fold3d_handle *h, double *T, int nx, int ny, int nz, double dx,
+                              double dy, double dz, void *stream, long long *n_exact);
+
 /* Tuning of the batched solve: waves per row of 64 nodes (1, 2, 4; default
  * 1), persistent blocks per compute unit in each launch (1..64; default 32)
  * and how a block finds its rows: 1..64 = that many rows per pull from the

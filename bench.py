@@ -190,7 +190,7 @@ def c2_run(torch, n=1280):
                "value": st.cell_updates / (st.sweep_ms / 1e3) / 1e9, "unit": "Gcell-updates/s",
                "cell_updates": st.cell_updates,
                "torch_device_bytes": torch.cuda.max_memory_allocated(),
-               "solver_scratch_bytes": 15 * (n + 2) ** 3 * 4}
+               "solver_scratch_bytes_max": 22 * (n + 2) ** 3 * 4}
         del W, T0, T
     finally:
         solver.close()

@@ -39,11 +39,12 @@ namespace fold3d {
 enum BatchLayout { kLayZ = 0, kLayX = 1, kLayY = 2 };
 
 // Device view of one instance.  Scratch per instance: Y (T in layout X or Y)
-// and two sets of kMetricFields fields (slowness and N = M^-1): MZ in the
-// reference layout, MX in the layout of the current sweep pair.
+// and kMetricFields fields (slowness and N = M^-1) per layout: MZ in the
+// reference layout, MX and MY in the x- and y-sweep layouts (made once per
+// batch), or MX = MY remade before each sweep pair when memory is short.
 template <typename R>
 struct BatchInst {
-  R *T, *Y, *MZ, *MX;
+  R *T, *Y, *MZ, *MX, *MY;
   long long count;
   long long stride[3][3];  // [layout][axis]
   int n[3];
@@ -164,7 +165,7 @@ __global__ __launch_bounds__(64 * kGroups) void batch_plane_kernel(
     const int tiles_a = (na + 63) / 64;
     const int a0 = (int)(r % tiles_a) * 64, b = 1 + (int)(r / tiles_a);
     R *T = lay == kLayZ ? I.T : I.Y;
-    const R *M = lay == kLayZ ? I.MZ : I.MX;
+    const R *M = lay == kLayZ ? I.MZ : lay == kLayX ? I.MX : I.MY;
     for (int idx = grp * 64 + tx; idx < 3 * 66; idx += 64 * kGroups) {
       const int l_a = idx % 66, l_b = idx / 66;
       const int sa = a0 + l_a;
@@ -230,7 +231,7 @@ constexpr int kBTile = 32, kBTileRows = 8;
 
 template <typename R>
 struct PermuteJob {
-  int which;          // 0: T -> Y, 1: Y -> T, 2: MZ -> MX
+  int which;          // 0: T -> Y, 1: Y -> T, 2: MZ -> MX, 3: MZ -> MY
   int lin, lout;      // layouts
   int fields;
 };
@@ -259,7 +260,7 @@ __global__ __launch_bounds__(kBTile *kBTileRows) void batch_permute_kernel(
   const long long *is = I.stride[job.lin], *os = I.stride[job.lout];
   const bool in_x = is[0] == 1, out_x = os[0] == 1;
   const R *src = job.which == 0 ? I.T : job.which == 1 ? I.Y : I.MZ;
-  R *dst = job.which == 0 ? I.Y : job.which == 1 ? I.T : I.MX;
+  R *dst = job.which == 0 ? I.Y : job.which == 1 ? I.T : job.which == 2 ? I.MX : I.MY;
   const int x0 = bx * kBTile, y0 = by * kBTile;
   const int tx = threadIdx.x;
   for (int f = 0; f < job.fields; ++f) {
@@ -315,12 +316,14 @@ static long long rows_of(const fold3d_instance &x, int d) {
   return (long long)((x.n[la] + 63) / 64) * x.n[lb];
 }
 
-size_t batch_scratch_bytes(int count, const fold3d_instance *inst, int precision) {
+size_t batch_scratch_bytes(int count, const fold3d_instance *inst, int precision,
+                           bool resident) {
   const size_t r = precision == FOLD3D_F64 ? 8 : 4;
+  const int fields = 1 + (resident ? 3 : 2) * kMetricFields;
   size_t total = 0;
   for (int i = 0; i < count; ++i) {
     const Grid G(inst[i].n[0], inst[i].n[1], inst[i].n[2]);
-    total += (size_t)G.count * r * (1 + 2 * kMetricFields);
+    total += (size_t)G.count * r * fields;
     total = (total + 255) & ~(size_t)255;
   }
   return total;
@@ -329,7 +332,8 @@ size_t batch_scratch_bytes(int count, const fold3d_instance *inst, int precision
 template <typename R>
 int batch_solve(hipStream_t user_st, int count, const fold3d_instance *inst, int max_iters,
                 double tol, int blocks, int groups, int rows_per_pull, int use_graph,
-                void *scratch, int *iters_out, double *max_dt_out, double *ms_out) {
+                void *scratch, bool resident, int *iters_out, double *max_dt_out,
+                double *ms_out) {
   int dev = 0;
   F3D_HIP(hipGetDevice(&dev));
   // The batch runs on a stream of its own (the legacy default stream cannot be
@@ -357,7 +361,8 @@ int batch_solve(hipStream_t user_st, int count, const fold3d_instance *inst, int
       b.Y = (R *)p;
       b.MZ = b.Y + GZ.count;
       b.MX = b.MZ + kMetricFields * GZ.count;
-      p += (size_t)GZ.count * sizeof(R) * (1 + 2 * kMetricFields);
+      b.MY = resident ? b.MX + kMetricFields * GZ.count : b.MX;
+      p += (size_t)GZ.count * sizeof(R) * (1 + (resident ? 3 : 2) * kMetricFields);
       p = (char *)(((uintptr_t)p + 255) & ~(uintptr_t)255);
       for (int a = 0; a < 3; ++a) {
         b.stride[kLayZ][a] = GZ.stride[a];
@@ -449,6 +454,13 @@ int batch_solve(hipStream_t user_st, int count, const fold3d_instance *inst, int
           (const R *)x.phi, (const R *)x.ra, (const R *)x.rb, (R)N[0], (R)N[1], (R)N[2],
           (R)N[3], (R)N[4], (R)N[5], hinst[i].MZ);
     }
+    if (resident) {  // the x- and y-sweep layouts of the metric fields, once per batch
+      const PermuteJob<R> jx{2, kLayZ, kLayX, kMetricFields}, jy{3, kLayZ, kLayY, kMetricFields};
+      batch_permute_kernel<R><<<(unsigned)total_tiles, dim3(kBTile, kBTileRows), 0, st>>>(
+          d_inst, d_tiles, count, jx, d_done);
+      batch_permute_kernel<R><<<(unsigned)total_tiles, dim3(kBTile, kBTileRows), 0, st>>>(
+          d_inst, d_tiles, count, jy, d_done);
+    }
     BATCH_HIP(hipGetLastError());
     BATCH_HIP(hipStreamSynchronize(st));
 
@@ -493,14 +505,14 @@ int batch_solve(hipStream_t user_st, int count, const fold3d_instance *inst, int
         }
       };
       permute(0, kLayZ, kLayX, 1);
-      permute(2, kLayZ, kLayX, kMetricFields);
+      if (!resident) permute(2, kLayZ, kLayX, kMetricFields);
       direction(FOLD3D_X_DOWN, kLayX);
       direction(FOLD3D_X_UP, kLayX);
       permute(1, kLayX, kLayZ, 1);
       direction(FOLD3D_K_UP, kLayZ);
       direction(FOLD3D_K_DOWN, kLayZ);
       permute(0, kLayZ, kLayY, 1);
-      permute(2, kLayZ, kLayY, kMetricFields);
+      if (!resident) permute(3, kLayZ, kLayY, kMetricFields);
       direction(FOLD3D_Y_UP, kLayY);
       direction(FOLD3D_Y_DOWN, kLayY);
       permute(1, kLayY, kLayZ, 1);
@@ -590,8 +602,8 @@ cleanup:
 }
 
 template int batch_solve<double>(hipStream_t, int, const fold3d_instance *, int, double, int, int,
-                                 int, int, void *, int *, double *, double *);
+                                 int, int, void *, bool, int *, double *, double *);
 template int batch_solve<float>(hipStream_t, int, const fold3d_instance *, int, double, int, int,
-                                int, int, void *, int *, double *, double *);
+                                int, int, void *, bool, int *, double *, double *);
 
 }  // namespace fold3d

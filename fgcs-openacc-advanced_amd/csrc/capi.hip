@@ -44,6 +44,24 @@ static int ensure_scratch(fold3d_handle *h, size_t bytes) {
   return FOLD3D_OK;
 }
 
+// As ensure_scratch, but a failed allocation is not an error: returns false
+// (with the handle's scratch released) so the caller can ask for less.
+static bool try_scratch(fold3d_handle *h, size_t bytes) {
+  if (h->scratch_bytes >= bytes) return true;
+  if (h->d_scratch) {
+    if (hipDeviceSynchronize() != hipSuccess || hipFree(h->d_scratch) != hipSuccess) return false;
+    h->d_scratch = nullptr;
+    h->scratch_bytes = 0;
+  }
+  if (hipMalloc(&h->d_scratch, bytes) != hipSuccess) {
+    (void)hipGetLastError();  // clear the failed allocation's error state
+    h->d_scratch = nullptr;
+    return false;
+  }
+  h->scratch_bytes = bytes;
+  return true;
+}
+
 static constexpr size_t kConstBytes = 6 * sizeof(FoldFast<double>);
 
 extern "C" {
@@ -248,17 +266,21 @@ int fold3d_solve(fold3d_handle *h, void *T, int nx, int ny, int nz, double dx,
                    h->d_max, &iters, &maxdt);
   } else if (model->theta != nullptr && o.engine != FOLD3D_ENGINE_PLANE &&
              o.engine != FOLD3D_ENGINE_HOISTED) {
-    // per-node metric: metric engine; scratch = T copy + two sets of metric fields
-    rc = ensure_scratch(h, ellip_metric_scratch_fields() * bytes);
-    if (rc != FOLD3D_OK) return rc;
+    // per-node metric: metric engine; scratch = T copy + three sets of metric
+    // fields (reference, x- and y-sweep layouts) when they fit, else two
+    const bool resident = try_scratch(h, ellip_metric_scratch_fields(true) * bytes);
+    if (!resident) {
+      rc = ensure_scratch(h, ellip_metric_scratch_fields(false) * bytes);
+      if (rc != FOLD3D_OK) return rc;
+    }
     const int groups = h->ellip_groups;
     rc = o.precision == FOLD3D_F64
              ? ellip_sweeps_metric_engine<double>(st, (double *)dT, (double *)h->d_scratch, G,
                                                   *model, hsp, o.max_iters, o.tol, groups,
-                                                  h->d_max, &iters, &maxdt)
+                                                  resident, h->d_max, &iters, &maxdt)
              : ellip_sweeps_metric_engine<float>(st, (float *)dT, (float *)h->d_scratch, G,
                                                  *model, hsp, o.max_iters, o.tol, groups,
-                                                 h->d_max, &iters, &maxdt);
+                                                 resident, h->d_max, &iters, &maxdt);
   } else
     rc = o.precision == FOLD3D_F64
              ? ellip_sweeps_plane_engine<double>(st, (double *)dT, G, *model, hsp, o.max_iters,
@@ -340,16 +362,19 @@ static int run_batch(fold3d_handle *h, int count, const fold3d_instance *inst,
   F3D_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, h->device));
   const int use_graph = o.engine != FOLD3D_ENGINE_PLANE;
   const int blocks = cus * blocks_per_cu;
-  const int rc = ensure_scratch(h, batch_scratch_bytes(count, inst, o.precision));
-  if (rc != FOLD3D_OK) return rc;
+  const bool resident = try_scratch(h, batch_scratch_bytes(count, inst, o.precision, true));
+  if (!resident) {
+    const int rc = ensure_scratch(h, batch_scratch_bytes(count, inst, o.precision, false));
+    if (rc != FOLD3D_OK) return rc;
+  }
   hipStream_t st = (hipStream_t)o.stream;
   return o.precision == FOLD3D_F64
              ? batch_solve<double>(st, count, inst, o.max_iters, o.tol, blocks, groups,
-                                   rows_per_pull, use_graph, h->d_scratch, iterations_out,
-                                   max_dt_out, ms_out)
+                                   rows_per_pull, use_graph, h->d_scratch, resident,
+                                   iterations_out, max_dt_out, ms_out)
              : batch_solve<float>(st, count, inst, o.max_iters, o.tol, blocks, groups,
-                                  rows_per_pull, use_graph, h->d_scratch, iterations_out,
-                                  max_dt_out, ms_out);
+                                  rows_per_pull, use_graph, h->d_scratch, resident,
+                                  iterations_out, max_dt_out, ms_out);
 }
 
 int fold3d_solve_batched(fold3d_handle *h, int count, const fold3d_instance *inst,

@@ -358,8 +358,9 @@ int ellip_sweeps_split_engine(hipStream_t st, R *T, R *Y, R *VX, R *VY, const Gr
 // ---------------------------------------------------------------------------
 // Metric engine: per-node metric (tilt and axis-ratio fields).  The slowness
 // and the six entries of N = M^-1 are computed once per solve into seven
-// fields of the reference layout (metric_prep_kernel) and copied, with T,
-// into the x- and y-sweep layouts before those sweeps; the plane steps are
+// fields of the reference layout (metric_prep_kernel) and copied into the x-
+// and y-sweep layouts — once per solve when the scratch holds both copies
+// (`resident`), else before each pair of x- and y-sweeps; the plane steps are
 // the split kernel's (LDS rows, units over kGroups waves) with the node's
 // lattice products read from those fields.  Same bits as ellip_plane_kernel
 // with kMetric.
@@ -525,42 +526,50 @@ static void ellip_metric_direction(hipStream_t st, R *T, const R *fields, const 
   }
 }
 
-size_t ellip_metric_scratch_fields() { return 1 + 2 * kMetricFields; }
+size_t ellip_metric_scratch_fields(bool resident) {
+  return 1 + (resident ? 3 : 2) * kMetricFields;
+}
 
 template <typename R>
 int ellip_sweeps_metric_engine(hipStream_t st, R *T, R *scratch, const Grid &G,
                                const fold3d_model &m, const double h[3], int max_iters,
-                               double tol, int groups, unsigned long long *d_max,
+                               double tol, int groups, bool resident, unsigned long long *d_max,
                                int *iters_out, double *max_dt_out) {
   const long long count = G.count;
   R *Y = scratch;                        // T in the x- or y-sweep layout
   R *MZ = scratch + count;               // metric fields, reference layout
-  R *MX = MZ + kMetricFields * count;    // metric fields, x- or y-sweep layout
+  R *MX = MZ + kMetricFields * count;    // metric fields, x-sweep layout (or x/y, reused)
+  R *MY = resident ? MX + kMetricFields * count : MX;  // metric fields, y-sweep layout
   metric_prep_kernel<R><<<2048, 256, 0, st>>>(count, (const R *)m.speed_field, (R)m.speed,
                                               (const R *)m.theta, (const R *)m.phi,
                                               (const R *)m.ra, (const R *)m.rb, MZ);
   F3D_HIP(hipGetLastError());
   const Grid GX = G.with_order(kOrderX), GZ = G.with_order(kOrderZ), GY = G.with_order(kOrderY);
-  auto permute_metric = [&](const Grid &Gout) {
+  auto permute_metric = [&](R *out, const Grid &Gout) {
     for (int q = 0; q < kMetricFields; ++q)
-      permute_layout<R>(st, MZ + q * count, GZ, MX + q * count, Gout);
+      permute_layout<R>(st, MZ + q * count, GZ, out + q * count, Gout);
   };
+  // with room for both, the x- and y-layout metric fields are made once per solve
+  if (resident) {
+    permute_metric(MX, GX);
+    permute_metric(MY, GY);
+  }
   int it = 0;
   double last = -1.0;
   for (; it < max_iters;) {
     unsigned long long *md = tol > 0 ? d_max : nullptr;
     if (md) ellip_zero_u64<<<1, 1, 0, st>>>(d_max);
     permute_layout<R>(st, T, GZ, Y, GX);
-    permute_metric(GX);
+    if (!resident) permute_metric(MX, GX);
     ellip_metric_direction<R>(st, Y, MX, GX, FOLD3D_X_DOWN, h, md, groups);
     ellip_metric_direction<R>(st, Y, MX, GX, FOLD3D_X_UP, h, md, groups);
     permute_layout<R>(st, Y, GX, T, GZ);
     ellip_metric_direction<R>(st, T, MZ, GZ, FOLD3D_K_UP, h, md, groups);
     ellip_metric_direction<R>(st, T, MZ, GZ, FOLD3D_K_DOWN, h, md, groups);
     permute_layout<R>(st, T, GZ, Y, GY);
-    permute_metric(GY);
-    ellip_metric_direction<R>(st, Y, MX, GY, FOLD3D_Y_UP, h, md, groups);
-    ellip_metric_direction<R>(st, Y, MX, GY, FOLD3D_Y_DOWN, h, md, groups);
+    if (!resident) permute_metric(MY, GY);
+    ellip_metric_direction<R>(st, Y, MY, GY, FOLD3D_Y_UP, h, md, groups);
+    ellip_metric_direction<R>(st, Y, MY, GY, FOLD3D_Y_DOWN, h, md, groups);
     permute_layout<R>(st, Y, GY, T, GZ);
     F3D_HIP(hipGetLastError());
     ++it;
@@ -579,11 +588,11 @@ int ellip_sweeps_metric_engine(hipStream_t st, R *T, R *scratch, const Grid &G,
 
 template int ellip_sweeps_metric_engine<double>(hipStream_t, double *, double *, const Grid &,
                                                 const fold3d_model &, const double *, int,
-                                                double, int, unsigned long long *, int *,
+                                                double, int, bool, unsigned long long *, int *,
                                                 double *);
 template int ellip_sweeps_metric_engine<float>(hipStream_t, float *, float *, const Grid &,
                                                const fold3d_model &, const double *, int, double,
-                                               int, unsigned long long *, int *, double *);
+                                               int, bool, unsigned long long *, int *, double *);
 
 template int ellip_sweeps_split_engine<double>(hipStream_t, double *, double *, double *,
                                                double *, const Grid &, const fold3d_model &,

@@ -256,7 +256,7 @@ def cpu_baseline(T0_host, n, threads):
     t0 = time.perf_counter()
     o.sweeps(T, n, 1)
     dt = time.perf_counter() - t0
-    return cell_updates(n, 1) / dt / 1e9, dt
+    return cell_updates(n, 1) / dt / 1e9, dt, T
 
 
 def main():
@@ -396,11 +396,17 @@ def main():
     if not args.no_cpu:
         threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or min(16, os.cpu_count() or 1)
         log(f"bench: CPU baseline, {threads} threads, 1 iteration of {n}^3")
-        v, dt = cpu_baseline(T0, n, threads)
+        v, dt, T_cpu = cpu_baseline(T0, n, threads)
+        # the same iteration on the GPU: the oracle's field is the checker
+        T1 = T0d.clone()
+        solver.solve(T1, n, max_iters=1)
+        same = bool(np.array_equal(T1.cpu().numpy().view(np.uint64), T_cpu.view(np.uint64)))
         cpu = {"value": v, "unit": "Gcell-updates/s", "cores": threads, "kind": "port",
                "sample": f"1 of the 8 iterations (6 sweeps) of the same {n}^3 FP64 solve, "
-                         f"{cell_updates(n, 1):.3e} cell-updates in {dt:.1f} s"}
-        log(f"bench: CPU {v:.3f} Gcell-upd/s")
+                         f"{cell_updates(n, 1):.3e} cell-updates in {dt:.1f} s",
+               "gpu_same_bits_after_1_iteration": same}
+        log(f"bench: CPU {v:.3f} Gcell-upd/s; GPU field after that iteration bit-identical: {same}")
+        del T1, T_cpu
 
     batch = None
     if not args.no_batch:

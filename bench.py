@@ -32,6 +32,9 @@ HBM_PEAK_GBS = 8000.0          # MI355X HBM3E
 # WRITE_SIZE x 1.0 KiB, the factors read off a plain device copy of known
 # size (profiles/fold_split_f64_512_pmc_traffic.md)
 PMC_TRAFFIC_512_F64 = (2901.2 * 2.0 + 849.6 * 1.0) * 1024
+# VALU wave-instructions per plane launch of the same kernel (rocprofv3 --pmc
+# SQ_INSTS_VALU, first iteration; 8192 waves of 702 instructions)
+PMC_VALU_INSTS_512_F64 = 5748589.5
 A100_V2_512_GCUPS = 8 * 6 * 511 * 512 ** 2 / 3.038 / 1e9  # plots/perfect-2-gpu.py:20,24
 
 
@@ -430,10 +433,20 @@ def main():
     log("bench: measured ceilings of this GPU (FMA rate, device copy)")
     peaks = fold3d.measure_peaks()
     log(f"bench: {peaks}")
-    # VALU view of the same kernel: wave-instructions per plane launch over the
-    # measured FP64 FMA issue rate is what binds (see roofline.valu)
     valu = {"measured_fp64_tflops": peaks["fp64_tflops"], "published_fp64_tflops": 78.6,
             "measured_copy_gbs": peaks["copy_gbs"]}
+    # VALU view of the same kernel, the bound that binds: wave-instructions per
+    # plane launch (rocprofv3 SQ_INSTS_VALU, first iteration at 512^3) over the
+    # issue rate of 256 CUs x 4 SIMDs at one wave-instruction per 4 cycles, the
+    # clock taken from the measured FP64 FMA rate (128 FLOP per CU per cycle)
+    clock_hz = peaks["fp64_tflops"] * 1e12 / (256 * 128)
+    issue_peak = 256 * 4 * clock_hz / 4
+    valu_issue = None
+    if n == 512 and groups == 2:
+        achieved_i = PMC_VALU_INSTS_512_F64 / (us * 1e-6)
+        valu_issue = {"wave_instructions_per_launch": PMC_VALU_INSTS_512_F64,
+                      "achieved_per_s": achieved_i, "peak_per_s": issue_peak,
+                      "frac": achieved_i / issue_peak, "clock_ghz": clock_hz / 1e9}
 
     out = {
         "metric": "Gcell-updates/s at 512^3 (fold-model PMM solve, 8 iterations)",
@@ -455,7 +468,7 @@ def main():
                      "frac": achieved / HBM_PEAK_GBS,
                      "traffic": PMC_TRAFFIC_512_F64 if (n == 512 and groups == 2) else None,
                      "kernel": "fold_split_kernel<double>", "kernel_us": us,
-                     "bytes_per_launch": bytes_per_launch},
+                     "bytes_per_launch": bytes_per_launch, "valu_issue": valu_issue},
         "engines_fp64": engines,
         "locked_engine_fp64": locked,
         "initializer": init,

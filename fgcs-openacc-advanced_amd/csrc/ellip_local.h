@@ -39,13 +39,17 @@ struct EllipConsts {
 template <typename R>
 __device__ __forceinline__ R ellip_sqrt(R x);
 template <>
-__device__ __forceinline__ double ellip_sqrt<double>(double x) { return sqrt(x); }
+__device__ __forceinline__ double ellip_sqrt<double>(double x) { return sqrt_rn(x); }
 template <>
 __device__ __forceinline__ float ellip_sqrt<float>(float x) { return __builtin_amdgcn_sqrtf(x); }
 
 // Unreached nodes carry MAXT (1e16); a simplex touching one is not used.
 template <typename R>
 __device__ __forceinline__ bool reached(R t) { return t < R(1e15); }
+
+// correctly rounded square root of a lattice length
+__device__ __forceinline__ double vertex_sqrt(double x) { return sqrt_rn(x); }
+__device__ __forceinline__ float vertex_sqrt(float x) { return sqrtf(x); }
 
 template <typename R>
 __device__ __forceinline__ void ellip_edge(R &best, R t0, R t1, R s2, const R *E) {
@@ -238,7 +242,7 @@ __device__ __forceinline__ void ellip_vertices_lattice(R &best, const R p[9], R 
   for (int k = 0; k < 9; ++k) {
     if (reached(p[k])) {
       const int a = k == 0 ? 0 : kRing[k - 1][0], b = k == 0 ? 0 : kRing[k - 1][1];
-      best = rmin(p[k] + s * sqrt(lat_dot(L, -a, -b, 1, -a, -b, 1)), best);
+      best = rmin(p[k] + s * vertex_sqrt(lat_dot(L, -a, -b, 1, -a, -b, 1)), best);
     }
   }
 }

@@ -20,6 +20,32 @@
 
 namespace fold3d {
 
+// Correctly rounded FP64 square root.  For 2^-767 <= x <= 2^1000 the
+// refinement below of the hardware reciprocal square root (one coupled
+// Newton step on (sqrt(x), 1/(2 sqrt(x))), then two residual corrections
+// g + (x - g*g) * h) is the IEEE result, and it is the same operation sequence
+// the compiler emits for sqrt() in that range; outside it (zero, tiny, huge,
+// inf, NaN, negative) the compiler's full sqrt(), with its scaling and special
+// cases, is used.  The test is wave-uniform, so a wave with every lane in range
+// skips the scaling and special-case selects.
+__device__ __forceinline__ double sqrt_refined(double x) {
+  const double y = __builtin_amdgcn_rsq(x);
+  double g = x * y, h = 0.5 * y;
+  const double r = fma(-h, g, 0.5);
+  g = fma(g, r, g);
+  h = fma(h, r, h);
+  double d = fma(-g, g, x);
+  g = fma(d, h, g);
+  d = fma(-g, g, x);
+  return fma(d, h, g);
+}
+
+__device__ __forceinline__ double sqrt_rn(double x) {
+  const bool in_range = x >= 0x1p-767 && x <= 0x1p+1000;
+  if (__builtin_amdgcn_ballot_w64(!in_range) == 0) return sqrt_refined(x);
+  return sqrt(x);
+}
+
 // Reference macro semantics: min(a,b) = a<b ? a : b (second operand on ties
 // and NaN, AN_ISO_1_OpenACC.c:32-34); sign(v) = v>0 ? 1 : -1 (Fold3dPMM.h:32-34).
 template <typename R>

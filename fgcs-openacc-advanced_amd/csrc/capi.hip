@@ -138,8 +138,8 @@ int fold3d_solve(fold3d_handle *h, void *T, int nx, int ny, int nz, double dx,
                  double dy, double dz, const fold3d_model *model,
                  const fold3d_opts *opts, fold3d_stats *stats) {
   F3D_CHECK(h && T && model, FOLD3D_E_INVALID, "fold3d_solve: NULL handle, T or model");
-  F3D_CHECK(nx >= 3 && ny >= 3 && nz >= 3, FOLD3D_E_INVALID,
-            "fold3d_solve: grid %dx%dx%d too small (need >= 3 per axis)", nx, ny, nz);
+  F3D_CHECK(nx >= 1 && ny >= 1 && nz >= 1, FOLD3D_E_INVALID,
+            "fold3d_solve: grid %dx%dx%d too small (need >= 1 per axis)", nx, ny, nz);
   F3D_CHECK(dx > 0 && dy > 0 && dz > 0 && std::isfinite(dx) && std::isfinite(dy) &&
                 std::isfinite(dz),
             FOLD3D_E_INVALID, "fold3d_solve: spacings must be positive and finite");
@@ -334,7 +334,7 @@ static int check_batch(fold3d_handle *h, int count, const fold3d_instance *inst,
   for (int i = 0; i < count; ++i) {
     const fold3d_instance &x = inst[i];
     F3D_CHECK(x.T != nullptr, FOLD3D_E_INVALID, "fold3d_solve_batched: instance %d has no T", i);
-    F3D_CHECK(x.n[0] >= 3 && x.n[1] >= 3 && x.n[2] >= 3, FOLD3D_E_INVALID,
+    F3D_CHECK(x.n[0] >= 1 && x.n[1] >= 1 && x.n[2] >= 1, FOLD3D_E_INVALID,
               "fold3d_solve_batched: instance %d grid too small", i);
     F3D_CHECK(x.h[0] > 0 && x.h[1] > 0 && x.h[2] > 0, FOLD3D_E_INVALID,
               "fold3d_solve_batched: instance %d spacings must be positive", i);
@@ -456,7 +456,7 @@ int fold3d_adjoint(fold3d_handle *h, const void *T, int nx, int ny, int nz, doub
             "fold3d_adjoint: NULL handle, T, model or grad_slowness");
   F3D_CHECK(model->kind == FOLD3D_MODEL_ELLIP, FOLD3D_E_INVALID,
             "fold3d_adjoint: the adjoint is for the elliptic model");
-  F3D_CHECK(nx >= 3 && ny >= 3 && nz >= 3, FOLD3D_E_INVALID, "fold3d_adjoint: grid too small");
+  F3D_CHECK(nx >= 1 && ny >= 1 && nz >= 1, FOLD3D_E_INVALID, "fold3d_adjoint: grid too small");
   F3D_CHECK(dx > 0 && dy > 0 && dz > 0, FOLD3D_E_INVALID, "fold3d_adjoint: bad spacings");
   F3D_CHECK(nrec >= 0 && (nrec == 0 || (rec_idx && rec_res)), FOLD3D_E_INVALID,
             "fold3d_adjoint: bad receiver arrays");
@@ -588,7 +588,7 @@ int fold3d_autotune(fold3d_handle *h, const void *T, int nx, int ny, int nz, dou
   fold3d_default_opts(&o);
   if (opts) o = *opts;
   F3D_CHECK(o.memory == FOLD3D_DEVICE, FOLD3D_E_INVALID, "fold3d_autotune: T must be on the device");
-  F3D_CHECK(nx >= 3 && ny >= 3 && nz >= 3, FOLD3D_E_INVALID, "fold3d_autotune: grid too small");
+  F3D_CHECK(nx >= 1 && ny >= 1 && nz >= 1, FOLD3D_E_INVALID, "fold3d_autotune: grid too small");
   const Grid G(nx, ny, nz);
   const size_t bytes = (size_t)G.count * (o.precision == FOLD3D_F64 ? 8 : 4);
   F3D_HIP(hipSetDevice(h->device));

@@ -60,7 +60,7 @@ def test_invalid_arguments_are_codes_not_crashes():
     assert _solve(None, dummy) == fold3d.FOLD3D_E_INVALID
     assert "NULL" in fold3d.last_error()
     assert _solve(dummy, None) == fold3d.FOLD3D_E_INVALID
-    assert _solve(dummy, dummy, n=2) == fold3d.FOLD3D_E_INVALID
+    assert _solve(dummy, dummy, n=0) == fold3d.FOLD3D_E_INVALID
     assert "too small" in fold3d.last_error()
     assert _solve(dummy, dummy, dx=-1.0) == fold3d.FOLD3D_E_INVALID
     assert _solve(dummy, dummy, dx=float("nan")) == fold3d.FOLD3D_E_INVALID

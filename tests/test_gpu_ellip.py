@@ -224,3 +224,31 @@ def test_metric_engine_matches_plane_engine(solver, oracle, groups, dtype):
         assert np.array_equal(reached, got < 1e15)
         rel = np.abs(got[reached] - want[reached]) / want[reached].clip(min=1e-300)
         assert rel.max() < 1e-12
+
+
+@pytest.mark.parametrize("n3", [(1, 1, 1), (2, 2, 2), (1, 5, 3), (3, 2, 9), (65, 1, 2)])
+def test_tiny_grids(solver, oracle, n3):
+    """Constant metric (split engine, bitwise vs the oracle) and per-node metric
+    (metric engine, bitwise vs the plane kernel) with one or two nodes along an axis."""
+    h = (0.1, 0.12, 0.09)
+    size = (n3[0] + 2) * (n3[1] + 2) * (n3[2] + 2)
+    rng = np.random.default_rng(sum(n3))
+    T0 = np.where(rng.random(size) < 0.3, rng.uniform(0.0, 1.0, size), 1e16)
+    W = wl.tilted_homogeneous(8)
+    want = T0.copy()
+    oracle.ellip_sweeps(want, n3, 2, 0.0, h, M=W.M, speed=1.0)
+    T = dev(T0)
+    solver.solve(T, n3, spacing=h, model=W.model(), max_iters=2)
+    assert bits_equal(T.cpu().numpy(), want)
+    fields = {"theta": rng.uniform(0, 1, size), "phi": rng.uniform(0, 6, size),
+              "ra": rng.uniform(0.6, 1, size), "rb": rng.uniform(0.6, 1, size)}
+    d = {k: dev(v) for k, v in fields.items()}
+    m = W.model()
+    for k, v in d.items():
+        setattr(m, k, v.data_ptr())
+    out = []
+    for eng in (fold3d.ENGINE_SPLIT, fold3d.ENGINE_PLANE):
+        T = dev(T0)
+        solver.solve(T, n3, spacing=h, model=m, max_iters=2, engine=eng)
+        out.append(T.cpu().numpy())
+    assert bits_equal(out[0], out[1])

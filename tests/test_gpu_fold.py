@@ -282,3 +282,18 @@ def test_locking_is_rejected_for_the_elliptic_model(solver):
     with pytest.raises(fold3d.Fold3dError) as e:
         solver.solve(T, 16, spacing=W.h, model=W.model(), engine=fold3d.ENGINE_LOCKED)
     assert e.value.code == fold3d.FOLD3D_E_UNSUPPORTED
+
+
+@pytest.mark.parametrize("engine", ENGINES)
+@pytest.mark.parametrize("n3", [(1, 1, 1), (2, 2, 2), (1, 5, 3), (2, 7, 1), (3, 2, 9), (65, 1, 2)])
+def test_tiny_grids_match_oracle(solver, oracle, n3, engine):
+    """Grids with one or two nodes along an axis (no or one plane step)."""
+    rng = np.random.default_rng(sum(n3) + engine)
+    size = (n3[0] + 2) * (n3[1] + 2) * (n3[2] + 2)
+    T0 = np.where(rng.random(size) < 0.3, -1.0, 1.0) * rng.uniform(1.0, 5.0, size)
+    h, a = (0.3, 0.25, 0.35), (-0.35, 0.4, 0.7)
+    want = T0.copy()
+    oracle.sweeps3(want, n3, 2, 1.1, a, h)
+    T = to_dev(T0)
+    solver.solve(T, n3, spacing=h, model=fold3d.fold_model(1.1, a), max_iters=2, engine=engine)
+    assert bits_equal(T.cpu().numpy(), want)

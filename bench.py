@@ -279,7 +279,7 @@ def main():
     ap.add_argument("--iters", type=int, default=8, help="iterations per solve (the reference runs 8)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
+        print(f"bench.py: this solver runs on one GPU only (asked for --gpus {args.gpus}); "
               "the solve does not shard across GPUs", file=sys.stderr)
         sys.exit(2)
 

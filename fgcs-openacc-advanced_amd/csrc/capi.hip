@@ -419,7 +419,7 @@ int fold3d_autotune_batched(fold3d_handle *h, int count, const fold3d_instance *
                                     {2, 8, 8}};
   float best_ms = 0;
   int best = -1;
-  for (int round = 0; round < 2 && rc == FOLD3D_OK; ++round) {  // the first round warms up
+  for (int pass = 0; pass < 2 && rc == FOLD3D_OK; ++pass) {  // the first pass warms up
     for (int c = 0; c < (int)(sizeof kConfigs / sizeof kConfigs[0]); ++c) {
       for (int i = 0; i < count && rc == FOLD3D_OK; ++i) {
         const Grid G(inst[i].n[0], inst[i].n[1], inst[i].n[2]);
@@ -432,7 +432,7 @@ int fold3d_autotune_batched(fold3d_handle *h, int count, const fold3d_instance *
         rc = run_batch(h, count, copy.data(), o, kConfigs[c][0], kConfigs[c][1], kConfigs[c][2],
                        nullptr, nullptr, &ms);
       if (rc != FOLD3D_OK) break;
-      if (round == 1 && (best < 0 || ms < best_ms)) {
+      if (pass == 1 && (best < 0 || ms < best_ms)) {
         best = c;
         best_ms = (float)ms;
       }
@@ -606,7 +606,7 @@ int fold3d_autotune(fold3d_handle *h, const void *T, int nx, int ny, int nz, dou
   int best = knob;
   float best_ms = -1.0f;
   int rc = FOLD3D_OK;
-  for (int r = 0; r < 2 && rc == FOLD3D_OK; ++r) {  // round 0 warms up
+  for (int r = 0; r < 2 && rc == FOLD3D_OK; ++r) {  // the first pass warms up
     for (int c = 0; c < 3 && rc == FOLD3D_OK; ++c) {
       knob = cand[c];
       if (hipMemcpyAsync(work, T, bytes, hipMemcpyDeviceToDevice, st) != hipSuccess) {

@@ -36,7 +36,7 @@ enum fold3d_status {
   FOLD3D_E_INVALID = -1,     /* bad argument (size, pointer, option) */
   FOLD3D_E_HIP = -2,         /* a HIP runtime call failed (see fold3d_last_error) */
   FOLD3D_E_NOMEM = -3,       /* device allocation failed */
-  FOLD3D_E_UNSUPPORTED = -4, /* valid request this build does not implement */
+  FOLD3D_E_UNSUPPORTED = -4, /* valid request this library does not implement */
   FOLD3D_E_NOT_CONVERGED = -5, /* max_iters reached before tol (result still written) */
   FOLD3D_E_IO = -6             /* a file could not be opened, read or written */
 };

@@ -113,7 +113,7 @@ double orc_surf(double tnew, double st, double xt, double yt, double F,
  * axis (ux, sign sx) and local y axis (uy, sign sy) of the upwind plane.
  * Read off the loads and model constants of the six kernels
  * (AN_ISO_1_OpenACC.c:195-212, 348-364, 500-516, 652-668, 805-821, 960-976)
- * and the driver's spacing arguments (:1198-1235).                        */
+ * and the reference driver's spacing arguments (:1198-1235).                        */
 typedef struct { int m, s, ux, sx, uy, sy; } frame_t;
 static const frame_t FRAMES[6] = {
     {0, +1, 2, +1, 1, +1}, /* x_down: i -> i+1, x^=+z, y^=+y */
@@ -132,7 +132,7 @@ typedef struct {
 
 static double diag(const double h[3], int a, int b)
 {
-  /* DXYP/DXZP/DYZP of the driver (:1159-1161): lower axis first. */
+  /* DXYP/DXZP/DYZP of the reference driver (:1159-1161): lower axis first. */
   int lo = a < b ? a : b, hi = a < b ? b : a;
   return sqrt(h[lo] * h[lo] + h[hi] * h[hi]);
 }

@@ -53,7 +53,7 @@ double ref_init_fold(double F, double ax, double ay, double az, double x,
 
 double ref_impsurf(double x, double y, double z) { return ImpSurfF(x, y, z); }
 
-/* Grid spacings exactly as the driver computes them (AN_ISO_1_OpenACC.c:1154-1161). */
+/* Grid spacings exactly as the reference driver computes them (AN_ISO_1_OpenACC.c:1154-1161). */
 static void spacings(double *DX, double *dy, double *dz, double *DXYP,
                      double *DXZP, double *DYZP)
 {
@@ -72,7 +72,7 @@ void ref_initialiser(double *T)
   ImplicitInitialiser(T, _nx, _ny, _nz, DX, dy, dz);
 }
 
-/* One plane step with the arguments the driver passes
+/* One plane step with the arguments the reference driver passes
  * (AN_ISO_1_OpenACC.c:1198,1205,1212,1219,1227,1234).
  * dir: 0 x_down, 1 x_up, 2 k_up, 3 k_down, 4 y_up, 5 y_down. */
 void ref_plane(int dir, double *T, int plane)

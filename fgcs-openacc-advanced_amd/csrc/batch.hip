@@ -4,16 +4,19 @@
 // Once per batch the slowness and the inverse metric N = M^-1 of every
 // instance are written to seven fields of its own (tilt fields or a constant
 // metric alike).  An iteration then runs, for all instances together, the
-// single-instance metric engine's schedule: move T and the metric fields into
-// the x-sweep layout (one launch per field set for the whole batch), run the
-// x-sweeps, move T back, run the z-sweeps in the reference layout, and the
-// same for y.  One launch per (sweep direction d, plane step q) serves every
-// instance that still has a plane q in direction d; instances are sorted by
-// their plane count along d, so those are a prefix of the sorted list.  A
-// launch runs a fixed grid of persistent blocks (`groups` waves each, which
-// share the candidate units of a row) that pull work items — one row of 64
-// target nodes of one instance — from a per-launch device counter until the
-// launch's items run out.  The launches of one iteration (plus the counter and
+// single-instance metric engine's schedule: move T into the x-sweep layout
+// (one launch for the whole batch; the metric fields are kept in every layout,
+// or moved along when memory is short), run the x-sweeps, move T back, run the
+// z-sweeps in the reference layout, and the same for y.  One launch per (sweep
+// direction d, plane step q) serves every instance that still has a plane q in
+// direction d; instances are sorted by their plane count along d, so those are
+// a prefix of the sorted list.  A launch runs a fixed grid of persistent
+// blocks (`groups` waves each, which share the candidate units of a row); a
+// work item is one row of 64 target nodes of one instance, and a row -> instance
+// table finds its owner in one load.  Blocks take items either from a
+// per-launch device counter (`rows_per_pull` at a time) or by a static stride
+// (block b takes items b, b + grid, ...); the batch autotuner picks.  The
+// launches of one iteration (plus the counter and
 // maximum resets) are captured once in a hipGraph and replayed; between
 // replays the host reads each instance's max|dT| and retires converged
 // instances through a device flag, so they cost only a flag test afterwards.

@@ -299,9 +299,35 @@ __global__ __launch_bounds__(256) void adjoint_plane_kernel(R *__restrict__ mu,
     unsigned long long m = in_mask[X];
     if (m != 0ull) {
       R acc = res[X];
+      // decode up to kBatch in-edges first so their loads are in flight together,
+      // then add them in mask order (the same order, so the same bits, as one
+      // at a time); a node with more in-edges finishes one at a time
+      constexpr int kBatch = 6;
+      const R *wp[kBatch];
+      long long zs[kBatch];
+      int cnt = 0;
+      while (m && cnt < kBatch) {
+        const int c = __builtin_ctzll(m) >> 1;
+        const int q = (int)((m >> (2 * c)) & 3ull) - 1;
+        m &= ~(3ull << (2 * c));
+        const int cc = c < 13 ? c : c + 1;
+        const int dx = cc % 3 - 1, dy = (cc / 3) % 3 - 1, dz = cc / 9 - 1;
+        zs[cnt] = X + dx * sx + dy * sy + dz * sz;
+        wp[cnt] = q == 0 ? S.w0 : q == 1 ? S.w1 : S.w2;
+        ++cnt;
+      }
+      R wv[kBatch], mv[kBatch];
+#pragma unroll
+      for (int k = 0; k < kBatch; ++k)
+        if (k < cnt) {
+          wv[k] = wp[k][zs[k]];
+          mv[k] = mu[zs[k]];
+        }
+#pragma unroll
+      for (int k = 0; k < kBatch; ++k)
+        if (k < cnt) acc += wv[k] * mv[k];
       while (m) {
-        const int bit = __builtin_ctzll(m);
-        const int c = bit >> 1;
+        const int c = __builtin_ctzll(m) >> 1;
         const int q = (int)((m >> (2 * c)) & 3ull) - 1;
         m &= ~(3ull << (2 * c));
         const int cc = c < 13 ? c : c + 1;

@@ -31,10 +31,10 @@ HBM_PEAK_GBS = 8000.0          # MI355X HBM3E
 # two rocprofv3 --pmc passes over the first iteration: FETCH_SIZE x 2.0 plus
 # WRITE_SIZE x 1.0 KiB, the factors read off a plain device copy of known
 # size (profiles/fold_split_f64_512_pmc_traffic.md)
-PMC_TRAFFIC_512_F64 = (2901.2 * 2.0 + 849.6 * 1.0) * 1024
+PMC_TRAFFIC_512_F64 = (2968.6 * 2.0 + 849.6 * 1.0) * 1024
 # VALU wave-instructions per plane launch of the same kernel (rocprofv3 --pmc
-# SQ_INSTS_VALU, first iteration; 8192 waves of 702 instructions)
-PMC_VALU_INSTS_512_F64 = 5748589.5
+# SQ_INSTS_VALU, first iteration; 8192 waves of 618 instructions)
+PMC_VALU_INSTS_512_F64 = 5061591.2
 A100_V2_512_GCUPS = 8 * 6 * 511 * 512 ** 2 / 3.038 / 1e9  # plots/perfect-2-gpu.py:20,24
 
 

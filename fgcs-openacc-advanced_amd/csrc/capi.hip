@@ -31,6 +31,32 @@ struct fold3d_handle {
 
 // Grow the handle's scratch field to at least `bytes` (stream-ordered free of
 // the old one is not available, so the device is synchronised first).
+#define F3D_CHECK_DEV(p, what)                                                             \
+  F3D_CHECK((p) == nullptr || is_device_ptr(p), FOLD3D_E_INVALID,                          \
+            "%s must be device memory (got a host or unknown pointer)", what)
+
+// True when the device can address p: device, managed or pinned host memory.
+// A plain (unregistered) host pointer passed where a device array is required
+// is rejected instead of being dereferenced by a kernel.
+static bool is_device_ptr(const void *p) {
+  if (!p) return false;
+  hipPointerAttribute_t a;
+  if (hipPointerGetAttributes(&a, p) != hipSuccess) {
+    (void)hipGetLastError();  // unregistered host memory reports an error: clear it
+    return false;
+  }
+  return a.type != hipMemoryTypeUnregistered;
+}
+
+static int check_model_fields(const fold3d_model *m) {
+  F3D_CHECK_DEV(m->speed_field, "model speed_field");
+  F3D_CHECK_DEV(m->theta, "model theta");
+  F3D_CHECK_DEV(m->phi, "model phi");
+  F3D_CHECK_DEV(m->ra, "model ra");
+  F3D_CHECK_DEV(m->rb, "model rb");
+  return FOLD3D_OK;
+}
+
 static int ensure_scratch(fold3d_handle *h, size_t bytes) {
   if (h->scratch_bytes >= bytes) return FOLD3D_OK;
   if (h->d_scratch) {
@@ -183,6 +209,9 @@ int fold3d_solve(fold3d_handle *h, void *T, int nx, int ny, int nz, double dx,
   }
 
   F3D_HIP(hipSetDevice(h->device));
+  if (o.memory == FOLD3D_DEVICE) F3D_CHECK_DEV(T, "fold3d_solve: T");
+  if (model->kind == FOLD3D_MODEL_ELLIP && check_model_fields(model) != FOLD3D_OK)
+    return FOLD3D_E_INVALID;
   hipStream_t st = (hipStream_t)o.stream;
   const Grid G(nx, ny, nz);
   const size_t esz = o.precision == FOLD3D_F64 ? 8 : 4;
@@ -334,6 +363,11 @@ static int check_batch(fold3d_handle *h, int count, const fold3d_instance *inst,
   for (int i = 0; i < count; ++i) {
     const fold3d_instance &x = inst[i];
     F3D_CHECK(x.T != nullptr, FOLD3D_E_INVALID, "fold3d_solve_batched: instance %d has no T", i);
+    F3D_CHECK(is_device_ptr(x.T) && (!x.speed_field || is_device_ptr(x.speed_field)) &&
+                  (!x.theta || (is_device_ptr(x.theta) && is_device_ptr(x.phi) &&
+                                is_device_ptr(x.ra) && is_device_ptr(x.rb))),
+              FOLD3D_E_INVALID, "fold3d_solve_batched: instance %d: fields must be device memory",
+              i);
     F3D_CHECK(x.n[0] >= 1 && x.n[1] >= 1 && x.n[2] >= 1, FOLD3D_E_INVALID,
               "fold3d_solve_batched: instance %d grid too small", i);
     F3D_CHECK(x.h[0] > 0 && x.h[1] > 0 && x.h[2] > 0, FOLD3D_E_INVALID,
@@ -470,6 +504,14 @@ int fold3d_adjoint(fold3d_handle *h, const void *T, int nx, int ny, int nz, doub
   F3D_CHECK(o.memory == FOLD3D_DEVICE, FOLD3D_E_INVALID, "fold3d_adjoint: T must be on the device");
   F3D_CHECK(o.max_iters >= 1 && o.tol >= 0, FOLD3D_E_INVALID, "fold3d_adjoint: bad max_iters/tol");
   F3D_HIP(hipSetDevice(h->device));
+  F3D_CHECK_DEV(T, "fold3d_adjoint: T");
+  F3D_CHECK_DEV(grad_slowness, "fold3d_adjoint: grad_slowness");
+  F3D_CHECK_DEV(grad_theta, "fold3d_adjoint: grad_theta");
+  if (nrec > 0) {
+    F3D_CHECK_DEV(rec_idx, "fold3d_adjoint: rec_idx");
+    F3D_CHECK_DEV(rec_res, "fold3d_adjoint: rec_res");
+  }
+  if (check_model_fields(model) != FOLD3D_OK) return FOLD3D_E_INVALID;
   const Grid G(nx, ny, nz);
   hipStream_t st = (hipStream_t)o.stream;
   if (nrec > 0) {
@@ -551,6 +593,7 @@ int fold3d_initialiser_device(fold3d_handle *h, double *T, int nx, int ny, int n
                 std::isfinite(dz),
             FOLD3D_E_INVALID, "fold3d_initialiser_device: spacings must be positive");
   F3D_HIP(hipSetDevice(h->device));
+  F3D_CHECK_DEV(T, "fold3d_initialiser_device: T");
   fold3d_set_grid(nx, ny, nz);
   return device_initialiser((hipStream_t)stream, T, nx, ny, nz, dx, dy, dz, n_exact);
 }
@@ -644,6 +687,7 @@ int fold3d_plane_step(fold3d_handle *h, int direction, double *T, int nx, int ny
   F3D_CHECK(ok, FOLD3D_E_INVALID, "fold3d_plane_step: plane %d out of range for direction %d",
             plane, direction);
   F3D_HIP(hipSetDevice(h->device));
+  F3D_CHECK_DEV(T, "fold3d_plane_step: T");
   FoldFrame<double> f{F, ax, ay, az, dzz, dxx, dyy, dxy, dxz, dyz};
   return fold_plane_step_f64((hipStream_t)stream, T, G, direction, plane, f);
 }

@@ -140,7 +140,7 @@ def test_fp32_gradient_close_to_fp64(solver):
         t0 = f["theta"].clone().requires_grad_(True)
         J = tomo.misfit(s0, t0)
         J.backward()
-        out[dt] = (float(J), s0.grad.double().cpu().numpy(), t0.grad.double().cpu().numpy())
+        out[dt] = (float(J.detach()), s0.grad.double().cpu().numpy(), t0.grad.double().cpu().numpy())
     J64, s64, t64 = out[torch.float64]
     J32, s32, t32 = out[torch.float32]
     assert abs(J32 - J64) <= 1e-2 * J64

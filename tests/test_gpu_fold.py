@@ -297,3 +297,33 @@ def test_tiny_grids_match_oracle(solver, oracle, n3, engine):
     T = to_dev(T0)
     solver.solve(T, n3, spacing=h, model=fold3d.fold_model(1.1, a), max_iters=2, engine=engine)
     assert bits_equal(T.cpu().numpy(), want)
+
+
+def test_host_pointers_are_rejected_not_dereferenced(solver):
+    """A plain host pointer where a device array is required is an error code,
+    not a kernel fault: solve with memory = DEVICE, a batch instance, an
+    adjoint receiver array."""
+    import ctypes as C
+    from fold3d import workloads as wl
+    n = 8
+    host = np.zeros((n + 2) ** 3)
+    L = fold3d.lib()
+    o = fold3d.Opts()
+    L.fold3d_default_opts(C.byref(o))
+    st = fold3d.Stats()
+    m = fold3d.reference_model()
+    rc = L.fold3d_solve(solver.h, C.c_void_p(host.ctypes.data), n, n, n, 0.1, 0.1, 0.1,
+                        C.byref(m), C.byref(o), C.byref(st))
+    assert rc == fold3d.FOLD3D_E_INVALID and "device memory" in fold3d.last_error()
+    W = wl.tilted_homogeneous(n)
+    Wd = wl.Workload(W.name, n, W.h, W.M, W.speed)
+    Wd.speed_field = torch.ones((n + 2) ** 3, dtype=torch.float64)  # a CPU tensor
+    T = torch.zeros((n + 2) ** 3, dtype=torch.float64, device="cuda")
+    with pytest.raises(fold3d.Fold3dError) as e:
+        solver.solve_batched([Wd], [T])
+    assert e.value.code == fold3d.FOLD3D_E_INVALID
+    me = W.model()
+    with pytest.raises(fold3d.Fold3dError) as e:
+        solver.adjoint(T, n, W.h, me, torch.zeros(4, dtype=torch.int32),
+                       torch.zeros(4, dtype=torch.float64), torch.zeros_like(T))
+    assert e.value.code == fold3d.FOLD3D_E_INVALID

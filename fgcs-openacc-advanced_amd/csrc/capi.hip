@@ -598,6 +598,29 @@ int fold3d_initialiser_device(fold3d_handle *h, double *T, int nx, int ny, int n
   return device_initialiser((hipStream_t)stream, T, nx, ny, nz, dx, dy, dz, n_exact);
 }
 
+int fold3d_point_source(fold3d_handle *h, void *T, int nx, int ny, int nz, double dx,
+                        double dy, double dz, const double src[3], const double M[6],
+                        double speed, double radius, int precision, void *stream) {
+  F3D_CHECK(h && T && src && M, FOLD3D_E_INVALID, "fold3d_point_source: NULL argument");
+  F3D_CHECK(nx >= 1 && ny >= 1 && nz >= 1, FOLD3D_E_INVALID, "fold3d_point_source: bad grid");
+  F3D_CHECK(dx > 0 && dy > 0 && dz > 0 && speed > 0 && radius >= 0 && std::isfinite(speed),
+            FOLD3D_E_INVALID, "fold3d_point_source: spacings, speed and radius must be positive");
+  F3D_CHECK(precision == FOLD3D_F64 || precision == FOLD3D_F32, FOLD3D_E_INVALID,
+            "fold3d_point_source: bad precision %d", precision);
+  const double m2 = M[0] * M[1] - M[3] * M[3];
+  const double m3 = M[0] * (M[1] * M[2] - M[5] * M[5]) - M[3] * (M[3] * M[2] - M[5] * M[4]) +
+                    M[4] * (M[3] * M[5] - M[1] * M[4]);
+  F3D_CHECK(M[0] > 0 && m2 > 0 && m3 > 0, FOLD3D_E_INVALID,
+            "fold3d_point_source: metric is not positive definite");
+  F3D_HIP(hipSetDevice(h->device));
+  F3D_CHECK_DEV(T, "fold3d_point_source: T");
+  double N[6];
+  sym3_inverse(M, N);
+  const double hh[3] = {dx, dy, dz};
+  return point_source_start((hipStream_t)stream, T, precision, nx, ny, nz, hh, src, N, speed,
+                            radius);
+}
+
 int fold3d_set_batch_tuning(fold3d_handle *h, int groups, int blocks_per_cu, int rows_per_pull) {
   F3D_CHECK(h != nullptr, FOLD3D_E_INVALID, "fold3d_set_batch_tuning: NULL handle");
   F3D_CHECK(groups == 0 || groups == 1 || groups == 2 || groups == 4, FOLD3D_E_INVALID,

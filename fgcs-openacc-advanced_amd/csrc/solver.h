@@ -106,6 +106,11 @@ size_t adjoint_scratch_bytes(const Grid &G, int precision);
 int device_initialiser(hipStream_t st, double *T, int nx, int ny, int nz, double dx, double dy,
                        double dz, long long *n_exact_out);
 
+// Start field of an elliptic solve from a point source (source_init.hip).
+int point_source_start(hipStream_t st, void *T, int precision, int nx, int ny, int nz,
+                       const double h[3], const double src[3], const double N[6], double v,
+                       double radius);
+
 // Copy a padded field from one axis order to another (layout.hip).
 template <typename R>
 void permute_layout(hipStream_t st, const R *in, const Grid &Gin, R *out, const Grid &Gout);

@@ -132,6 +132,7 @@ def lib() -> C.CDLL:
     L.fold3d_adjoint.argtypes = [vp, vp, i, i, i, d, d, d, C.POINTER(Model), i, vp, vp,
                                  C.POINTER(Opts), vp, vp, C.POINTER(Stats)]
     L.fold3d_synchronize.argtypes = [vp]
+    L.fold3d_point_source.argtypes = [vp, vp, i, i, i, d, d, d, dp, dp, d, d, i, vp]
     L.fold3d_initialiser_device.argtypes = [vp, vp, i, i, i, d, d, d, vp, C.POINTER(C.c_longlong)]
     cp = C.c_char_p
     L.fold3d_write_text.argtypes = [cp, dp, i, i, i]
@@ -360,6 +361,20 @@ class Solver:
 
     def set_tuning(self, groups: int, kind: int = MODEL_FOLD):
         check(lib().fold3d_set_tuning(self.h, kind, groups))
+
+    def point_source(self, T, n, spacing, source, M, speed, radius=1.6, stream=None):
+        """Closed-form start of an elliptic solve around a point source, on the
+        device into the torch tensor T (its dtype sets the precision)."""
+        import torch
+        nx, ny, nz = _dims(n)
+        dx, dy, dz = (spacing,) * 3 if np.isscalar(spacing) else spacing
+        src = (C.c_double * 3)(*[float(v) for v in source])
+        Mv = (C.c_double * 6)(*[float(v) for v in M])
+        if stream is None:
+            stream = torch.cuda.current_stream(T.device).cuda_stream
+        prec = F64 if T.dtype == torch.float64 else F32
+        check(lib().fold3d_point_source(self.h, C.c_void_p(T.data_ptr()), nx, ny, nz, dx, dy,
+                                        dz, src, Mv, float(speed), float(radius), prec, stream))
 
     def initialiser_device(self, T, n, spacing=None, stream=None) -> int:
         """The reference's input field built on the device into the FP64 torch

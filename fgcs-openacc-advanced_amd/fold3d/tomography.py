@@ -11,9 +11,9 @@ for the backward pass while the checkpoints fit `budget_bytes`; the rest
 are recomputed in the backward pass, group by group.
 
 Sources sit a few cells below the top face z = 1; receivers are top-face
-nodes.  The start of every solve is the closed form sqrt(d^T N d) * s at
-nodes within 1.6 cells (max norm) of the source, with N and s taken at the
-node nearest the source; those nodes are held fixed by the adjoint (they do
+nodes.  The start of every solve is the closed form sqrt(d^T N d) / v at
+nodes within 1.6 cells (max norm) of the source, with N and v taken at the
+node nearest the source (fold3d_point_source); those nodes are held fixed by the adjoint (they do
 not depend on the parameters except through that one node).
 """
 from __future__ import annotations
@@ -24,7 +24,7 @@ from dataclasses import dataclass
 import numpy as np
 
 from . import (F32, F64, MODEL_ELLIP, Model, Solver, check, lib)  # noqa: F401
-from .workloads import MAXT, _axis, _gauss_sum, tilted_metric
+from .workloads import _axis, _gauss_sum, tilted_metric
 
 
 @dataclass
@@ -136,7 +136,8 @@ class Tomography:
         return m
 
     def start_field(self, k: int, slowness, theta):
-        """Closed-form start of source k (device tensor, the field dtype)."""
+        """Closed-form start of source k (device tensor, the field dtype),
+        written by the library's point-source kernel."""
         import torch
         S = self.survey
         n, h = S.n, S.h
@@ -146,24 +147,9 @@ class Tomography:
         flat = i + H * (j + H * kk)
         M = tilted_metric(float(theta[flat]), float(self.phi[flat]), 1.0,
                           float(self.ra[flat]), float(self.rb[flat]))
-        A = np.array([[M[0], M[3], M[4]], [M[3], M[1], M[5]], [M[4], M[5], M[2]]])
-        N = np.linalg.inv(A)
-        s0 = float(slowness[flat])
-        dev = slowness.device
-        x = _axis(torch, n, None, dev) * h
-        dx = (x - src[0])[None, None, :]
-        dy = (x - src[1])[None, :, None]
-        dz = (x - src[2])[:, None, None]
-        q = (N[0, 0] * dx * dx + N[1, 1] * dy * dy + N[2, 2] * dz * dz
-             + 2 * N[0, 1] * dx * dy + 2 * N[0, 2] * dx * dz + 2 * N[1, 2] * dy * dz)
-        T = torch.sqrt(q) * s0
-        r = 1.6 * h
-        near = (dx.abs() <= r) & (dy.abs() <= r) & (dz.abs() <= r)
-        idx = _axis(torch, n, None, dev)
-        inside = (idx >= 1) & (idx <= n)
-        near = near & inside[:, None, None] & inside[None, :, None] & inside[None, None, :]
-        T = torch.where(near, T, torch.full_like(T, MAXT))
-        return T.to(slowness.dtype).contiguous().reshape(-1)
+        T = torch.empty(H ** 3, dtype=slowness.dtype, device=slowness.device)
+        self.solver.point_source(T, n, h, src, M, 1.0 / float(slowness[flat]), radius=1.6)
+        return T
 
     def solve_group(self, ks, slowness, theta):
         """Solved fields of the sources in `ks` (list of tensors)."""

@@ -194,6 +194,17 @@ int fold3d_autotune(fold3d_handle *h, const void *T, int nx, int ny, int nz,
                     double dx, double dy, double dz, const fold3d_model *model,
                     const fold3d_opts *opts, int iterations, float *ms_out);
 
+/* Start field of an elliptic-model solve from a point source at `src` (in
+ * the grid's coordinates: node (i,j,k) sits at ((i-1)dx, (j-1)dy, (k-1)dz)):
+ * T = sqrt(d^T M^-1 d) / speed, d = node - src, at the interior nodes within
+ * `radius` cells of the source in every axis (1.6 in the tests and
+ * benchmarks), FOLD3D_MAXT elsewhere and on the ghost layer.  With per-node
+ * fields, pass the metric and speed at the node nearest the source.  T is a
+ * device array in `precision`; asynchronous on `stream`. */
+int fold3d_point_source(fold3d_handle *h, void *T, int nx, int ny, int nz, double dx,
+                        double dy, double dz, const double src[3], const double M[6],
+                        double speed, double radius, int precision, void *stream);
+
 /* The reference's input field (ImplicitInitialiser, Fold3dPMM.c:129-389)
  * built on the device into a device FP64 array T on the padded grid, with the
  * same bits as the host initializer: the device evaluates the implicit

@@ -84,6 +84,28 @@ def test_invalid_arguments_are_codes_not_crashes():
     assert L.fold3d_plane_step(dummy, 1, dummy, 16, 16, 16, 1, *([1.0] * 10), None) == fold3d.FOLD3D_E_INVALID
 
 
+def test_point_source_arguments_are_checked():
+    L = fold3d.lib()
+    dummy = C.c_void_p(0x1000)  # never dereferenced: validation fails first
+    src = (C.c_double * 3)(1.0, 1.0, 1.0)
+    eye = (C.c_double * 6)(1.0, 1.0, 1.0, 0.0, 0.0, 0.0)
+    bad = (C.c_double * 6)(1.0, 1.0, 1.0, 2.0, 0.0, 0.0)  # not positive definite
+
+    def call(h=dummy, T=dummy, n=8, d=0.5, M=eye, v=1.0, r=1.6, prec=fold3d.F64):
+        return L.fold3d_point_source(h, T, n, n, n, d, d, d, src, M, v, r, prec, None)
+
+    assert call(h=None) == fold3d.FOLD3D_E_INVALID
+    assert call(T=None) == fold3d.FOLD3D_E_INVALID
+    assert call(n=0) == fold3d.FOLD3D_E_INVALID
+    assert call(d=0.0) == fold3d.FOLD3D_E_INVALID
+    assert call(v=-1.0) == fold3d.FOLD3D_E_INVALID
+    assert call(v=float("inf")) == fold3d.FOLD3D_E_INVALID
+    assert call(r=-1.0) == fold3d.FOLD3D_E_INVALID
+    assert call(prec=5) == fold3d.FOLD3D_E_INVALID
+    assert call(M=bad) == fold3d.FOLD3D_E_INVALID
+    assert "positive definite" in fold3d.last_error()
+
+
 def test_create_without_gpu_is_an_error_code():
     import torch
     if torch.cuda.is_available():

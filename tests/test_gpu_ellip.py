@@ -252,3 +252,26 @@ def test_tiny_grids(solver, oracle, n3):
         solver.solve(T, n3, spacing=h, model=m, max_iters=2, engine=eng)
         out.append(T.cpu().numpy())
     assert bits_equal(out[0], out[1])
+
+
+@pytest.mark.parametrize("name,n", [("tilted_homogeneous", 20), ("tilted_field", 24)])
+def test_device_point_source_start(solver, name, n):
+    """fold3d_point_source against the host start field: MAXT at the same nodes,
+    values within 1e-14 relative (the host inverts the metric by LU, the library
+    by cofactors).  FP32 is the FP64 value rounded, to one ulp."""
+    print(f"point source {name} n={n}", flush=True)
+    W = getattr(wl, name)(n)
+    want = wl.point_source_start(W)
+    M = W.meta.get("metric_at_source", W.M)
+    Mv = [float(x) for x in np.asarray(M).ravel()]  # (xx, yy, zz, xy, xz, yz)
+    for dtype in (torch.float64, torch.float32):
+        T = torch.empty((n + 2) ** 3, dtype=dtype, device="cuda")
+        solver.point_source(T, n, W.h, W.source, Mv, W.v_source)
+        torch.cuda.synchronize()
+        got = T.cpu().numpy().astype(np.float64)
+        far = want >= fold3d.MAXT
+        assert np.array_equal(got >= fold3d.MAXT * (1 - 1e-6), far)
+        assert int((~far).sum()) >= 8
+        ref = want[~far] if dtype == torch.float64 else want[~far].astype(np.float32).astype(np.float64)
+        rtol = 1e-14 if dtype == torch.float64 else 1.2e-7
+        np.testing.assert_allclose(got[~far], ref, rtol=rtol, atol=0)

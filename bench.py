@@ -249,6 +249,31 @@ def adjoint_run(solver, torch, n=256):
     return out
 
 
+def host_cpu():
+    """Model, sockets and physical cores of this host (from /proc/cpuinfo)."""
+    model, cores, sockets = None, set(), set()
+    phys = core = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                key, _, val = line.partition(":")
+                key, val = key.strip(), val.strip()
+                if key == "model name" and model is None:
+                    model = val
+                elif key == "physical id":
+                    phys = val
+                    sockets.add(val)
+                elif key == "core id":
+                    core = val
+                elif not key and phys is not None:
+                    cores.add((phys, core))
+                    phys = core = None
+    except OSError:
+        pass
+    return {"model": model, "sockets": len(sockets) or None, "physical_cores": len(cores) or None,
+            "logical_cpus": os.cpu_count()}
+
+
 def cpu_baseline(T0_host, n, threads):
     """The oracle (FP64 OpenMP restatement, bit-identical to the reference)
     on a bounded sample: one of the 8 iterations of the same 512^3 solve."""
@@ -265,7 +290,7 @@ def cpu_baseline(T0_host, n, threads):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--n", type=int, default=512)
     ap.add_argument("--no-cpu", action="store_true")
@@ -317,9 +342,11 @@ def main():
     W = cell_updates(n, iters)
     launches = iters * 6 * (n - 1)
     log(f"bench: FP64 solve x{args.steps} (+{args.warmup} warmup), split engine (default)")
+    head_stats = []
     ms, T, sweep_ms = time_solves(solver, T0d, n, args.steps, args.warmup, torch.float64, torch,
-                                  iters=iters)
+                                  iters=iters, stats=head_stats)
     ms_step = ms / args.steps
+    sweep_ms_steps = sorted(st.sweep_ms for st in head_stats)
     value = W * args.steps / (ms / 1e3) / 1e9
     log(f"bench: FP64 {ms_step:.1f} ms/solve, {value:.2f} Gcell-upd/s")
     # average plane-kernel duration from the solve's own HIP events (the
@@ -407,7 +434,7 @@ def main():
         cpu = {"value": v, "unit": "Gcell-updates/s", "cores": threads, "kind": "port",
                "sample": f"1 of the 8 iterations (6 sweeps) of the same {n}^3 FP64 solve, "
                          f"{cell_updates(n, 1):.3e} cell-updates in {dt:.1f} s",
-               "gpu_same_bits_after_1_iteration": same}
+               "gpu_same_bits_after_1_iteration": same, "host": host_cpu()}
         log(f"bench: CPU {v:.3f} Gcell-upd/s; GPU field after that iteration bit-identical: {same}")
         del T1, T_cpu
 
@@ -463,7 +490,9 @@ def main():
         "data": "synthetic: the reference's implicit-surface initial field (Fold3dPMM.c)",
         "config": {"workload": f"fold-model PMM solve, {n}^3 nodes, FP64, {iters} iterations x 6 sweeps",
                    "n": n, "iterations": iters, "cell_updates_per_step": W,
-                   "engine": "split", "split_groups": groups},
+                   "engine": "split", "split_groups": groups,
+                   "sweep_ms_median": sweep_ms_steps[len(sweep_ms_steps) // 2],
+                   "sweep_ms_min": sweep_ms_steps[0], "sweep_ms_max": sweep_ms_steps[-1]},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
                      "traffic": PMC_TRAFFIC_512_F64 if (n == 512 and groups == 2) else None,

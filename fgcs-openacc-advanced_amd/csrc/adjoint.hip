@@ -16,10 +16,18 @@
 //   dJ/ds_X = mu_X dT_X/ds_X,  dJ/dtheta_X = mu_X dT_X/dtheta_X.
 // Nodes no candidate reproduces (the closed-form start around a source) are
 // fixed: they pass nothing upstream and contribute no gradient.
+//
+// Several sources on one grid and model are solved together: every plane
+// launch serves all sources still running (blockIdx.z = source), which gives
+// the small planes of one source enough waves to fill the GPU.  Each source's
+// values are computed exactly as alone, it stops on its own tolerance, and the
+// gradients are added in source order, so the bits are those of one solve per
+// source in that order.
 #include <hip/hip_runtime.h>
 
 #include <cmath>
 #include <cstring>
+#include <vector>
 
 #include "common.h"
 #include "ellip_local.h"
@@ -281,16 +289,31 @@ __global__ void in_mask_kernel(Stencil<R> S, long long count, long long sx, long
   }
 }
 
-// One plane of an adjoint sweep: mu_X = r_X + sum over the neighbours Z that
-// list X in their stencil (in_mask[X]) of weight * mu_Z.
+// Device view of one source: its forward field, adjoint state, residuals,
+// in-edge masks and winning stencils.
 template <typename R>
-__global__ __launch_bounds__(256) void adjoint_plane_kernel(R *__restrict__ mu,
-                                                            const R *__restrict__ res,
-                                                            Stencil<R> S, PlaneGeom g,
-                                                            long long sx, long long sy,
-                                                            long long sz,
-                                                            const unsigned long long *__restrict__ in_mask,
+struct AdjSrc {
+  const R *T;
+  R *mu, *res;
+  unsigned long long *in_mask;
+  Stencil<R> S;
+};
+
+// One plane of an adjoint sweep for every source still running (source =
+// blockIdx.z): mu_X = r_X + sum over the neighbours Z that list X in their
+// stencil (in_mask[X]) of weight * mu_Z.
+template <typename R>
+__global__ __launch_bounds__(256) void adjoint_plane_kernel(const AdjSrc<R> *__restrict__ src,
+                                                            const int *__restrict__ done,
+                                                            PlaneGeom g, long long sx,
+                                                            long long sy, long long sz,
                                                             unsigned long long *__restrict__ maxd) {
+  const int zsrc = blockIdx.z;
+  if (done[zsrc]) return;  // uniform over the block
+  R *__restrict__ mu = src[zsrc].mu;
+  const R *__restrict__ res = src[zsrc].res;
+  const Stencil<R> S = src[zsrc].S;
+  const unsigned long long *__restrict__ in_mask = src[zsrc].in_mask;
   const int a = 1 + blockIdx.x * 64 + threadIdx.x;
   const int b = 1 + blockIdx.y * 4 + threadIdx.y;
   R d = R(0);
@@ -344,18 +367,31 @@ __global__ __launch_bounds__(256) void adjoint_plane_kernel(R *__restrict__ mu,
   double v = (double)d;
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, 64));
-  if ((threadIdx.x & 63) == 0 && v > 0.0) atomicMax(maxd, (unsigned long long)__double_as_longlong(v));
+  if ((threadIdx.x & 63) == 0 && v > 0.0)
+    atomicMax(&maxd[zsrc], (unsigned long long)__double_as_longlong(v));
 }
 
+// grad += mu_k * dT/dparam_k over the sources k in order: the same additions,
+// in the same order, as one source at a time.
 template <typename R>
-__global__ void adjoint_gradient_kernel(const R *__restrict__ mu, Stencil<R> S, long long count,
-                                        R *__restrict__ gs, R *__restrict__ gth) {
+__global__ void adjoint_gradient_kernel(const AdjSrc<R> *__restrict__ src, int nsrc,
+                                        long long count, R *__restrict__ gs,
+                                        R *__restrict__ gth) {
   for (long long x = blockIdx.x * (long long)blockDim.x + threadIdx.x; x < count;
        x += (long long)gridDim.x * blockDim.x) {
-    const R m = mu[x];
-    if (m != R(0)) {
-      gs[x] += m * S.dTds[x];
-      if (gth) gth[x] += m * S.dTdth[x];
+    R a = gs[x], t = gth ? gth[x] : R(0);
+    bool any = false;
+    for (int k = 0; k < nsrc; ++k) {
+      const R m = src[k].mu[x];
+      if (m != R(0)) {
+        a += m * src[k].S.dTds[x];
+        if (gth) t += m * src[k].S.dTdth[x];
+        any = true;
+      }
+    }
+    if (any) {
+      gs[x] = a;
+      if (gth) gth[x] = t;
     }
   }
 }
@@ -367,39 +403,51 @@ __global__ void scatter_residual_kernel(R *__restrict__ res, const int *__restri
   if (k < count) res[idx[k]] += val[k];
 }
 
-__global__ void adj_zero_u64(unsigned long long *p) { *p = 0ull; }
+__global__ void adj_zero_u64(unsigned long long *p, int n) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < n) p[k] = 0ull;
+}
+
+static size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
+
+// Bytes of one source's scratch: in-edge masks (u64), mu, res, three weights,
+// two partials (R) and three offsets (int) per node.
+static size_t per_source_bytes(const Grid &G, size_t r) {
+  return align256((size_t)G.count * (sizeof(unsigned long long) + 7 * r + 3 * sizeof(int)));
+}
 
 template <typename R>
-int adjoint_solve(hipStream_t st, const R *T, const Grid &G, const fold3d_model &m,
-                  const double h[3], int nrec, const int *rec_idx, const R *rec_res,
-                  int max_iters, double tol, R *grad_s, R *grad_th, void *scratch,
-                  unsigned long long *d_max, int *iters_out, double *maxd_out) {
+int adjoint_solve(hipStream_t st, int nsrc, const AdjointSource *srcs, const Grid &G,
+                  const fold3d_model &m, const double h[3], int max_iters, R *grad_s,
+                  R *grad_th, void *scratch, int *iters_out, double *maxd_out) {
   const long long count = G.count;
-  // scratch: in-edge masks (u64), mu, res, 3 weights, 2 partials (R), 3 offsets (int)
-  char *p = (char *)scratch;
-  unsigned long long *in_mask = (unsigned long long *)p;
-  p += count * sizeof(unsigned long long);
-  R *mu = (R *)p;
-  p += count * sizeof(R);
-  R *res = (R *)p;
-  p += count * sizeof(R);
-  Stencil<R> S;
-  S.w0 = (R *)p;
-  p += count * sizeof(R);
-  S.w1 = (R *)p;
-  p += count * sizeof(R);
-  S.w2 = (R *)p;
-  p += count * sizeof(R);
-  S.dTds = (R *)p;
-  p += count * sizeof(R);
-  S.dTdth = (R *)p;
-  p += count * sizeof(R);
-  S.off0 = (int *)p;
-  p += count * sizeof(int);
-  S.off1 = (int *)p;
-  p += count * sizeof(int);
-  S.off2 = (int *)p;
-  F3D_HIP(hipMemsetAsync(scratch, 0, adjoint_scratch_bytes(G, sizeof(R) == 8 ? FOLD3D_F64 : FOLD3D_F32), st));
+  const size_t per = per_source_bytes(G, sizeof(R));
+  char *base = (char *)scratch;
+  // after the sources: the device table of sources, max|d mu| and done flags
+  AdjSrc<R> *d_src = (AdjSrc<R> *)(base + per * nsrc);
+  unsigned long long *d_max =
+      (unsigned long long *)((char *)d_src + align256(sizeof(AdjSrc<R>) * nsrc));
+  int *d_done = (int *)((char *)d_max + align256(sizeof(unsigned long long) * nsrc));
+  std::vector<AdjSrc<R>> hs(nsrc);
+  for (int k = 0; k < nsrc; ++k) {
+    char *p = base + per * k;
+    AdjSrc<R> &A = hs[k];
+    A.T = (const R *)srcs[k].T;
+    A.in_mask = (unsigned long long *)p;
+    p += count * sizeof(unsigned long long);
+    R **fields[7] = {&A.mu, &A.res, &A.S.w0, &A.S.w1, &A.S.w2, &A.S.dTds, &A.S.dTdth};
+    for (R **f : fields) {
+      *f = (R *)p;
+      p += count * sizeof(R);
+    }
+    A.S.off0 = (int *)p;
+    p += count * sizeof(int);
+    A.S.off1 = (int *)p;
+    p += count * sizeof(int);
+    A.S.off2 = (int *)p;
+  }
+  F3D_HIP(hipMemsetAsync(scratch, 0, adjoint_scratch_bytes(G, sizeof(R) == 8 ? FOLD3D_F64 : FOLD3D_F32, nsrc), st));
+  F3D_HIP(hipMemcpyAsync(d_src, hs.data(), sizeof(AdjSrc<R>) * nsrc, hipMemcpyHostToDevice, st));
   AdjArgs<R> A;
   memset(&A, 0, sizeof A);
   for (int a = 0; a < 3; ++a) {
@@ -418,24 +466,35 @@ int adjoint_solve(hipStream_t st, const R *T, const Grid &G, const fold3d_model 
   for (int q = 0; q < 6; ++q) A.Nc[q] = (R)N[q];
   A.rel_tol = sizeof(R) == 8 ? (R)1e-12 : (R)1e-6;
   const long long interior = (long long)G.n[0] * G.n[1] * G.n[2];
-  stencil_kernel<R><<<(unsigned)((interior + 255) / 256), 256, 0, st>>>(T, A, S);
-  in_mask_kernel<R><<<2048, 256, 0, st>>>(S, count, G.stride[0], G.stride[1], G.stride[2],
-                                          in_mask);
-  if (nrec > 0) scatter_residual_kernel<R><<<(nrec + 255) / 256, 256, 0, st>>>(res, rec_idx, rec_res, nrec);
-  F3D_HIP(hipGetLastError());
-  // nodes no other node lists keep mu = res; the sweeps update the rest
-  F3D_HIP(hipMemcpyAsync(mu, res, count * sizeof(R), hipMemcpyDeviceToDevice, st));
-  // One sweep pass (all six directions, reverse order) is captured once in a
-  // hipGraph on a stream of its own (the legacy default stream cannot be
-  // captured) and replayed until no adjoint value changes by more than tol.
+  for (int k = 0; k < nsrc; ++k) {
+    stencil_kernel<R><<<(unsigned)((interior + 255) / 256), 256, 0, st>>>(hs[k].T, A, hs[k].S);
+    in_mask_kernel<R><<<2048, 256, 0, st>>>(hs[k].S, count, G.stride[0], G.stride[1],
+                                            G.stride[2], hs[k].in_mask);
+    const int nrec = srcs[k].nrec;
+    if (nrec > 0)
+      scatter_residual_kernel<R><<<(nrec + 255) / 256, 256, 0, st>>>(
+          hs[k].res, srcs[k].rec_idx, (const R *)srcs[k].rec_res, nrec);
+    F3D_HIP(hipGetLastError());
+    // nodes no other node lists keep mu = res; the sweeps update the rest
+    F3D_HIP(hipMemcpyAsync(hs[k].mu, hs[k].res, count * sizeof(R), hipMemcpyDeviceToDevice, st));
+  }
+  // One sweep pass (all six directions, reverse order, all sources) is captured
+  // once in a hipGraph on a stream of its own (the legacy default stream cannot
+  // be captured) and replayed; after each pass a source whose adjoint values
+  // all changed by no more than its tolerance is marked done and skipped.
   hipStream_t ws = nullptr;
   hipEvent_t ev = nullptr;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   bool capturing = false;
   int rc = FOLD3D_OK;
-  int it = 0;
-  double last = -1.0;
+  int pass = 0, running = nsrc;
+  std::vector<unsigned long long> bits(nsrc);
+  std::vector<int> done(nsrc, 0);
+  for (int k = 0; k < nsrc; ++k) {
+    iters_out[k] = 0;
+    maxd_out[k] = -1.0;
+  }
   const dim3 block(64, 4);
 #define ADJ_HIP(call)                                                                  \
   do {                                                                                 \
@@ -452,7 +511,7 @@ int adjoint_solve(hipStream_t st, const R *T, const Grid &G, const fold3d_model 
   ADJ_HIP(hipStreamWaitEvent(ws, ev, 0));
   ADJ_HIP(hipStreamBeginCapture(ws, hipStreamCaptureModeThreadLocal));
   capturing = true;
-  adj_zero_u64<<<1, 1, 0, ws>>>(d_max);
+  adj_zero_u64<<<(nsrc + 255) / 256, 256, 0, ws>>>(d_max, nsrc);
   // reverse of the forward order: each direction's planes from last to first
   for (int dd = 5; dd >= 0; --dd) {
     const Frame &fr = kFrames[dd];
@@ -462,24 +521,37 @@ int adjoint_solve(hipStream_t st, const R *T, const Grid &G, const fold3d_model 
       const int pl = fr.s > 0 ? n - q : 1 + q;
       PlaneGeom g = plane_geom(G, dd, pl);
       g.sm = 0;  // the plane itself (plane_geom addresses a source plane)
-      const dim3 grid((g.na + 63) / 64, (g.nb + 3) / 4);
-      adjoint_plane_kernel<R><<<grid, block, 0, ws>>>(mu, res, S, g, G.stride[0], G.stride[1],
-                                                      G.stride[2], in_mask, d_max);
+      const dim3 grid((g.na + 63) / 64, (g.nb + 3) / 4, nsrc);
+      adjoint_plane_kernel<R><<<grid, block, 0, ws>>>(d_src, d_done, g, G.stride[0],
+                                                      G.stride[1], G.stride[2], d_max);
     }
   }
   capturing = false;
   ADJ_HIP(hipStreamEndCapture(ws, &graph));
   ADJ_HIP(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
-  for (; it < max_iters;) {
+  for (; pass < max_iters && running > 0;) {
     ADJ_HIP(hipGraphLaunch(exec, ws));
-    ++it;
-    unsigned long long bits = 0;
-    ADJ_HIP(hipMemcpyAsync(&bits, d_max, sizeof bits, hipMemcpyDeviceToHost, ws));
+    ++pass;
+    ADJ_HIP(hipMemcpyAsync(bits.data(), d_max, sizeof(unsigned long long) * nsrc,
+                           hipMemcpyDeviceToHost, ws));
     ADJ_HIP(hipStreamSynchronize(ws));
-    memcpy(&last, &bits, sizeof last);
-    if (last <= tol) break;
+    bool changed = false;
+    for (int k = 0; k < nsrc; ++k) {
+      if (done[k]) continue;
+      iters_out[k] = pass;
+      memcpy(&maxd_out[k], &bits[k], sizeof(double));
+      if (maxd_out[k] <= srcs[k].tol) {
+        done[k] = 1;
+        --running;
+        changed = true;
+      }
+    }
+    if (changed && running > 0) {
+      ADJ_HIP(hipMemcpyAsync(d_done, done.data(), sizeof(int) * nsrc, hipMemcpyHostToDevice, ws));
+      ADJ_HIP(hipStreamSynchronize(ws));
+    }
   }
-  adjoint_gradient_kernel<R><<<1024, 256, 0, ws>>>(mu, S, count, grad_s, grad_th);
+  adjoint_gradient_kernel<R><<<1024, 256, 0, ws>>>(d_src, nsrc, count, grad_s, grad_th);
   ADJ_HIP(hipGetLastError());
   // later work on the caller's stream sees the gradients
   ADJ_HIP(hipEventRecord(ev, ws));
@@ -498,22 +570,22 @@ done:
     (void)hipStreamDestroy(ws);
   }
   if (ev) (void)hipEventDestroy(ev);
-  *iters_out = it;
-  *maxd_out = last;
   return rc;
 }
 
-size_t adjoint_scratch_bytes(const Grid &G, int precision) {
+size_t adjoint_scratch_bytes(const Grid &G, int precision, int nsrc) {
   const size_t r = precision == FOLD3D_F64 ? 8 : 4;
-  return (size_t)G.count * (sizeof(unsigned long long) + 7 * r + 3 * sizeof(int));
+  const size_t tab = sizeof(AdjSrc<double>) > sizeof(AdjSrc<float>) ? sizeof(AdjSrc<double>)
+                                                                    : sizeof(AdjSrc<float>);
+  return per_source_bytes(G, r) * nsrc + align256(tab * nsrc) +
+         align256(sizeof(unsigned long long) * nsrc) + align256(sizeof(int) * nsrc);
 }
 
-template int adjoint_solve<double>(hipStream_t, const double *, const Grid &, const fold3d_model &,
-                                   const double *, int, const int *, const double *, int, double,
-                                   double *, double *, void *, unsigned long long *, int *,
-                                   double *);
-template int adjoint_solve<float>(hipStream_t, const float *, const Grid &, const fold3d_model &,
-                                  const double *, int, const int *, const float *, int, double,
-                                  float *, float *, void *, unsigned long long *, int *, double *);
+template int adjoint_solve<double>(hipStream_t, int, const AdjointSource *, const Grid &,
+                                   const fold3d_model &, const double *, int, double *, double *,
+                                   void *, int *, double *);
+template int adjoint_solve<float>(hipStream_t, int, const AdjointSource *, const Grid &,
+                                  const fold3d_model &, const double *, int, float *, float *,
+                                  void *, int *, double *);
 
 }  // namespace fold3d

@@ -482,84 +482,169 @@ int fold3d_autotune_batched(fold3d_handle *h, int count, const fold3d_instance *
   return FOLD3D_OK;
 }
 
-int fold3d_adjoint(fold3d_handle *h, const void *T, int nx, int ny, int nz, double dx,
-                   double dy, double dz, const fold3d_model *model, int nrec,
-                   const int *rec_idx, const void *rec_res, const fold3d_opts *opts,
-                   void *grad_slowness, void *grad_theta, fold3d_stats *stats) {
-  F3D_CHECK(h && T && model && grad_slowness, FOLD3D_E_INVALID,
-            "fold3d_adjoint: NULL handle, T, model or grad_slowness");
+}  // extern "C"
+
+// Checks shared by the adjoint entry points: model, grid, options.
+static int check_adjoint_common(const char *who, int nx, int ny, int nz, double dx, double dy,
+                                double dz, const fold3d_model *model, void *grad_slowness,
+                                void *grad_theta, const fold3d_opts &o) {
+  F3D_CHECK(model && grad_slowness, FOLD3D_E_INVALID, "%s: NULL model or grad_slowness", who);
   F3D_CHECK(model->kind == FOLD3D_MODEL_ELLIP, FOLD3D_E_INVALID,
-            "fold3d_adjoint: the adjoint is for the elliptic model");
-  F3D_CHECK(nx >= 1 && ny >= 1 && nz >= 1, FOLD3D_E_INVALID, "fold3d_adjoint: grid too small");
-  F3D_CHECK(dx > 0 && dy > 0 && dz > 0, FOLD3D_E_INVALID, "fold3d_adjoint: bad spacings");
-  F3D_CHECK(nrec >= 0 && (nrec == 0 || (rec_idx && rec_res)), FOLD3D_E_INVALID,
-            "fold3d_adjoint: bad receiver arrays");
+            "%s: the adjoint is for the elliptic model", who);
+  F3D_CHECK(nx >= 1 && ny >= 1 && nz >= 1, FOLD3D_E_INVALID, "%s: grid too small", who);
+  F3D_CHECK(dx > 0 && dy > 0 && dz > 0, FOLD3D_E_INVALID, "%s: bad spacings", who);
   F3D_CHECK(grad_theta == nullptr || model->theta != nullptr, FOLD3D_E_INVALID,
-            "fold3d_adjoint: a tilt gradient needs per-node tilt fields");
+            "%s: a tilt gradient needs per-node tilt fields", who);
   F3D_CHECK(model->speed_field != nullptr || model->speed > 0, FOLD3D_E_INVALID,
-            "fold3d_adjoint: speed must be positive");
-  fold3d_opts o;
-  fold3d_default_opts(&o);
-  if (opts) o = *opts;
-  F3D_CHECK(o.memory == FOLD3D_DEVICE, FOLD3D_E_INVALID, "fold3d_adjoint: T must be on the device");
-  F3D_CHECK(o.max_iters >= 1 && o.tol >= 0, FOLD3D_E_INVALID, "fold3d_adjoint: bad max_iters/tol");
-  F3D_HIP(hipSetDevice(h->device));
-  F3D_CHECK_DEV(T, "fold3d_adjoint: T");
-  F3D_CHECK_DEV(grad_slowness, "fold3d_adjoint: grad_slowness");
-  F3D_CHECK_DEV(grad_theta, "fold3d_adjoint: grad_theta");
-  if (nrec > 0) {
-    F3D_CHECK_DEV(rec_idx, "fold3d_adjoint: rec_idx");
-    F3D_CHECK_DEV(rec_res, "fold3d_adjoint: rec_res");
+            "%s: speed must be positive", who);
+  F3D_CHECK(o.memory == FOLD3D_DEVICE, FOLD3D_E_INVALID, "%s: T must be on the device", who);
+  F3D_CHECK(o.precision == FOLD3D_F64 || o.precision == FOLD3D_F32, FOLD3D_E_INVALID,
+            "%s: bad precision %d", who, o.precision);
+  F3D_CHECK(o.max_iters >= 1 && o.tol >= 0, FOLD3D_E_INVALID, "%s: bad max_iters/tol", who);
+  return FOLD3D_OK;
+}
+
+// Device-side checks of one source: device arrays, receivers at interior nodes
+// (their residuals are scattered by index).
+static int check_adjoint_source(const char *who, int k, const Grid &G, const void *T, int nrec,
+                                const int *rec_idx, const void *rec_res, double tol,
+                                hipStream_t st) {
+  F3D_CHECK(T != nullptr, FOLD3D_E_INVALID, "%s: source %d: NULL T", who, k);
+  F3D_CHECK(nrec >= 0 && (nrec == 0 || (rec_idx && rec_res)), FOLD3D_E_INVALID,
+            "%s: source %d: bad receiver arrays", who, k);
+  F3D_CHECK(tol >= 0, FOLD3D_E_INVALID, "%s: source %d: negative tolerance", who, k);
+  F3D_CHECK_DEV(T, "adjoint: T");
+  if (nrec == 0) return FOLD3D_OK;
+  F3D_CHECK_DEV(rec_idx, "adjoint: rec_idx");
+  F3D_CHECK_DEV(rec_res, "adjoint: rec_res");
+  std::vector<int> idx(nrec);
+  F3D_HIP(hipMemcpyAsync(idx.data(), rec_idx, sizeof(int) * nrec, hipMemcpyDeviceToHost, st));
+  F3D_HIP(hipStreamSynchronize(st));
+  const int nx = G.n[0], ny = G.n[1], nz = G.n[2];
+  for (int r = 0; r < nrec; ++r) {
+    const long long x = idx[r];
+    const long long i = x % G.stride[1], j = (x / G.stride[1]) % (ny + 2), kk = x / G.stride[2];
+    F3D_CHECK(x >= 0 && x < G.count && i >= 1 && i <= nx && j >= 1 && j <= ny && kk >= 1 &&
+                  kk <= nz,
+              FOLD3D_E_INVALID, "%s: source %d: receiver %d index %lld is not an interior node",
+              who, k, r, x);
   }
-  if (check_model_fields(model) != FOLD3D_OK) return FOLD3D_E_INVALID;
+  return FOLD3D_OK;
+}
+
+// Runs the checked adjoint of `nsrc` sources; per-source outputs.
+static int run_adjoint(fold3d_handle *h, const char *who, int nsrc, const AdjointSource *srcs,
+                       int nx, int ny, int nz, double dx, double dy, double dz,
+                       const fold3d_model *model, const fold3d_opts &o, void *grad_slowness,
+                       void *grad_theta, int *iters, double *maxd, double *ms_out) {
   const Grid G(nx, ny, nz);
   hipStream_t st = (hipStream_t)o.stream;
-  if (nrec > 0) {
-    // receivers must be interior nodes: their residuals are scattered by index
-    std::vector<int> idx(nrec);
-    F3D_HIP(hipMemcpyAsync(idx.data(), rec_idx, sizeof(int) * nrec, hipMemcpyDeviceToHost, st));
-    F3D_HIP(hipStreamSynchronize(st));
-    for (int r = 0; r < nrec; ++r) {
-      const long long x = idx[r];
-      const long long i = x % G.stride[1], j = (x / G.stride[1]) % (ny + 2), k = x / G.stride[2];
-      F3D_CHECK(x >= 0 && x < G.count && i >= 1 && i <= nx && j >= 1 && j <= ny && k >= 1 &&
-                    k <= nz,
-                FOLD3D_E_INVALID, "fold3d_adjoint: receiver %d index %lld is not an interior node",
-                r, x);
-    }
-  }
-  const int rc0 = ensure_scratch(h, adjoint_scratch_bytes(G, o.precision));
+  const int rc0 = ensure_scratch(h, adjoint_scratch_bytes(G, o.precision, nsrc));
   if (rc0 != FOLD3D_OK) return rc0;
   const double hsp[3] = {dx, dy, dz};
-  int iters = 0;
-  double maxd = -1;
   F3D_HIP(hipEventRecord(h->ev0, st));
   const int rc =
       o.precision == FOLD3D_F64
-          ? adjoint_solve<double>(st, (const double *)T, G, *model, hsp, nrec, rec_idx,
-                                  (const double *)rec_res, o.max_iters, o.tol,
+          ? adjoint_solve<double>(st, nsrc, srcs, G, *model, hsp, o.max_iters,
                                   (double *)grad_slowness, (double *)grad_theta, h->d_scratch,
-                                  h->d_max, &iters, &maxd)
-          : adjoint_solve<float>(st, (const float *)T, G, *model, hsp, nrec, rec_idx,
-                                 (const float *)rec_res, o.max_iters, o.tol,
-                                 (float *)grad_slowness, (float *)grad_theta, h->d_scratch,
-                                 h->d_max, &iters, &maxd);
+                                  iters, maxd)
+          : adjoint_solve<float>(st, nsrc, srcs, G, *model, hsp, o.max_iters,
+                                 (float *)grad_slowness, (float *)grad_theta, h->d_scratch, iters,
+                                 maxd);
   F3D_HIP(hipEventRecord(h->ev1, st));
   if (rc != FOLD3D_OK) return rc;
   F3D_HIP(hipEventSynchronize(h->ev1));
   float ms = 0;
   F3D_HIP(hipEventElapsedTime(&ms, h->ev0, h->ev1));
-  if (stats) {
+  if (ms_out) *ms_out = ms;
+  for (int k = 0; k < nsrc; ++k)
+    if (!(maxd[k] <= srcs[k].tol)) {
+      set_error("%s: source %d: adjoint change %g > tol %g after %d sweeps", who, k, maxd[k],
+                srcs[k].tol, iters[k]);
+      return FOLD3D_E_NOT_CONVERGED;
+    }
+  return FOLD3D_OK;
+}
+
+extern "C" {
+
+int fold3d_adjoint(fold3d_handle *h, const void *T, int nx, int ny, int nz, double dx,
+                   double dy, double dz, const fold3d_model *model, int nrec,
+                   const int *rec_idx, const void *rec_res, const fold3d_opts *opts,
+                   void *grad_slowness, void *grad_theta, fold3d_stats *stats) {
+  static const char *who = "fold3d_adjoint";
+  F3D_CHECK(h && T, FOLD3D_E_INVALID, "fold3d_adjoint: NULL handle or T");
+  fold3d_opts o;
+  fold3d_default_opts(&o);
+  if (opts) o = *opts;
+  int rc = check_adjoint_common(who, nx, ny, nz, dx, dy, dz, model, grad_slowness, grad_theta, o);
+  if (rc != FOLD3D_OK) return rc;
+  F3D_CHECK(nrec >= 0 && (nrec == 0 || (rec_idx && rec_res)), FOLD3D_E_INVALID,
+            "fold3d_adjoint: bad receiver arrays");
+  F3D_HIP(hipSetDevice(h->device));
+  F3D_CHECK_DEV(grad_slowness, "fold3d_adjoint: grad_slowness");
+  F3D_CHECK_DEV(grad_theta, "fold3d_adjoint: grad_theta");
+  if (check_model_fields(model) != FOLD3D_OK) return FOLD3D_E_INVALID;
+  const Grid G(nx, ny, nz);
+  rc = check_adjoint_source(who, 0, G, T, nrec, rec_idx, rec_res, o.tol, (hipStream_t)o.stream);
+  if (rc != FOLD3D_OK) return rc;
+  const AdjointSource src{T, nrec, rec_idx, rec_res, o.tol};
+  int iters = 0;
+  double maxd = -1, ms = 0;
+  rc = run_adjoint(h, who, 1, &src, nx, ny, nz, dx, dy, dz, model, o, grad_slowness, grad_theta,
+                   &iters, &maxd, &ms);
+  if (stats && (rc == FOLD3D_OK || rc == FOLD3D_E_NOT_CONVERGED)) {
     stats->iterations = iters;
     stats->max_dT = maxd;
     stats->cell_updates = 0;
     stats->sweep_ms = ms;
   }
-  if (!(maxd <= o.tol)) {
-    set_error("fold3d_adjoint: adjoint change %g > tol %g after %d sweeps", maxd, o.tol, iters);
-    return FOLD3D_E_NOT_CONVERGED;
+  return rc;
+}
+
+int fold3d_adjoint_batched(fold3d_handle *h, int count, const fold3d_adjoint_source *src,
+                           int nx, int ny, int nz, double dx, double dy, double dz,
+                           const fold3d_model *model, const fold3d_opts *opts,
+                           void *grad_slowness, void *grad_theta, int *iterations_out,
+                           double *max_d_out, double *ms_out) {
+  static const char *who = "fold3d_adjoint_batched";
+  F3D_CHECK(h && src, FOLD3D_E_INVALID, "fold3d_adjoint_batched: NULL handle or sources");
+  F3D_CHECK(count >= 1 && count <= 4096, FOLD3D_E_INVALID,
+            "fold3d_adjoint_batched: count must be 1..4096");
+  fold3d_opts o;
+  fold3d_default_opts(&o);
+  if (opts) o = *opts;
+  int rc = check_adjoint_common(who, nx, ny, nz, dx, dy, dz, model, grad_slowness, grad_theta, o);
+  if (rc != FOLD3D_OK) return rc;
+  for (int k = 0; k < count; ++k)
+    F3D_CHECK(src[k].T && src[k].nrec >= 0 && (src[k].nrec == 0 || (src[k].rec_idx && src[k].rec_res)) &&
+                  src[k].tol >= 0,
+              FOLD3D_E_INVALID, "fold3d_adjoint_batched: source %d: bad T, receivers or tol", k);
+  F3D_HIP(hipSetDevice(h->device));
+  F3D_CHECK_DEV(grad_slowness, "fold3d_adjoint_batched: grad_slowness");
+  F3D_CHECK_DEV(grad_theta, "fold3d_adjoint_batched: grad_theta");
+  if (check_model_fields(model) != FOLD3D_OK) return FOLD3D_E_INVALID;
+  const Grid G(nx, ny, nz);
+  std::vector<AdjointSource> srcs(count);
+  for (int k = 0; k < count; ++k) {
+    rc = check_adjoint_source(who, k, G, src[k].T, src[k].nrec, src[k].rec_idx, src[k].rec_res,
+                              src[k].tol, (hipStream_t)o.stream);
+    if (rc != FOLD3D_OK) return rc;
+    srcs[k] = AdjointSource{src[k].T, src[k].nrec, src[k].rec_idx, src[k].rec_res, src[k].tol};
   }
-  return FOLD3D_OK;
+  std::vector<int> iters(count, 0);
+  std::vector<double> maxd(count, -1.0);
+  double ms = 0;
+  rc = run_adjoint(h, who, count, srcs.data(), nx, ny, nz, dx, dy, dz, model, o, grad_slowness,
+                   grad_theta, iters.data(), maxd.data(), &ms);
+  if (rc == FOLD3D_OK || rc == FOLD3D_E_NOT_CONVERGED) {
+    for (int k = 0; k < count; ++k) {
+      if (iterations_out) iterations_out[k] = iters[k];
+      if (max_d_out) max_d_out[k] = maxd[k];
+    }
+    if (ms_out) *ms_out = ms;
+  }
+  return rc;
 }
 
 int fold3d_set_tuning(fold3d_handle *h, int model_kind, int groups) {

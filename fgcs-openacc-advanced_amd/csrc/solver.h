@@ -92,14 +92,21 @@ int batch_solve(hipStream_t st, int count, const fold3d_instance *inst, int max_
                 double *ms_out);
 size_t batch_scratch_bytes(int count, const fold3d_instance *inst, int precision, bool resident);
 
-// Adjoint state and gradients (adjoint.hip); `scratch` holds
-// adjoint_scratch_bytes() bytes.
+// Adjoint state and gradients of `nsrc` sources on one grid and model
+// (adjoint.hip); `scratch` holds adjoint_scratch_bytes(G, precision, nsrc)
+// bytes.  iters_out / maxd_out: per source.
+struct AdjointSource {
+  const void *T;        // converged forward field (device)
+  int nrec;
+  const int *rec_idx;   // receiver node indices (device)
+  const void *rec_res;  // residuals at the receivers (device)
+  double tol;           // stop rule: max change of an adjoint value in a pass
+};
 template <typename R>
-int adjoint_solve(hipStream_t st, const R *T, const Grid &G, const fold3d_model &m,
-                  const double h[3], int nrec, const int *rec_idx, const R *rec_res,
-                  int max_iters, double tol, R *grad_s, R *grad_th, void *scratch,
-                  unsigned long long *d_max, int *iters_out, double *maxd_out);
-size_t adjoint_scratch_bytes(const Grid &G, int precision);
+int adjoint_solve(hipStream_t st, int nsrc, const AdjointSource *srcs, const Grid &G,
+                  const fold3d_model &m, const double h[3], int max_iters, R *grad_s,
+                  R *grad_th, void *scratch, int *iters_out, double *maxd_out);
+size_t adjoint_scratch_bytes(const Grid &G, int precision, int nsrc);
 
 // The reference's input field built on the device, same bits as the host
 // initializer (gpu_init.hip); n_exact_out = nodes whose IF the host evaluated.

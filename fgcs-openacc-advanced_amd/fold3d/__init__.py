@@ -80,6 +80,16 @@ class Stats(C.Structure):
     ]
 
 
+class AdjointSource(C.Structure):
+    _fields_ = [
+        ("T", C.c_void_p),
+        ("nrec", C.c_int),
+        ("rec_idx", C.c_void_p),
+        ("rec_res", C.c_void_p),
+        ("tol", C.c_double),
+    ]
+
+
 class Instance(C.Structure):
     _fields_ = [
         ("T", C.c_void_p),
@@ -131,6 +141,10 @@ def lib() -> C.CDLL:
                                   C.POINTER(Opts), i, C.POINTER(C.c_float)]
     L.fold3d_adjoint.argtypes = [vp, vp, i, i, i, d, d, d, C.POINTER(Model), i, vp, vp,
                                  C.POINTER(Opts), vp, vp, C.POINTER(Stats)]
+    L.fold3d_adjoint_batched.argtypes = [vp, i, C.POINTER(AdjointSource), i, i, i, d, d, d,
+                                         C.POINTER(Model), C.POINTER(Opts), vp, vp,
+                                         C.POINTER(C.c_int), C.POINTER(C.c_double),
+                                         C.POINTER(C.c_double)]
     L.fold3d_synchronize.argtypes = [vp]
     L.fold3d_point_source.argtypes = [vp, vp, i, i, i, d, d, d, dp, dp, d, d, i, vp]
     L.fold3d_initialiser_device.argtypes = [vp, vp, i, i, i, d, d, d, vp, C.POINTER(C.c_longlong)]
@@ -358,6 +372,44 @@ class Solver:
         if not (rc == FOLD3D_E_NOT_CONVERGED and not raise_not_converged):
             check(rc)
         return st
+
+    def adjoint_batched(self, Ts, n, spacing, model: Model, rec_idx, rec_res, tols,
+                        grad_slowness, grad_theta=None, max_iters: int = 200, stream=None,
+                        raise_not_converged: bool = True):
+        """adjoint() for several sources on one grid and model at once: Ts, rec_idx,
+        rec_res and tols are per-source lists (rec_idx may be one tensor shared by
+        all).  Gradients of all sources are added in list order.  Returns
+        (sweeps per source, last max change per source, device ms)."""
+        import torch
+        count = len(Ts)
+        nx, ny, nz = (n, n, n) if np.isscalar(n) else n
+        dx, dy, dz = (spacing,) * 3 if np.isscalar(spacing) else spacing
+        idx = rec_idx if isinstance(rec_idx, (list, tuple)) else [rec_idx] * count
+        arr = (AdjointSource * count)()
+        for k in range(count):
+            if idx[k].dtype != torch.int32 or rec_res[k].dtype != Ts[k].dtype:
+                raise TypeError("rec_idx must be int32 and rec_res of T's dtype")
+            arr[k].T = Ts[k].data_ptr()
+            arr[k].nrec = int(idx[k].numel())
+            arr[k].rec_idx = idx[k].data_ptr()
+            arr[k].rec_res = rec_res[k].data_ptr()
+            arr[k].tol = float(tols[k])
+        o = Opts()
+        lib().fold3d_default_opts(C.byref(o))
+        o.precision = F64 if Ts[0].dtype == torch.float64 else F32
+        o.max_iters = max_iters
+        o.stream = stream if stream is not None else torch.cuda.current_stream(Ts[0].device).cuda_stream
+        its = (C.c_int * count)()
+        mds = (C.c_double * count)()
+        ms = C.c_double()
+        rc = lib().fold3d_adjoint_batched(
+            self.h, count, arr, nx, ny, nz, dx, dy, dz, C.byref(model), C.byref(o),
+            C.c_void_p(grad_slowness.data_ptr()),
+            None if grad_theta is None else C.c_void_p(grad_theta.data_ptr()), its, mds,
+            C.byref(ms))
+        if not (rc == FOLD3D_E_NOT_CONVERGED and not raise_not_converged):
+            check(rc)
+        return list(its), list(mds), ms.value
 
     def set_tuning(self, groups: int, kind: int = MODEL_FOLD):
         check(lib().fold3d_set_tuning(self.h, kind, groups))

@@ -133,6 +133,7 @@ class Tomography:
         m.speed_field = speed.data_ptr()
         m.theta, m.phi = theta.data_ptr(), self.phi.data_ptr()
         m.ra, m.rb = self.ra.data_ptr(), self.rb.data_ptr()
+        m._refs = (speed, theta, self.phi, self.ra, self.rb)  # the pointers stay valid with m
         return m
 
     def start_field(self, k: int, slowness, theta):
@@ -215,22 +216,24 @@ class Tomography:
         missing = [k for k in range(nsrc) if ckpt[k] is None]
         fields = dict((k, ckpt[k]) for k in range(nsrc) if ckpt[k] is not None)
         adj_iters, adj_ms = [], []
+
+        def adjoint_group(ks, Ts):
+            rs = [(res[k] * scale).contiguous() for k in ks]
+            its, _, ms = self.solver.adjoint_batched(Ts, S.n, S.h, m, idx, rs,
+                                                     [adjoint_tol(r) for r in rs], gs, gt,
+                                                     max_iters=self.adj_iters)
+            adj_iters.extend(its)
+            adj_ms.append(ms)
+
+        # checkpointed sources first, in groups of `batch`, then the recomputed ones
         order = [k for k in range(nsrc) if ckpt[k] is not None]
-        for k in order:
-            r = (res[k] * scale).contiguous()
-            st = self.solver.adjoint(fields[k], S.n, S.h, m, idx, r, gs, gt,
-                                     max_iters=self.adj_iters, tol=adjoint_tol(r))
-            adj_iters.append(st.iterations)
-            adj_ms.append(st.sweep_ms)
+        for g0 in range(0, len(order), self.batch):
+            ks = order[g0:g0 + self.batch]
+            adjoint_group(ks, [fields[k] for k in ks])
         for g0 in range(0, len(missing), self.batch):
             ks = missing[g0:g0 + self.batch]
             Ts = self.solve_group(ks, slowness, theta)
-            for k, T in zip(ks, Ts):
-                r = (res[k] * scale).contiguous()
-                st = self.solver.adjoint(T, S.n, S.h, m, idx, r, gs, gt,
-                                         max_iters=self.adj_iters, tol=adjoint_tol(r))
-                adj_iters.append(st.iterations)
-                adj_ms.append(st.sweep_ms)
+            adjoint_group(ks, Ts)
             del Ts
         self.stats["adjoint_iters"] = adj_iters
         self.stats["adjoint_ms"] = adj_ms

@@ -67,9 +67,12 @@ class Workload:
         for k in range(6):
             m.M[k] = float(self.M[k])
         m.speed = float(self.speed)
+        refs = []
         for name in ("speed_field", "theta", "phi", "ra", "rb"):
             f = getattr(self, name)
             setattr(m, name, None if f is None else f.data_ptr())
+            refs.append(f)
+        m._refs = tuple(refs)  # the pointers stay valid with m
         return m
 
 

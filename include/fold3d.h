@@ -182,6 +182,31 @@ int fold3d_adjoint(fold3d_handle *h, const void *T, int nx, int ny, int nz,
                    const fold3d_opts *opts, void *grad_slowness, void *grad_theta,
                    fold3d_stats *stats);
 
+/* fold3d_adjoint for `count` sources on one grid and model at once (e.g. the
+ * sources of a tomography survey): every plane launch of the reverse sweeps
+ * serves all sources still running, so small grids fill the GPU.  Each source
+ * has its converged field, receivers, residuals and stop rule (`tol`, as
+ * opts->tol of fold3d_adjoint; opts->tol is not used) and stops on its own.
+ * The gradients of all sources are added to grad_slowness / grad_theta in
+ * source order, with the same bits as `count` calls of fold3d_adjoint in that
+ * order.  Scratch: the handle holds 48 B per node per source in FP32 (76 B in
+ * FP64).  Outputs (may be NULL): sweeps run and last max change per source
+ * (arrays of length `count`), device time.  FOLD3D_E_NOT_CONVERGED if any
+ * source ran out of opts->max_iters sweeps (gradients written). */
+typedef struct fold3d_adjoint_source {
+  const void *T;          /* converged field of the source (device, opts->precision) */
+  int nrec;               /* receivers: linear node indices (device int32) and */
+  const int *rec_idx;     /*   residuals T[rec] - T_obs (device, opts->precision) */
+  const void *rec_res;
+  double tol;             /* stop when no adjoint value changes by more than tol */
+} fold3d_adjoint_source;
+
+int fold3d_adjoint_batched(fold3d_handle *h, int count, const fold3d_adjoint_source *src,
+                           int nx, int ny, int nz, double dx, double dy, double dz,
+                           const fold3d_model *model, const fold3d_opts *opts,
+                           void *grad_slowness, void *grad_theta, int *iterations_out,
+                           double *max_d_out, double *ms_out);
+
 /* Tuning of the split engines, per model kind: how many wavefronts share one
  * row of 64 nodes (fold model: 2, 4 or 8, default 4; elliptic model with a
  * constant metric: 1, 2 or 4, default 2; 0 restores the default).

@@ -106,6 +106,38 @@ def test_point_source_arguments_are_checked():
     assert "positive definite" in fold3d.last_error()
 
 
+def test_batched_adjoint_arguments_are_checked():
+    L = fold3d.lib()
+    dummy = C.c_void_p(0x1000)  # never dereferenced: validation fails first
+    m = fold3d.Model()
+    m.kind = fold3d.MODEL_ELLIP
+    m.speed = 1.0
+    o = fold3d.Opts()
+    L.fold3d_default_opts(C.byref(o))
+    src = (fold3d.AdjointSource * 2)()
+    src[0].T = src[1].T = 0x1000
+
+    def call(h=dummy, count=2, s=src, n=8, model=m, grad=dummy, gt=None):
+        return L.fold3d_adjoint_batched(h, count, s, n, n, n, 0.1, 0.1, 0.1,
+                                        None if model is None else C.byref(model), C.byref(o),
+                                        grad, gt, None, None, None)
+
+    assert call(h=None) == fold3d.FOLD3D_E_INVALID
+    assert call(s=None) == fold3d.FOLD3D_E_INVALID
+    assert call(count=0) == fold3d.FOLD3D_E_INVALID
+    assert call(n=0) == fold3d.FOLD3D_E_INVALID
+    assert call(model=None) == fold3d.FOLD3D_E_INVALID
+    assert call(grad=None) == fold3d.FOLD3D_E_INVALID
+    assert call(gt=dummy) == fold3d.FOLD3D_E_INVALID  # no tilt fields
+    assert "tilt" in fold3d.last_error()
+    src[1].nrec = 3  # receivers without arrays
+    assert call() == fold3d.FOLD3D_E_INVALID
+    assert "source 1" in fold3d.last_error()
+    src[1].nrec = 0
+    src[1].tol = -1.0
+    assert call() == fold3d.FOLD3D_E_INVALID
+
+
 def test_create_without_gpu_is_an_error_code():
     import torch
     if torch.cuda.is_available():

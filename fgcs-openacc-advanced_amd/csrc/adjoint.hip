@@ -410,10 +410,11 @@ __global__ void adj_zero_u64(unsigned long long *p, int n) {
 
 static size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
 
-// Bytes of one source's scratch: in-edge masks (u64), mu, res, three weights,
-// two partials (R) and three offsets (int) per node.
+// Bytes of one source's scratch per node: in-edge masks (u64), mu, res, three
+// weights, two partials (R) and three offsets (int) in the reference layout,
+// plus the masks, mu, res and weights again in the x-sweep layout.
 static size_t per_source_bytes(const Grid &G, size_t r) {
-  return align256((size_t)G.count * (sizeof(unsigned long long) + 7 * r + 3 * sizeof(int)));
+  return align256((size_t)G.count * (2 * sizeof(unsigned long long) + 12 * r + 3 * sizeof(int)));
 }
 
 template <typename R>
@@ -424,11 +425,16 @@ int adjoint_solve(hipStream_t st, int nsrc, const AdjointSource *srcs, const Gri
   const size_t per = per_source_bytes(G, sizeof(R));
   char *base = (char *)scratch;
   // after the sources: the device table of sources, max|d mu| and done flags
+  // x-sweeps run on copies in layout X (x slowest), where a plane x = const is
+  // contiguous: in the reference layout its rows are strided by n + 2 and
+  // every lane of a wave would load its own cache line
+  const Grid GX = G.with_order(kOrderX);
   AdjSrc<R> *d_src = (AdjSrc<R> *)(base + per * nsrc);
+  AdjSrc<R> *d_srcX = d_src + nsrc;
   unsigned long long *d_max =
-      (unsigned long long *)((char *)d_src + align256(sizeof(AdjSrc<R>) * nsrc));
+      (unsigned long long *)((char *)d_src + align256(sizeof(AdjSrc<R>) * 2 * nsrc));
   int *d_done = (int *)((char *)d_max + align256(sizeof(unsigned long long) * nsrc));
-  std::vector<AdjSrc<R>> hs(nsrc);
+  std::vector<AdjSrc<R>> hs(2 * nsrc);  // [0, nsrc): reference layout, then layout X
   for (int k = 0; k < nsrc; ++k) {
     char *p = base + per * k;
     AdjSrc<R> &A = hs[k];
@@ -445,9 +451,19 @@ int adjoint_solve(hipStream_t st, int nsrc, const AdjointSource *srcs, const Gri
     A.S.off1 = (int *)p;
     p += count * sizeof(int);
     A.S.off2 = (int *)p;
+    p += count * sizeof(int);
+    AdjSrc<R> &X = hs[nsrc + k];
+    memset(&X, 0, sizeof X);
+    X.in_mask = (unsigned long long *)p;
+    p += count * sizeof(unsigned long long);
+    R **fx[5] = {&X.mu, &X.res, &X.S.w0, &X.S.w1, &X.S.w2};
+    for (R **f : fx) {
+      *f = (R *)p;
+      p += count * sizeof(R);
+    }
   }
   F3D_HIP(hipMemsetAsync(scratch, 0, adjoint_scratch_bytes(G, sizeof(R) == 8 ? FOLD3D_F64 : FOLD3D_F32, nsrc), st));
-  F3D_HIP(hipMemcpyAsync(d_src, hs.data(), sizeof(AdjSrc<R>) * nsrc, hipMemcpyHostToDevice, st));
+  F3D_HIP(hipMemcpyAsync(d_src, hs.data(), sizeof(AdjSrc<R>) * 2 * nsrc, hipMemcpyHostToDevice, st));
   AdjArgs<R> A;
   memset(&A, 0, sizeof A);
   for (int a = 0; a < 3; ++a) {
@@ -477,6 +493,15 @@ int adjoint_solve(hipStream_t st, int nsrc, const AdjointSource *srcs, const Gri
     F3D_HIP(hipGetLastError());
     // nodes no other node lists keep mu = res; the sweeps update the rest
     F3D_HIP(hipMemcpyAsync(hs[k].mu, hs[k].res, count * sizeof(R), hipMemcpyDeviceToDevice, st));
+    // the fixed fields of the x-sweeps in layout X (the 8-byte masks move as
+    // doubles: the copy keeps their bits)
+    const AdjSrc<R> &A = hs[k], &X = hs[nsrc + k];
+    permute_layout<double>(st, (const double *)A.in_mask, G, (double *)X.in_mask, GX);
+    permute_layout<R>(st, A.res, G, X.res, GX);
+    permute_layout<R>(st, A.S.w0, G, X.S.w0, GX);
+    permute_layout<R>(st, A.S.w1, G, X.S.w1, GX);
+    permute_layout<R>(st, A.S.w2, G, X.S.w2, GX);
+    F3D_HIP(hipGetLastError());
   }
   // One sweep pass (all six directions, reverse order, all sources) is captured
   // once in a hipGraph on a stream of its own (the legacy default stream cannot
@@ -512,19 +537,28 @@ int adjoint_solve(hipStream_t st, int nsrc, const AdjointSource *srcs, const Gri
   ADJ_HIP(hipStreamBeginCapture(ws, hipStreamCaptureModeThreadLocal));
   capturing = true;
   adj_zero_u64<<<(nsrc + 255) / 256, 256, 0, ws>>>(d_max, nsrc);
-  // reverse of the forward order: each direction's planes from last to first
+  // reverse of the forward order: each direction's planes from last to first;
+  // the y- and z-sweeps in the reference layout, the x-sweeps (last) in layout
+  // X, with mu moved there and back around them
   for (int dd = 5; dd >= 0; --dd) {
     const Frame &fr = kFrames[dd];
-    const int n = G.n[fr.m];
+    const bool xs = fr.m == 0;
+    const Grid &GL = xs ? GX : G;
+    if (dd == 1)
+      for (int k = 0; k < nsrc; ++k) permute_layout<R>(ws, hs[k].mu, G, hs[nsrc + k].mu, GX);
+    const int n = GL.n[fr.m];
     for (int q = 0; q < n; ++q) {
       // sweep plane index along the march axis, against the characteristics
       const int pl = fr.s > 0 ? n - q : 1 + q;
-      PlaneGeom g = plane_geom(G, dd, pl);
+      PlaneGeom g = plane_geom(GL, dd, pl);
       g.sm = 0;  // the plane itself (plane_geom addresses a source plane)
       const dim3 grid((g.na + 63) / 64, (g.nb + 3) / 4, nsrc);
-      adjoint_plane_kernel<R><<<grid, block, 0, ws>>>(d_src, d_done, g, G.stride[0],
-                                                      G.stride[1], G.stride[2], d_max);
+      adjoint_plane_kernel<R><<<grid, block, 0, ws>>>(xs ? d_srcX : d_src, d_done, g,
+                                                      GL.stride[0], GL.stride[1], GL.stride[2],
+                                                      d_max);
     }
+    if (dd == 0)
+      for (int k = 0; k < nsrc; ++k) permute_layout<R>(ws, hs[nsrc + k].mu, GX, hs[k].mu, G);
   }
   capturing = false;
   ADJ_HIP(hipStreamEndCapture(ws, &graph));
@@ -577,7 +611,7 @@ size_t adjoint_scratch_bytes(const Grid &G, int precision, int nsrc) {
   const size_t r = precision == FOLD3D_F64 ? 8 : 4;
   const size_t tab = sizeof(AdjSrc<double>) > sizeof(AdjSrc<float>) ? sizeof(AdjSrc<double>)
                                                                     : sizeof(AdjSrc<float>);
-  return per_source_bytes(G, r) * nsrc + align256(tab * nsrc) +
+  return per_source_bytes(G, r) * nsrc + align256(tab * 2 * nsrc) +
          align256(sizeof(unsigned long long) * nsrc) + align256(sizeof(int) * nsrc);
 }
 

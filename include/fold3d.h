@@ -189,8 +189,8 @@ int fold3d_adjoint(fold3d_handle *h, const void *T, int nx, int ny, int nz,
  * opts->tol of fold3d_adjoint; opts->tol is not used) and stops on its own.
  * The gradients of all sources are added to grad_slowness / grad_theta in
  * source order, with the same bits as `count` calls of fold3d_adjoint in that
- * order.  Scratch: the handle holds 48 B per node per source in FP32 (76 B in
- * FP64).  Outputs (may be NULL): sweeps run and last max change per source
+ * order.  Scratch: the handle holds 76 B per node per source in FP32 (124 B
+ * in FP64).  Outputs (may be NULL): sweeps run and last max change per source
  * (arrays of length `count`), device time.  FOLD3D_E_NOT_CONVERGED if any
  * source ran out of opts->max_iters sweeps (gradients written). */
 typedef struct fold3d_adjoint_source {

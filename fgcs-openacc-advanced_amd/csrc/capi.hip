@@ -128,7 +128,7 @@ int fold3d_create(fold3d_handle **out, int device) {
   h->split_groups = 4;
   h->ellip_groups = 2;
   h->batch_groups = 1;
-  h->batch_blocks_per_cu = 32;
+  h->batch_blocks_per_cu = 64;
   h->batch_rows_per_pull = -1;
   hipError_t e = hipMalloc((void **)&h->d_max, sizeof(unsigned long long));
   if (e != hipSuccess) {
@@ -715,7 +715,7 @@ int fold3d_set_batch_tuning(fold3d_handle *h, int groups, int blocks_per_cu, int
   F3D_CHECK(rows_per_pull >= -1 && rows_per_pull <= 64, FOLD3D_E_INVALID,
             "fold3d_set_batch_tuning: rows_per_pull must be -1..64");
   h->batch_groups = groups == 0 ? 1 : groups;
-  h->batch_blocks_per_cu = blocks_per_cu == 0 ? 32 : blocks_per_cu;
+  h->batch_blocks_per_cu = blocks_per_cu == 0 ? 64 : blocks_per_cu;
   h->batch_rows_per_pull = rows_per_pull;
   return FOLD3D_OK;
 }

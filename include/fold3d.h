@@ -242,7 +242,7 @@ int fold3d_initialiser_device(fold3d_handle *h, double *T, int nx, int ny, int n
                               double dy, double dz, void *stream, long long *n_exact);
 
 /* Tuning of the batched solve: waves per row of 64 nodes (1, 2, 4; default
- * 1), persistent blocks per compute unit in each launch (1..64; default 32)
+ * 1), persistent blocks per compute unit in each launch (1..64; default 64)
  * and how a block finds its rows: 1..64 = that many rows per pull from the
  * launch's device queue, 0 = queue with an automatic pull size, -1 = static
  * (block b takes rows b, b + grid, ...; the default).  0 arguments of

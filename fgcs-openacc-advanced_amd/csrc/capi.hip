@@ -29,8 +29,6 @@ struct fold3d_handle {
   int batch_rows_per_pull;    // batched solve: rows per queue pull (0 = automatic)
 };
 
-// Grow the handle's scratch field to at least `bytes` (stream-ordered free of
-// the old one is not available, so the device is synchronised first).
 #define F3D_CHECK_DEV(p, what)                                                             \
   F3D_CHECK((p) == nullptr || is_device_ptr(p), FOLD3D_E_INVALID,                          \
             "%s must be device memory (got a host or unknown pointer)", what)
@@ -57,6 +55,16 @@ static int check_model_fields(const fold3d_model *m) {
   return FOLD3D_OK;
 }
 
+// Device memory released when it goes out of scope (error paths included).
+struct DeviceBuffer {
+  void *p = nullptr;
+  ~DeviceBuffer() {
+    if (p) (void)hipFree(p);
+  }
+};
+
+// Grow the handle's scratch field to at least `bytes` (stream-ordered free of
+// the old one is not available, so the device is synchronised first).
 static int ensure_scratch(fold3d_handle *h, size_t bytes) {
   if (h->scratch_bytes >= bytes) return FOLD3D_OK;
   if (h->d_scratch) {
@@ -123,7 +131,7 @@ int fold3d_create(fold3d_handle **out, int device) {
   F3D_CHECK(device >= 0 && device < count, FOLD3D_E_INVALID,
             "fold3d_create: device %d not in [0,%d)", device, count);
   F3D_HIP(hipSetDevice(device));
-  fold3d_handle *h = new fold3d_handle();
+  fold3d_handle *h = new fold3d_handle();  // value-initialised: no resources yet
   h->device = device;
   h->split_groups = 4;
   h->ellip_groups = 2;
@@ -131,16 +139,16 @@ int fold3d_create(fold3d_handle **out, int device) {
   h->batch_blocks_per_cu = 64;
   h->batch_rows_per_pull = -1;
   hipError_t e = hipMalloc((void **)&h->d_max, sizeof(unsigned long long));
+  if (e == hipSuccess) e = hipEventCreate(&h->ev0);
+  if (e == hipSuccess) e = hipEventCreate(&h->ev1);
+  if (e == hipSuccess) e = hipEventCreateWithFlags(&h->consts_copied, hipEventDisableTiming);
+  if (e == hipSuccess) e = hipMalloc(&h->d_consts, kConstBytes);
+  if (e == hipSuccess) e = hipHostMalloc(&h->h_consts, kConstBytes, hipHostMallocDefault);
   if (e != hipSuccess) {
-    delete h;
-    set_error("fold3d_create: hipMalloc failed: %s", hipGetErrorString(e));
-    return FOLD3D_E_NOMEM;
+    set_error("fold3d_create: %s", hipGetErrorString(e));
+    fold3d_destroy(h);  // releases what was created
+    return e == hipErrorOutOfMemory ? FOLD3D_E_NOMEM : FOLD3D_E_HIP;
   }
-  F3D_HIP(hipEventCreate(&h->ev0));
-  F3D_HIP(hipEventCreate(&h->ev1));
-  F3D_HIP(hipEventCreateWithFlags(&h->consts_copied, hipEventDisableTiming));
-  F3D_HIP(hipMalloc(&h->d_consts, kConstBytes));
-  F3D_HIP(hipHostMalloc(&h->h_consts, kConstBytes, hipHostMallocDefault));
   *out = h;
   return FOLD3D_OK;
 }
@@ -148,14 +156,14 @@ int fold3d_create(fold3d_handle **out, int device) {
 int fold3d_destroy(fold3d_handle *h) {
   if (!h) return FOLD3D_OK;
   (void)hipSetDevice(h->device);
-  (void)hipEventSynchronize(h->consts_copied);
+  if (h->consts_copied) (void)hipEventSynchronize(h->consts_copied);
   if (h->d_scratch) (void)hipFree(h->d_scratch);
-  (void)hipFree(h->d_consts);
-  (void)hipHostFree(h->h_consts);
-  (void)hipEventDestroy(h->consts_copied);
-  (void)hipFree(h->d_max);
-  (void)hipEventDestroy(h->ev0);
-  (void)hipEventDestroy(h->ev1);
+  if (h->d_consts) (void)hipFree(h->d_consts);
+  if (h->h_consts) (void)hipHostFree(h->h_consts);
+  if (h->consts_copied) (void)hipEventDestroy(h->consts_copied);
+  if (h->d_max) (void)hipFree(h->d_max);
+  if (h->ev0) (void)hipEventDestroy(h->ev0);
+  if (h->ev1) (void)hipEventDestroy(h->ev1);
   delete h;
   return FOLD3D_OK;
 }
@@ -217,8 +225,10 @@ int fold3d_solve(fold3d_handle *h, void *T, int nx, int ny, int nz, double dx,
   const size_t esz = o.precision == FOLD3D_F64 ? 8 : 4;
   const size_t bytes = (size_t)G.count * esz;
   void *dT = T;
+  DeviceBuffer staging;  // host-resident T: its device copy, freed on every path
   if (o.memory == FOLD3D_HOST) {
-    F3D_HIP(hipMalloc(&dT, bytes));
+    F3D_HIP(hipMalloc(&staging.p, bytes));
+    dT = staging.p;
     F3D_HIP(hipMemcpyAsync(dT, T, bytes, hipMemcpyHostToDevice, st));
   }
   const double hsp[3] = {dx, dy, dz};
@@ -321,7 +331,6 @@ int fold3d_solve(fold3d_handle *h, void *T, int nx, int ny, int nz, double dx,
   if (o.memory == FOLD3D_HOST) {
     F3D_HIP(hipMemcpyAsync(T, dT, bytes, hipMemcpyDeviceToHost, st));
     F3D_HIP(hipStreamSynchronize(st));
-    F3D_HIP(hipFree(dT));
   }
   F3D_HIP(hipEventSynchronize(h->ev1));
   float ms = 0;
@@ -370,13 +379,14 @@ static int check_batch(fold3d_handle *h, int count, const fold3d_instance *inst,
               i);
     F3D_CHECK(x.n[0] >= 1 && x.n[1] >= 1 && x.n[2] >= 1, FOLD3D_E_INVALID,
               "fold3d_solve_batched: instance %d grid too small", i);
-    F3D_CHECK(x.h[0] > 0 && x.h[1] > 0 && x.h[2] > 0, FOLD3D_E_INVALID,
-              "fold3d_solve_batched: instance %d spacings must be positive", i);
+    F3D_CHECK(x.h[0] > 0 && x.h[1] > 0 && x.h[2] > 0 && std::isfinite(x.h[0]) &&
+                  std::isfinite(x.h[1]) && std::isfinite(x.h[2]),
+              FOLD3D_E_INVALID, "fold3d_solve_batched: instance %d spacings must be positive", i);
     const int nf = (x.theta != nullptr) + (x.phi != nullptr) + (x.ra != nullptr) + (x.rb != nullptr);
     F3D_CHECK(nf == 0 || nf == 4, FOLD3D_E_INVALID,
               "fold3d_solve_batched: instance %d: give all of theta, phi, ra, rb or none", i);
-    F3D_CHECK(x.speed_field != nullptr || x.speed > 0, FOLD3D_E_INVALID,
-              "fold3d_solve_batched: instance %d speed must be positive", i);
+    F3D_CHECK(x.speed_field != nullptr || (x.speed > 0 && std::isfinite(x.speed)),
+              FOLD3D_E_INVALID, "fold3d_solve_batched: instance %d speed must be positive", i);
     if (nf == 0) {
       const double *M = x.M;
       const double m2 = M[0] * M[1] - M[3] * M[3];
@@ -492,11 +502,13 @@ static int check_adjoint_common(const char *who, int nx, int ny, int nz, double 
   F3D_CHECK(model->kind == FOLD3D_MODEL_ELLIP, FOLD3D_E_INVALID,
             "%s: the adjoint is for the elliptic model", who);
   F3D_CHECK(nx >= 1 && ny >= 1 && nz >= 1, FOLD3D_E_INVALID, "%s: grid too small", who);
-  F3D_CHECK(dx > 0 && dy > 0 && dz > 0, FOLD3D_E_INVALID, "%s: bad spacings", who);
+  F3D_CHECK(dx > 0 && dy > 0 && dz > 0 && std::isfinite(dx) && std::isfinite(dy) &&
+                std::isfinite(dz),
+            FOLD3D_E_INVALID, "%s: bad spacings", who);
   F3D_CHECK(grad_theta == nullptr || model->theta != nullptr, FOLD3D_E_INVALID,
             "%s: a tilt gradient needs per-node tilt fields", who);
-  F3D_CHECK(model->speed_field != nullptr || model->speed > 0, FOLD3D_E_INVALID,
-            "%s: speed must be positive", who);
+  F3D_CHECK(model->speed_field != nullptr || (model->speed > 0 && std::isfinite(model->speed)),
+            FOLD3D_E_INVALID, "%s: speed must be positive", who);
   F3D_CHECK(o.memory == FOLD3D_DEVICE, FOLD3D_E_INVALID, "%s: T must be on the device", who);
   F3D_CHECK(o.precision == FOLD3D_F64 || o.precision == FOLD3D_F32, FOLD3D_E_INVALID,
             "%s: bad precision %d", who, o.precision);
@@ -743,6 +755,7 @@ int fold3d_autotune(fold3d_handle *h, const void *T, int nx, int ny, int nz, dou
   const Grid G(nx, ny, nz);
   const size_t bytes = (size_t)G.count * (o.precision == FOLD3D_F64 ? 8 : 4);
   F3D_HIP(hipSetDevice(h->device));
+  F3D_CHECK_DEV(T, "fold3d_autotune: T");
   void *work = nullptr;
   F3D_HIP(hipMalloc(&work, bytes));
   hipStream_t st = (hipStream_t)o.stream;

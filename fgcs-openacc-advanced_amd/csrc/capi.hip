@@ -509,6 +509,20 @@ static int check_adjoint_common(const char *who, int nx, int ny, int nz, double 
             "%s: a tilt gradient needs per-node tilt fields", who);
   F3D_CHECK(model->speed_field != nullptr || (model->speed > 0 && std::isfinite(model->speed)),
             FOLD3D_E_INVALID, "%s: speed must be positive", who);
+  const int nf = (model->theta != nullptr) + (model->phi != nullptr) + (model->ra != nullptr) +
+                 (model->rb != nullptr);
+  F3D_CHECK(nf == 0 || nf == 4, FOLD3D_E_INVALID,
+            "%s: give all four of theta, phi, ra, rb or none", who);
+  if (nf == 0) {
+    const double *M = model->M;
+    bool fin = true;
+    for (int k = 0; k < 6; ++k) fin = fin && std::isfinite(M[k]);
+    const double m2 = M[0] * M[1] - M[3] * M[3];
+    const double m3 = M[0] * (M[1] * M[2] - M[5] * M[5]) - M[3] * (M[3] * M[2] - M[5] * M[4]) +
+                      M[4] * (M[3] * M[5] - M[1] * M[4]);
+    F3D_CHECK(fin && M[0] > 0 && m2 > 0 && m3 > 0, FOLD3D_E_INVALID,
+              "%s: metric M is not symmetric positive definite", who);
+  }
   F3D_CHECK(o.memory == FOLD3D_DEVICE, FOLD3D_E_INVALID, "%s: T must be on the device", who);
   F3D_CHECK(o.precision == FOLD3D_F64 || o.precision == FOLD3D_F32, FOLD3D_E_INVALID,
             "%s: bad precision %d", who, o.precision);

@@ -112,6 +112,7 @@ def test_batched_adjoint_arguments_are_checked():
     m = fold3d.Model()
     m.kind = fold3d.MODEL_ELLIP
     m.speed = 1.0
+    m.M[0] = m.M[1] = m.M[2] = 1.0
     o = fold3d.Opts()
     L.fold3d_default_opts(C.byref(o))
     src = (fold3d.AdjointSource * 2)()
@@ -130,6 +131,14 @@ def test_batched_adjoint_arguments_are_checked():
     assert call(grad=None) == fold3d.FOLD3D_E_INVALID
     assert call(gt=dummy) == fold3d.FOLD3D_E_INVALID  # no tilt fields
     assert "tilt" in fold3d.last_error()
+    m.theta = 0x1000  # tilt fields come as all four or none
+    assert call() == fold3d.FOLD3D_E_INVALID
+    assert "all four" in fold3d.last_error()
+    m.theta = None
+    m.M[3] = 2.0  # not positive definite
+    assert call() == fold3d.FOLD3D_E_INVALID
+    assert "positive definite" in fold3d.last_error()
+    m.M[3] = 0.0
     src[1].nrec = 3  # receivers without arrays
     assert call() == fold3d.FOLD3D_E_INVALID
     assert "source 1" in fold3d.last_error()

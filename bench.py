@@ -300,6 +300,8 @@ def main():
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--no-c0", action="store_true")
+    ap.add_argument("--no-fixpt", action="store_true",
+                    help="skip the solves to the exact fixed point (split and locked engines)")
     ap.add_argument("--c2-n", type=int, default=1280, help="side of the per-node metric run (0: skip)")
     ap.add_argument("--iters", type=int, default=8, help="iterations per solve (the reference runs 8)")
     args = ap.parse_args()
@@ -371,6 +373,31 @@ def main():
         f"{locked['computed_fraction']:.3f} of the updates computed, same bits: "
         f"{locked['same_bits_as_split']}")
     del T_l
+
+    # time to the exact fixed point (no node changes in a whole iteration),
+    # split and locked engines: the reference's 8 iterations stop well short of it
+    fixpt = None
+    if not args.no_fixpt:
+        log("bench: FP64 solves to the exact fixed point, split and locked engines")
+        fixpt = {}
+        res = {}
+        for name, eng in (("split", fold3d.ENGINE_SPLIT), ("locked", fold3d.ENGINE_LOCKED)):
+            Tf = T0d.clone()
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            st_f = solver.solve(Tf, n, max_iters=1000, tol=1e-300, engine=eng)
+            torch.cuda.synchronize()
+            fixpt[name] = {"s": time.perf_counter() - t0, "iterations": st_f.iterations,
+                           "computed_fraction": st_f.updates_computed / st_f.cell_updates}
+            res[name] = Tf
+        fixpt["same_bits"] = bool(torch.equal(res["split"].view(torch.int64),
+                                              res["locked"].view(torch.int64)))
+        fixpt["locked_speedup"] = fixpt["split"]["s"] / fixpt["locked"]["s"]
+        log(f"bench: fixed point after {fixpt['split']['iterations']} iterations: split "
+            f"{fixpt['split']['s']:.2f} s, locked {fixpt['locked']['s']:.2f} s "
+            f"({fixpt['locked']['computed_fraction']:.3f} of the updates computed), same bits: "
+            f"{fixpt['same_bits']}")
+        del res, Tf
 
     engines = {}
     if not args.no_ablation:
@@ -500,6 +527,7 @@ def main():
                      "bytes_per_launch": bytes_per_launch, "valu_issue": valu_issue},
         "engines_fp64": engines,
         "locked_engine_fp64": locked,
+        "fixed_point_fp64": fixpt,
         "initializer": init,
         "fp32": fp32,
         "elliptic_blob_speed": ellip,

@@ -87,28 +87,29 @@ static LatticeIdx lattice_idx(int d) {
 }
 
 template <typename R, int G0, int kStride>
-__device__ __forceinline__ void batch_units(R &best, const R p[9], R s2, const LatticeN<R> &L) {
+__device__ __forceinline__ void batch_units(R &best, const R p[9], R s2, const LatticeN<R> &L,
+                                            R gap) {
   if constexpr (G0 < 8) {
-    ellip_unit_lattice<R, G0>(best, p, s2, L);
-    batch_units<R, G0 + kStride, kStride>(best, p, s2, L);
+    ellip_unit_lattice<R, G0, true>(best, p, s2, L, gap);
+    batch_units<R, G0 + kStride, kStride>(best, p, s2, L, gap);
   }
 }
 
 template <typename R, int kGroups>
 __device__ __forceinline__ void batch_dispatch_units(int grp, R &best, const R p[9], R s2,
-                                                     const LatticeN<R> &L) {
+                                                     const LatticeN<R> &L, R gap) {
   if constexpr (kGroups == 1) {
-    batch_units<R, 0, 1>(best, p, s2, L);
+    batch_units<R, 0, 1>(best, p, s2, L, gap);
   } else if constexpr (kGroups == 2) {
-    if (grp == 0) batch_units<R, 0, 2>(best, p, s2, L);
-    else batch_units<R, 1, 2>(best, p, s2, L);
+    if (grp == 0) batch_units<R, 0, 2>(best, p, s2, L, gap);
+    else batch_units<R, 1, 2>(best, p, s2, L, gap);
   } else {
     static_assert(kGroups == 4, "one, two or four waves per node row");
     switch (grp) {
-      case 0: batch_units<R, 0, 4>(best, p, s2, L); break;
-      case 1: batch_units<R, 1, 4>(best, p, s2, L); break;
-      case 2: batch_units<R, 2, 4>(best, p, s2, L); break;
-      default: batch_units<R, 3, 4>(best, p, s2, L); break;
+      case 0: batch_units<R, 0, 4>(best, p, s2, L, gap); break;
+      case 1: batch_units<R, 1, 4>(best, p, s2, L, gap); break;
+      case 2: batch_units<R, 2, 4>(best, p, s2, L, gap); break;
+      default: batch_units<R, 3, 4>(best, p, s2, L, gap); break;
     }
   }
 }
@@ -205,7 +206,7 @@ __global__ __launch_bounds__(64 * kGroups) void batch_plane_kernel(
         pv[1 + k] = s[1 + ii * ub + jj * vb][ca + ii * ua + jj * va];
       }
       ellip_vertices_lattice<R>(best, pv, sl, L);
-      batch_dispatch_units<R, kGroups>(grp, best, pv, sl * sl, L);
+      batch_dispatch_units<R, kGroups>(grp, best, pv, sl * sl, L, plane_gap(L, sl));
     }
     part[grp][tx] = best;
     __syncthreads();

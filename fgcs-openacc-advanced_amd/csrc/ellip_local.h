@@ -15,7 +15,11 @@
 //     lambda = l0 - (root/r) mu in the simplex  =>  candidate T_0 + delta.l0 + root.
 // A candidate of S is never below min over S's vertices, so a simplex whose
 // smallest vertex is not below the running minimum is skipped: pruning that
-// cannot change the result.
+// cannot change the result.  The split, metric and batched kernels prune
+// harder: S lies in the upwind plane, so a candidate is also at least that
+// minimum plus s times the N-distance from X to the plane (plane_gap below).
+// The one-thread-per-node plane kernel keeps the plain test, and the tests
+// hold the two to the same bits.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -51,9 +55,26 @@ __device__ __forceinline__ bool reached(R t) { return t < R(1e15); }
 __device__ __forceinline__ double vertex_sqrt(double x) { return sqrt_rn(x); }
 __device__ __forceinline__ float vertex_sqrt(float x) { return sqrtf(x); }
 
+// The pruning test: can a simplex with smallest vertex `lo` improve on `best`?
+// kGap: the bound adds `gap` (a lower bound of s |X - Y|_N over the upwind
+// plane, already shrunk by plane_gap) and allows 16 ulps of `best` for the
+// candidate's own rounding.  Without kGap: the plain vertex test.
 template <typename R>
-__device__ __forceinline__ void ellip_edge(R &best, R t0, R t1, R s2, const R *E) {
-  if (!(rmin(t0, t1) < best) || !(reached(t0) && reached(t1))) return;
+__device__ __forceinline__ R prune_slack();
+template <>
+__device__ __forceinline__ double prune_slack<double>() { return 1.0 + 16.0 * 2.220446049250313e-16; }
+template <>
+__device__ __forceinline__ float prune_slack<float>() { return 1.0f + 16.0f * 1.1920929e-7f; }
+
+template <typename R, bool kGap>
+__device__ __forceinline__ bool may_win(R lo, R gap, R best) {
+  if constexpr (kGap) return lo + gap < best * prune_slack<R>();
+  return lo < best;
+}
+
+template <typename R, bool kGap = false>
+__device__ __forceinline__ void ellip_edge(R &best, R t0, R t1, R s2, const R *E, R gap = R(0)) {
+  if (!may_win<R, kGap>(rmin(t0, t1), gap, best) || !(reached(t0) && reached(t1))) return;
   const R P = E[0], l0 = E[1], kap = E[2];
   const R d = t1 - t0;
   const R mu = P * d;
@@ -66,9 +87,11 @@ __device__ __forceinline__ void ellip_edge(R &best, R t0, R t1, R s2, const R *E
   if (c < best) best = c;
 }
 
-template <typename R>
-__device__ __forceinline__ void ellip_tri(R &best, R t0, R t1, R t2, R s2, const R *Tr) {
-  if (!(rmin(t0, rmin(t1, t2)) < best) || !(reached(t0) && reached(t1) && reached(t2)))
+template <typename R, bool kGap = false>
+__device__ __forceinline__ void ellip_tri(R &best, R t0, R t1, R t2, R s2, const R *Tr,
+                                          R gap = R(0)) {
+  if (!may_win<R, kGap>(rmin(t0, rmin(t1, t2)), gap, best) ||
+      !(reached(t0) && reached(t1) && reached(t2)))
     return;
   const R d1 = t1 - t0, d2 = t2 - t0;
   const R m1 = Tr[0] * d1 + Tr[1] * d2;
@@ -197,21 +220,39 @@ __device__ __forceinline__ LatticeN<R> lattice_of(const R N[6], const int u[3], 
   return L;
 }
 
+// Lower bound, for the pruning test, of s |X - Y|_N over the points Y of the
+// upwind plane: the minimum of |w|_N^2 over w = (a dxx, b dyy, dzz), a and b
+// real, is the Schur complement C - (B Fx^2 - 2 E Fx Fy + A Fy^2) / (A B - E^2).
+// It is shrunk to leave room for the rounding of a candidate: near a grazing
+// simplex the square root of the rounded s^2 - delta.mu moves a candidate by up
+// to about 5 sqrt(eps) s h sqrt(lambda_max(N)): in FP32 2e-3 of the bound for
+// a metric of condition number 1 and 0.1 at about 3000, in FP64 1e-3 of the
+// bound at 2e8.  So FP32 keeps 90 % of the bound and FP64 99.9 %.  Ill-conditioned cases (the complement below
+// 1e-3 of C, or a nearly singular in-plane block) get no bound.
+template <typename R>
+__device__ __forceinline__ R plane_gap(const LatticeN<R> &L, R s) {
+  const R det = L.A * L.B - L.E * L.E;
+  if (!(det > R(1e-6) * L.A * L.B)) return R(0);
+  const R q = L.C - (L.B * L.Fx * L.Fx - R(2) * L.E * L.Fx * L.Fy + L.A * L.Fy * L.Fy) / det;
+  if (!(q > R(1e-3) * L.C)) return R(0);
+  return (sizeof(R) == 8 ? R(0.999) : R(0.9)) * s * sqrt(q);
+}
+
 // Unit U of the node (triangle U, spoke U, rim U) with constants from L.
-template <typename R, int U>
+template <typename R, int U, bool kGap = false>
 __device__ __forceinline__ void ellip_unit_lattice(R &best, const R p[9], R s2,
-                                                   const LatticeN<R> &L) {
+                                                   const LatticeN<R> &L, R gap = R(0)) {
   constexpr int k1 = 1 + U, k2 = 1 + (U + 1) % 8;
   constexpr int a1 = kRing[U][0], b1 = kRing[U][1];
   constexpr int a2 = kRing[(U + 1) % 8][0], b2 = kRing[(U + 1) % 8][1];
   const R t0 = p[0], t1 = p[k1], t2 = p[k2];
-  if (!(rmin(t0, rmin(t1, t2)) < best)) return;  // nothing of this unit can win
+  if (!may_win<R, kGap>(rmin(t0, rmin(t1, t2)), gap, best)) return;  // nothing here can win
   {  // spoke centre -> ring U: E = (a1, b1, 0), D = (0, 0, 1)
     const R g = lat_dot(L, a1, b1, 0, a1, b1, 0);
     const R b = lat_dot(L, a1, b1, 0, 0, 0, 1);
     const R P = R(1) / g, l0 = P * b;
     const R K[3] = {P, l0, L.C - b * l0};
-    ellip_edge(best, t0, t1, s2, K);
+    ellip_edge<R, kGap>(best, t0, t1, s2, K, gap);
   }
   {  // rim ring U -> ring U+1: E = (a2-a1, b2-b1, 0), D = (-a1, -b1, 1)
     constexpr int ea = a2 - a1, eb = b2 - b1;
@@ -220,7 +261,7 @@ __device__ __forceinline__ void ellip_unit_lattice(R &best, const R p[9], R s2,
     const R c = lat_dot(L, -a1, -b1, 1, -a1, -b1, 1);
     const R P = R(1) / g, l0 = P * b;
     const R K[3] = {P, l0, c - b * l0};
-    ellip_edge(best, t1, t2, s2, K);
+    ellip_edge<R, kGap>(best, t1, t2, s2, K, gap);
   }
   {  // triangle centre, ring U, ring U+1
     const R g11 = lat_dot(L, a1, b1, 0, a1, b1, 0), g22 = lat_dot(L, a2, b2, 0, a2, b2, 0);
@@ -230,7 +271,7 @@ __device__ __forceinline__ void ellip_unit_lattice(R &best, const R p[9], R s2,
     const R P11 = g22 * rdet, P12 = -g12 * rdet, P22 = g11 * rdet;
     const R l1 = P11 * q1 + P12 * q2, l2 = P12 * q1 + P22 * q2;
     const R K[6] = {P11, P12, P22, l1, l2, L.C - (q1 * l1 + q2 * l2)};
-    ellip_tri(best, t0, t1, t2, s2, K);
+    ellip_tri<R, kGap>(best, t0, t1, t2, s2, K, gap);
   }
 }
 

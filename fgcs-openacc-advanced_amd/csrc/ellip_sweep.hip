@@ -25,6 +25,7 @@ struct EllipArgs {
   int u[3];                // local frame axes x^, y^, z^ (global axis ids)
   R sg[3];                 // their signs
   R hl[3];                 // spacings along x^, y^, z^
+  R gk;                    // constant metric: plane_gap of the direction for s = 1
 };
 
 template <typename R, bool kReduce, bool kSpeed, bool kMetric>
@@ -119,6 +120,20 @@ static EllipArgs<R> ellip_args(int dir, const fold3d_model &m, const double h[3]
     const double *src = &Kd.vert[0];
     R *dst = &A.K.vert[0];
     for (size_t i = 0; i < sizeof(Kd) / sizeof(double); ++i) dst[i] = (R)src[i];
+    // plane_gap of the direction for unit slowness, as the device computes it
+    // from the lattice products (the same guards and shrink factor)
+    auto nl = [&](int i, int j) {
+      const double Ng[3][3] = {{N[0], N[3], N[4]}, {N[3], N[1], N[5]}, {N[4], N[5], N[2]}};
+      return sg[i] * sg[j] * Ng[u[i]][u[j]];
+    };
+    const double La = nl(0, 0) * hl[0] * hl[0], Lb = nl(1, 1) * hl[1] * hl[1];
+    const double Lc = nl(2, 2) * hl[2] * hl[2], Le = nl(0, 1) * hl[0] * hl[1];
+    const double Lfx = nl(0, 2) * hl[0] * hl[2], Lfy = nl(1, 2) * hl[1] * hl[2];
+    const double det = La * Lb - Le * Le;
+    const double q = det > 1e-6 * La * Lb
+                         ? Lc - (Lb * Lfx * Lfx - 2.0 * Le * Lfx * Lfy + La * Lfy * Lfy) / det
+                         : 0.0;
+    A.gk = q > 1e-3 * Lc ? (R)((sizeof(R) == 8 ? 0.999 : 0.9) * std::sqrt(q)) : R(0);
   }
   return A;
 }
@@ -189,36 +204,39 @@ template int ellip_sweeps_plane_engine<float>(hipStream_t, float *, const Grid &
 // The partial minima meet in LDS.  Same bits as ellip_plane_kernel.
 // ---------------------------------------------------------------------------
 template <typename R, int U>
-__device__ __forceinline__ void ellip_unit(R &best, const R p[9], R s2, const EllipConsts<R> &K) {
+__device__ __forceinline__ void ellip_unit(R &best, const R p[9], R s2, const EllipConsts<R> &K,
+                                           R gap) {
   constexpr int k1 = 1 + U, k2 = 1 + (U + 1) % 8;
-  ellip_edge(best, p[0], p[k1], s2, K.edge[U]);
-  ellip_edge(best, p[k1], p[k2], s2, K.edge[8 + U]);
-  ellip_tri(best, p[0], p[k1], p[k2], s2, K.tri[U]);
+  if (!may_win<R, true>(rmin(p[0], rmin(p[k1], p[k2])), gap, best)) return;
+  ellip_edge<R, true>(best, p[0], p[k1], s2, K.edge[U], gap);
+  ellip_edge<R, true>(best, p[k1], p[k2], s2, K.edge[8 + U], gap);
+  ellip_tri<R, true>(best, p[0], p[k1], p[k2], s2, K.tri[U], gap);
 }
 
 template <typename R, int G0, int kStride>
-__device__ __forceinline__ void ellip_units(R &best, const R p[9], R s2, const EllipConsts<R> &K) {
+__device__ __forceinline__ void ellip_units(R &best, const R p[9], R s2, const EllipConsts<R> &K,
+                                            R gap) {
   if constexpr (G0 < 8) {
-    ellip_unit<R, G0>(best, p, s2, K);
-    ellip_units<R, G0 + kStride, kStride>(best, p, s2, K);
+    ellip_unit<R, G0>(best, p, s2, K, gap);
+    ellip_units<R, G0 + kStride, kStride>(best, p, s2, K, gap);
   }
 }
 
 template <typename R, int kGroups>
 __device__ __forceinline__ void ellip_dispatch_units(int grp, R &best, const R p[9], R s2,
-                                                     const EllipConsts<R> &K) {
+                                                     const EllipConsts<R> &K, R gap) {
   if constexpr (kGroups == 1) {
-    ellip_units<R, 0, 1>(best, p, s2, K);
+    ellip_units<R, 0, 1>(best, p, s2, K, gap);
   } else if constexpr (kGroups == 2) {
-    if (grp == 0) ellip_units<R, 0, 2>(best, p, s2, K);
-    else ellip_units<R, 1, 2>(best, p, s2, K);
+    if (grp == 0) ellip_units<R, 0, 2>(best, p, s2, K, gap);
+    else ellip_units<R, 1, 2>(best, p, s2, K, gap);
   } else {
     static_assert(kGroups == 4, "one, two or four waves per node row");
     switch (grp) {
-      case 0: ellip_units<R, 0, 4>(best, p, s2, K); break;
-      case 1: ellip_units<R, 1, 4>(best, p, s2, K); break;
-      case 2: ellip_units<R, 2, 4>(best, p, s2, K); break;
-      default: ellip_units<R, 3, 4>(best, p, s2, K); break;
+      case 0: ellip_units<R, 0, 4>(best, p, s2, K, gap); break;
+      case 1: ellip_units<R, 1, 4>(best, p, s2, K, gap); break;
+      case 2: ellip_units<R, 2, 4>(best, p, s2, K, gap); break;
+      default: ellip_units<R, 3, 4>(best, p, s2, K, gap); break;
     }
   }
 }
@@ -258,7 +276,7 @@ __global__ __launch_bounds__(64 * kGroups) void ellip_split_kernel(
 #pragma unroll
     for (int k = 0; k < 9; ++k)
       if (reached(p[k])) best = rmin(p[k] + sl * A.K.vert[k], best);
-    ellip_dispatch_units<R, kGroups>(grp, best, p, sl * sl, A.K);
+    ellip_dispatch_units<R, kGroups>(grp, best, p, sl * sl, A.K, sl * A.gk);
   }
   part[grp][tx] = best;
   __syncthreads();
@@ -409,28 +427,29 @@ static LatticeMap<R> lattice_map(int d, const R *fields, long long count, const 
 }
 
 template <typename R, int G0, int kStride>
-__device__ __forceinline__ void lattice_units(R &best, const R p[9], R s2, const LatticeN<R> &L) {
+__device__ __forceinline__ void lattice_units(R &best, const R p[9], R s2, const LatticeN<R> &L,
+                                              R gap) {
   if constexpr (G0 < 8) {
-    ellip_unit_lattice<R, G0>(best, p, s2, L);
-    lattice_units<R, G0 + kStride, kStride>(best, p, s2, L);
+    ellip_unit_lattice<R, G0, true>(best, p, s2, L, gap);
+    lattice_units<R, G0 + kStride, kStride>(best, p, s2, L, gap);
   }
 }
 
 template <typename R, int kGroups>
 __device__ __forceinline__ void lattice_dispatch_units(int grp, R &best, const R p[9], R s2,
-                                                       const LatticeN<R> &L) {
+                                                       const LatticeN<R> &L, R gap) {
   if constexpr (kGroups == 1) {
-    lattice_units<R, 0, 1>(best, p, s2, L);
+    lattice_units<R, 0, 1>(best, p, s2, L, gap);
   } else if constexpr (kGroups == 2) {
-    if (grp == 0) lattice_units<R, 0, 2>(best, p, s2, L);
-    else lattice_units<R, 1, 2>(best, p, s2, L);
+    if (grp == 0) lattice_units<R, 0, 2>(best, p, s2, L, gap);
+    else lattice_units<R, 1, 2>(best, p, s2, L, gap);
   } else {
     static_assert(kGroups == 4, "one, two or four waves per node row");
     switch (grp) {
-      case 0: lattice_units<R, 0, 4>(best, p, s2, L); break;
-      case 1: lattice_units<R, 1, 4>(best, p, s2, L); break;
-      case 2: lattice_units<R, 2, 4>(best, p, s2, L); break;
-      default: lattice_units<R, 3, 4>(best, p, s2, L); break;
+      case 0: lattice_units<R, 0, 4>(best, p, s2, L, gap); break;
+      case 1: lattice_units<R, 1, 4>(best, p, s2, L, gap); break;
+      case 2: lattice_units<R, 2, 4>(best, p, s2, L, gap); break;
+      default: lattice_units<R, 3, 4>(best, p, s2, L, gap); break;
     }
   }
 }
@@ -478,7 +497,7 @@ __global__ __launch_bounds__(64 * kGroups) void ellip_metric_kernel(
       p[1 + k] = s[1 + i * ub + j * vb][ca + i * ua + j * va];
     }
     ellip_vertices_lattice<R>(best, p, sl, L);
-    lattice_dispatch_units<R, kGroups>(grp, best, p, sl * sl, L);
+    lattice_dispatch_units<R, kGroups>(grp, best, p, sl * sl, L, plane_gap(L, sl));
   }
   part[grp][tx] = best;
   __syncthreads();

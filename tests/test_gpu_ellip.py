@@ -275,3 +275,45 @@ def test_device_point_source_start(solver, name, n):
         ref = want[~far] if dtype == torch.float64 else want[~far].astype(np.float32).astype(np.float64)
         rtol = 1e-14 if dtype == torch.float64 else 1.2e-7
         np.testing.assert_allclose(got[~far], ref, rtol=rtol, atol=0)
+
+
+@pytest.mark.parametrize("dtype", ["float64", "float32"])
+@pytest.mark.parametrize("ratios", [(0.8, 0.6), (0.25, 0.1)])
+def test_plane_bound_pruning_keeps_the_bits(solver, oracle, dtype, ratios):
+    """The split engine skips simplices by the plane bound (smallest vertex +
+    s times the N-distance to the upwind plane), the plane engine by the vertex
+    test alone: the same bits in FP64 (both equal to the oracle) and in FP32,
+    for a mild and a strongly anisotropic metric (condition numbers of N 2.8
+    and 100), with a speed field."""
+    torch_dtype = getattr(torch, dtype)
+    n3 = (36, 41, 33)
+    h = (1.0 / 35, 1.0 / 37, 1.0 / 30)
+    print(f"plane bound {dtype} ratios={ratios}", flush=True)
+    M = np.asarray(wl.tilted_metric(0.6, 2.2, 1.0, ratios[0], ratios[1]))
+    k, j, i = np.indices((n3[2] + 2, n3[1] + 2, n3[0] + 2)).reshape(3, -1)
+    X = np.stack([(i - 1) * h[0], (j - 1) * h[1], (k - 1) * h[2]]) - np.array([[0.45], [0.5], [0.4]])
+    N = np.linalg.inv(wl.sym(M))
+    r = np.sqrt(np.einsum("in,ij,jn->n", X, N, X))
+    near = (np.abs(X) <= 1.6 * np.array(h)[:, None]).all(axis=0)  # closed form within 1.6 cells
+    T0 = np.where(near, r, 1e16)
+    ghost = (i == 0) | (j == 0) | (k == 0) | (i == n3[0] + 1) | (j == n3[1] + 1) | (k == n3[2] + 1)
+    T0[ghost] = 1e16
+    assert near.sum() >= 8
+    v = 1.0 + 0.5 * np.random.default_rng(7).random(T0.size)
+    m = fold3d.Model()
+    m.kind = fold3d.MODEL_ELLIP
+    for q in range(6):
+        m.M[q] = float(M[q])
+    vd = dev(v, torch_dtype)
+    m.speed_field = vd.data_ptr()
+    out = {}
+    for name, eng in (("split", fold3d.ENGINE_SPLIT), ("plane", fold3d.ENGINE_PLANE)):
+        T = dev(T0, torch_dtype)
+        solver.solve(T, n3, spacing=h, model=m, max_iters=4, engine=eng)
+        out[name] = T.cpu().numpy()
+    assert bits_equal(out["split"], out["plane"])
+    assert (out["split"][~ghost] < 1e15).all()  # every interior node reached
+    if dtype == "float64":
+        want = T0.copy()
+        oracle.ellip_sweeps(want, n3, 4, 0.0, h, M=M, speed_field=v)
+        assert bits_equal(out["split"], want)

@@ -59,13 +59,6 @@ __device__ __forceinline__ float vertex_sqrt(float x) { return sqrtf(x); }
 // kGap: the bound adds `gap` (a lower bound of s |X - Y|_N over the upwind
 // plane, already shrunk by plane_gap) and allows 16 ulps of `best` for the
 // candidate's own rounding.  Without kGap: the plain vertex test.
-template <typename R>
-__device__ __forceinline__ R prune_slack();
-template <>
-__device__ __forceinline__ double prune_slack<double>() { return 1.0 + 16.0 * 2.220446049250313e-16; }
-template <>
-__device__ __forceinline__ float prune_slack<float>() { return 1.0f + 16.0f * 1.1920929e-7f; }
-
 template <typename R, bool kGap>
 __device__ __forceinline__ bool may_win(R lo, R gap, R best) {
   if constexpr (kGap) return lo + gap < best * prune_slack<R>();

@@ -16,6 +16,17 @@
 //     i.e. the same bits as the IEEE division.
 // Everything else keeps the reference's operation order, so results stay
 // bit-identical to fold_local.h (and to the reference) in FP64.
+//
+// One more early exit, beyond the reference's: every candidate of a simplex
+// in the upwind plane is its foot point's interpolated value (>= the
+// simplex's smallest vertex) plus the travel time from the foot point to the
+// target, and for F|p| + a.p = 1 the least travel time from the plane z = 0
+// to (0, 0, dzz) is dzz / (F + az) (the largest p_z on the z axis with
+// H(p) <= 1).  So a simplex whose smallest vertex plus that bound (shrunk to
+// 99.9 % in FP64, 90 % in FP32, against the rounding of a candidate near a
+// grazing characteristic) is not below the running value by 16 ulps cannot
+// lower it, and is skipped.  The literal kernel (fold_local.h) does not skip;
+// the tests hold both to the reference's bits.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -71,6 +82,7 @@ struct TriSite {
 template <typename R>
 struct FoldSign {
   R F, az, dzz;
+  R gap;            // shrunk dzz / (F + az): least travel time from the upwind plane
   R ga, rga;        // triangles: F^2 - az^2 (shared by all eight) and reciprocal
   R k0;             // centre one-point constant dzz^2/d
   int k0_ok;
@@ -84,8 +96,10 @@ struct FoldFast {
 };
 
 template <typename R>
-__device__ __forceinline__ void fast_edge(R &tnew, R st, R xt, R F, const EdgeSite<R> &e) {
+__device__ __forceinline__ void fast_edge(R &tnew, R st, R xt, R F, const EdgeSite<R> &e,
+                                          R gap) {
   if (tnew < rmin(xt, st) || (e.ax_pos && xt > st)) return;
+  if (!(rmin(xt, st) + gap < tnew * prune_slack<R>())) return;  // cannot go below tnew
   const R gx = div_const(xt - st, e.dxx, e.rdxx);
   const R q = R(1) - gx * e.ax;
   const R disc = e.ga * (q * q - gx * gx * e.gaz);
@@ -107,6 +121,7 @@ template <typename R>
 __device__ __forceinline__ void fast_tri(R &tnew, R st, R xt, R yt, const FoldSign<R> &S,
                                          const TriSite<R> &t, const EdgeSite<R> &sub) {
   if (tnew < rmin(st, rmin(xt, yt)) || (t.ax_pos && xt > st) || (t.ay_pos && yt > xt)) return;
+  if (!(rmin(st, rmin(xt, yt)) + S.gap < tnew * prune_slack<R>())) return;  // nor its sub-edge
   const R F = S.F, az = S.az, dzz = S.dzz;
   const R gx = div_const(xt - st, t.dxx, t.rdxx);
   const R gy = div_const(yt - xt, t.dyy, t.rdyy);
@@ -124,7 +139,7 @@ __device__ __forceinline__ void fast_tri(R &tnew, R st, R xt, R yt, const FoldSi
       if (R(0) <= rmin(fx, fy) && fy * t.dxx <= fx * t.dyy && fx <= t.dxx) tnew = cand;
     }
   }
-  if (tnew > rmin(xt, yt) && (!(t.sub_pos && xt > yt))) fast_edge(tnew, xt, yt, F, sub);
+  if (tnew > rmin(xt, yt) && (!(t.sub_pos && xt > yt))) fast_edge(tnew, xt, yt, F, sub, S.gap);
 }
 
 // Same result as fold_node_update(), bit for bit.
@@ -144,14 +159,14 @@ __device__ __forceinline__ R fold_node_fast(R told, const Patch<R> &p, const Fol
     fast_tri(tnew, st, p.my, p.pxmy, S, S.tri[5], S.edge[13]);
     fast_tri(tnew, st, p.py, p.pxpy, S, S.tri[6], S.edge[14]);
     fast_tri(tnew, st, p.py, p.mxpy, S, S.tri[7], S.edge[15]);
-    fast_edge(tnew, st, p.px, S.F, S.edge[0]);
-    fast_edge(tnew, st, p.pxpy, S.F, S.edge[1]);
-    fast_edge(tnew, st, p.py, S.F, S.edge[2]);
-    fast_edge(tnew, st, p.mxpy, S.F, S.edge[3]);
-    fast_edge(tnew, st, p.mx, S.F, S.edge[4]);
-    fast_edge(tnew, st, p.mxmy, S.F, S.edge[5]);
-    fast_edge(tnew, st, p.my, S.F, S.edge[6]);
-    fast_edge(tnew, st, p.pxmy, S.F, S.edge[7]);
+    fast_edge(tnew, st, p.px, S.F, S.edge[0], S.gap);
+    fast_edge(tnew, st, p.pxpy, S.F, S.edge[1], S.gap);
+    fast_edge(tnew, st, p.py, S.F, S.edge[2], S.gap);
+    fast_edge(tnew, st, p.mxpy, S.F, S.edge[3], S.gap);
+    fast_edge(tnew, st, p.mx, S.F, S.edge[4], S.gap);
+    fast_edge(tnew, st, p.mxmy, S.F, S.edge[5], S.gap);
+    fast_edge(tnew, st, p.my, S.F, S.edge[6], S.gap);
+    fast_edge(tnew, st, p.pxmy, S.F, S.edge[7], S.gap);
   }
   return ((told > R(0)) ? R(1) : R(-1)) * tnew;
 }
@@ -185,35 +200,35 @@ __device__ __forceinline__ void fold_unit(R &tnew, const Patch<R> &p, const Fold
   switch (u) {
     case 0:
       fast_tri(tnew, st, p.px, p.pxpy, S, S.tri[0], S.edge[8]);
-      fast_edge(tnew, st, p.px, S.F, S.edge[0]);
+      fast_edge(tnew, st, p.px, S.F, S.edge[0], S.gap);
       break;
     case 1:
       fast_tri(tnew, st, p.px, p.pxmy, S, S.tri[1], S.edge[9]);
-      fast_edge(tnew, st, p.pxmy, S.F, S.edge[7]);
+      fast_edge(tnew, st, p.pxmy, S.F, S.edge[7], S.gap);
       break;
     case 2:
       fast_tri(tnew, st, p.mx, p.mxmy, S, S.tri[2], S.edge[10]);
-      fast_edge(tnew, st, p.mx, S.F, S.edge[4]);
+      fast_edge(tnew, st, p.mx, S.F, S.edge[4], S.gap);
       break;
     case 3:
       fast_tri(tnew, st, p.mx, p.mxpy, S, S.tri[3], S.edge[11]);
-      fast_edge(tnew, st, p.mxpy, S.F, S.edge[3]);
+      fast_edge(tnew, st, p.mxpy, S.F, S.edge[3], S.gap);
       break;
     case 4:
       fast_tri(tnew, st, p.my, p.mxmy, S, S.tri[4], S.edge[12]);
-      fast_edge(tnew, st, p.mxmy, S.F, S.edge[5]);
+      fast_edge(tnew, st, p.mxmy, S.F, S.edge[5], S.gap);
       break;
     case 5:
       fast_tri(tnew, st, p.my, p.pxmy, S, S.tri[5], S.edge[13]);
-      fast_edge(tnew, st, p.my, S.F, S.edge[6]);
+      fast_edge(tnew, st, p.my, S.F, S.edge[6], S.gap);
       break;
     case 6:
       fast_tri(tnew, st, p.py, p.pxpy, S, S.tri[6], S.edge[14]);
-      fast_edge(tnew, st, p.pxpy, S.F, S.edge[1]);
+      fast_edge(tnew, st, p.pxpy, S.F, S.edge[1], S.gap);
       break;
     default:
       fast_tri(tnew, st, p.py, p.mxpy, S, S.tri[7], S.edge[15]);
-      fast_edge(tnew, st, p.py, S.F, S.edge[2]);
+      fast_edge(tnew, st, p.py, S.F, S.edge[2], S.gap);
       break;
   }
 }
@@ -257,6 +272,7 @@ inline void host_fold_sign(const FoldFrame<double> &f, double sgn, FoldSign<doub
   S.F = F;
   S.az = az;
   S.dzz = dzz;
+  S.gap = 0.999 * dzz / (F + az);  // F > |a|, so F + az > 0
   S.ga = F * F - az * az;
   S.rga = 1.0 / S.ga;
   double d = (F * F - (ax * ax + ay * ay)) * dzz * dzz;
@@ -319,6 +335,7 @@ inline void narrow<float>(const FoldFast<double> &in, FoldFast<float> &out) {
     b.F = (float)a.F;
     b.az = (float)a.az;
     b.dzz = (float)a.dzz;
+    b.gap = (float)(0.9 * a.dzz / (a.F + a.az));
     b.ga = (float)a.ga;
     b.rga = 1.0f / b.ga;
     b.k0 = (float)a.k0;

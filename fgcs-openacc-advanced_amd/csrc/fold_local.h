@@ -46,6 +46,15 @@ __device__ __forceinline__ double sqrt_rn(double x) {
   return sqrt(x);
 }
 
+// 1 + 16 ulps: the pruning tests that add a travel-time bound to a simplex's
+// smallest vertex allow this much for a candidate's own rounding.
+template <typename R>
+__device__ __forceinline__ R prune_slack();
+template <>
+__device__ __forceinline__ double prune_slack<double>() { return 1.0 + 16.0 * 2.220446049250313e-16; }
+template <>
+__device__ __forceinline__ float prune_slack<float>() { return 1.0f + 16.0f * 1.1920929e-7f; }
+
 // Reference macro semantics: min(a,b) = a<b ? a : b (second operand on ties
 // and NaN, AN_ISO_1_OpenACC.c:32-34); sign(v) = v>0 ? 1 : -1 (Fold3dPMM.h:32-34).
 template <typename R>

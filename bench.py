@@ -31,10 +31,14 @@ HBM_PEAK_GBS = 8000.0          # MI355X HBM3E
 # two rocprofv3 --pmc passes over the first iteration: FETCH_SIZE x 2.0 plus
 # WRITE_SIZE x 1.0 KiB, the factors read off a plain device copy of known
 # size (profiles/fold_split_f64_512_pmc_traffic.md)
-PMC_TRAFFIC_512_F64 = (2968.6 * 2.0 + 849.6 * 1.0) * 1024
-# VALU wave-instructions per plane launch of the same kernel (rocprofv3 --pmc
-# SQ_INSTS_VALU, first iteration; 8192 waves of 618 instructions)
-PMC_VALU_INSTS_512_F64 = 5061591.2
+PMC_TRAFFIC_512_F64 = (2962.5 * 2.0 + 849.6 * 1.0) * 1024
+# The VALU view of the same launches, all from one profiled run over the first
+# iteration (profiles/fold_split_f64_512_pmc_traffic.md): wave-instructions
+# per plane launch (SQ_INSTS_VALU: 8192 waves of 552.8), the launch's average
+# duration (kernel trace) and the clock implied by that run's FP64 FMA rate.
+PMC_VALU_INSTS_512_F64 = 4528418.4
+PMC_VALU_LAUNCH_US_512_F64 = 15.927
+PMC_VALU_CLOCK_GHZ = 1.917
 A100_V2_512_GCUPS = 8 * 6 * 511 * 512 ** 2 / 3.038 / 1e9  # plots/perfect-2-gpu.py:20,24
 
 
@@ -300,6 +304,8 @@ def main():
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--no-c0", action="store_true")
+    ap.add_argument("--groups", type=int, default=0, choices=[0, 2, 4, 8],
+                    help="waves per node row of the fold split kernel (0: autotune)")
     ap.add_argument("--no-fixpt", action="store_true",
                     help="skip the solves to the exact fixed point (split and locked engines)")
     ap.add_argument("--c2-n", type=int, default=1280, help="side of the per-node metric run (0: skip)")
@@ -336,9 +342,14 @@ def main():
         f"same bits: {init['same_bits']}")
     del Td
 
-    log("bench: autotuning waves per node row (one iteration each of 2, 4, 8)")
-    groups, tune_ms = solver.autotune(T0d, n, iterations=1)
-    log(f"bench: autotuner picked {groups} waves per node row ({tune_ms:.1f} ms/iteration)")
+    if args.groups:
+        solver.set_tuning(args.groups)
+        groups = args.groups
+        log(f"bench: {groups} waves per node row (set, no autotuning)")
+    else:
+        log("bench: autotuning waves per node row (one iteration each of 2, 4, 8)")
+        groups, tune_ms = solver.autotune(T0d, n, iterations=1)
+        log(f"bench: autotuner picked {groups} waves per node row ({tune_ms:.1f} ms/iteration)")
 
     iters = args.iters
     W = cell_updates(n, iters)
@@ -490,17 +501,21 @@ def main():
     valu = {"measured_fp64_tflops": peaks["fp64_tflops"], "published_fp64_tflops": 78.6,
             "measured_copy_gbs": peaks["copy_gbs"]}
     # VALU view of the same kernel, the bound that binds: wave-instructions per
-    # plane launch (rocprofv3 SQ_INSTS_VALU, first iteration at 512^3) over the
-    # issue rate of 256 CUs x 4 SIMDs at one wave-instruction per 4 cycles, the
-    # clock taken from the measured FP64 FMA rate (128 FLOP per CU per cycle)
-    clock_hz = peaks["fp64_tflops"] * 1e12 / (256 * 128)
-    issue_peak = 256 * 4 * clock_hz / 4
+    # plane launch over the issue rate of 256 CUs x 4 SIMDs at one
+    # wave-instruction per 4 cycles.  Counts, launch time and clock come from
+    # one profiled run of the first iteration (the counters cannot be taken
+    # live); that iteration is the heaviest (its launches average 15.9 us, all
+    # eight 14.0 us).
     valu_issue = None
     if n == 512 and groups == 2:
-        achieved_i = PMC_VALU_INSTS_512_F64 / (us * 1e-6)
+        clock_hz = PMC_VALU_CLOCK_GHZ * 1e9
+        issue_peak = 256 * 4 * clock_hz / 4
+        achieved_i = PMC_VALU_INSTS_512_F64 / (PMC_VALU_LAUNCH_US_512_F64 * 1e-6)
         valu_issue = {"wave_instructions_per_launch": PMC_VALU_INSTS_512_F64,
-                      "achieved_per_s": achieved_i, "peak_per_s": issue_peak,
-                      "frac": achieved_i / issue_peak, "clock_ghz": clock_hz / 1e9}
+                      "launch_us": PMC_VALU_LAUNCH_US_512_F64, "achieved_per_s": achieved_i,
+                      "peak_per_s": issue_peak, "frac": achieved_i / issue_peak,
+                      "clock_ghz": PMC_VALU_CLOCK_GHZ, "iteration": 1,
+                      "source": "profiles/fold_split_f64_512_pmc_traffic.md"}
 
     out = {
         "metric": "Gcell-updates/s at 512^3 (fold-model PMM solve, 8 iterations)",

@@ -19,17 +19,23 @@
 //
 // One more early exit, beyond the reference's: every candidate of a simplex
 // in the upwind plane is its foot point's interpolated value (>= the
-// simplex's smallest vertex) plus the travel time from the foot point to the
-// target, and for F|p| + a.p = 1 the least travel time from the plane z = 0
-// to (0, 0, dzz) is dzz / (F + az) (the largest p_z on the z axis with
-// H(p) <= 1).  So a simplex whose smallest vertex plus that bound (shrunk to
-// 99.9 % in FP64, 90 % in FP32, against the rounding of a candidate near a
-// grazing characteristic) is not below the running value by 16 ulps cannot
-// lower it, and is skipped.  The literal kernel (fold_local.h) does not skip;
-// the tests hold both to the reference's bits.
+// simplex's smallest vertex) plus the travel time tau(X - Y) from the foot
+// point Y to the target X.  For F|p| + a.p = 1, tau(w) = max p.w over
+// H(p) <= 1, so by minimax the least travel time from a set S of points is
+// max over the p orthogonal to S's directions: from the plane z = 0 to
+// (0, 0, dzz) it is dzz / (F + az) (p on the z axis); from an edge's line (an
+// edge site's own frame: the line is the x axis) it is dzz / (sqrt(F^2 - ay^2)
+// + az) (p in the y-z plane).  A triangle (with its sub-edge) or an edge
+// whose smallest vertex plus its bound (shrunk to 99.9 % in FP64, 90 % in
+// FP32, against the rounding of a candidate near a grazing characteristic) is
+// not below the running value by 16 ulps cannot lower it, and is skipped.  The
+// literal kernel (fold_local.h) does not skip; the tests hold both to the
+// reference's bits.
 #pragma once
 
 #include <hip/hip_runtime.h>
+
+#include <cmath>
 
 #include "fold_local.h"
 
@@ -66,6 +72,7 @@ struct EdgeSite {
   R gaz, rgaz;      // ga - az^2 and its reciprocal
   R rdxx;           // 1/dxx
   R k1;             // one-point constant len2/c of the far end
+  R lgap;           // shrunk least travel time from the edge's line to the target
   int ax_pos;       // ax > 0
   int k1_ok;        // the one-point candidate exists
 };
@@ -96,10 +103,9 @@ struct FoldFast {
 };
 
 template <typename R>
-__device__ __forceinline__ void fast_edge(R &tnew, R st, R xt, R F, const EdgeSite<R> &e,
-                                          R gap) {
+__device__ __forceinline__ void fast_edge(R &tnew, R st, R xt, R F, const EdgeSite<R> &e) {
   if (tnew < rmin(xt, st) || (e.ax_pos && xt > st)) return;
-  if (!(rmin(xt, st) + gap < tnew * prune_slack<R>())) return;  // cannot go below tnew
+  if (!(rmin(xt, st) + e.lgap < tnew * prune_slack<R>())) return;  // cannot go below tnew
   const R gx = div_const(xt - st, e.dxx, e.rdxx);
   const R q = R(1) - gx * e.ax;
   const R disc = e.ga * (q * q - gx * gx * e.gaz);
@@ -139,7 +145,7 @@ __device__ __forceinline__ void fast_tri(R &tnew, R st, R xt, R yt, const FoldSi
       if (R(0) <= rmin(fx, fy) && fy * t.dxx <= fx * t.dyy && fx <= t.dxx) tnew = cand;
     }
   }
-  if (tnew > rmin(xt, yt) && (!(t.sub_pos && xt > yt))) fast_edge(tnew, xt, yt, F, sub, S.gap);
+  if (tnew > rmin(xt, yt) && (!(t.sub_pos && xt > yt))) fast_edge(tnew, xt, yt, F, sub);
 }
 
 // Same result as fold_node_update(), bit for bit.
@@ -159,14 +165,14 @@ __device__ __forceinline__ R fold_node_fast(R told, const Patch<R> &p, const Fol
     fast_tri(tnew, st, p.my, p.pxmy, S, S.tri[5], S.edge[13]);
     fast_tri(tnew, st, p.py, p.pxpy, S, S.tri[6], S.edge[14]);
     fast_tri(tnew, st, p.py, p.mxpy, S, S.tri[7], S.edge[15]);
-    fast_edge(tnew, st, p.px, S.F, S.edge[0], S.gap);
-    fast_edge(tnew, st, p.pxpy, S.F, S.edge[1], S.gap);
-    fast_edge(tnew, st, p.py, S.F, S.edge[2], S.gap);
-    fast_edge(tnew, st, p.mxpy, S.F, S.edge[3], S.gap);
-    fast_edge(tnew, st, p.mx, S.F, S.edge[4], S.gap);
-    fast_edge(tnew, st, p.mxmy, S.F, S.edge[5], S.gap);
-    fast_edge(tnew, st, p.my, S.F, S.edge[6], S.gap);
-    fast_edge(tnew, st, p.pxmy, S.F, S.edge[7], S.gap);
+    fast_edge(tnew, st, p.px, S.F, S.edge[0]);
+    fast_edge(tnew, st, p.pxpy, S.F, S.edge[1]);
+    fast_edge(tnew, st, p.py, S.F, S.edge[2]);
+    fast_edge(tnew, st, p.mxpy, S.F, S.edge[3]);
+    fast_edge(tnew, st, p.mx, S.F, S.edge[4]);
+    fast_edge(tnew, st, p.mxmy, S.F, S.edge[5]);
+    fast_edge(tnew, st, p.my, S.F, S.edge[6]);
+    fast_edge(tnew, st, p.pxmy, S.F, S.edge[7]);
   }
   return ((told > R(0)) ? R(1) : R(-1)) * tnew;
 }
@@ -200,35 +206,35 @@ __device__ __forceinline__ void fold_unit(R &tnew, const Patch<R> &p, const Fold
   switch (u) {
     case 0:
       fast_tri(tnew, st, p.px, p.pxpy, S, S.tri[0], S.edge[8]);
-      fast_edge(tnew, st, p.px, S.F, S.edge[0], S.gap);
+      fast_edge(tnew, st, p.px, S.F, S.edge[0]);
       break;
     case 1:
       fast_tri(tnew, st, p.px, p.pxmy, S, S.tri[1], S.edge[9]);
-      fast_edge(tnew, st, p.pxmy, S.F, S.edge[7], S.gap);
+      fast_edge(tnew, st, p.pxmy, S.F, S.edge[7]);
       break;
     case 2:
       fast_tri(tnew, st, p.mx, p.mxmy, S, S.tri[2], S.edge[10]);
-      fast_edge(tnew, st, p.mx, S.F, S.edge[4], S.gap);
+      fast_edge(tnew, st, p.mx, S.F, S.edge[4]);
       break;
     case 3:
       fast_tri(tnew, st, p.mx, p.mxpy, S, S.tri[3], S.edge[11]);
-      fast_edge(tnew, st, p.mxpy, S.F, S.edge[3], S.gap);
+      fast_edge(tnew, st, p.mxpy, S.F, S.edge[3]);
       break;
     case 4:
       fast_tri(tnew, st, p.my, p.mxmy, S, S.tri[4], S.edge[12]);
-      fast_edge(tnew, st, p.mxmy, S.F, S.edge[5], S.gap);
+      fast_edge(tnew, st, p.mxmy, S.F, S.edge[5]);
       break;
     case 5:
       fast_tri(tnew, st, p.my, p.pxmy, S, S.tri[5], S.edge[13]);
-      fast_edge(tnew, st, p.my, S.F, S.edge[6], S.gap);
+      fast_edge(tnew, st, p.my, S.F, S.edge[6]);
       break;
     case 6:
       fast_tri(tnew, st, p.py, p.pxpy, S, S.tri[6], S.edge[14]);
-      fast_edge(tnew, st, p.pxpy, S.F, S.edge[1], S.gap);
+      fast_edge(tnew, st, p.pxpy, S.F, S.edge[1]);
       break;
     default:
       fast_tri(tnew, st, p.py, p.mxpy, S, S.tri[7], S.edge[15]);
-      fast_edge(tnew, st, p.py, S.F, S.edge[2], S.gap);
+      fast_edge(tnew, st, p.py, S.F, S.edge[2]);
       break;
   }
 }
@@ -236,6 +242,14 @@ __device__ __forceinline__ void fold_unit(R &tnew, const Patch<R> &p, const Fold
 // ---------------------------------------------------------------------------
 // Host-side construction (plain double arithmetic, same expressions as the
 // reference, so the hoisted values carry the bits the node loop would compute).
+
+// Least travel time from the x axis to (0, 0, dzz) in an edge site's frame:
+// dzz max_phi sin(phi) / (F + |ay| cos(phi) + az sin(phi)); the derivative
+// vanishes only at cos(phi) = -|ay| / F, which gives dzz / (sqrt(F^2 - ay^2) +
+// az) (>= the plane bound dzz / (F + az); positive because F > |a|).
+inline double edge_line_bound(double F, double ay, double az, double dzz) {
+  return dzz / (std::sqrt(F * F - ay * ay) + az);
+}
 
 inline void host_edge_site(double F, double ax, double ay, double az, double dxx,
                            double dzz, EdgeSite<double> &e) {
@@ -248,6 +262,7 @@ inline void host_edge_site(double F, double ax, double ay, double az, double dxx
   e.gaz = e.ga - az * az;
   e.rgaz = 1.0 / e.gaz;
   e.rdxx = 1.0 / dxx;
+  e.lgap = 0.999 * edge_line_bound(F, ay, az, dzz);
   e.ax_pos = ax > 0.0;
   double c = (-dxx * ax + dzz * az);
   const double len2 = dxx * dxx + dzz * dzz;
@@ -366,6 +381,7 @@ inline void narrow<float>(const FoldFast<double> &in, FoldFast<float> &out) {
       y.rgaz = 1.0f / y.gaz;
       y.rdxx = 1.0f / y.dxx;
       y.k1 = (float)x.k1;
+      y.lgap = (float)(x.lgap / 0.999 * 0.9);
       y.ax_pos = x.ax_pos;
       y.k1_ok = x.k1_ok;
     }

@@ -31,14 +31,14 @@ HBM_PEAK_GBS = 8000.0          # MI355X HBM3E
 # two rocprofv3 --pmc passes over the first iteration: FETCH_SIZE x 2.0 plus
 # WRITE_SIZE x 1.0 KiB, the factors read off a plain device copy of known
 # size (profiles/fold_split_f64_512_pmc_traffic.md)
-PMC_TRAFFIC_512_F64 = (2962.5 * 2.0 + 849.6 * 1.0) * 1024
+PMC_TRAFFIC_512_F64 = (2961.6 * 2.0 + 849.6 * 1.0) * 1024
 # The VALU view of the same launches, all from one profiled run over the first
 # iteration (profiles/fold_split_f64_512_pmc_traffic.md): wave-instructions
-# per plane launch (SQ_INSTS_VALU: 8192 waves of 552.8), the launch's average
+# per plane launch (SQ_INSTS_VALU: 8192 waves of 523.2), the launch's average
 # duration (kernel trace) and the clock implied by that run's FP64 FMA rate.
-PMC_VALU_INSTS_512_F64 = 4528418.4
-PMC_VALU_LAUNCH_US_512_F64 = 15.927
-PMC_VALU_CLOCK_GHZ = 1.917
+PMC_VALU_INSTS_512_F64 = 4285692.1
+PMC_VALU_LAUNCH_US_512_F64 = 15.322
+PMC_VALU_CLOCK_GHZ = 1.955
 A100_V2_512_GCUPS = 8 * 6 * 511 * 512 ** 2 / 3.038 / 1e9  # plots/perfect-2-gpu.py:20,24
 
 
@@ -504,8 +504,8 @@ def main():
     # plane launch over the issue rate of 256 CUs x 4 SIMDs at one
     # wave-instruction per 4 cycles.  Counts, launch time and clock come from
     # one profiled run of the first iteration (the counters cannot be taken
-    # live); that iteration is the heaviest (its launches average 15.9 us, all
-    # eight 14.0 us).
+    # live); that iteration is the heaviest (its launches average 15.3 us, all
+    # eight 13.3 us).
     valu_issue = None
     if n == 512 and groups == 2:
         clock_hz = PMC_VALU_CLOCK_GHZ * 1e9

@@ -53,7 +53,6 @@ struct AdjArgs {
   const R *th, *ph, *ra, *rb;      // per-node metric or null
   R Nc[6];                         // constant inverse metric
   R rel_tol;                       // a candidate within rel_tol of T counts as the winner
-  R gapc;                          // constant metric: a lower bound of sqrt(lambda_min(N))
 };
 
 // dN/dtheta of N = Q diag(1, 1/ra^2, 1/rb^2) Q^T, Q = Rz(phi) Ry(theta).
@@ -115,16 +114,14 @@ __global__ __launch_bounds__(256) void stencil_kernel(const R *__restrict__ T, A
   }
   // Only a candidate <= tX (1 + rel_tol) can make the node a stencil node.  A
   // candidate is its foot point's interpolated value (>= the smallest vertex
-  // of the simplex) plus s |w|_N, and w spans at least the march-axis spacing
-  // h_m (the simplex lies in the upwind plane), so |w|_N >= h_m
-  // sqrt(lambda_min(N)).  Simplices whose smallest vertex plus that bound
-  // (shrunk by 0.1 %) exceeds tX (1 + rel_tol) (1 + margin), the margin far
-  // above the rounding of a candidate, are skipped: they cannot win, so the
-  // stencil is the one the full search finds.
+  // of the simplex) plus s |w|_N with w from a point of the upwind plane to
+  // the node, so at least the smallest vertex plus plane_gap (the least
+  // N-distance from the node to that plane, times s, shrunk).  Simplices whose
+  // bound exceeds tX (1 + rel_tol) (1 + margin), the margin far above the
+  // rounding of a candidate, are skipped: they cannot win, so the stencil is
+  // the one the full search finds.
   const R thr = tX * (R(1) + A.rel_tol) * (R(1) + (sizeof(R) == 8 ? R(1e-10) : R(1e-4)));
   const R s = A.V ? R(1) / A.V[X] : R(1) / A.speed;
-  // sqrt(lambda_min(N)): N = Q diag(1, 1/ra^2, 1/rb^2) Q^T with a per-node tilt
-  const R lam = A.th ? R(1) / fmax(R(1), fmax(A.ra[X], A.rb[X])) : A.gapc;
   R N[6], dN[6];
   if (A.th) {
     tilted_inverse_metric<R>(A.th[X], A.ph[X], A.ra[X], A.rb[X], N);
@@ -159,7 +156,7 @@ __global__ __launch_bounds__(256) void stencil_kernel(const R *__restrict__ T, A
       if (reached(p[q])) minv = rmin(p[q], minv);
     }
     (void)c0;
-    const R gap = R(0.999) * s * hl[2] * lam;  // <= s |w|_N of every candidate here
+    const R gap = plane_gap(L, s);  // <= s |w|_N of every candidate here
     if (!(minv + gap <= thr)) continue;        // no simplex of this direction can win
     // local lattice coordinates of the patch nodes relative to the target
     auto lat = [&](int q, int &a, int &b) {
@@ -506,13 +503,6 @@ int adjoint_solve(hipStream_t st, int nsrc, const AdjointSource *srcs, const Gri
   sym3_inverse(m.M, N);
   for (int q = 0; q < 6; ++q) A.Nc[q] = (R)N[q];
   A.rel_tol = sizeof(R) == 8 ? (R)1e-12 : (R)1e-6;
-  {
-    // lambda_max(M) <= |M|_F, so sqrt(lambda_min(N)) >= |M|_F^(-1/2)
-    const double *M = m.M;
-    const double fro = std::sqrt(M[0] * M[0] + M[1] * M[1] + M[2] * M[2] +
-                                 2.0 * (M[3] * M[3] + M[4] * M[4] + M[5] * M[5]));
-    A.gapc = fro > 0.0 ? (R)(1.0 / std::sqrt(fro)) : R(0);
-  }
   const long long interior = (long long)G.n[0] * G.n[1] * G.n[2];
   for (int k = 0; k < nsrc; ++k) {
     stencil_kernel<R><<<(unsigned)((interior + 255) / 256), 256, 0, st>>>(hs[k].T, A, hs[k].S);

@@ -21,8 +21,9 @@
 // in the upwind plane is its foot point's interpolated value (>= the
 // simplex's smallest vertex) plus the travel time tau(X - Y) from the foot
 // point Y to the target X.  For F|p| + a.p = 1, tau(w) = max p.w over
-// H(p) <= 1, so by minimax the least travel time from a set S of points is
-// max over the p orthogonal to S's directions: from the plane z = 0 to
+// H(p) <= 1, so by minimax the least travel time to X from an affine set S
+// (a line or a plane through Y0) is the largest p.(X - Y0) over the p with
+// H(p) <= 1 orthogonal to S: from the plane z = 0 to
 // (0, 0, dzz) it is dzz / (F + az) (p on the z axis); from an edge's line (an
 // edge site's own frame: the line is the x axis) it is dzz / (sqrt(F^2 - ay^2)
 // + az) (p in the y-z plane).  A triangle (with its sub-edge) or an edge

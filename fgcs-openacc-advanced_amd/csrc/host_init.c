@@ -33,9 +33,9 @@ void fold3d_get_grid(int *nx, int *ny, int *nz)
   *nz = g_nz;
 }
 
-static inline long node(int i, int j, int k)
+static inline long node_of(int i, int j, int k, int nx, int ny)
 {
-  return i + (long)(g_nx + 2) * (j + (long)(g_ny + 2) * k);
+  return i + (long)(nx + 2) * (j + (long)(ny + 2) * k);
 }
 
 static inline double sgn(double v) { return (v > 0) ? 1.0 : -1.0; }
@@ -62,13 +62,21 @@ double ImpSurfF(double x, double y, double z)
          (x - 5.0) * (y - 5.0) * (z - 5) / 55.0;
 }
 
-/* Fold3dPMM.c:35-54: (x,y,z) is the surface point relative to node (i,j,k). */
-void InitNode(double x, double y, double z, int i, int j, int k, double *T)
+/* Fold3dPMM.c:35-54: (x,y,z) is the surface point relative to node (i,j,k)
+ * of a grid with nx x ny interior nodes per plane. */
+void fold3d_init_node(double x, double y, double z, int i, int j, int k, double *T, int nx,
+                      int ny, int nz)
 {
-  double *t = T + node(i, j, k);
+  (void)nz;  /* the layout needs only the two faster strides */
+  double *t = T + node_of(i, j, k, nx, ny);
   const double s = (*t < 0.0) ? -1.0 : 1.0;
   const double tn = InitFoldFromPoint(_Fc, s * _axx, s * _ayy, s * _azz, -x, -y, -z);
   if (tn < fabs(*t)) *t = sgn(*t) * tn;
+}
+
+void InitNode(double x, double y, double z, int i, int j, int k, double *T)
+{
+  fold3d_init_node(x, y, z, i, j, k, T, g_nx, g_ny, g_nz);
 }
 
 /* Fold3dPMM.c:56-112: the 44 nodes around voxel (i,j,k).  Row = {node offset
@@ -100,14 +108,20 @@ static inline double shifted(double p, int c, double h)
   return c == 0 ? p : c == 1 ? p - h : c == -1 ? p + h : p - 2 * h;
 }
 
-void updateVoxel(double *T, int i, int j, int k, double xp, double yp,
-                 double zp, double dxx, double dyy, double dzz)
+void fold3d_update_voxel(double *T, int i, int j, int k, double xp, double yp, double zp,
+                         double dxx, double dyy, double dzz, int nx, int ny, int nz)
 {
   for (int e = 0; e < 44; ++e) {
     const signed char *r = kVoxel[e];
-    InitNode(shifted(xp, r[3], dxx), shifted(yp, r[4], dyy),
-             shifted(zp, r[5], dzz), i + r[0], j + r[1], k + r[2], T);
+    fold3d_init_node(shifted(xp, r[3], dxx), shifted(yp, r[4], dyy), shifted(zp, r[5], dzz),
+                     i + r[0], j + r[1], k + r[2], T, nx, ny, nz);
   }
+}
+
+void updateVoxel(double *T, int i, int j, int k, double xp, double yp,
+                 double zp, double dxx, double dyy, double dzz)
+{
+  fold3d_update_voxel(T, i, j, k, xp, yp, zp, dxx, dyy, dzz, g_nx, g_ny, g_nz);
 }
 
 double x(int i, double DX) { return (i - 1) * DX + _xmin; } /* Fold3dPMM.c:115-126 */
@@ -182,7 +196,7 @@ void ImplicitInitialiser(double *T, int nx, int ny, int nz, double dxx,
   for (int k = 0; k < nz + 2; k++)
     for (int j = 0; j < ny + 2; j++)
       for (int i = 0; i < nx + 2; i++) {
-        const long c = node(i, j, k);
+        const long c = node_of(i, j, k, nx, ny);
         IF[c] = ImpSurfF(x(i, dxx), y(j, dyy), z(k, dzz));
         T[c] = sgn(IF[c]) * MAXT;
       }
@@ -192,7 +206,7 @@ void ImplicitInitialiser(double *T, int nx, int ny, int nz, double dxx,
       for (int k = 1; k < nz; k++) {
         /* corner values c[di + 2*dj + 4*dk] */
         double c[8];
-        for (int q = 0; q < 8; ++q) c[q] = IF[node(i + (q & 1), j + (q >> 1 & 1), k + (q >> 2))];
+        for (int q = 0; q < 8; ++q) c[q] = IF[node_of(i + (q & 1), j + (q >> 1 & 1), k + (q >> 2), nx, ny)];
         points_t P;
         P.n = 0;
         double t;

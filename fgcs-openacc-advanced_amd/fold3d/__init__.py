@@ -158,6 +158,15 @@ def lib() -> C.CDLL:
     L.ImpSurfF.restype = d
     L.ImpSurfF.argtypes = [d] * 3
     L.fold3d_set_grid.argtypes = [i, i, i]
+    L.fold3d_get_grid.argtypes = [C.POINTER(i)] * 3
+    L.InitNode.argtypes = [d, d, d, i, i, i, dp]
+    L.InitNode.restype = None
+    L.fold3d_init_node.argtypes = [d, d, d, i, i, i, dp, i, i, i]
+    L.fold3d_init_node.restype = None
+    L.updateVoxel.argtypes = [dp, i, i, i] + [d] * 6
+    L.updateVoxel.restype = None
+    L.fold3d_update_voxel.argtypes = [dp, i, i, i] + [d] * 6 + [i, i, i]
+    L.fold3d_update_voxel.restype = None
     for name in DIRECTIONS:
         fn = getattr(L, "function_" + name)
         fn.argtypes = [vp, i] + [d] * 10

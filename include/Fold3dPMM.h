@@ -82,6 +82,16 @@ void ImplicitInitialiser(_DOUBLE_ *T, int nx, int ny, int nz, _DOUBLE_ dxx,
 
 /* Grid used by InitNode/updateVoxel and the plane steps below. */
 void fold3d_set_grid(int nx, int ny, int nz);
+void fold3d_get_grid(int *nx, int *ny, int *nz);
+
+/* InitNode / updateVoxel on an explicit grid of nx x ny x nz interior nodes
+ * (same results; they ignore the grid recorded by ImplicitInitialiser or
+ * fold3d_set_grid).  As in the reference, the node indices, with updateVoxel's
+ * offsets of -1..2, must lie in 0..n+1 per axis. */
+void fold3d_init_node(double x, double y, double z, int i, int j, int k, double *T, int nx,
+                      int ny, int nz);
+void fold3d_update_voxel(double *T, int i, int j, int k, double xp, double yp, double zp,
+                         double dxx, double dyy, double dzz, int nx, int ny, int nz);
 
 /* Plane steps on a device field (reference AN_ISO_1_OpenACC.c:180, 334, 485,
  * 636, 790, 944).  Synchronous. */

@@ -44,6 +44,46 @@ def test_host_point_source_matches_reference(golden):
     assert bits_equal(got, want)
 
 
+def test_explicit_grid_voxel_update_matches_implicit():
+    """fold3d_update_voxel / fold3d_init_node on an explicit (nx,ny,nz) give the
+    same bits as updateVoxel / InitNode after fold3d_set_grid on that grid, and
+    ignore whatever grid was recorded last.  The implicit pair is the one the
+    golden 16^3 and 24^3 initializer fields pin (ImplicitInitialiser calls it)."""
+    L = fold3d.lib()
+    dp = C.POINTER(C.c_double)
+    nx, ny, nz = 7, 9, 11
+    h = (0.6, 0.45, 0.8)
+    rng = np.random.default_rng(5)
+    base = np.where(rng.random((nx + 2) * (ny + 2) * (nz + 2)) < 0.5, -1.0, 1.0) * 1e5
+    # voxels whose 44 neighbours (offsets -1..2) stay inside the (n+2)^3 box
+    vox = [(int(rng.integers(1, nx - 1)), int(rng.integers(1, ny - 1)),
+            int(rng.integers(1, nz - 1))) for _ in range(12)]
+    pts = rng.random((len(vox), 3)) * np.array(h)
+
+    implicit = base.copy()
+    L.fold3d_set_grid(nx, ny, nz)
+    got = (C.c_int(), C.c_int(), C.c_int())
+    L.fold3d_get_grid(*map(C.byref, got))
+    assert [g.value for g in got] == [nx, ny, nz]
+    for (i, j, k), p in zip(vox, pts):
+        L.updateVoxel(implicit.ctypes.data_as(dp), i, j, k, *map(float, p), *h)
+    L.InitNode(0.1, -0.2, 0.3, 2, 3, 4, implicit.ctypes.data_as(dp))
+
+    explicit = base.copy()
+    L.fold3d_set_grid(30, 5, 2)  # a different recorded grid must not matter
+    for (i, j, k), p in zip(vox, pts):
+        L.fold3d_update_voxel(explicit.ctypes.data_as(dp), i, j, k, *map(float, p), *h,
+                              nx, ny, nz)
+    L.fold3d_init_node(0.1, -0.2, 0.3, 2, 3, 4, explicit.ctypes.data_as(dp), nx, ny, nz)
+    L.fold3d_set_grid(16, 16, 16)
+
+    assert bits_equal(explicit, implicit)
+    assert (implicit != base).sum() > 100  # the updates did reach many nodes
+    # the sign of every node is kept; only magnitudes shrink
+    assert np.all(np.sign(implicit) == np.sign(base))
+    assert np.all(np.abs(implicit) <= np.abs(base))
+
+
 def _solve(handle, T, n=16, dx=0.5, model=None, opts=None):
     L = fold3d.lib()
     m = model if model is not None else fold3d.reference_model()

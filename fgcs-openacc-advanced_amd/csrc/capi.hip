@@ -827,6 +827,20 @@ int fold3d_plane_step(fold3d_handle *h, int direction, double *T, int nx, int ny
   return fold_plane_step_f64((hipStream_t)stream, T, G, direction, plane, f);
 }
 
+#define F3D_PLANE_STEP(name, dir)                                                  \
+  int name(fold3d_handle *h, double *T, int nx, int ny, int nz, int plane, double dzz,  \
+           double dxx, double dyy, double dxy, double dxz, double dyz, double F,         \
+           double ay, double ax, double az, void *stream) {                             \
+    return fold3d_plane_step(h, dir, T, nx, ny, nz, plane, dzz, dxx, dyy, dxy, dxz, dyz, \
+                             F, ay, ax, az, stream);                                    \
+  }
+F3D_PLANE_STEP(fold3d_function_x_down, FOLD3D_X_DOWN)
+F3D_PLANE_STEP(fold3d_function_x_up, FOLD3D_X_UP)
+F3D_PLANE_STEP(fold3d_function_k_up, FOLD3D_K_UP)
+F3D_PLANE_STEP(fold3d_function_k_down, FOLD3D_K_DOWN)
+F3D_PLANE_STEP(fold3d_function_y_up, FOLD3D_Y_UP)
+F3D_PLANE_STEP(fold3d_function_y_down, FOLD3D_Y_DOWN)
+
 int fold3d_device_alloc(void **ptr, size_t bytes) {
   F3D_CHECK(ptr != nullptr, FOLD3D_E_INVALID, "fold3d_device_alloc: ptr is NULL");
   F3D_HIP(hipMalloc(ptr, bytes));

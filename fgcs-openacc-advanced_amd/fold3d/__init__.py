@@ -171,6 +171,8 @@ def lib() -> C.CDLL:
         fn = getattr(L, "function_" + name)
         fn.argtypes = [vp, i] + [d] * 10
         fn.restype = None
+        getattr(L, "fold3d_function_" + name).argtypes = L.fold3d_plane_step.argtypes[:1] + \
+            L.fold3d_plane_step.argtypes[2:]
     _lib = L
     return L
 

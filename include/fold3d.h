@@ -265,6 +265,23 @@ int fold3d_plane_step(fold3d_handle *h, int direction, double *T, int nx,
                       double dyy, double dxy, double dxz, double dyz, double F,
                       double ay, double ax, double az, void *stream);
 
+/* The same per direction, named after the reference's six kernels
+ * (function_x_down AN_ISO_1_OpenACC.c:180, function_x_up :334, function_y_up
+ * :485, function_y_down :636, function_k_up :790, function_k_down :944), with
+ * the grid, the handle and the stream added: fold3d_function_x_down(h, T, nx,
+ * ny, nz, plane, ...) == fold3d_plane_step(h, FOLD3D_X_DOWN, T, nx, ny, nz,
+ * plane, ...).  A driver loop over several grids or streams calls these. */
+#define FOLD3D_PLANE_STEP_ARGS                                                  \
+  fold3d_handle *h, double *T, int nx, int ny, int nz, int plane, double dzz,  \
+      double dxx, double dyy, double dxy, double dxz, double dyz, double F,    \
+      double ay, double ax, double az, void *stream
+int fold3d_function_x_down(FOLD3D_PLANE_STEP_ARGS);
+int fold3d_function_x_up(FOLD3D_PLANE_STEP_ARGS);
+int fold3d_function_k_up(FOLD3D_PLANE_STEP_ARGS);
+int fold3d_function_k_down(FOLD3D_PLANE_STEP_ARGS);
+int fold3d_function_y_up(FOLD3D_PLANE_STEP_ARGS);
+int fold3d_function_y_down(FOLD3D_PLANE_STEP_ARGS);
+
 /* Measured ceilings of the current device, for reporting next to the
  * published peaks: sustained FP64 and FP32 FMA rate (FLOP/s) and the
  * bandwidth of a plain device-to-device copy (bytes read + written per s).

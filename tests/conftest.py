@@ -11,6 +11,7 @@ for p in (REPO, PKG):
         sys.path.insert(0, p)
 
 GOLDEN = os.path.join(REPO, "tests", "golden", "fold_ref_golden.npz")
+GOLDEN_LARGE = os.path.join(REPO, "tests", "golden", "fold_ref_large.npz")
 
 
 def pytest_configure(config):
@@ -27,6 +28,24 @@ def oracle():
 @pytest.fixture(scope="session")
 def golden():
     return np.load(GOLDEN)
+
+
+@pytest.fixture(scope="session")
+def golden_large():
+    """Digests of the reference's 64^3 and 128^3 solves (tests/golden/make_golden.py)."""
+    return np.load(GOLDEN_LARGE)
+
+
+def matches_digest(T, g, key) -> bool:
+    """Field T against the digest `key` of golden_large: sampled nodes, then
+    per-plane sums, then the SHA-256 of the bytes (each names a different failure)."""
+    import hashlib
+    T = np.ascontiguousarray(T, dtype=np.float64)
+    assert bits_equal(T[g[f"{key}_idx"]], g[f"{key}_val"]), f"{key}: sampled nodes differ"
+    H = round(T.size ** (1 / 3))
+    sums = np.abs(T.reshape(H, H, H)).clip(max=1e3).sum(axis=(1, 2))
+    assert np.array_equal(sums, g[f"{key}_plane_sums"]), f"{key}: plane sums differ"
+    return hashlib.sha256(T.tobytes()).digest() == g[f"{key}_sha256"].tobytes()
 
 
 @pytest.fixture(scope="session")

@@ -7,8 +7,11 @@ comes from running its code:
                                        plane kernels :180-1092, driver order :1192-1237
   Stencil-OpenACC/Fold3dPMM.c          InitFoldFromPoint :9, ImplicitInitialiser :129
 
-Run:  make -C oracle ref && python tests/golden/make_golden.py
-Output: tests/golden/fold_ref_golden.npz (+ .sha256)
+Run:  make -C oracle ref && python tests/golden/make_golden.py [--large-only]
+Output: tests/golden/fold_ref_golden.npz and fold_ref_large.npz (+ .sha256 each).
+The second holds digests (SHA-256 of the field's bytes, 4,096 seeded sample
+nodes, per-plane sums) of the 64^3 solve after 1 and 8 iterations and at the
+fixed point, and of the 128^3 solve at the fixed point.
 """
 from __future__ import annotations
 
@@ -158,13 +161,52 @@ def main() -> None:
     H = 130
     out["solve8_128_plane_sums"] = np.abs(T.reshape(H, H, H)).clip(max=1e3).sum(axis=(1, 2))
 
-    path = os.path.join(HERE, "fold_ref_golden.npz")
+    save(out, "fold_ref_golden.npz")
+
+
+def save(out: dict, name: str) -> None:
+    path = os.path.join(HERE, name)
     np.savez_compressed(path, **out)
     digest = hashlib.sha256(open(path, "rb").read()).hexdigest()
     with open(path + ".sha256", "w") as f:
-        f.write(f"{digest}  fold_ref_golden.npz\n")
+        f.write(f"{digest}  {name}\n")
     print(path, os.path.getsize(path), "bytes", digest)
 
 
+def field_digest(out: dict, key: str, T: np.ndarray, n: int, rng: np.random.Generator) -> None:
+    out[f"{key}_sha256"] = np.frombuffer(hashlib.sha256(T.tobytes()).digest(), np.uint8)
+    idx = np.sort(rng.choice(T.size, 4096, replace=False))
+    out[f"{key}_idx"] = idx
+    out[f"{key}_val"] = T[idx]
+    H = n + 2
+    out[f"{key}_plane_sums"] = np.abs(T.reshape(H, H, H)).clip(max=1e3).sum(axis=(1, 2))
+
+
+def large() -> None:
+    """Digests of the 64^3 and 128^3 solves (the full fields are 2.3 and 18 MB)."""
+    rng = np.random.default_rng(SEED + 1)
+    out: dict[str, np.ndarray] = {}
+    for n in (64, 128):
+        ref = RefBuild(n)
+        A = ref.initialiser()
+        it, fixed_at = 0, None
+        while fixed_at is None:
+            B = A.copy()
+            ref.sweeps(A, 1)
+            it += 1
+            if n == 64 and it == 1:
+                field_digest(out, "solve1_64", A, n, rng)
+            if n == 64 and it == 8:
+                field_digest(out, "solve8_64", A, n, rng)
+            if np.array_equal(A, B):
+                fixed_at = it
+                field_digest(out, f"fixed_{n}", A, n, rng)
+            print(f"n={n} iteration {it}", flush=True)
+        out[f"fixed_iters_{n}"] = np.array([fixed_at])
+    save(out, "fold_ref_large.npz")
+
+
 if __name__ == "__main__":
-    main()
+    if "--large-only" not in sys.argv:
+        main()
+    large()

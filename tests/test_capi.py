@@ -122,6 +122,10 @@ def test_invalid_arguments_are_codes_not_crashes():
     assert L.fold3d_plane_step(dummy, 6, dummy, 16, 16, 16, 3, *([1.0] * 10), None) == fold3d.FOLD3D_E_INVALID
     assert L.fold3d_plane_step(dummy, 0, dummy, 16, 16, 16, 16, *([1.0] * 10), None) == fold3d.FOLD3D_E_INVALID
     assert L.fold3d_plane_step(dummy, 1, dummy, 16, 16, 16, 1, *([1.0] * 10), None) == fold3d.FOLD3D_E_INVALID
+    # the per-direction entry points check the same way
+    assert L.fold3d_function_x_down(None, dummy, 16, 16, 16, 3, *([1.0] * 10), None) == fold3d.FOLD3D_E_INVALID
+    assert L.fold3d_function_k_up(dummy, dummy, 16, 16, 16, 16, *([1.0] * 10), None) == fold3d.FOLD3D_E_INVALID
+    assert L.fold3d_function_y_up(dummy, dummy, 16, 16, 16, 1, *([1.0] * 10), None) == fold3d.FOLD3D_E_INVALID
 
 
 def test_point_source_arguments_are_checked():

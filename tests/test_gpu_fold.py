@@ -52,6 +52,27 @@ def test_plane_step_random_field(solver, golden, d):
     assert bits_equal(T.cpu().numpy(), golden[f"plane_rand_{d}_16"])
 
 
+def test_per_direction_entry_points(solver, golden):
+    """fold3d_function_x_down & co. (handle, grid and stream added to the
+    reference's plane-step signature) on the 16^3 random field, each against
+    the reference's own plane step."""
+    import ctypes as C
+    L = fold3d.lib()
+    planes = {0: 7, 1: 9, 2: 7, 3: 9, 4: 9, 5: 7}
+    h = fold3d.reference_spacing(16)
+    for d, name in enumerate(fold3d.DIRECTIONS):
+        T = to_dev(golden["plane_rand_in_16"])
+        fold3d.check(getattr(L, "fold3d_function_" + name)(
+            solver.h, C.c_void_p(T.data_ptr()), 16, 16, 16, planes[d], *driver_args(d, h),
+            None))
+        torch.cuda.synchronize()
+        assert bits_equal(T.cpu().numpy(), golden[f"plane_rand_{d}_16"]), name
+    # out-of-range plane is an error code, not a launch
+    assert getattr(L, "fold3d_function_x_up")(solver.h, C.c_void_p(T.data_ptr()), 16, 16, 16,
+                                              1, *driver_args(1, h), None) \
+        == fold3d.FOLD3D_E_INVALID
+
+
 @pytest.mark.parametrize("d", range(6))
 def test_direction_on_initial_field(solver, golden, d):
     T = to_dev(golden["init_16"])
@@ -152,6 +173,34 @@ def test_solve_128_matches_reference_digest(solver, golden):
     assert np.array_equal(out[golden["solve8_128_idx"]], golden["solve8_128_val"])
     assert hashlib.sha256(out.tobytes()).digest() == golden["solve8_128_sha256"].tobytes()
     assert st.cell_updates == 8 * 6 * 127 * 128 * 128
+
+
+@pytest.mark.parametrize("engine", ENGINES)
+def test_solves_64_match_reference_digests(solver, golden_large, engine):
+    """64^3 after 1 and 8 iterations and at the fixed point (the convergence
+    stop), each against the digest of the reference's own solve."""
+    from conftest import matches_digest
+    n = 64
+    print(f"64^3 digests, engine {engine}", flush=True)
+    for iters, key in ((1, "solve1_64"), (8, "solve8_64")):
+        T = to_dev(fold3d.implicit_initialiser(n))
+        solver.solve(T, n, max_iters=iters, engine=engine)
+        assert matches_digest(T.cpu().numpy(), golden_large, key)
+    T = to_dev(fold3d.implicit_initialiser(n))
+    st = solver.solve(T, n, max_iters=100, tol=1e-300, engine=engine)
+    assert st.iterations == int(golden_large["fixed_iters_64"][0])
+    assert matches_digest(T.cpu().numpy(), golden_large, "fixed_64")
+
+
+@pytest.mark.parametrize("engine", [fold3d.ENGINE_SPLIT, fold3d.ENGINE_LOCKED])
+def test_fixed_point_128_matches_reference_digest(solver, golden_large, engine):
+    from conftest import matches_digest
+    n = 128
+    print(f"128^3 to the fixed point, engine {engine}", flush=True)
+    T = to_dev(fold3d.implicit_initialiser(n))
+    st = solver.solve(T, n, max_iters=100, tol=1e-300, engine=engine)
+    assert st.iterations == int(golden_large["fixed_iters_128"][0])
+    assert matches_digest(T.cpu().numpy(), golden_large, "fixed_128")
 
 
 def test_solve_64_matches_oracle(solver, oracle):

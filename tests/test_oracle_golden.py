@@ -88,3 +88,32 @@ def test_solve_128_digest(oracle, golden):
     oracle.sweeps(T, 128, 8)
     assert np.array_equal(T[golden["solve8_128_idx"]], golden["solve8_128_val"])
     assert hashlib.sha256(T.tobytes()).digest() == golden["solve8_128_sha256"].tobytes()
+
+
+def test_solves_64_digests(oracle, golden_large):
+    """The oracle against the reference's 64^3 solve after 1 and 8 iterations
+    and at its fixed point (18 iterations, the first that changes nothing)."""
+    from conftest import matches_digest
+    n = 64
+    T = oracle.initialiser(n)
+    oracle.sweeps(T, n, 1)
+    assert matches_digest(T, golden_large, "solve1_64")
+    oracle.sweeps(T, n, 7)
+    assert matches_digest(T, golden_large, "solve8_64")
+    its = int(golden_large["fixed_iters_64"][0])
+    assert its == 18
+    oracle.sweeps(T, n, its - 8)
+    assert matches_digest(T, golden_large, "fixed_64")
+    U = T.copy()
+    oracle.sweeps(U, n, 1)
+    assert bits_equal(U, T)
+
+
+@pytest.mark.slow
+def test_fixed_point_128_digest(oracle, golden_large):
+    from conftest import matches_digest
+    n = 128
+    its = int(golden_large["fixed_iters_128"][0])
+    T = oracle.initialiser(n)
+    oracle.sweeps(T, n, its)
+    assert matches_digest(T, golden_large, "fixed_128")

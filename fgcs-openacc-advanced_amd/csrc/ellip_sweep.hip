@@ -343,21 +343,23 @@ int ellip_sweeps_split_engine(hipStream_t st, R *T, R *Y, R *VX, R *VY, const Gr
     permute_layout<R>(st, V, GZ, VX, GX);
     permute_layout<R>(st, V, GZ, VY, GY);
   }
+  FieldMover<R> mv(st, T, Y, GZ);
+  const bool extra = FieldMover<R>::extra_move_first(max_iters, tol);
   int it = 0;
   double last = -1.0;
   for (; it < max_iters;) {
     unsigned long long *md = tol > 0 ? d_max : nullptr;
     if (md) ellip_zero_u64<<<1, 1, 0, st>>>(d_max);
-    permute_layout<R>(st, T, GZ, Y, GX);
-    ellip_split_direction<R>(st, Y, V ? VX : nullptr, GX, FOLD3D_X_DOWN, args[FOLD3D_X_DOWN], md, groups);
-    ellip_split_direction<R>(st, Y, V ? VX : nullptr, GX, FOLD3D_X_UP, args[FOLD3D_X_UP], md, groups);
-    permute_layout<R>(st, Y, GX, T, GZ);
-    ellip_split_direction<R>(st, T, V, GZ, FOLD3D_K_UP, args[FOLD3D_K_UP], md, groups);
-    ellip_split_direction<R>(st, T, V, GZ, FOLD3D_K_DOWN, args[FOLD3D_K_DOWN], md, groups);
-    permute_layout<R>(st, T, GZ, Y, GY);
-    ellip_split_direction<R>(st, Y, V ? VY : nullptr, GY, FOLD3D_Y_UP, args[FOLD3D_Y_UP], md, groups);
-    ellip_split_direction<R>(st, Y, V ? VY : nullptr, GY, FOLD3D_Y_DOWN, args[FOLD3D_Y_DOWN], md, groups);
-    permute_layout<R>(st, Y, GY, T, GZ);
+    mv.to(GX);
+    ellip_split_direction<R>(st, mv.field(), V ? VX : nullptr, GX, FOLD3D_X_DOWN, args[FOLD3D_X_DOWN], md, groups);
+    ellip_split_direction<R>(st, mv.field(), V ? VX : nullptr, GX, FOLD3D_X_UP, args[FOLD3D_X_UP], md, groups);
+    mv.to(GZ);
+    ellip_split_direction<R>(st, mv.field(), V, GZ, FOLD3D_K_UP, args[FOLD3D_K_UP], md, groups);
+    ellip_split_direction<R>(st, mv.field(), V, GZ, FOLD3D_K_DOWN, args[FOLD3D_K_DOWN], md, groups);
+    mv.to(GY);
+    ellip_split_direction<R>(st, mv.field(), V ? VY : nullptr, GY, FOLD3D_Y_UP, args[FOLD3D_Y_UP], md, groups);
+    ellip_split_direction<R>(st, mv.field(), V ? VY : nullptr, GY, FOLD3D_Y_DOWN, args[FOLD3D_Y_DOWN], md, groups);
+    if (it == 0 && extra) mv.to(GZ);
     F3D_HIP(hipGetLastError());
     ++it;
     if (md) {
@@ -368,6 +370,7 @@ int ellip_sweeps_split_engine(hipStream_t st, R *T, R *Y, R *VX, R *VY, const Gr
       if (last < tol) break;
     }
   }
+  F3D_HIP(mv.finish(GZ));
   *iters_out = it;
   *max_dt_out = last;
   return FOLD3D_OK;
@@ -573,23 +576,25 @@ int ellip_sweeps_metric_engine(hipStream_t st, R *T, R *scratch, const Grid &G,
     permute_metric(MX, GX);
     permute_metric(MY, GY);
   }
+  FieldMover<R> mv(st, T, Y, GZ);
+  const bool extra = FieldMover<R>::extra_move_first(max_iters, tol);
   int it = 0;
   double last = -1.0;
   for (; it < max_iters;) {
     unsigned long long *md = tol > 0 ? d_max : nullptr;
     if (md) ellip_zero_u64<<<1, 1, 0, st>>>(d_max);
-    permute_layout<R>(st, T, GZ, Y, GX);
+    mv.to(GX);
     if (!resident) permute_metric(MX, GX);
-    ellip_metric_direction<R>(st, Y, MX, GX, FOLD3D_X_DOWN, h, md, groups);
-    ellip_metric_direction<R>(st, Y, MX, GX, FOLD3D_X_UP, h, md, groups);
-    permute_layout<R>(st, Y, GX, T, GZ);
-    ellip_metric_direction<R>(st, T, MZ, GZ, FOLD3D_K_UP, h, md, groups);
-    ellip_metric_direction<R>(st, T, MZ, GZ, FOLD3D_K_DOWN, h, md, groups);
-    permute_layout<R>(st, T, GZ, Y, GY);
+    ellip_metric_direction<R>(st, mv.field(), MX, GX, FOLD3D_X_DOWN, h, md, groups);
+    ellip_metric_direction<R>(st, mv.field(), MX, GX, FOLD3D_X_UP, h, md, groups);
+    mv.to(GZ);
+    ellip_metric_direction<R>(st, mv.field(), MZ, GZ, FOLD3D_K_UP, h, md, groups);
+    ellip_metric_direction<R>(st, mv.field(), MZ, GZ, FOLD3D_K_DOWN, h, md, groups);
+    mv.to(GY);
     if (!resident) permute_metric(MY, GY);
-    ellip_metric_direction<R>(st, Y, MY, GY, FOLD3D_Y_UP, h, md, groups);
-    ellip_metric_direction<R>(st, Y, MY, GY, FOLD3D_Y_DOWN, h, md, groups);
-    permute_layout<R>(st, Y, GY, T, GZ);
+    ellip_metric_direction<R>(st, mv.field(), MY, GY, FOLD3D_Y_UP, h, md, groups);
+    ellip_metric_direction<R>(st, mv.field(), MY, GY, FOLD3D_Y_DOWN, h, md, groups);
+    if (it == 0 && extra) mv.to(GZ);
     F3D_HIP(hipGetLastError());
     ++it;
     if (md) {
@@ -600,6 +605,7 @@ int ellip_sweeps_metric_engine(hipStream_t st, R *T, R *scratch, const Grid &G,
       if (last < tol) break;
     }
   }
+  F3D_HIP(mv.finish(GZ));
   *iters_out = it;
   *max_dt_out = last;
   return FOLD3D_OK;

@@ -550,41 +550,27 @@ int fold_sweeps_tiled_engine(hipStream_t st, R *T, R *Y, const Grid &G,
   const Grid GX = G.with_order(kOrderX), GZ = G.with_order(kOrderZ), GY = G.with_order(kOrderY);
   const bool lock = stamp != nullptr && groups > 0;
   if (lock) F3D_HIP(hipMemsetAsync(stamp, 0, sizeof(int) * G.count, st));
-  // stamps move with T (4-byte fields: the float permute copies the bits)
-  auto permute_stamp = [&](const int *in, const Grid &Gi, int *out, const Grid &Go) {
-    if (lock) permute_layout<float>(st, (const float *)in, Gi, (float *)out, Go);
-  };
-  auto lk = [&](int *sp, int it, int d) {
-    return LockState{lock ? sp : nullptr, 6 * it + d, updates_done};
+  FieldMover<R> mv(st, T, Y, GZ, lock ? stamp : nullptr, lock ? stampY : nullptr);
+  const bool extra = FieldMover<R>::extra_move_first(max_iters, tol);
+  auto dir = [&](const Grid &Gd, int it, int d, unsigned long long *md) {
+    tiled_direction<R>(st, mv.field(), Gd, d, d_consts + d, md, groups,
+                       LockState{lock ? mv.stamps() : nullptr, 6 * it + d, updates_done});
   };
   int it = 0;
   double last = -1.0;
   for (; it < max_iters;) {
     unsigned long long *md = tol > 0 ? d_max : nullptr;
     if (md) zero_u64_kernel<<<1, 1, 0, st>>>(d_max);
-    // x_down, x_up with x slowest
-    permute_layout<R>(st, T, GZ, Y, GX);
-    permute_stamp(stamp, GZ, stampY, GX);
-    tiled_direction<R>(st, Y, GX, FOLD3D_X_DOWN, d_consts + FOLD3D_X_DOWN, md, groups,
-                       lk(stampY, it, FOLD3D_X_DOWN));
-    tiled_direction<R>(st, Y, GX, FOLD3D_X_UP, d_consts + FOLD3D_X_UP, md, groups,
-                       lk(stampY, it, FOLD3D_X_UP));
-    permute_layout<R>(st, Y, GX, T, GZ);
-    permute_stamp(stampY, GX, stamp, GZ);
-    // k_up, k_down in the reference layout
-    tiled_direction<R>(st, T, GZ, FOLD3D_K_UP, d_consts + FOLD3D_K_UP, md, groups,
-                       lk(stamp, it, FOLD3D_K_UP));
-    tiled_direction<R>(st, T, GZ, FOLD3D_K_DOWN, d_consts + FOLD3D_K_DOWN, md, groups,
-                       lk(stamp, it, FOLD3D_K_DOWN));
-    // y_up, y_down with y slowest
-    permute_layout<R>(st, T, GZ, Y, GY);
-    permute_stamp(stamp, GZ, stampY, GY);
-    tiled_direction<R>(st, Y, GY, FOLD3D_Y_UP, d_consts + FOLD3D_Y_UP, md, groups,
-                       lk(stampY, it, FOLD3D_Y_UP));
-    tiled_direction<R>(st, Y, GY, FOLD3D_Y_DOWN, d_consts + FOLD3D_Y_DOWN, md, groups,
-                       lk(stampY, it, FOLD3D_Y_DOWN));
-    permute_layout<R>(st, Y, GY, T, GZ);
-    permute_stamp(stampY, GY, stamp, GZ);
+    mv.to(GX);  // x_down, x_up with x slowest
+    dir(GX, it, FOLD3D_X_DOWN, md);
+    dir(GX, it, FOLD3D_X_UP, md);
+    mv.to(GZ);  // k_up, k_down in the reference layout
+    dir(GZ, it, FOLD3D_K_UP, md);
+    dir(GZ, it, FOLD3D_K_DOWN, md);
+    mv.to(GY);  // y_up, y_down with y slowest
+    dir(GY, it, FOLD3D_Y_UP, md);
+    dir(GY, it, FOLD3D_Y_DOWN, md);
+    if (it == 0 && extra) mv.to(GZ);
     F3D_HIP(hipGetLastError());
     ++it;
     if (md) {
@@ -595,6 +581,7 @@ int fold_sweeps_tiled_engine(hipStream_t st, R *T, R *Y, const Grid &G,
       if (last < tol) break;
     }
   }
+  F3D_HIP(mv.finish(GZ));
   *iters_out = it;
   *max_dt_out = last;
   return FOLD3D_OK;

@@ -122,6 +122,49 @@ int point_source_start(hipStream_t st, void *T, int precision, int nx, int ny, i
 template <typename R>
 void permute_layout(hipStream_t st, const R *in, const Grid &Gin, R *out, const Grid &Gout);
 
+// Where the field of a running solve lives.  The x-, z- and y-sweeps each run
+// in their own layout (X, Z, Y), so an iteration moves the field X -> Z -> Y,
+// and the move Y -> X opens the next iteration: three moves per iteration, and
+// one back to the reference layout at the end.  Every move writes the other
+// of two buffers (the caller's field and a scratch field of the same size).
+// The buffer the field ends in follows from the number of moves; the engines
+// move it back to Z once more after the first iteration when that makes a
+// fixed iteration count end in the caller's buffer, and finish() copies it
+// there otherwise (a solve stopped by its tolerance).  Optional int fields
+// (locking stamps) travel with it.
+template <typename R>
+struct FieldMover {
+  hipStream_t st;
+  R *buf[2];
+  int *sbuf[2];
+  const Grid *lay;
+  int cur = 0;
+
+  FieldMover(hipStream_t s, R *T, R *scratch, const Grid &GZ, int *stamp = nullptr,
+             int *stamp_scratch = nullptr)
+      : st(s), buf{T, scratch}, sbuf{stamp, stamp_scratch}, lay(&GZ) {}
+  R *field() const { return buf[cur]; }
+  int *stamps() const { return sbuf[cur]; }
+  void to(const Grid &G) {
+    if (lay == &G) return;
+    permute_layout<R>(st, buf[cur], *lay, buf[1 - cur], G);
+    if (sbuf[0]) permute_layout<float>(st, (const float *)sbuf[cur], *lay, (float *)sbuf[1 - cur], G);
+    cur = 1 - cur;
+    lay = &G;
+  }
+  // true: one more move to Z after the first iteration, so that a solve of
+  // exactly max_iters iterations (3 max_iters + 2 moves) ends in buf[0]
+  static bool extra_move_first(int max_iters, double tol) {
+    return tol <= 0 && max_iters > 1 && max_iters % 2 == 0;
+  }
+  hipError_t finish(const Grid &GZ) {
+    to(GZ);
+    if (cur == 0) return hipSuccess;
+    cur = 0;
+    return hipMemcpyAsync(buf[0], buf[1], sizeof(R) * GZ.count, hipMemcpyDeviceToDevice, st);
+  }
+};
+
 int fold_plane_step_f64(hipStream_t st, double *T, const Grid &G, int dir,
                         int plane, const FoldFrame<double> &f);
 

@@ -114,6 +114,11 @@ __device__ __forceinline__ void batch_dispatch_units(int grp, R &best, const R p
   }
 }
 
+// Largest |dT| of each instance in an iteration: kBatchMaxSlots words per
+// instance, a block updating the one its index picks, so that the rows of a
+// plane do not all meet on one word (the host takes the largest).
+constexpr int kBatchMaxSlots = 32;
+
 // One plane step q of direction `dir` for every instance that has it:
 // persistent blocks of kGroups waves pull rows of 64 target nodes from the
 // launch's counter, `chunk` consecutive rows per pull.
@@ -222,8 +227,9 @@ __global__ __launch_bounds__(64 * kGroups) void batch_plane_kernel(
       double vmax = (double)d;
 #pragma unroll
       for (int off = 32; off > 0; off >>= 1) vmax = fmax(vmax, __shfl_xor(vmax, off, 64));
-      if (tx == 0 && vmax > 0.0)
-        atomicMax(&max_dt[i], (unsigned long long)__double_as_longlong(vmax));
+      if (tx == 0 && vmax > 0.0)  // the instance's word for this block (see kBatchMaxSlots)
+        atomicMax(&max_dt[(long long)i * kBatchMaxSlots + blockIdx.x % kBatchMaxSlots],
+                  (unsigned long long)__double_as_longlong(vmax));
     }
     __syncthreads();  // s[][] and part[][] are reused by the next item
   }
@@ -433,7 +439,7 @@ int batch_solve(hipStream_t user_st, int count, const fold3d_instance *inst, int
     BATCH_HIP(hipMalloc(&d_prefix, sizeof(long long) * prefix_all.size()));
     BATCH_HIP(hipMalloc(&d_done, sizeof(int) * count));
     BATCH_HIP(hipMalloc(&d_counters, sizeof(unsigned int) * launches));
-    BATCH_HIP(hipMalloc(&d_max, sizeof(unsigned long long) * count));
+    BATCH_HIP(hipMalloc(&d_max, sizeof(unsigned long long) * count * kBatchMaxSlots));
     BATCH_HIP(hipMalloc(&d_tiles, sizeof(long long) * (count + 1)));
     BATCH_HIP(hipMalloc(&d_owner, sizeof(int) * owner_all.size()));
     BATCH_HIP(hipMemcpyAsync(d_owner, owner_all.data(), sizeof(int) * owner_all.size(),
@@ -470,8 +476,8 @@ int batch_solve(hipStream_t user_st, int count, const fold3d_instance *inst, int
 
     // one iteration as a launch sequence
     auto enqueue_iteration = [&](hipStream_t s) {
-      batch_reset_kernel<<<(launches + count + 255) / 256, 256, 0, s>>>(d_counters, launches,
-                                                                      d_max, count);
+      batch_reset_kernel<<<(launches + count * kBatchMaxSlots + 255) / 256, 256, 0, s>>>(
+          d_counters, launches, d_max, count * kBatchMaxSlots);
       int slot = 0;
       auto permute = [&](int which, int lin, int lout, int fields) {
         PermuteJob<R> job{which, lin, lout, fields};
@@ -533,7 +539,7 @@ int batch_solve(hipStream_t user_st, int count, const fold3d_instance *inst, int
     BATCH_HIP(hipEventCreate(&e0));
     BATCH_HIP(hipEventCreate(&e1));
     BATCH_HIP(hipEventRecord(e0, st));
-    std::vector<unsigned long long> hmax(count);
+    std::vector<unsigned long long> hmax((size_t)count * kBatchMaxSlots);
     std::vector<int> iters(count, 0);
     std::vector<double> last(count, -1.0);
     int remaining = count;
@@ -541,14 +547,17 @@ int batch_solve(hipStream_t user_st, int count, const fold3d_instance *inst, int
       if (use_graph) BATCH_HIP(hipGraphLaunch(exec, st));
       else enqueue_iteration(st);
       BATCH_HIP(hipGetLastError());
-      BATCH_HIP(hipMemcpyAsync(hmax.data(), d_max, sizeof(unsigned long long) * count,
+      BATCH_HIP(hipMemcpyAsync(hmax.data(), d_max, sizeof(unsigned long long) * hmax.size(),
                                hipMemcpyDeviceToHost, st));
       BATCH_HIP(hipStreamSynchronize(st));
       bool changed = false;
       for (int i = 0; i < count; ++i) {
         if (done[i]) continue;
+        unsigned long long m = 0;  // largest of the instance's words
+        for (int k = 0; k < kBatchMaxSlots; ++k)
+          m = std::max(m, hmax[(size_t)i * kBatchMaxSlots + k]);
         double v;
-        memcpy(&v, &hmax[i], sizeof v);
+        memcpy(&v, &m, sizeof v);
         iters[i] = it + 1;
         last[i] = v;
         if (tol > 0 && v < tol) {

@@ -15,7 +15,7 @@ using namespace fold3d;
 
 struct fold3d_handle {
   int device;
-  unsigned long long *d_max;  // max |dT| of the running iteration
+  unsigned long long *d_max;  // max |dT| of the running iteration (kMaxSlots words)
   hipEvent_t ev0, ev1;
   void *d_consts;             // per-direction constants of the running solve
   void *h_consts;             // pinned staging for them
@@ -138,7 +138,7 @@ int fold3d_create(fold3d_handle **out, int device) {
   h->batch_groups = 1;
   h->batch_blocks_per_cu = 64;
   h->batch_rows_per_pull = -1;
-  hipError_t e = hipMalloc((void **)&h->d_max, sizeof(unsigned long long));
+  hipError_t e = hipMalloc((void **)&h->d_max, sizeof(unsigned long long) * kMaxSlots);
   if (e == hipSuccess) e = hipEventCreate(&h->ev0);
   if (e == hipSuccess) e = hipEventCreate(&h->ev1);
   if (e == hipSuccess) e = hipEventCreateWithFlags(&h->consts_copied, hipEventDisableTiming);

@@ -7,6 +7,7 @@
 
 #include <cstdint>
 #include <cstdio>
+#include <cstring>
 #include <string>
 
 #include "../../include/fold3d.h"
@@ -56,6 +57,30 @@ constexpr Frame kFrames[6] = {
 constexpr int kOrderZ[3] = {0, 1, 2};
 constexpr int kOrderX[3] = {1, 2, 0};
 constexpr int kOrderY[3] = {0, 2, 1};
+
+// The largest |dT| of an iteration (tolerance stop), kept as the bit pattern
+// of a non-negative double, which orders like its value, in kMaxSlots words.
+// A block updates the word its block index picks: the thousands of blocks of
+// a plane launch then share each word with a few dozen others rather than
+// all meeting on one, whose serialised atomics cost more than the update.
+constexpr int kMaxSlots = 256;
+__device__ __forceinline__ void max_dt_update(unsigned long long *slots, double v) {
+  const unsigned b = blockIdx.x + 7u * blockIdx.y + 13u * blockIdx.z;
+  atomicMax(&slots[b % kMaxSlots], (unsigned long long)__double_as_longlong(v));
+}
+// Clears the words: one block of kMaxSlots threads.
+static __global__ void max_dt_reset_kernel(unsigned long long *slots) { slots[threadIdx.x] = 0ull; }
+// Largest word, on the host, once the stream's work so far has finished.
+static inline hipError_t max_dt_read(hipStream_t st, const unsigned long long *slots,
+                                     double *out) {
+  unsigned long long h[kMaxSlots];
+  hipError_t e = hipMemcpyAsync(h, slots, sizeof h, hipMemcpyDeviceToHost, st);
+  if (e == hipSuccess) e = hipStreamSynchronize(st);
+  unsigned long long m = 0;
+  for (int k = 0; k < kMaxSlots; ++k) m = h[k] > m ? h[k] : m;
+  memcpy(out, &m, sizeof m);
+  return e;
+}
 
 // Padded grid: nodes 1..n[a] on each axis, ghosts at 0 and n[a]+1.
 struct Grid {

@@ -61,11 +61,9 @@ __global__ __launch_bounds__(kLaneE *kRowsE) void ellip_plane_kernel(
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, 64));
     if ((threadIdx.x & 63) == 0 && v > 0.0)
-      atomicMax(max_dt, (unsigned long long)__double_as_longlong(v));
+      max_dt_update(max_dt, v);
   }
 }
-
-__global__ void ellip_zero_u64(unsigned long long *p) { *p = 0ull; }
 
 template <typename R, bool kSpeed, bool kMetric>
 static void ellip_launch(hipStream_t st, R *T, const PlaneGeom &g,
@@ -162,7 +160,7 @@ int ellip_sweeps_plane_engine(hipStream_t st, R *T, const Grid &G,
   int it = 0;
   double last = -1.0;
   for (; it < max_iters;) {
-    if (tol > 0) ellip_zero_u64<<<1, 1, 0, st>>>(d_max);
+    if (tol > 0) max_dt_reset_kernel<<<1, kMaxSlots, 0, st>>>(d_max);
     for (int d = 0; d < 6; ++d) {
       const Frame &fr = kFrames[d];
       const int n = G.n[fr.m];
@@ -175,10 +173,7 @@ int ellip_sweeps_plane_engine(hipStream_t st, R *T, const Grid &G,
     F3D_HIP(hipGetLastError());
     ++it;
     if (tol > 0) {
-      unsigned long long bits = 0;
-      F3D_HIP(hipMemcpyAsync(&bits, d_max, sizeof bits, hipMemcpyDeviceToHost, st));
-      F3D_HIP(hipStreamSynchronize(st));
-      memcpy(&last, &bits, sizeof last);
+      F3D_HIP(max_dt_read(st, d_max, &last));
       if (last < tol) break;
     }
   }
@@ -293,7 +288,7 @@ __global__ __launch_bounds__(64 * kGroups) void ellip_split_kernel(
       double v = (double)d;
 #pragma unroll
       for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, 64));
-      if (tx == 0 && v > 0.0) atomicMax(max_dt, (unsigned long long)__double_as_longlong(v));
+      if (tx == 0 && v > 0.0) max_dt_update(max_dt, v);
     }
   }
 }
@@ -349,7 +344,7 @@ int ellip_sweeps_split_engine(hipStream_t st, R *T, R *Y, R *VX, R *VY, const Gr
   double last = -1.0;
   for (; it < max_iters;) {
     unsigned long long *md = tol > 0 ? d_max : nullptr;
-    if (md) ellip_zero_u64<<<1, 1, 0, st>>>(d_max);
+    if (md) max_dt_reset_kernel<<<1, kMaxSlots, 0, st>>>(d_max);
     mv.to(GX);
     ellip_split_direction<R>(st, mv.field(), V ? VX : nullptr, GX, FOLD3D_X_DOWN, args[FOLD3D_X_DOWN], md, groups);
     ellip_split_direction<R>(st, mv.field(), V ? VX : nullptr, GX, FOLD3D_X_UP, args[FOLD3D_X_UP], md, groups);
@@ -363,10 +358,7 @@ int ellip_sweeps_split_engine(hipStream_t st, R *T, R *Y, R *VX, R *VY, const Gr
     F3D_HIP(hipGetLastError());
     ++it;
     if (md) {
-      unsigned long long bits = 0;
-      F3D_HIP(hipMemcpyAsync(&bits, d_max, sizeof bits, hipMemcpyDeviceToHost, st));
-      F3D_HIP(hipStreamSynchronize(st));
-      memcpy(&last, &bits, sizeof last);
+      F3D_HIP(max_dt_read(st, d_max, &last));
       if (last < tol) break;
     }
   }
@@ -517,7 +509,7 @@ __global__ __launch_bounds__(64 * kGroups) void ellip_metric_kernel(
       double v = (double)d;
 #pragma unroll
       for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, 64));
-      if (tx == 0 && v > 0.0) atomicMax(max_dt, (unsigned long long)__double_as_longlong(v));
+      if (tx == 0 && v > 0.0) max_dt_update(max_dt, v);
     }
   }
 }
@@ -582,7 +574,7 @@ int ellip_sweeps_metric_engine(hipStream_t st, R *T, R *scratch, const Grid &G,
   double last = -1.0;
   for (; it < max_iters;) {
     unsigned long long *md = tol > 0 ? d_max : nullptr;
-    if (md) ellip_zero_u64<<<1, 1, 0, st>>>(d_max);
+    if (md) max_dt_reset_kernel<<<1, kMaxSlots, 0, st>>>(d_max);
     mv.to(GX);
     if (!resident) permute_metric(MX, GX);
     ellip_metric_direction<R>(st, mv.field(), MX, GX, FOLD3D_X_DOWN, h, md, groups);
@@ -598,10 +590,7 @@ int ellip_sweeps_metric_engine(hipStream_t st, R *T, R *scratch, const Grid &G,
     F3D_HIP(hipGetLastError());
     ++it;
     if (md) {
-      unsigned long long bits = 0;
-      F3D_HIP(hipMemcpyAsync(&bits, d_max, sizeof bits, hipMemcpyDeviceToHost, st));
-      F3D_HIP(hipStreamSynchronize(st));
-      memcpy(&last, &bits, sizeof last);
+      F3D_HIP(max_dt_read(st, d_max, &last));
       if (last < tol) break;
     }
   }

@@ -75,11 +75,9 @@ __global__ __launch_bounds__(kLaneX *kRowsY) void fold_plane_kernel(
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, 64));
     if ((threadIdx.x & 63) == 0 && v > 0.0)
-      atomicMax(max_dt, (unsigned long long)__double_as_longlong(v));
+      max_dt_update(max_dt, v);
   }
 }
-
-__global__ void zero_u64_kernel(unsigned long long *p) { *p = 0ull; }
 
 // Local frame of direction `dir` from the global model (F, a) and spacings.
 template <typename R>
@@ -161,16 +159,13 @@ int fold_sweeps_plane_engine(hipStream_t st, R *T, const Grid &G,
   int it = 0;
   double last = -1.0;
   for (; it < max_iters;) {
-    if (tol > 0) zero_u64_kernel<<<1, 1, 0, st>>>(d_max);
+    if (tol > 0) max_dt_reset_kernel<<<1, kMaxSlots, 0, st>>>(d_max);
     for (int d = 0; d < 6; ++d)
       fold_direction<R>(st, T, G, d, frames[d], tol > 0 ? d_max : nullptr);
     F3D_HIP(hipGetLastError());
     ++it;
     if (tol > 0) {
-      unsigned long long bits = 0;
-      F3D_HIP(hipMemcpyAsync(&bits, d_max, sizeof bits, hipMemcpyDeviceToHost, st));
-      F3D_HIP(hipStreamSynchronize(st));
-      memcpy(&last, &bits, sizeof last);
+      F3D_HIP(max_dt_read(st, d_max, &last));
       if (last < tol) break;
     }
   }
@@ -234,7 +229,7 @@ __global__ __launch_bounds__(kLaneX *kRowsY) void fold_plane_fast_kernel(
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, 64));
     if ((threadIdx.x & 63) == 0 && v > 0.0)
-      atomicMax(max_dt, (unsigned long long)__double_as_longlong(v));
+      max_dt_update(max_dt, v);
   }
 }
 
@@ -257,7 +252,7 @@ int fold_sweeps_fast_engine(hipStream_t st, R *T, const Grid &G, const FoldFast<
   double last = -1.0;
   const dim3 block(kLaneX, kRowsY);
   for (; it < max_iters;) {
-    if (tol > 0) zero_u64_kernel<<<1, 1, 0, st>>>(d_max);
+    if (tol > 0) max_dt_reset_kernel<<<1, kMaxSlots, 0, st>>>(d_max);
     for (int d = 0; d < 6; ++d) {
       const Frame &fr = kFrames[d];
       const int n = G.n[fr.m];
@@ -274,10 +269,7 @@ int fold_sweeps_fast_engine(hipStream_t st, R *T, const Grid &G, const FoldFast<
     F3D_HIP(hipGetLastError());
     ++it;
     if (tol > 0) {
-      unsigned long long bits = 0;
-      F3D_HIP(hipMemcpyAsync(&bits, d_max, sizeof bits, hipMemcpyDeviceToHost, st));
-      F3D_HIP(hipStreamSynchronize(st));
-      memcpy(&last, &bits, sizeof last);
+      F3D_HIP(max_dt_read(st, d_max, &last));
       if (last < tol) break;
     }
   }
@@ -346,7 +338,7 @@ __global__ __launch_bounds__(kTileA *kTileB) void fold_tile_kernel(
     double v = (double)d;
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, 64));
-    if (tx == 0 && v > 0.0) atomicMax(max_dt, (unsigned long long)__double_as_longlong(v));
+    if (tx == 0 && v > 0.0) max_dt_update(max_dt, v);
   }
 }
 
@@ -482,7 +474,7 @@ __global__ __launch_bounds__(64 * kGroups) void fold_split_kernel(
       double v = (double)d;
 #pragma unroll
       for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, 64));
-      if (tx == 0 && v > 0.0) atomicMax(max_dt, (unsigned long long)__double_as_longlong(v));
+      if (tx == 0 && v > 0.0) max_dt_update(max_dt, v);
     }
   }
 }
@@ -560,7 +552,7 @@ int fold_sweeps_tiled_engine(hipStream_t st, R *T, R *Y, const Grid &G,
   double last = -1.0;
   for (; it < max_iters;) {
     unsigned long long *md = tol > 0 ? d_max : nullptr;
-    if (md) zero_u64_kernel<<<1, 1, 0, st>>>(d_max);
+    if (md) max_dt_reset_kernel<<<1, kMaxSlots, 0, st>>>(d_max);
     mv.to(GX);  // x_down, x_up with x slowest
     dir(GX, it, FOLD3D_X_DOWN, md);
     dir(GX, it, FOLD3D_X_UP, md);
@@ -574,10 +566,7 @@ int fold_sweeps_tiled_engine(hipStream_t st, R *T, R *Y, const Grid &G,
     F3D_HIP(hipGetLastError());
     ++it;
     if (md) {
-      unsigned long long bits = 0;
-      F3D_HIP(hipMemcpyAsync(&bits, d_max, sizeof bits, hipMemcpyDeviceToHost, st));
-      F3D_HIP(hipStreamSynchronize(st));
-      memcpy(&last, &bits, sizeof last);
+      F3D_HIP(max_dt_read(st, d_max, &last));
       if (last < tol) break;
     }
   }

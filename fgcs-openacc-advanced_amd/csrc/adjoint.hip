@@ -27,6 +27,7 @@
 
 #include <cmath>
 #include <cstring>
+#include <algorithm>
 #include <vector>
 
 #include "common.h"
@@ -321,6 +322,11 @@ struct AdjSrc {
   Stencil<R> S;
 };
 
+// Largest change of each source's adjoint values in a pass: kAdjMaxSlots
+// words per source, a block updating the one its index picks, so that the
+// blocks of a plane do not all meet on one word (the host takes the largest).
+constexpr int kAdjMaxSlots = 32;
+
 // One plane of an adjoint sweep for every source still running (source =
 // blockIdx.z): mu_X = r_X + sum over the neighbours Z that list X in their
 // stencil (in_mask[X]) of weight * mu_Z.
@@ -389,8 +395,9 @@ __global__ __launch_bounds__(256) void adjoint_plane_kernel(const AdjSrc<R> *__r
   double v = (double)d;
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, 64));
-  if ((threadIdx.x & 63) == 0 && v > 0.0)
-    atomicMax(&maxd[zsrc], (unsigned long long)__double_as_longlong(v));
+  if ((threadIdx.x & 63) == 0 && v > 0.0)  // the source's word for this block
+    atomicMax(&maxd[zsrc * kAdjMaxSlots + (blockIdx.x + 7u * blockIdx.y) % kAdjMaxSlots],
+              (unsigned long long)__double_as_longlong(v));
 }
 
 // grad += mu_k * dT/dparam_k over the sources k in order: the same additions,
@@ -455,7 +462,7 @@ int adjoint_solve(hipStream_t st, int nsrc, const AdjointSource *srcs, const Gri
   AdjSrc<R> *d_srcX = d_src + nsrc;
   unsigned long long *d_max =
       (unsigned long long *)((char *)d_src + align256(sizeof(AdjSrc<R>) * 2 * nsrc));
-  int *d_done = (int *)((char *)d_max + align256(sizeof(unsigned long long) * nsrc));
+  int *d_done = (int *)((char *)d_max + align256(sizeof(unsigned long long) * nsrc * kAdjMaxSlots));
   std::vector<AdjSrc<R>> hs(2 * nsrc);  // [0, nsrc): reference layout, then layout X
   for (int k = 0; k < nsrc; ++k) {
     char *p = base + per * k;
@@ -536,7 +543,7 @@ int adjoint_solve(hipStream_t st, int nsrc, const AdjointSource *srcs, const Gri
   bool capturing = false;
   int rc = FOLD3D_OK;
   int pass = 0, running = nsrc;
-  std::vector<unsigned long long> bits(nsrc);
+  std::vector<unsigned long long> bits((size_t)nsrc * kAdjMaxSlots);
   std::vector<int> done(nsrc, 0);
   for (int k = 0; k < nsrc; ++k) {
     iters_out[k] = 0;
@@ -558,7 +565,7 @@ int adjoint_solve(hipStream_t st, int nsrc, const AdjointSource *srcs, const Gri
   ADJ_HIP(hipStreamWaitEvent(ws, ev, 0));
   ADJ_HIP(hipStreamBeginCapture(ws, hipStreamCaptureModeThreadLocal));
   capturing = true;
-  adj_zero_u64<<<(nsrc + 255) / 256, 256, 0, ws>>>(d_max, nsrc);
+  adj_zero_u64<<<(nsrc * kAdjMaxSlots + 255) / 256, 256, 0, ws>>>(d_max, nsrc * kAdjMaxSlots);
   // reverse of the forward order: each direction's planes from last to first;
   // the y- and z-sweeps in the reference layout, the x-sweeps (last) in layout
   // X, with mu moved there and back around them
@@ -588,14 +595,16 @@ int adjoint_solve(hipStream_t st, int nsrc, const AdjointSource *srcs, const Gri
   for (; pass < max_iters && running > 0;) {
     ADJ_HIP(hipGraphLaunch(exec, ws));
     ++pass;
-    ADJ_HIP(hipMemcpyAsync(bits.data(), d_max, sizeof(unsigned long long) * nsrc,
+    ADJ_HIP(hipMemcpyAsync(bits.data(), d_max, sizeof(unsigned long long) * bits.size(),
                            hipMemcpyDeviceToHost, ws));
     ADJ_HIP(hipStreamSynchronize(ws));
     bool changed = false;
     for (int k = 0; k < nsrc; ++k) {
       if (done[k]) continue;
       iters_out[k] = pass;
-      memcpy(&maxd_out[k], &bits[k], sizeof(double));
+      unsigned long long m = 0;  // largest of the source's words
+      for (int w = 0; w < kAdjMaxSlots; ++w) m = std::max(m, bits[(size_t)k * kAdjMaxSlots + w]);
+      memcpy(&maxd_out[k], &m, sizeof(double));
       if (maxd_out[k] <= srcs[k].tol) {
         done[k] = 1;
         --running;
@@ -634,7 +643,7 @@ size_t adjoint_scratch_bytes(const Grid &G, int precision, int nsrc) {
   const size_t tab = sizeof(AdjSrc<double>) > sizeof(AdjSrc<float>) ? sizeof(AdjSrc<double>)
                                                                     : sizeof(AdjSrc<float>);
   return per_source_bytes(G, r) * nsrc + align256(tab * 2 * nsrc) +
-         align256(sizeof(unsigned long long) * nsrc) + align256(sizeof(int) * nsrc);
+         align256(sizeof(unsigned long long) * nsrc * kAdjMaxSlots) + align256(sizeof(int) * nsrc);
 }
 
 template int adjoint_solve<double>(hipStream_t, int, const AdjointSource *, const Grid &,

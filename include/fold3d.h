@@ -136,7 +136,9 @@ void fold3d_reference_model(fold3d_model *m);       /* F = 1.1, a = (-0.35, 0.4,
 
 /* Whole solve, in place (replaces AN_ISO_1_OpenACC.c:1185-1241).
  * dx, dy, dz: node spacings.  Returns FOLD3D_E_NOT_CONVERGED (with T and
- * stats written) when tol > 0 and max_iters ran out first. */
+ * stats written) when tol > 0 and max_iters ran out first.  During the solve
+ * T also serves as one of the two buffers the field moves between layouts
+ * through, so after any other error its contents are unspecified. */
 int fold3d_solve(fold3d_handle *h, void *T, int nx, int ny, int nz,
                  double dx, double dy, double dz, const fold3d_model *model,
                  const fold3d_opts *opts, fold3d_stats *stats);

@@ -8,8 +8,8 @@
 //   layout X = (y, z, x)  for the x-sweeps   T[j + H1*(k + H2*i)]
 //   layout Z = (x, y, z)  for the z-sweeps   (the reference layout)
 //   layout Y = (x, z, y)  for the y-sweeps   T[i + H0*(k + H2*j)]
-// and moves the field between them with the kernels below (32 x 32 tiles of
-// the (x, y) plane staged through LDS when the fastest axis changes, plain row
+// and moves the field between them with the kernels below (tiles of the
+// (x, y) plane staged through LDS when the fastest axis changes, plain row
 // copies when it does not).
 #include <hip/hip_runtime.h>
 
@@ -20,8 +20,11 @@
 
 namespace fold3d {
 
-constexpr int kTile = 32;      // transposing moves: a 32 x 32 tile of the (x, y) plane
-constexpr int kTileRows = 8;   //   per block of 32 x 8 threads
+// Transposing moves: a square tile of the (x, y) plane whose rows are 256
+// bytes (32 doubles or 64 floats), per block of (tile side) x 8 threads.
+template <typename R>
+constexpr int kTile = 256 / sizeof(R);
+constexpr int kTileRows = 8;
 constexpr int kRowLanes = 64;  // other moves: 64 lanes along x
 constexpr int kRowRows = 4;    //   by 4 rows of y per block, 16 rows per block
 
@@ -32,22 +35,23 @@ struct Strides3 {
 // Moves that change the fastest axis (x <-> y): the tile is read along the
 // input's fastest axis and written along the output's, through LDS.
 template <typename R, bool kInFastX>
-__global__ __launch_bounds__(kTile *kTileRows) void transpose_kernel(
+__global__ __launch_bounds__(kTile<R> *kTileRows) void transpose_kernel(
     const R *__restrict__ in, R *__restrict__ out, Strides3 is, Strides3 os, int Hx, int Hy) {
-  __shared__ R tile[kTile][kTile + 1];  // [y - y0][x - x0]
-  const int x0 = blockIdx.x * kTile, y0 = blockIdx.y * kTile;
+  constexpr int kT = kTile<R>;
+  __shared__ R tile[kT][kT + 1];  // [y - y0][x - x0]
+  const int x0 = blockIdx.x * kT, y0 = blockIdx.y * kT;
   const long long z = blockIdx.z;
   const long long ib = z * is.s[2], ob = z * os.s[2];
   const int tx = threadIdx.x;
 #pragma unroll
-  for (int r = threadIdx.y; r < kTile; r += kTileRows) {
+  for (int r = threadIdx.y; r < kT; r += kTileRows) {
     const int x = x0 + (kInFastX ? tx : r), y = y0 + (kInFastX ? r : tx);
     if (x < Hx && y < Hy)
       tile[y - y0][x - x0] = in[ib + x * is.s[0] + y * is.s[1]];
   }
   __syncthreads();
 #pragma unroll
-  for (int r = threadIdx.y; r < kTile; r += kTileRows) {
+  for (int r = threadIdx.y; r < kT; r += kTileRows) {
     const int x = x0 + (kInFastX ? r : tx), y = y0 + (kInFastX ? tx : r);
     if (x < Hx && y < Hy) out[ob + x * os.s[0] + y * os.s[1]] = tile[y - y0][x - x0];
   }
@@ -100,8 +104,9 @@ void permute_layout(hipStream_t st, const R *in, const Grid &Gin, R *out, const 
       restride_kernel<R><<<grid, block, 0, st>>>(in, out, is2, os2, Hy, Hx);
     }
   } else {
-    const dim3 block(kTile, kTileRows);
-    const dim3 grid((Hx + kTile - 1) / kTile, (Hy + kTile - 1) / kTile, Hz);
+    constexpr int kT = kTile<R>;
+    const dim3 block(kT, kTileRows);
+    const dim3 grid((Hx + kT - 1) / kT, (Hy + kT - 1) / kT, Hz);
     if (in_x) transpose_kernel<R, true><<<grid, block, 0, st>>>(in, out, is, os, Hx, Hy);
     else transpose_kernel<R, false><<<grid, block, 0, st>>>(in, out, is, os, Hx, Hy);
   }

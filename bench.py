@@ -21,6 +21,8 @@ import os
 import sys
 import time
 
+import numpy as np
+
 REPO = os.path.dirname(os.path.abspath(__file__))
 for p in (REPO, os.path.join(REPO, "fgcs-openacc-advanced_amd")):
     if p not in sys.path:
@@ -70,7 +72,7 @@ def time_solves(solver, T0d, n, steps, warmup, dtype, torch, engine=0, iters=8, 
     return (time.perf_counter() - t0) * 1e3, T, sweep_ms
 
 
-def ellip_run(solver, n, dtype, tol, torch):
+def ellip_run(solver, n, dtype, tol, torch, threads=0):
     """The 512^3 tilted-metric, Gaussian-blob speed workload, to tolerance."""
     from fold3d import workloads as wl
     W = wl.blob_speed(n, xp=torch, dtype=dtype, device="cuda")
@@ -85,9 +87,30 @@ def ellip_run(solver, n, dtype, tol, torch):
     torch.cuda.synchronize()
     wall = (time.perf_counter() - t0) * 1e3
     W_upd = st.cell_updates
-    return {"iterations": st.iterations, "max_dT": st.max_dT, "sweep_ms": st.sweep_ms,
-            "wall_ms": wall, "value": W_upd / (st.sweep_ms / 1e3) / 1e9,
-            "unit": "Gcell-updates/s", "cell_updates": W_upd, "split_groups": groups}, T
+    out = {"iterations": st.iterations, "max_dT": st.max_dT, "sweep_ms": st.sweep_ms,
+           "wall_ms": wall, "value": W_upd / (st.sweep_ms / 1e3) / 1e9,
+           "unit": "Gcell-updates/s", "cell_updates": W_upd, "split_groups": groups}
+    if threads and dtype == torch.float64:
+        # bounded sample: the first iteration, on the host (elliptic oracle)
+        # and on the GPU from the same start field
+        from oracle.oracle import Oracle
+        o = Oracle()
+        o.set_threads(threads)
+        Tc = T0.cpu().numpy().copy()
+        V = W.speed_field.cpu().numpy()
+        t0 = time.perf_counter()
+        o.ellip_sweeps(Tc, n, 1, 0.0, W.h, M=W.M, speed=W.speed, speed_field=V)
+        dt = time.perf_counter() - t0
+        T1 = T0.clone()
+        solver.solve(T1, n, spacing=W.h, model=W.model(), max_iters=1)
+        out["cpu_baseline"] = {
+            "value": cell_updates(n, 1) / dt / 1e9, "unit": "Gcell-updates/s", "cores": threads,
+            "kind": "port", "sample": f"1 iteration (6 sweeps) of the same {n}^3 FP64 solve, "
+            f"{cell_updates(n, 1):.3e} cell-updates in {dt:.1f} s",
+            "gpu_same_bits_after_1_iteration": bool(np.array_equal(
+                Tc.view(np.uint64), T1.cpu().numpy().view(np.uint64)))}
+        del T1
+    return out, T
 
 
 def batch_run(solver, torch, count=96, check=6):
@@ -152,7 +175,7 @@ def batch_run(solver, torch, count=96, check=6):
             "fp32_max_rel_err_vs_fp64_on_smallest": {"instances": check, "value": worst}}
 
 
-def c0_run(solver, torch, n=128):
+def c0_run(solver, torch, n=128, threads=0):
     """Homogeneous tilted medium, 128^3 FP64 on [0,1]^3, source at the centre
     (between nodes), to max|dT| < 1e-10; error against sqrt(x^T M^-1 x)/v."""
     from fold3d import workloads as wl
@@ -167,11 +190,24 @@ def c0_run(solver, torch, n=128):
     reached = T < 1e15
     err = (T - exact).abs()[reached]
     far = reached & (exact > 0.25)
-    return {"n": n, "dtype": "f64", "iterations": st.iterations, "max_dT": st.max_dT,
-            "sweep_ms": st.sweep_ms, "value": st.cell_updates / (st.sweep_ms / 1e3) / 1e9,
-            "unit": "Gcell-updates/s", "max_abs_err_vs_closed_form": float(err.max()),
-            "mean_abs_err_vs_closed_form": float(err.mean()),
-            "max_rel_err_beyond_r0.25": float(((T - exact).abs()[far] / exact[far]).max())}
+    out = {"n": n, "dtype": "f64", "iterations": st.iterations, "max_dT": st.max_dT,
+           "sweep_ms": st.sweep_ms, "value": st.cell_updates / (st.sweep_ms / 1e3) / 1e9,
+           "unit": "Gcell-updates/s", "max_abs_err_vs_closed_form": float(err.max()),
+           "mean_abs_err_vs_closed_form": float(err.mean()),
+           "max_rel_err_beyond_r0.25": float(((T - exact).abs()[far] / exact[far]).max())}
+    if threads:  # the elliptic oracle on the host, same start field and stop rule
+        from oracle.oracle import Oracle
+        o = Oracle()
+        o.set_threads(threads)
+        Tc = T0.cpu().numpy().copy()
+        t0 = time.perf_counter()
+        it, _ = o.ellip_sweeps(Tc, n, 200, 1e-10, W.h, M=W.M, speed=W.speed)
+        dt = time.perf_counter() - t0
+        out["cpu_baseline"] = {"value": cell_updates(n, it) / dt / 1e9, "unit": "Gcell-updates/s",
+                               "cores": threads, "kind": "port", "s": dt, "iterations": it,
+                               "same_bits_as_gpu": bool(np.array_equal(
+                                   Tc.view(np.uint64), T.cpu().numpy().view(np.uint64)))}
+    return out
 
 
 def c2_run(torch, n=1280):
@@ -435,12 +471,16 @@ def main():
         log(f"bench: FP32 {fp32['value']:.2f} Gcell-upd/s, max rel err {rel.max():.2e}")
         del T32
 
+    # host threads of the CPU baselines (0: none are run)
+    cpu_threads = 0 if args.no_cpu else (
+        int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or min(16, os.cpu_count() or 1))
+
     ellip = None
     if not args.no_ellip:
         ellip = {}
         for name, dt, tol in (("fp64", torch.float64, 1e-10), ("fp32", torch.float32, 1e-6)):
             log(f"bench: elliptic blob-speed workload {n}^3 {name}, tol {tol}")
-            r, Te = ellip_run(solver, n, dt, tol, torch)
+            r, Te = ellip_run(solver, n, dt, tol, torch, threads=cpu_threads)
             ellip[name] = r
             if name == "fp64":
                 ref64 = Te
@@ -451,14 +491,23 @@ def main():
                 far = reached & (a > 8 / (n - 1) / 2.3)
                 r["max_rel_err_vs_fp64"] = float(((b - a).abs()[far] / a[far]).max())
             log(f"bench: elliptic {name}: {r['iterations']} iterations, {r['value']:.2f} Gcell-upd/s")
+            if "cpu_baseline" in r:
+                cb = r["cpu_baseline"]
+                log(f"bench: elliptic {name} CPU ({cb['cores']} threads, 1 iteration): "
+                    f"{cb['value']:.3f} Gcell-upd/s, GPU same bits: "
+                    f"{cb['gpu_same_bits_after_1_iteration']}")
             del Te
         del ref64
 
     c0 = None
     if not args.no_c0:
         log("bench: homogeneous tilted medium 128^3 FP64 vs closed form")
-        c0 = c0_run(solver, torch)
+        c0 = c0_run(solver, torch, threads=cpu_threads)
         log(f"bench: C0 {c0['iterations']} iterations, max abs err {c0['max_abs_err_vs_closed_form']:.3e}")
+        if "cpu_baseline" in c0:
+            cb = c0["cpu_baseline"]
+            log(f"bench: C0 CPU ({cb['cores']} threads): {cb['s']:.2f} s, {cb['iterations']} "
+                f"iterations, same bits as the GPU: {cb['same_bits_as_gpu']}")
 
     cpu = None
     if not args.no_cpu:

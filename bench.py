@@ -210,6 +210,39 @@ def c0_run(solver, torch, n=128, threads=0):
     return out
 
 
+def c0_ref_run(torch, threads, n=128):
+    """The reference's own workload at 128^3 (implicit-surface input, fold
+    model, FP64, 8 iterations) on the GPU and, with `threads`, in the oracle on
+    the host: time and the bits of the two results."""
+    import fold3d
+    T0 = fold3d.implicit_initialiser(n)
+    T = torch.from_numpy(T0).cuda()
+    solver = fold3d.Solver(0)  # a handle of its own, tuned for this size
+    try:
+        groups, _ = solver.autotune(T, n, iterations=1)
+        solver.solve(T, n, max_iters=8)  # warm
+        T.copy_(torch.from_numpy(T0))
+        torch.cuda.synchronize()
+        st = solver.solve(T, n, max_iters=8)
+    finally:
+        solver.close()
+    out = {"n": n, "dtype": "f64", "iterations": 8, "sweep_ms": st.sweep_ms, "split_groups": groups,
+           "value": cell_updates(n, 8) / (st.sweep_ms / 1e3) / 1e9, "unit": "Gcell-updates/s"}
+    if threads:
+        from oracle.oracle import Oracle
+        o = Oracle()
+        o.set_threads(threads)
+        Tc = T0.copy()
+        t0 = time.perf_counter()
+        o.sweeps(Tc, n, 8)
+        dt = time.perf_counter() - t0
+        out["cpu_baseline"] = {"value": cell_updates(n, 8) / dt / 1e9, "unit": "Gcell-updates/s",
+                               "cores": threads, "kind": "port", "s": dt,
+                               "same_bits_as_gpu": bool(np.array_equal(
+                                   Tc.view(np.uint64), T.cpu().numpy().view(np.uint64)))}
+    return out
+
+
 def c2_run(torch, n=1280):
     """Per-node tilt and axis-ratio fields with v = 1.5 + 2z + 5 % blobs,
     FP32, source near the top face, to max|dT| < 1e-6 (metric engine)."""
@@ -500,6 +533,16 @@ def main():
         del ref64
 
     c0 = None
+    c0_ref = None
+    if not args.no_c0:
+        log("bench: the reference workload at 128^3 FP64, 8 iterations")
+        c0_ref = c0_ref_run(torch, cpu_threads)
+        msg = f"bench: C0-REF {c0_ref['sweep_ms']:.1f} ms, {c0_ref['value']:.2f} Gcell-upd/s"
+        if "cpu_baseline" in c0_ref:
+            cb = c0_ref["cpu_baseline"]
+            msg += (f"; CPU ({cb['cores']} threads) {cb['s']:.2f} s, same bits: "
+                    f"{cb['same_bits_as_gpu']}")
+        log(msg)
     if not args.no_c0:
         log("bench: homogeneous tilted medium 128^3 FP64 vs closed form")
         c0 = c0_run(solver, torch, threads=cpu_threads)
@@ -598,6 +641,7 @@ def main():
         "cpu_baseline": cpu,
         "batched": batch,
         "adjoint": adjoint,
+        "c0_ref_128": c0_ref,
         "c0_closed_form": c0,
         "c2_tilted_field": c2,
         "device_ceilings": valu,

@@ -351,6 +351,10 @@ int fold3d_solve(fold3d_handle *h, void *T, int nx, int ny, int nz, double dx,
       for (int k = 0; k < 256; ++k) sum += (double)done[k];
       stats->updates_computed = sum;
     }
+    const bool ellip = model->kind == FOLD3D_MODEL_ELLIP;
+    const int params = ellip ? (model->speed_field ? 1 : 0) + (model->theta ? 4 : 0) : 0;
+    stats->bytes_moved = stats->updates_computed * (2 + params) *
+                         (o.precision == FOLD3D_F64 ? 8.0 : 4.0);
   }
   if (o.tol > 0 && !(maxdt < o.tol)) {
     set_error("fold3d_solve: max|dT| = %g >= tol %g after %d iterations", maxdt, o.tol, iters);
@@ -624,6 +628,8 @@ int fold3d_adjoint(fold3d_handle *h, const void *T, int nx, int ny, int nz, doub
     stats->max_dT = maxd;
     stats->cell_updates = 0;
     stats->sweep_ms = ms;
+    stats->updates_computed = 0;
+    stats->bytes_moved = 0;
   }
   return rc;
 }

@@ -77,6 +77,7 @@ class Stats(C.Structure):
         ("cell_updates", C.c_double),
         ("sweep_ms", C.c_double),
         ("updates_computed", C.c_double),
+        ("bytes_moved", C.c_double),
     ]
 
 

@@ -120,6 +120,11 @@ typedef struct fold3d_stats {
   double sweep_ms;       /* device time of the sweep loop (HIP events) */
   double updates_computed; /* cell-updates evaluated: cell_updates, or fewer with
                               FOLD3D_ENGINE_LOCKED (rows skipped by locking) */
+  double bytes_moved;    /* algorithmic HBM bytes of the plane steps: (2 + P) *
+                            sizeof(T) per evaluated update (read and write the
+                            target, read P per-node fields: speed_field, and
+                            theta, phi, ra, rb when given); layout moves are not
+                            counted.  0 from the adjoint entry points */
 } fold3d_stats;
 
 typedef struct fold3d_handle fold3d_handle;

@@ -61,6 +61,7 @@ def test_blob_speed_bitwise_and_converges(solver, oracle):
     T = dev(T0)
     st = solver.solve(T, n, spacing=W.h, model=Wd.model(), max_iters=40, tol=1e-10)
     assert st.iterations == it_o and st.max_dT == md_o
+    assert st.bytes_moved == st.updates_computed * 3 * 8  # target read + write, speed read
     assert bits_equal(T.cpu().numpy(), want)
 
 

@@ -173,6 +173,7 @@ def test_solve_128_matches_reference_digest(solver, golden):
     assert np.array_equal(out[golden["solve8_128_idx"]], golden["solve8_128_val"])
     assert hashlib.sha256(out.tobytes()).digest() == golden["solve8_128_sha256"].tobytes()
     assert st.cell_updates == 8 * 6 * 127 * 128 * 128
+    assert st.bytes_moved == st.updates_computed * 16  # read and write each target, FP64
 
 
 @pytest.mark.parametrize("engine", ENGINES)

@@ -175,7 +175,7 @@ def batch_run(solver, torch, count=96, check=6):
             "fp32_max_rel_err_vs_fp64_on_smallest": {"instances": check, "value": worst}}
 
 
-def c0_run(solver, torch, n=128, threads=0):
+def tilted_homogeneous_run(solver, torch, n=128, threads=0):
     """Homogeneous tilted medium, 128^3 FP64 on [0,1]^3, source at the centre
     (between nodes), to max|dT| < 1e-10; error against sqrt(x^T M^-1 x)/v."""
     from fold3d import workloads as wl
@@ -210,7 +210,7 @@ def c0_run(solver, torch, n=128, threads=0):
     return out
 
 
-def c0_ref_run(torch, threads, n=128):
+def reference_small_run(torch, threads, n=128):
     """The reference's own workload at 128^3 (implicit-surface input, fold
     model, FP64, 8 iterations) on the GPU and, with `threads`, in the oracle on
     the host: time and the bits of the two results."""
@@ -243,7 +243,7 @@ def c0_ref_run(torch, threads, n=128):
     return out
 
 
-def c2_run(torch, n=1280):
+def tilted_field_run(torch, n=1280):
     """Per-node tilt and axis-ratio fields with v = 1.5 + 2z + 5 % blobs,
     FP32, source near the top face, to max|dT| < 1e-6 (metric engine)."""
     import fold3d
@@ -254,7 +254,7 @@ def c2_run(torch, n=1280):
         T0 = wl.point_source_start(W, xp=torch, dtype=torch.float32, device="cuda")
         torch.cuda.empty_cache()
         groups, tune_ms = solver.autotune(T0, n, spacing=W.h, model=W.model(), iterations=1)
-        log(f"bench: C2 autotuner picked {groups} waves per node row ({tune_ms:.0f} ms/iteration)")
+        log(f"bench: tilted field: autotuner picked {groups} waves per node row ({tune_ms:.0f} ms/iteration)")
         T = T0
         torch.cuda.synchronize()
         t0 = time.perf_counter()
@@ -372,12 +372,14 @@ def main():
     ap.add_argument("--no-ablation", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
-    ap.add_argument("--no-c0", action="store_true")
+    ap.add_argument("--no-small", "--no-c0", dest="no_small", action="store_true",
+                    help="skip the 128^3 runs")
     ap.add_argument("--groups", type=int, default=0, choices=[0, 2, 4, 8],
                     help="waves per node row of the fold split kernel (0: autotune)")
     ap.add_argument("--no-fixpt", action="store_true",
                     help="skip the solves to the exact fixed point (split and locked engines)")
-    ap.add_argument("--c2-n", type=int, default=1280, help="side of the per-node metric run (0: skip)")
+    ap.add_argument("--field-n", "--c2-n", dest="field_n", type=int, default=1280,
+                    help="side of the per-node tilt-field run (0: skip)")
     ap.add_argument("--iters", type=int, default=8, help="iterations per solve (the reference runs 8)")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -532,24 +534,25 @@ def main():
             del Te
         del ref64
 
-    c0 = None
-    c0_ref = None
-    if not args.no_c0:
+    homog = None
+    ref_small = None
+    if not args.no_small:
         log("bench: the reference workload at 128^3 FP64, 8 iterations")
-        c0_ref = c0_ref_run(torch, cpu_threads)
-        msg = f"bench: C0-REF {c0_ref['sweep_ms']:.1f} ms, {c0_ref['value']:.2f} Gcell-upd/s"
-        if "cpu_baseline" in c0_ref:
-            cb = c0_ref["cpu_baseline"]
+        ref_small = reference_small_run(torch, cpu_threads)
+        msg = (f"bench: reference workload 128^3: {ref_small['sweep_ms']:.1f} ms, "
+               f"{ref_small['value']:.2f} Gcell-upd/s")
+        if "cpu_baseline" in ref_small:
+            cb = ref_small["cpu_baseline"]
             msg += (f"; CPU ({cb['cores']} threads) {cb['s']:.2f} s, same bits: "
                     f"{cb['same_bits_as_gpu']}")
         log(msg)
-    if not args.no_c0:
+    if not args.no_small:
         log("bench: homogeneous tilted medium 128^3 FP64 vs closed form")
-        c0 = c0_run(solver, torch, threads=cpu_threads)
-        log(f"bench: C0 {c0['iterations']} iterations, max abs err {c0['max_abs_err_vs_closed_form']:.3e}")
-        if "cpu_baseline" in c0:
-            cb = c0["cpu_baseline"]
-            log(f"bench: C0 CPU ({cb['cores']} threads): {cb['s']:.2f} s, {cb['iterations']} "
+        homog = tilted_homogeneous_run(solver, torch, threads=cpu_threads)
+        log(f"bench: tilted homogeneous 128^3: {homog['iterations']} iterations, max abs err {homog['max_abs_err_vs_closed_form']:.3e}")
+        if "cpu_baseline" in homog:
+            cb = homog["cpu_baseline"]
+            log(f"bench: tilted homogeneous CPU ({cb['cores']} threads): {cb['s']:.2f} s, {cb['iterations']} "
                 f"iterations, same bits as the GPU: {cb['same_bits_as_gpu']}")
 
     cpu = None
@@ -579,13 +582,13 @@ def main():
         log(f"bench: gradient {adjoint['gradient_s']:.2f} s (forward {adjoint['forward_batched_s']:.2f} s, "
             f"adjoint {adjoint['adjoint_device_ms_total'] / 1e3:.2f} s)")
 
-    c2 = None
-    if args.c2_n:
+    field = None
+    if args.field_n:
         del solver
         torch.cuda.empty_cache()
-        log(f"bench: per-node metric workload {args.c2_n}^3 FP32")
-        c2 = c2_run(torch, args.c2_n)
-        log(f"bench: C2 {c2['iterations']} iterations, {c2['value']:.2f} Gcell-upd/s")
+        log(f"bench: per-node tilt-field workload {args.field_n}^3 FP32")
+        field = tilted_field_run(torch, args.field_n)
+        log(f"bench: tilted field: {field['iterations']} iterations, {field['value']:.2f} Gcell-upd/s")
 
     log("bench: measured ceilings of this GPU (FMA rate, device copy)")
     peaks = fold3d.measure_peaks()
@@ -641,9 +644,9 @@ def main():
         "cpu_baseline": cpu,
         "batched": batch,
         "adjoint": adjoint,
-        "c0_ref_128": c0_ref,
-        "c0_closed_form": c0,
-        "c2_tilted_field": c2,
+        "reference_128": ref_small,
+        "tilted_homogeneous_128": homog,
+        "tilted_field": field,
         "device_ceilings": valu,
     }
     print(json.dumps(out), flush=True)

@@ -279,9 +279,9 @@ def adjoint_run(solver, torch, n=256):
     below the top face, 64x64 top-face receivers, observed times from the
     background plus 3 anomalies of +-5 %.  One gradient evaluation = the 16
     forward solves (one batch, checkpointed) + 16 adjoint solves."""
-    from fold3d.tomography import Tomography, c4_fields, top_face_survey
-    f = c4_fields(n, xp=torch, dtype=torch.float32, device="cuda")
-    ft = c4_fields(n, xp=torch, dtype=torch.float32, device="cuda", anomalies=True)
+    from fold3d.tomography import Tomography, tomography_fields, top_face_survey
+    f = tomography_fields(n, xp=torch, dtype=torch.float32, device="cuda")
+    ft = tomography_fields(n, xp=torch, dtype=torch.float32, device="cuda", anomalies=True)
     S = top_face_survey(n, 4, 64)
     tomo = Tomography(solver, S, f["phi"], f["ra"], f["rb"], batch=16)
     tomo.observe(ft["slowness"], ft["theta"])       # also the warm-up

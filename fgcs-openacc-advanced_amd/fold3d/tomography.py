@@ -54,8 +54,8 @@ def top_face_survey(n: int, src_side: int = 4, rec_side: int = 64) -> Survey:
     return Survey(n, h, sources, np.asarray(idx, dtype=np.int32))
 
 
-def c4_fields(n: int = 256, seed: int = 20261206, xp=np, dtype=np.float32, device=None,
-              anomalies: bool = False):
+def tomography_fields(n: int = 256, seed: int = 20261206, xp=np, dtype=np.float32,
+                      device=None, anomalies: bool = False):
     """Fields of the tomography workload on the padded grid.
 
     Speed: v = 1 + 6 d + sum_8 A_b exp(-|x-c_b|^2/(2*0.1^2)), A_b ~ U(+-.3),

@@ -8,14 +8,14 @@ import numpy as np
 import pytest
 
 import fold3d
-from fold3d.tomography import Tomography, adjoint_tol, c4_fields, top_face_survey
+from fold3d.tomography import Tomography, adjoint_tol, tomography_fields, top_face_survey
 
 pytestmark = pytest.mark.gpu
 
 
 def _fields(n, dtype, anomalies=False):
     import torch
-    return c4_fields(n, xp=torch, dtype=dtype, device="cuda", anomalies=anomalies)
+    return tomography_fields(n, xp=torch, dtype=dtype, device="cuda", anomalies=anomalies)
 
 
 def _tomo(solver, n, dtype, src_side, rec_side, **kw):

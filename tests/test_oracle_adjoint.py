@@ -7,7 +7,7 @@ angle; and it is linear in the receiver residuals."""
 import numpy as np
 import pytest
 
-from fold3d.tomography import c4_fields, top_face_survey
+from fold3d.tomography import tomography_fields, top_face_survey
 from fold3d.workloads import Workload, point_source_start, tilted_metric
 
 N = 12
@@ -30,8 +30,8 @@ def _solve(oracle, S, fl):
 
 @pytest.fixture(scope="module")
 def case(oracle):
-    f = c4_fields(N, xp=np, dtype=np.float64)
-    ft = c4_fields(N, xp=np, dtype=np.float64, anomalies=True)
+    f = tomography_fields(N, xp=np, dtype=np.float64)
+    ft = tomography_fields(N, xp=np, dtype=np.float64, anomalies=True)
     S = top_face_survey(N, src_side=1, rec_side=N)
     obs = _solve(oracle, S, ft)[0][S.rec_idx]
     T, src = _solve(oracle, S, f)

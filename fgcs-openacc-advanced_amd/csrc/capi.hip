@@ -98,6 +98,14 @@ static bool try_scratch(fold3d_handle *h, size_t bytes) {
 
 static constexpr size_t kConstBytes = 6 * sizeof(FoldFast<double>);
 
+// False when one FP64 field of the padded grid would exceed 2^56 bytes (far
+// beyond any device, and below the point where the size arithmetic of the
+// engines, 15 fields of it, could overflow 64 bits).
+static bool grid_bytes_fit(long long nx, long long ny, long long nz) {
+  const double bytes = 8.0 * (double)(nx + 2) * (double)(ny + 2) * (double)(nz + 2);
+  return bytes < 0x1p56;
+}
+
 extern "C" {
 
 const char *fold3d_version(void) { return "fold3d-mi355x 0.1 (gfx950)"; }
@@ -192,9 +200,18 @@ int fold3d_solve(fold3d_handle *h, void *T, int nx, int ny, int nz, double dx,
             FOLD3D_E_UNSUPPORTED, "fold3d_solve: locking is implemented for the fold model only");
   F3D_CHECK(model->kind == FOLD3D_MODEL_FOLD || model->kind == FOLD3D_MODEL_ELLIP,
             FOLD3D_E_INVALID, "fold3d_solve: unknown model kind %d", model->kind);
+  F3D_CHECK(grid_bytes_fit(nx, ny, nz), FOLD3D_E_NOMEM,
+            "fold3d_solve: a %dx%dx%d field does not fit in device memory", nx, ny, nz);
   if (model->kind == FOLD3D_MODEL_FOLD) {
     F3D_CHECK(std::isfinite(model->F) && model->F > 0, FOLD3D_E_INVALID,
               "fold3d_solve: F must be positive");
+    // F|p| + a.p = 1 is an eikonal equation only for |a| < F (the wind must be
+    // slower than the medium); the pruning bounds dzz/(F + az) and
+    // dzz/(sqrt(F^2 - ay^2) + az) assume it too
+    const double *a = model->a;
+    F3D_CHECK(std::isfinite(a[0]) && std::isfinite(a[1]) && std::isfinite(a[2]) &&
+                  a[0] * a[0] + a[1] * a[1] + a[2] * a[2] < model->F * model->F,
+              FOLD3D_E_INVALID, "fold3d_solve: a must be finite with |a| < F");
   } else {
     const int nf = (model->theta != nullptr) + (model->phi != nullptr) +
                    (model->ra != nullptr) + (model->rb != nullptr);
@@ -376,13 +393,10 @@ static int check_batch(fold3d_handle *h, int count, const fold3d_instance *inst,
   for (int i = 0; i < count; ++i) {
     const fold3d_instance &x = inst[i];
     F3D_CHECK(x.T != nullptr, FOLD3D_E_INVALID, "fold3d_solve_batched: instance %d has no T", i);
-    F3D_CHECK(is_device_ptr(x.T) && (!x.speed_field || is_device_ptr(x.speed_field)) &&
-                  (!x.theta || (is_device_ptr(x.theta) && is_device_ptr(x.phi) &&
-                                is_device_ptr(x.ra) && is_device_ptr(x.rb))),
-              FOLD3D_E_INVALID, "fold3d_solve_batched: instance %d: fields must be device memory",
-              i);
     F3D_CHECK(x.n[0] >= 1 && x.n[1] >= 1 && x.n[2] >= 1, FOLD3D_E_INVALID,
               "fold3d_solve_batched: instance %d grid too small", i);
+    F3D_CHECK(grid_bytes_fit(x.n[0], x.n[1], x.n[2]), FOLD3D_E_NOMEM,
+              "fold3d_solve_batched: instance %d does not fit in device memory", i);
     F3D_CHECK(x.h[0] > 0 && x.h[1] > 0 && x.h[2] > 0 && std::isfinite(x.h[0]) &&
                   std::isfinite(x.h[1]) && std::isfinite(x.h[2]),
               FOLD3D_E_INVALID, "fold3d_solve_batched: instance %d spacings must be positive", i);
@@ -393,12 +407,20 @@ static int check_batch(fold3d_handle *h, int count, const fold3d_instance *inst,
               FOLD3D_E_INVALID, "fold3d_solve_batched: instance %d speed must be positive", i);
     if (nf == 0) {
       const double *M = x.M;
+      bool fin = true;
+      for (int k = 0; k < 6; ++k) fin = fin && std::isfinite(M[k]);
       const double m2 = M[0] * M[1] - M[3] * M[3];
       const double m3 = M[0] * (M[1] * M[2] - M[5] * M[5]) - M[3] * (M[3] * M[2] - M[5] * M[4]) +
                         M[4] * (M[3] * M[5] - M[1] * M[4]);
-      F3D_CHECK(M[0] > 0 && m2 > 0 && m3 > 0, FOLD3D_E_INVALID,
-                "fold3d_solve_batched: instance %d metric is not positive definite", i);
+      F3D_CHECK(fin && M[0] > 0 && m2 > 0 && m3 > 0, FOLD3D_E_INVALID,
+                "fold3d_solve_batched: instance %d metric is not finite positive definite", i);
     }
+    // host-side checks first; then the pointers (a runtime query)
+    F3D_CHECK(is_device_ptr(x.T) && (!x.speed_field || is_device_ptr(x.speed_field)) &&
+                  (!x.theta || (is_device_ptr(x.theta) && is_device_ptr(x.phi) &&
+                                is_device_ptr(x.ra) && is_device_ptr(x.rb))),
+              FOLD3D_E_INVALID, "fold3d_solve_batched: instance %d: fields must be device memory",
+              i);
   }
   return FOLD3D_OK;
 }

@@ -16,12 +16,16 @@ namespace fold3d {
 
 void set_error(const char *fmt, ...);
 
+// A failed call is reported and its error taken off the runtime's last-error
+// slot, so that a later hipGetLastError() check (after a launch) does not
+// report it a second time: a failed allocation leaves the handle usable.
 #define F3D_HIP(call)                                                       \
   do {                                                                      \
     hipError_t e_ = (call);                                                 \
     if (e_ != hipSuccess) {                                                 \
       ::fold3d::set_error("%s failed: %s (%s:%d)", #call,                   \
                           hipGetErrorString(e_), __FILE__, __LINE__);       \
+      (void)hipGetLastError();                                              \
       return e_ == hipErrorOutOfMemory ? FOLD3D_E_NOMEM : FOLD3D_E_HIP;     \
     }                                                                       \
   } while (0)

@@ -37,8 +37,9 @@ int fold3d_write_text(const char *path, const double *T, int nx, int ny, int nz)
   FILE *f = fopen(path, "w");
   if (!f) return io_fail(FOLD3D_E_IO, "fold3d_write_text: cannot open", path);
   const long long H0 = nx + 2, H1 = ny + 2, H2 = nz + 2;
-  static char buf[1 << 20];
-  setvbuf(f, buf, _IOFBF, sizeof buf);
+  /* a 1 MiB buffer that stdio allocates for this stream alone (writers on
+     several threads do not share it) and frees at fclose */
+  setvbuf(f, NULL, _IOFBF, 1 << 20);
   for (long long i = 0; i < H0; ++i)
     for (long long j = 0; j < H1; ++j)
       for (long long k = 0; k < H2; ++k)

@@ -251,6 +251,63 @@ def implicit_initialiser(n: int) -> np.ndarray:
     return T
 
 
+def padded_count(nx: int, ny: int, nz: int) -> int:
+    """Nodes of the padded grid, ghosts included: (nx+2)(ny+2)(nz+2)."""
+    return (nx + 2) * (ny + 2) * (nz + 2)
+
+
+def _check_tensor(x, count, what, device, dtype=None):
+    """A torch tensor the C-ABI may read and write as `count` contiguous FP64
+    or FP32 values on GPU `device` (of `dtype` when given).  The library sees
+    only a raw pointer, so a wrong size, layout, dtype or device is rejected
+    here, before any launch."""
+    import torch
+    if not isinstance(x, torch.Tensor):
+        raise TypeError(f"{what} must be a torch tensor, not {type(x).__name__}")
+    if x.dtype not in (torch.float32, torch.float64):
+        raise TypeError(f"{what} must be float32 or float64, not {x.dtype}")
+    if dtype is not None and x.dtype != dtype:
+        raise TypeError(f"{what} is {x.dtype}; it must be {dtype}, like T")
+    if not x.is_contiguous():
+        raise ValueError(f"{what} must be contiguous")
+    if x.numel() != count:
+        raise ValueError(f"{what} has {x.numel()} elements; the padded grid has {count}")
+    if not x.is_cuda or (device is not None and x.device.index != device):
+        raise TypeError(f"{what} must be on cuda:{device}, not {x.device}")
+
+
+def _check_host_array(x, count, what):
+    if x.dtype not in (np.float64, np.float32):
+        raise TypeError(f"{what} must be float64 or float32, not {x.dtype}")
+    if not x.flags.c_contiguous or not x.flags.writeable:
+        raise ValueError(f"{what} must be a writeable C-contiguous array")
+    if x.size != count:
+        raise ValueError(f"{what} has {x.size} elements; the padded grid has {count}")
+
+
+def _check_model_fields(m, count, device, dtype):
+    """Per-node fields a Model was built from (Workload.model() and the
+    tomography harness keep them in m._refs): same dtype, device and size as T."""
+    for name, f in zip(("speed_field", "theta", "phi", "ra", "rb"), getattr(m, "_refs", ())):
+        if f is not None:
+            _check_tensor(f, count, f"model {name}", device, dtype)
+
+
+def _check_receivers(rec_idx, rec_res, device, dtype, what=""):
+    import torch
+    if not isinstance(rec_idx, torch.Tensor) or rec_idx.dtype != torch.int32:
+        raise TypeError(f"rec_idx{what} must be an int32 torch tensor")
+    if not isinstance(rec_res, torch.Tensor) or rec_res.dtype != dtype:
+        raise TypeError(f"rec_res{what} must be a torch tensor of T's dtype {dtype}")
+    for name, x in (("rec_idx", rec_idx), ("rec_res", rec_res)):
+        if not x.is_cuda or x.device.index != device:
+            raise TypeError(f"{name}{what} must be on cuda:{device}")
+        if not x.is_contiguous():
+            raise ValueError(f"{name}{what} must be contiguous")
+    if rec_res.numel() != rec_idx.numel():
+        raise ValueError(f"rec_res{what} has {rec_res.numel()} values for {rec_idx.numel()} receivers")
+
+
 class Solver:
     """One fold3d handle bound to a device."""
 
@@ -258,6 +315,7 @@ class Solver:
         h = C.c_void_p()
         check(lib().fold3d_create(C.byref(h), device))
         self.h = h
+        self.device = device
 
     def close(self):
         if self.h:
@@ -281,12 +339,19 @@ class Solver:
         o = Opts()
         lib().fold3d_default_opts(C.byref(o))
         o.max_iters, o.tol, o.engine = max_iters, tol, engine
+        count = padded_count(nx, ny, nz)
+        m = model if model is not None else reference_model()
         if isinstance(T, np.ndarray):
+            _check_host_array(T, count, "T")
+            if m.kind == MODEL_ELLIP and any(f is not None for f in getattr(m, "_refs", ())):
+                raise TypeError("per-node model fields need a device-resident T")
             o.memory = HOST
             o.precision = F64 if T.dtype == np.float64 else F32
             ptr = T.ctypes.data
         else:
             import torch
+            _check_tensor(T, count, "T", self.device)
+            _check_model_fields(m, count, self.device, T.dtype)
             o.memory = DEVICE
             o.precision = F64 if T.dtype == torch.float64 else F32
             ptr = T.data_ptr()
@@ -294,7 +359,6 @@ class Solver:
                 stream = torch.cuda.current_stream(T.device).cuda_stream
         o.stream = stream
         st = Stats()
-        m = model if model is not None else reference_model()
         rc = lib().fold3d_solve(self.h, C.c_void_p(ptr), nx, ny, nz, dx, dy, dz,
                                 C.byref(m), C.byref(o), C.byref(st))
         if rc == FOLD3D_E_NOT_CONVERGED and not raise_not_converged:
@@ -302,14 +366,21 @@ class Solver:
         check(rc)
         return st
 
-    @staticmethod
-    def _instances(workloads, Ts):
+    def _instances(self, workloads, Ts):
         count = len(workloads)
+        if count < 1 or len(Ts) != count:
+            raise ValueError(f"{count} workloads for {len(Ts)} fields")
         arr = (Instance * count)()
         for k, (W, T) in enumerate(zip(workloads, Ts)):
             x = arr[k]
-            x.T = T.data_ptr()
             nn = (W.n,) * 3 if np.isscalar(W.n) else W.n
+            cnt = padded_count(*map(int, nn))
+            _check_tensor(T, cnt, f"T of instance {k}", self.device, Ts[0].dtype)
+            for name in ("speed_field", "theta", "phi", "ra", "rb"):
+                f = getattr(W, name)
+                if f is not None:
+                    _check_tensor(f, cnt, f"{name} of instance {k}", self.device, T.dtype)
+            x.T = T.data_ptr()
             hh = (W.h,) * 3 if np.isscalar(W.h) else W.h
             for a in range(3):
                 x.n[a] = int(nn[a])
@@ -368,8 +439,13 @@ class Solver:
         import torch
         nx, ny, nz = (n, n, n) if np.isscalar(n) else n
         dx, dy, dz = (spacing,) * 3 if np.isscalar(spacing) else spacing
-        if rec_idx.dtype != torch.int32 or rec_res.dtype != T.dtype:
-            raise TypeError("rec_idx must be int32 and rec_res of T's dtype")
+        count = padded_count(nx, ny, nz)
+        _check_tensor(T, count, "T", self.device)
+        _check_model_fields(model, count, self.device, T.dtype)
+        _check_receivers(rec_idx, rec_res, self.device, T.dtype)
+        _check_tensor(grad_slowness, count, "grad_slowness", self.device, T.dtype)
+        if grad_theta is not None:
+            _check_tensor(grad_theta, count, "grad_theta", self.device, T.dtype)
         o = Opts()
         lib().fold3d_default_opts(C.byref(o))
         o.precision = F64 if T.dtype == torch.float64 else F32
@@ -397,10 +473,17 @@ class Solver:
         nx, ny, nz = (n, n, n) if np.isscalar(n) else n
         dx, dy, dz = (spacing,) * 3 if np.isscalar(spacing) else spacing
         idx = rec_idx if isinstance(rec_idx, (list, tuple)) else [rec_idx] * count
+        if count < 1 or len(idx) != count or len(rec_res) != count or len(tols) != count:
+            raise ValueError("Ts, rec_idx, rec_res and tols must have one entry per source")
+        cnt = padded_count(nx, ny, nz)
+        _check_model_fields(model, cnt, self.device, Ts[0].dtype)
+        _check_tensor(grad_slowness, cnt, "grad_slowness", self.device, Ts[0].dtype)
+        if grad_theta is not None:
+            _check_tensor(grad_theta, cnt, "grad_theta", self.device, Ts[0].dtype)
         arr = (AdjointSource * count)()
         for k in range(count):
-            if idx[k].dtype != torch.int32 or rec_res[k].dtype != Ts[k].dtype:
-                raise TypeError("rec_idx must be int32 and rec_res of T's dtype")
+            _check_tensor(Ts[k], cnt, f"T of source {k}", self.device, Ts[0].dtype)
+            _check_receivers(idx[k], rec_res[k], self.device, Ts[0].dtype, f" of source {k}")
             arr[k].T = Ts[k].data_ptr()
             arr[k].nrec = int(idx[k].numel())
             arr[k].rec_idx = idx[k].data_ptr()
@@ -431,6 +514,7 @@ class Solver:
         device into the torch tensor T (its dtype sets the precision)."""
         import torch
         nx, ny, nz = _dims(n)
+        _check_tensor(T, padded_count(nx, ny, nz), "T", self.device)
         dx, dy, dz = (spacing,) * 3 if np.isscalar(spacing) else spacing
         src = (C.c_double * 3)(*[float(v) for v in source])
         Mv = (C.c_double * 6)(*[float(v) for v in M])
@@ -449,8 +533,7 @@ class Solver:
         if spacing is None:
             spacing = reference_spacing(nx)
         dx, dy, dz = (spacing,) * 3 if np.isscalar(spacing) else spacing
-        if T.dtype != torch.float64 or not T.is_cuda:
-            raise TypeError("T must be a float64 device tensor")
+        _check_tensor(T, padded_count(nx, ny, nz), "T", self.device, torch.float64)
         if stream is None:
             stream = torch.cuda.current_stream(T.device).cuda_stream
         ne = C.c_longlong()
@@ -480,11 +563,13 @@ class Solver:
         if spacing is None:
             spacing = reference_spacing(nx)
         dx, dy, dz = (spacing,) * 3 if np.isscalar(spacing) else spacing
+        m = model if model is not None else reference_model()
+        _check_tensor(T, padded_count(nx, ny, nz), "T", self.device)
+        _check_model_fields(m, padded_count(nx, ny, nz), self.device, T.dtype)
         o = Opts()
         lib().fold3d_default_opts(C.byref(o))
         o.precision = F64 if T.dtype == torch.float64 else F32
         o.stream = stream if stream is not None else torch.cuda.current_stream(T.device).cuda_stream
-        m = model if model is not None else reference_model()
         ms = C.c_float()
         check(lib().fold3d_autotune(self.h, C.c_void_p(T.data_ptr()), nx, ny, nz, dx, dy, dz,
                                     C.byref(m), C.byref(o), iterations, C.byref(ms)))
@@ -492,9 +577,10 @@ class Solver:
 
     def plane_step(self, direction: int, T, n, plane: int, dzz, dxx, dyy, dxy,
                    dxz, dyz, F, ay, ax, az, stream=None):
+        import torch
         nx, ny, nz = (n, n, n) if np.isscalar(n) else n
+        _check_tensor(T, padded_count(nx, ny, nz), "T", self.device, torch.float64)
         if stream is None:
-            import torch
             stream = torch.cuda.current_stream(T.device).cuda_stream
         check(lib().fold3d_plane_step(self.h, direction, C.c_void_p(T.data_ptr()),
                                       nx, ny, nz, plane, dzz, dxx, dyy, dxy, dxz,

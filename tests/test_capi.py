@@ -128,6 +128,61 @@ def test_invalid_arguments_are_codes_not_crashes():
     assert L.fold3d_function_y_up(dummy, dummy, 16, 16, 16, 1, *([1.0] * 10), None) == fold3d.FOLD3D_E_INVALID
 
 
+def test_fold_model_needs_a_finite_wind_slower_than_F():
+    """F|grad T| + a.grad T = 1 is an eikonal equation only for |a| < F."""
+    dummy = C.c_void_p(0x1000)  # never dereferenced: validation fails first
+    for a in ((float("nan"), 0.0, 0.0), (0.0, float("inf"), 0.0), (1.1, 0.0, 0.0),
+              (0.8, 0.8, 0.0), (0.0, 0.0, -2.0)):
+        m = fold3d.fold_model(1.1, a)
+        assert _solve(dummy, dummy, model=m) == fold3d.FOLD3D_E_INVALID, a
+        assert "|a| < F" in fold3d.last_error()
+
+
+def test_huge_grid_is_nomem_before_any_allocation():
+    """A field too large for any device is FOLD3D_E_NOMEM from the size check,
+    with no allocation tried (and no 64-bit overflow in the size arithmetic)."""
+    L = fold3d.lib()
+    dummy = C.c_void_p(0x1000)
+    m = fold3d.reference_model()
+    o = fold3d.Opts()
+    L.fold3d_default_opts(C.byref(o))
+    st = fold3d.Stats()
+    for nx, ny, nz in ((2_000_000, 2_000_000, 2_000_000), (1 << 30, 1 << 30, 4)):
+        rc = L.fold3d_solve(dummy, dummy, nx, ny, nz, 0.1, 0.1, 0.1, C.byref(m), C.byref(o),
+                            C.byref(st))
+        assert rc == fold3d.FOLD3D_E_NOMEM
+        assert "does not fit" in fold3d.last_error()
+
+
+def test_batch_instances_are_checked_on_the_host_first():
+    """Non-finite or indefinite metrics and oversized instances are rejected
+    before any pointer is queried."""
+    L = fold3d.lib()
+    dummy = C.c_void_p(0x1000)
+    o = fold3d.Opts()
+    L.fold3d_default_opts(C.byref(o))
+    inst = (fold3d.Instance * 1)()
+    x = inst[0]
+    x.T = 0x1000
+    x.n[0] = x.n[1] = x.n[2] = 8
+    x.h[0] = x.h[1] = x.h[2] = 0.1
+    x.M[0] = x.M[1] = x.M[2] = 1.0
+    x.speed = 1.0
+
+    def call():
+        return L.fold3d_solve_batched(dummy, 1, inst, C.byref(o), None, None, None)
+
+    x.M[4] = float("nan")
+    assert call() == fold3d.FOLD3D_E_INVALID
+    assert "not finite positive definite" in fold3d.last_error()
+    x.M[4] = 0.0
+    x.n[0] = x.n[1] = x.n[2] = 3_000_000
+    assert call() == fold3d.FOLD3D_E_NOMEM
+    x.n[0] = x.n[1] = x.n[2] = 8
+    x.speed = float("nan")
+    assert call() == fold3d.FOLD3D_E_INVALID
+
+
 def test_point_source_arguments_are_checked():
     L = fold3d.lib()
     dummy = C.c_void_p(0x1000)  # never dereferenced: validation fails first
@@ -208,3 +263,33 @@ def test_defaults_follow_the_reference():
     assert (o.precision, o.memory, o.max_iters, o.tol) == (fold3d.F64, fold3d.DEVICE, 8, 0.0)
     m = fold3d.reference_model()
     assert (m.F, tuple(m.a)) == (1.1, (-0.35, 0.4, 0.7))
+
+
+def test_python_layer_checks_host_arrays_and_tensors_without_a_gpu():
+    """The wrapper checks what the C-ABI cannot (it sees raw pointers): size,
+    contiguity and dtype of every array, before any library call."""
+    import torch
+    s = fold3d.Solver.__new__(fold3d.Solver)  # no handle: nothing may reach the library
+    s.h, s.device = None, 0
+    n = 6
+    cnt = (n + 2) ** 3
+    with pytest.raises(ValueError):
+        s.solve(np.zeros(cnt - 1), n)
+    with pytest.raises(ValueError):
+        s.solve(np.zeros(2 * cnt)[::2], n)
+    with pytest.raises(TypeError):
+        s.solve(np.zeros(cnt, dtype=np.float16), n)
+    with pytest.raises(TypeError):
+        s.solve(np.zeros(cnt, dtype=np.int64), n)
+    with pytest.raises(TypeError):
+        s.solve(torch.zeros(cnt, dtype=torch.bfloat16), n)
+    with pytest.raises(ValueError):
+        s.solve(torch.zeros(cnt + 1), n)
+    with pytest.raises(ValueError):
+        s.solve(torch.zeros(cnt, 2)[:, 0], n)
+    with pytest.raises(TypeError):  # right size and dtype, but not on a GPU
+        s.solve(torch.zeros(cnt, dtype=torch.float64), n)
+    with pytest.raises(TypeError):
+        s.initialiser_device(torch.zeros(cnt, dtype=torch.float32), n)
+    with pytest.raises(ValueError):
+        s.point_source(torch.zeros(cnt - 3), n, 0.1, (0.3, 0.3, 0.3), (1, 1, 1, 0, 0, 0), 1.0)

@@ -350,9 +350,10 @@ def test_tiny_grids_match_oracle(solver, oracle, n3, engine):
 
 
 def test_host_pointers_are_rejected_not_dereferenced(solver):
-    """A plain host pointer where a device array is required is an error code,
-    not a kernel fault: solve with memory = DEVICE, a batch instance, an
-    adjoint receiver array."""
+    """A plain host pointer where a device array is required is an error code
+    from the C-ABI, not a kernel fault: solve with memory = DEVICE, a batch
+    instance, an adjoint receiver array (called through the C-ABI directly;
+    the Python layer would stop these earlier)."""
     import ctypes as C
     from fold3d import workloads as wl
     n = 8
@@ -366,14 +367,60 @@ def test_host_pointers_are_rejected_not_dereferenced(solver):
                         C.byref(m), C.byref(o), C.byref(st))
     assert rc == fold3d.FOLD3D_E_INVALID and "device memory" in fold3d.last_error()
     W = wl.tilted_homogeneous(n)
-    Wd = wl.Workload(W.name, n, W.h, W.M, W.speed)
-    Wd.speed_field = torch.ones((n + 2) ** 3, dtype=torch.float64)  # a CPU tensor
     T = torch.zeros((n + 2) ** 3, dtype=torch.float64, device="cuda")
-    with pytest.raises(fold3d.Fold3dError) as e:
-        solver.solve_batched([Wd], [T])
-    assert e.value.code == fold3d.FOLD3D_E_INVALID
+    inst = (fold3d.Instance * 1)()
+    x = inst[0]
+    x.T = T.data_ptr()
+    for a in range(3):
+        x.n[a], x.h[a] = n, W.h
+    for a in range(6):
+        x.M[a] = float(W.M[a])
+    x.speed = 1.0
+    x.speed_field = host.ctypes.data  # host memory
+    assert L.fold3d_solve_batched(solver.h, 1, inst, C.byref(o), None, None, None) \
+        == fold3d.FOLD3D_E_INVALID
+    assert "device memory" in fold3d.last_error()
     me = W.model()
-    with pytest.raises(fold3d.Fold3dError) as e:
-        solver.adjoint(T, n, W.h, me, torch.zeros(4, dtype=torch.int32),
-                       torch.zeros(4, dtype=torch.float64), torch.zeros_like(T))
-    assert e.value.code == fold3d.FOLD3D_E_INVALID
+    idx = np.zeros(4, dtype=np.int32)
+    res = np.zeros(4)
+    g = torch.zeros_like(T)
+    rc = L.fold3d_adjoint(solver.h, C.c_void_p(T.data_ptr()), n, n, n, W.h, W.h, W.h,
+                          C.byref(me), 4, C.c_void_p(idx.ctypes.data),
+                          C.c_void_p(res.ctypes.data), C.byref(o), C.c_void_p(g.data_ptr()),
+                          None, C.byref(st))
+    assert rc == fold3d.FOLD3D_E_INVALID
+
+
+def test_python_layer_rejects_bad_device_tensors(solver):
+    """Wrong size, layout, dtype or device of a device tensor is a TypeError /
+    ValueError in the Python layer, before the C-ABI sees a raw pointer."""
+    from fold3d import workloads as wl
+    n = 8
+    cnt = (n + 2) ** 3
+    ok = torch.zeros(cnt, dtype=torch.float64, device="cuda")
+    with pytest.raises(ValueError):
+        solver.solve(torch.zeros(cnt - 1, dtype=torch.float64, device="cuda"), n)
+    with pytest.raises(ValueError):
+        solver.solve(torch.zeros(2 * cnt, dtype=torch.float64, device="cuda")[::2], n)
+    with pytest.raises(TypeError):
+        solver.solve(torch.zeros(cnt, dtype=torch.float16, device="cuda"), n)
+    with pytest.raises(TypeError):
+        solver.solve(torch.zeros(cnt, dtype=torch.float64), n)  # host tensor
+    W = wl.tilted_homogeneous(n)
+    Wd = wl.Workload(W.name, n, W.h, W.M, W.speed)
+    Wd.speed_field = torch.ones(cnt, dtype=torch.float32, device="cuda")  # dtype differs from T
+    with pytest.raises(TypeError):
+        solver.solve(ok, n, spacing=W.h, model=Wd.model())
+    with pytest.raises(TypeError):
+        solver.solve_batched([Wd], [ok])
+    me = W.model()
+    idx = torch.zeros(4, dtype=torch.int32, device="cuda")
+    with pytest.raises(TypeError):
+        solver.adjoint(ok, n, W.h, me, idx, torch.zeros(4, dtype=torch.float32, device="cuda"),
+                       torch.zeros_like(ok))
+    with pytest.raises(ValueError):
+        solver.adjoint(ok, n, W.h, me, idx, torch.zeros(3, dtype=torch.float64, device="cuda"),
+                       torch.zeros_like(ok))
+    with pytest.raises(ValueError):
+        solver.adjoint(ok, n, W.h, me, idx, torch.zeros(4, dtype=torch.float64, device="cuda"),
+                       torch.zeros(cnt + 1, dtype=torch.float64, device="cuda"))

@@ -15,7 +15,9 @@
 //     q is within one ulp of x/d, q + (x - q d) * RN(1/d) rounds to RN(x/d)),
 //     i.e. the same bits as the IEEE division.
 // Everything else keeps the reference's operation order, so results stay
-// bit-identical to fold_local.h (and to the reference) in FP64.
+// bit-identical to fold_local.h: to the reference in FP64, and to the literal
+// kernel run in FP32 (the constants are then computed in float, with the
+// same expressions, and the square root is the correctly rounded one).
 //
 // One more early exit, beyond the reference's: every candidate of a simplex
 // in the upwind plane is its foot point's interpolated value (>= the
@@ -62,8 +64,10 @@ template <typename R>
 __device__ __forceinline__ R fast_sqrt(R x);
 template <>
 __device__ __forceinline__ double fast_sqrt<double>(double x) { return sqrt_rn(x); }
+// FP32: the correctly rounded square root, as sqrt() of the literal kernel
+// (fold_local.h) compiles to, so both give the same bits.
 template <>
-__device__ __forceinline__ float fast_sqrt<float>(float x) { return __builtin_amdgcn_sqrtf(x); }
+__device__ __forceinline__ float fast_sqrt<float>(float x) { return __builtin_sqrtf(x); }
 
 // One edge call site (FoldEdge2Points with fixed model/spacing arguments).
 template <typename R>
@@ -241,8 +245,11 @@ __device__ __forceinline__ void fold_unit(R &tnew, const Patch<R> &p, const Fold
 }
 
 // ---------------------------------------------------------------------------
-// Host-side construction (plain double arithmetic, same expressions as the
-// reference, so the hoisted values carry the bits the node loop would compute).
+// Host-side construction, in the arithmetic of R (IEEE, no contraction) with
+// the expressions of fold_local.h, so the hoisted values carry the bits the
+// node loop of the literal kernel computes in the same precision.  Only the
+// pruning bounds (gap, lgap) are computed in double: they are conservative
+// bounds, not results.
 
 // Least travel time from the x axis to (0, 0, dzz) in an edge site's frame:
 // dzz max_phi sin(phi) / (F + |ay| cos(phi) + az sin(phi)); the derivative
@@ -252,141 +259,95 @@ inline double edge_line_bound(double F, double ay, double az, double dzz) {
   return dzz / (std::sqrt(F * F - ay * ay) + az);
 }
 
-inline void host_edge_site(double F, double ax, double ay, double az, double dxx,
-                           double dzz, EdgeSite<double> &e) {
+// Pruning bounds are shrunk by this factor: 99.9% in FP64, 90% in FP32.
+template <typename R>
+inline double prune_shrink() { return sizeof(R) == 8 ? 0.999 : 0.9; }
+
+template <typename R>
+inline void host_edge_site(R F, R ax, R ay, R az, R dxx, R dzz, EdgeSite<R> &e) {
   e.ax = ax;
   e.az = az;
   e.dxx = dxx;
   e.dzz = dzz;
   e.ga = F * F - ay * ay;
-  e.rga = 1.0 / e.ga;
+  e.rga = R(1) / e.ga;
   e.gaz = e.ga - az * az;
-  e.rgaz = 1.0 / e.gaz;
-  e.rdxx = 1.0 / dxx;
-  e.lgap = 0.999 * edge_line_bound(F, ay, az, dzz);
-  e.ax_pos = ax > 0.0;
-  double c = (-dxx * ax + dzz * az);
-  const double len2 = dxx * dxx + dzz * dzz;
-  const double d = (F * F - ax * ax - ay * ay - az * az) * len2 + c * c;
+  e.rgaz = R(1) / e.gaz;
+  e.rdxx = R(1) / dxx;
+  e.lgap = (R)(prune_shrink<R>() * edge_line_bound(F, ay, az, dzz));
+  e.ax_pos = ax > R(0);
+  R c = (-dxx * ax + dzz * az);
+  const R len2 = dxx * dxx + dzz * dzz;
+  const R d = (F * F - ax * ax - ay * ay - az * az) * len2 + c * c;
   e.k1_ok = 0;
-  e.k1 = 0.0;
-  if (d > 0.0) {
+  e.k1 = R(0);
+  if (d > R(0)) {
     c = c + std::sqrt(d);
-    if (c > 0.0) {
+    if (c > R(0)) {
       e.k1_ok = 1;
       e.k1 = len2 / c;
     }
   }
 }
 
-// FoldFrame f holds (F, ax, ay, az, dzz, dxx, dyy, dxy, dxz, dyz) of the direction.
-inline void host_fold_sign(const FoldFrame<double> &f, double sgn, FoldSign<double> &S) {
-  const double F = f.F;
+// FoldFrame f holds (F, ax, ay, az, dzz, dxx, dyy, dxy, dxz, dyz) of the
+// direction in R, as the literal kernel receives it.
+template <typename R>
+inline void host_fold_sign(const FoldFrame<R> &f, R sgn, FoldSign<R> &S) {
+  const R F = f.F;
   // a flips on the negative side exactly as `a *= -1` does
-  const double ax = f.ax * sgn, ay = f.ay * sgn, az = f.az * sgn;
-  const double dzz = f.dzz, dxx = f.dxx, dyy = f.dyy, dxy = f.dxy, dxz = f.dxz, dyz = f.dyz;
+  const R ax = f.ax * sgn, ay = f.ay * sgn, az = f.az * sgn;
+  const R dzz = f.dzz, dxx = f.dxx, dyy = f.dyy, dxy = f.dxy, dxz = f.dxz, dyz = f.dyz;
   S.F = F;
   S.az = az;
   S.dzz = dzz;
-  S.gap = 0.999 * dzz / (F + az);  // F > |a|, so F + az > 0
+  S.gap = (R)(prune_shrink<R>() * (double)dzz / ((double)F + (double)az));  // F > |a|
   S.ga = F * F - az * az;
-  S.rga = 1.0 / S.ga;
-  double d = (F * F - (ax * ax + ay * ay)) * dzz * dzz;
+  S.rga = R(1) / S.ga;
+  R d = (F * F - (ax * ax + ay * ay)) * dzz * dzz;
   S.k0_ok = 0;
-  S.k0 = 0.0;
-  if (d > 0.0) {
+  S.k0 = R(0);
+  if (d > R(0)) {
     d = az * dzz + std::sqrt(d);
-    if (d > 0.0) {
+    if (d > R(0)) {
       S.k0_ok = 1;
       S.k0 = dzz * dzz / d;
     }
   }
   // triangles (AN_ISO_1_OpenACC.c:233-240): (ax_t, ay_t, dxx_t, dyy_t, dxz_t)
-  const double tp[8][5] = {
+  const R tp[8][5] = {
       {ax, ay, dxx, dyy, dxz},   {ax, -ay, dxx, dyy, dxz},  {-ax, -ay, dxx, dyy, dxz},
       {-ax, ay, dxx, dyy, dxz},  {-ay, -ax, dyy, dxx, dyz}, {-ay, ax, dyy, dxx, dyz},
       {ay, ax, dyy, dxx, dyz},   {ay, -ax, dyy, dxx, dyz}};
   for (int k = 0; k < 8; ++k) {
-    TriSite<double> &t = S.tri[k];
+    TriSite<R> &t = S.tri[k];
     t.ax = tp[k][0];
     t.ay = tp[k][1];
     t.dxx = tp[k][2];
     t.dyy = tp[k][3];
-    t.rdxx = 1.0 / t.dxx;
-    t.rdyy = 1.0 / t.dyy;
-    t.ax_pos = t.ax > 0.0;
-    t.ay_pos = t.ay > 0.0;
-    t.sub_pos = t.ax * dzz + az * t.dxx > 0.0;
-    const double txz = tp[k][4];
+    t.rdxx = R(1) / t.dxx;
+    t.rdyy = R(1) / t.dyy;
+    t.ax_pos = t.ax > R(0);
+    t.ay_pos = t.ay > R(0);
+    t.sub_pos = t.ax * dzz + az * t.dxx > R(0);
+    const R txz = tp[k][4];
     // sub-edge xt -> yt in the rotated frame (:170-174)
-    host_edge_site(F, t.ay, (t.ax * dzz + az * t.dxx) / txz, (-t.ax * t.dxx + az * dzz) / txz,
-                   t.dyy, txz, S.edge[8 + k]);
+    host_edge_site<R>(F, t.ay, (t.ax * dzz + az * t.dxx) / txz, (-t.ax * t.dxx + az * dzz) / txz,
+                      t.dyy, txz, S.edge[8 + k]);
   }
   // the node's own edges (:242-253), order px, pxpy, py, mxpy, mx, mxmy, my, pxmy
-  host_edge_site(F, ax, ay, az, dxx, dzz, S.edge[0]);
-  host_edge_site(F, (ax * dxx + ay * dyy) / dxy, (-ax * dyy + ay * dxx) / dxy, az, dxy, dzz,
-                 S.edge[1]);
-  host_edge_site(F, ay, ax, az, dyy, dzz, S.edge[2]);
-  host_edge_site(F, -(ax * dxx - ay * dyy) / dxy, (ax * dyy + ay * dxx) / dxy, az, dxy, dzz,
-                 S.edge[3]);
-  host_edge_site(F, -ax, ay, az, dxx, dzz, S.edge[4]);
-  host_edge_site(F, -(ax * dxx + ay * dyy) / dxy, (-ax * dyy + ay * dxx) / dxy, az, dxy, dzz,
-                 S.edge[5]);
-  host_edge_site(F, -ay, ax, az, dyy, dzz, S.edge[6]);
-  host_edge_site(F, (ax * dxx - ay * dyy) / dxy, (ax * dyy + ay * dxx) / dxy, az, dxy, dzz,
-                 S.edge[7]);
-}
-
-template <typename R>
-inline void narrow(const FoldFast<double> &in, FoldFast<R> &out);
-
-template <>
-inline void narrow<double>(const FoldFast<double> &in, FoldFast<double> &out) { out = in; }
-
-template <>
-inline void narrow<float>(const FoldFast<double> &in, FoldFast<float> &out) {
-  for (int s = 0; s < 2; ++s) {
-    const FoldSign<double> &a = in.sgn[s];
-    FoldSign<float> &b = out.sgn[s];
-    b.F = (float)a.F;
-    b.az = (float)a.az;
-    b.dzz = (float)a.dzz;
-    b.gap = (float)(0.9 * a.dzz / (a.F + a.az));
-    b.ga = (float)a.ga;
-    b.rga = 1.0f / b.ga;
-    b.k0 = (float)a.k0;
-    b.k0_ok = a.k0_ok;
-    for (int k = 0; k < 8; ++k) {
-      const TriSite<double> &x = a.tri[k];
-      TriSite<float> &y = b.tri[k];
-      y.ax = (float)x.ax;
-      y.ay = (float)x.ay;
-      y.dxx = (float)x.dxx;
-      y.dyy = (float)x.dyy;
-      y.rdxx = 1.0f / y.dxx;
-      y.rdyy = 1.0f / y.dyy;
-      y.ax_pos = x.ax_pos;
-      y.ay_pos = x.ay_pos;
-      y.sub_pos = x.sub_pos;
-    }
-    for (int k = 0; k < 16; ++k) {
-      const EdgeSite<double> &x = a.edge[k];
-      EdgeSite<float> &y = b.edge[k];
-      y.ax = (float)x.ax;
-      y.az = (float)x.az;
-      y.dxx = (float)x.dxx;
-      y.dzz = (float)x.dzz;
-      y.ga = (float)x.ga;
-      y.rga = 1.0f / y.ga;
-      y.gaz = (float)x.gaz;
-      y.rgaz = 1.0f / y.gaz;
-      y.rdxx = 1.0f / y.dxx;
-      y.k1 = (float)x.k1;
-      y.lgap = (float)(x.lgap / 0.999 * 0.9);
-      y.ax_pos = x.ax_pos;
-      y.k1_ok = x.k1_ok;
-    }
-  }
+  host_edge_site<R>(F, ax, ay, az, dxx, dzz, S.edge[0]);
+  host_edge_site<R>(F, (ax * dxx + ay * dyy) / dxy, (-ax * dyy + ay * dxx) / dxy, az, dxy, dzz,
+                    S.edge[1]);
+  host_edge_site<R>(F, ay, ax, az, dyy, dzz, S.edge[2]);
+  host_edge_site<R>(F, -(ax * dxx - ay * dyy) / dxy, (ax * dyy + ay * dxx) / dxy, az, dxy, dzz,
+                    S.edge[3]);
+  host_edge_site<R>(F, -ax, ay, az, dxx, dzz, S.edge[4]);
+  host_edge_site<R>(F, -(ax * dxx + ay * dyy) / dxy, (-ax * dyy + ay * dxx) / dxy, az, dxy, dzz,
+                    S.edge[5]);
+  host_edge_site<R>(F, -ay, ax, az, dyy, dzz, S.edge[6]);
+  host_edge_site<R>(F, (ax * dxx - ay * dyy) / dxy, (ax * dyy + ay * dxx) / dxy, az, dxy, dzz,
+                    S.edge[7]);
 }
 
 }  // namespace fold3d

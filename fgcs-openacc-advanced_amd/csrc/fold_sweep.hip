@@ -235,11 +235,10 @@ __global__ __launch_bounds__(kLaneX *kRowsY) void fold_plane_fast_kernel(
 
 template <typename R>
 void build_fold_fast(int dir, const fold3d_model &m, const double h[3], FoldFast<R> &out) {
-  const FoldFrame<double> f = fold_frame<double>(dir, m.F, m.a, h);
-  FoldFast<double> d;
-  host_fold_sign(f, 1.0, d.sgn[0]);
-  host_fold_sign(f, -1.0, d.sgn[1]);
-  narrow<R>(d, out);
+  // the frame in R, as the literal kernel receives it, then everything in R
+  const FoldFrame<R> f = fold_frame<R>(dir, m.F, m.a, h);
+  host_fold_sign<R>(f, R(1), out.sgn[0]);
+  host_fold_sign<R>(f, R(-1), out.sgn[1]);
 }
 template void build_fold_fast<double>(int, const fold3d_model &, const double *, FoldFast<double> &);
 template void build_fold_fast<float>(int, const fold3d_model &, const double *, FoldFast<float> &);

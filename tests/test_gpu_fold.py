@@ -264,15 +264,24 @@ def test_deterministic(solver):
     assert torch.equal(A.view(torch.int64), B.view(torch.int64))
 
 
-@pytest.mark.parametrize("n", [32, 64])
-def test_fp32_within_tolerance_of_fp64(solver, oracle, n):
-    """FP32 field vs the FP64 oracle on the same input, relative error over
-    reached nodes.  The reference's acceptance tests are discontinuous (its
-    gate ignores the -x^/-y^ neighbours; candidates are accepted on strict
-    foot-point bounds), so where FP32 rounding flips one of them a node takes
-    another candidate, and the difference is carried downstream.  Stated
-    bound: signs agree everywhere, at most 1% of reached nodes exceed 1e-5
-    relative (0.4% at 64^3), none exceeds 1e-2."""
+# The FP32 bound, stated for the reference workload (8 iterations) up to 512^3,
+# over reached nodes, relative to the FP64 field (which equals the reference's
+# bits): signs and the reached set identical; max < 1e-2; at most 0.5 % of
+# nodes above 1e-3, 10 % above 1e-4, 25 % above 1e-5.  Measured at 512^3:
+# max 4.8e-3, 0.25 %, 8.4 %, 19.2 %; at 128^3: 4.1e-3, 0.08 %, 2.6 %, 6.2 %.
+FP32_BOUND = {"max": 1e-2, 1e-3: 0.005, 1e-4: 0.10, 1e-5: 0.25}
+
+
+@pytest.mark.parametrize("n", [32, 64, 128])
+def test_fp32_within_stated_bound_of_fp64(solver, oracle, n):
+    """FP32 field vs the FP64 oracle on the same input.  The FP32 engines give
+    the literal kernel's FP32 bits (test_fp32_engines_equal_the_fp32_literal_kernel),
+    so this is the reference's arithmetic run in float.  Its acceptance tests
+    are discontinuous (the gate ignores the -x^/-y^ neighbours; candidates are
+    accepted on strict foot-point bounds): where FP32 rounding flips one, a
+    node takes another candidate and the difference is carried downstream, so
+    the 1e-5 target of the elliptic model does not hold here for every node."""
+    print(f"FP32 vs the FP64 oracle at {n}^3", flush=True)
     T0 = fold3d.implicit_initialiser(n)
     want = T0.copy()
     oracle.sweeps(want, n, 8)
@@ -281,11 +290,55 @@ def test_fp32_within_tolerance_of_fp64(solver, oracle, n):
     got = T.cpu().double().numpy()
     assert np.array_equal(np.sign(got), np.sign(want))
     reached = np.abs(want) < 1e15
+    assert np.array_equal(reached, np.abs(got) < 1e15)
     rel = np.abs(got[reached] - want[reached]) / np.abs(want[reached])
-    frac = np.mean(rel > 1e-5)
+    fr = {t: float(np.mean(rel > t)) for t in (1e-3, 1e-4, 1e-5)}
     print(f"FP32 n={n}: max rel err {rel.max():.3e}, median {np.median(rel):.3e}, "
-          f"fraction above 1e-5 {frac:.2e}")
-    assert frac < 1e-2 and rel.max() < 1e-2
+          f"above 1e-3/1e-4/1e-5: {fr[1e-3]:.2e} {fr[1e-4]:.2e} {fr[1e-5]:.2e}")
+    assert rel.max() < FP32_BOUND["max"]
+    for t in (1e-3, 1e-4, 1e-5):
+        assert fr[t] <= FP32_BOUND[t], (t, fr[t])
+
+
+FAST_ENGINES = [fold3d.ENGINE_HOISTED, fold3d.ENGINE_TILED, fold3d.ENGINE_SPLIT,
+                fold3d.ENGINE_LOCKED]
+
+
+@pytest.mark.parametrize("groups", [2, 4, 8])
+@pytest.mark.parametrize("engine", FAST_ENGINES)
+def test_fp32_engines_equal_the_fp32_literal_kernel(solver, engine, groups):
+    """FP32: hoisted constants (computed in float with the literal kernel's
+    expressions), pruning by the plane and edge-line bounds (shrunk to 90 %),
+    LDS tiles, units split over waves and locking change no bit against the
+    literal FP32 kernel (fold_local.h run in float, which prunes nothing).
+    On the reference workload at 64^3 (8 iterations and to the fixed point),
+    and on a box grid with a random signed field and another model."""
+    if groups != 2 and engine not in (fold3d.ENGINE_SPLIT, fold3d.ENGINE_LOCKED):
+        pytest.skip("waves per row apply to the split and locked engines")
+    print(f"FP32 engine {engine}, {groups} waves per row, vs the literal FP32 kernel", flush=True)
+    solver.set_tuning(groups, fold3d.MODEL_FOLD)
+    try:
+        n = 64
+        T0 = to_dev(fold3d.implicit_initialiser(n), torch.float32)
+        for iters, tol in ((8, 0.0), (200, 1e-30)):
+            A, B = T0.clone(), T0.clone()
+            sa = solver.solve(A, n, max_iters=iters, tol=tol, engine=fold3d.ENGINE_PLANE)
+            sb = solver.solve(B, n, max_iters=iters, tol=tol, engine=engine)
+            assert sa.iterations == sb.iterations
+            assert torch.equal(A.view(torch.int32), B.view(torch.int32)), (iters, tol)
+        n3 = (45, 70, 29)
+        rng = np.random.default_rng(groups + 10 * engine)
+        size = (n3[0] + 2) * (n3[1] + 2) * (n3[2] + 2)
+        R0 = np.where(rng.random(size) < 0.3, -1.0, 1.0) * rng.uniform(1.0, 5.0, size)
+        R0[rng.random(size) < 0.05] = fold3d.MAXT
+        h, a = (0.31, 0.27, 0.35), (0.25, -0.4, 0.3)
+        m = fold3d.fold_model(1.2, a)
+        A, B = to_dev(R0, torch.float32), to_dev(R0, torch.float32)
+        solver.solve(A, n3, spacing=h, model=m, max_iters=3, engine=fold3d.ENGINE_PLANE)
+        solver.solve(B, n3, spacing=h, model=m, max_iters=3, engine=engine)
+        assert torch.equal(A.view(torch.int32), B.view(torch.int32))
+    finally:
+        solver.set_tuning(0, fold3d.MODEL_FOLD)
 
 
 @pytest.mark.parametrize("groups", [2, 4, 8])

@@ -44,6 +44,30 @@ PMC_VALU_CLOCK_GHZ = 1.955
 # FP32 fold solve vs FP64 on the reference workload (8 iterations, up to 512^3)
 FP32_BOUND_TEXT = ("signs and reached set equal; relative error < 1e-2 everywhere, above 1e-3 "
                    "at <= 0.5% of reached nodes, above 1e-4 at <= 10%, above 1e-5 at <= 25%")
+# HBM traffic per launch of the other rows' dominant kernels, from rocprofv3
+# --pmc passes (FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 KiB, the factors of the
+# copy-kernel calibration above), keyed by row; None until profiled.  Each
+# entry: (bytes per launch, the profile file that holds the passes).
+PMC_ROW_TRAFFIC = {}
+
+
+def roofline_row(updates, bytes_per_update, sweep_ms, launches, nodes_per_launch, kernel,
+                 row=None):
+    """HBM roofline of one row: algorithmic bytes (bytes_per_update per
+    cell-update, the read and write of the target node plus the per-node
+    parameters the workload names) over the device time of the row's sweeps
+    (HIP events around them, so layout moves and launch gaps are included: a
+    lower bound on the kernel's own rate), against the 8 TB/s peak."""
+    achieved = updates * bytes_per_update / (sweep_ms / 1e3) / 1e9
+    out = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+           "frac": achieved / HBM_PEAK_GBS, "kernel": kernel, "bytes_per_update": bytes_per_update,
+           "bytes_per_launch": bytes_per_update * nodes_per_launch,
+           "kernel_us": sweep_ms * 1e3 / launches if launches else None, "traffic": None}
+    if row in PMC_ROW_TRAFFIC:
+        out["traffic"], out["traffic_source"] = PMC_ROW_TRAFFIC[row]
+    return out
+
+
 A100_V2_512_GCUPS = 8 * 6 * 511 * 512 ** 2 / 3.038 / 1e9  # plots/perfect-2-gpu.py:20,24
 
 
@@ -90,9 +114,13 @@ def ellip_run(solver, n, dtype, tol, torch, threads=0):
     torch.cuda.synchronize()
     wall = (time.perf_counter() - t0) * 1e3
     W_upd = st.cell_updates
+    r = 8 if dtype == torch.float64 else 4
     out = {"iterations": st.iterations, "max_dT": st.max_dT, "sweep_ms": st.sweep_ms,
            "wall_ms": wall, "value": W_upd / (st.sweep_ms / 1e3) / 1e9,
-           "unit": "Gcell-updates/s", "cell_updates": W_upd, "split_groups": groups}
+           "unit": "Gcell-updates/s", "cell_updates": W_upd, "split_groups": groups,
+           "roofline": roofline_row(W_upd, 3 * r, st.sweep_ms, st.iterations * 6 * (n - 1), n * n,
+                                    f"ellip_split_kernel<{'double' if r == 8 else 'float'}>",
+                                    f"blob_speed_{'f64' if r == 8 else 'f32'}")}
     if threads and dtype == torch.float64:
         # bounded sample: the first iteration, on the host (elliptic oracle)
         # and on the GPU from the same start field
@@ -173,9 +201,24 @@ def batch_run(solver, torch, count=96, check=6):
             "looped_solves_per_s": count / loop_s, "speedup": loop_s / batch_s,
             "iterations_min": min(its), "iterations_max": max(its),
             "batched_cell_updates": upd, "batched_gcups": upd / batch_s / 1e9,
+            # 7 x 4 B per cell-update (T read and write; v, theta, phi, ra, rb),
+            # over the batch's device time; a launch serves every instance
+            "roofline": roofline_row(upd, 28, dev_ms, batch_launches(sides, its),
+                                     sum(n * n for n in sides) / len(sides),
+                                     "batch_plane_kernel<float>", "batch"),
             "batch_tuning": {"waves_per_row": tuned[0], "blocks_per_cu": tuned[1],
                              "rows_per_pull": tuned[2]},
             "fp32_max_rel_err_vs_fp64_on_smallest": {"instances": check, "value": worst}}
+
+
+def batch_launches(sides, its):
+    """Plane launches of a batch: one per (iteration, direction, plane step)
+    while any instance still has that step (instances are sorted by size)."""
+    total = 0
+    for it in range(max(its)):
+        live = [n for n, k in zip(sides, its) if k > it]
+        total += 6 * (max(live) - 1)
+    return total
 
 
 def tilted_homogeneous_run(solver, torch, n=128, threads=0):
@@ -268,6 +311,10 @@ def tilted_field_run(torch, n=1280):
                "sweep_ms": st.sweep_ms, "wall_s": wall, "groups": groups,
                "value": st.cell_updates / (st.sweep_ms / 1e3) / 1e9, "unit": "Gcell-updates/s",
                "cell_updates": st.cell_updates,
+               # 7 x 4 B: T read and write, v, theta, phi, ra, rb per node
+               "roofline": roofline_row(st.cell_updates, 28, st.sweep_ms,
+                                        st.iterations * 6 * (n - 1), n * n,
+                                        "ellip_metric_kernel<float>", "tilted_field"),
                "torch_device_bytes": torch.cuda.max_memory_allocated(),
                "solver_scratch_bytes_max": 22 * (n + 2) ** 3 * 4}
         del W, T0, T
@@ -319,9 +366,128 @@ def adjoint_run(solver, torch, n=256):
            "adjoint_iterations": [min(st["adjoint_iters"]), max(st["adjoint_iters"])],
            "forward_iterations": [min(st["forward_iters"]), max(st["forward_iters"])],
            "checkpoint_bytes": st["checkpoint_bytes"], "recomputed": st["recomputed"],
+           "nodes_held_per_source": list(solver.last_held),
+           # adjoint plane steps: per node visit at least the in-edge mask (8 B),
+           # the residual and the new value (2 x 4 B); the in-edges' weights and
+           # values gathered on top are not counted, so this is a lower bound
+           "roofline": roofline_row(nsrc * 6 * n ** 3 * float(np.mean(st["adjoint_iters"])), 16,
+                                    adj_ms, 6 * n * max(st["adjoint_iters"]), nsrc * n * n,
+                                    "adjoint_plane_kernel<float>", "adjoint"),
            "grad_norm_slowness": float(s0.grad.norm()), "grad_norm_theta": float(t0_.grad.norm())}
     del tomo, f, s0, t0_
     torch.cuda.empty_cache()
+    return out
+
+
+def _seeded_cells(g, srcs, H, count, seed):
+    """`count` seeded cells whose gradient is above 1e-3 of the largest, at
+    least 3 cells (max norm) from every source."""
+    idx = np.flatnonzero(np.abs(g) > 1e-3 * np.abs(g).max())
+    i, j, k = idx % H, (idx // H) % H, idx // (H * H)
+    far = np.ones(idx.size, bool)
+    for s_ in srcs:
+        si, sj, sk = s_ % H, (s_ // H) % H, s_ // (H * H)
+        far &= np.maximum(np.maximum(abs(i - si), abs(j - sj)), abs(k - sk)) >= 3
+    return np.random.default_rng(seed).choice(idx[far], count, replace=False)
+
+
+def adjoint_fd_run(solver, torch, n=256, cells=32, threads=0):
+    """The tomography gradient at its full configuration (256^3, 16 sources on
+    a 4x4 grid, 64x64 top-face receivers, per-node speed and tilt), FP64,
+    against central finite differences of the forward solve (relative step
+    1e-3, and its Richardson combination with step 5e-4) at `cells` seeded
+    cells per parameter; and, at 64^3 with the same survey layout, against the
+    FP64 CPU adjoint (the oracle) on the same converged fields."""
+    from fold3d.tomography import Tomography, tomography_fields, top_face_survey
+    out = {"n": n, "dtype": "f64", "sources": 16, "receivers": 64 * 64, "cells_per_parameter": cells,
+           "relative_steps": [1e-3, 5e-4]}
+    t_start = time.perf_counter()
+    f = tomography_fields(n, xp=torch, dtype=torch.float64, device="cuda")
+    ft = tomography_fields(n, xp=torch, dtype=torch.float64, device="cuda", anomalies=True)
+    S = top_face_survey(n, 4, 64)
+    tomo = Tomography(solver, S, f["phi"], f["ra"], f["rb"], batch=16)
+    tomo.observe(ft["slowness"], ft["theta"])
+    del ft
+    s0 = f["slowness"].clone().requires_grad_(True)
+    t0 = f["theta"].clone().requires_grad_(True)
+    J0 = tomo.misfit(s0, t0)
+    J0.backward()
+    out["misfit"] = float(J0.detach())
+    out["nodes_held_per_source"] = list(solver.last_held)
+    H = n + 2
+    srcs = []
+    for src in S.sources:
+        i, j, k = [min(n, max(1, int(round(c / S.h)) + 1)) for c in src]
+        srcs.append(i + H * (j + H * k))
+
+    def J_of(name, c, delta):
+        s_, t_ = f["slowness"].clone(), f["theta"].clone()
+        (s_ if name == "slowness" else t_)[c] += delta
+        with torch.no_grad():
+            return float(tomo.forward_pass(s_, t_)[0])
+
+    for name, grad in (("slowness", s0.grad), ("theta", t0.grad)):
+        g = grad.cpu().numpy()
+        sel = _seeded_cells(g, srcs, H, cells, seed=20261209)
+        rows = []
+        for q, c in enumerate(sel):
+            base = float(f[name][c])
+            d = 1e-3 * abs(base)
+            d1 = (J_of(name, c, d) - J_of(name, c, -d)) / (2 * d)
+            d2 = (J_of(name, c, d / 2) - J_of(name, c, -d / 2)) / d
+            rows.append((g[c], d1, (4 * d2 - d1) / 3))
+            if q % 8 == 7:
+                log(f"bench: adjoint FD {name}: {q + 1} of {len(sel)} cells")
+        R = np.array(rows)
+        raw = np.abs(R[:, 0] - R[:, 1]) / np.abs(R[:, 1])
+        ric = np.abs(R[:, 0] - R[:, 2]) / np.abs(R[:, 2])
+        out[name] = {
+            "raw_central_1e-3": {"median_rel": float(np.median(raw)), "max_rel": float(raw.max()),
+                                 "frac_within_1e-4": float(np.mean(raw < 1e-4)),
+                                 "frac_within_1e-3": float(np.mean(raw < 1e-3))},
+            "richardson": {"median_rel": float(np.median(ric)), "max_rel": float(ric.max()),
+                           "frac_within_1e-4": float(np.mean(ric < 1e-4)),
+                           "frac_within_1e-3": float(np.mean(ric < 1e-3)),
+                           "max_abs_over_max_grad": float(np.abs(R[:, 0] - R[:, 2]).max() /
+                                                          np.abs(R[:, 2]).max())}}
+        log(f"bench: adjoint FD {name}: raw central median {np.median(raw):.2e}, "
+            f"Richardson median {np.median(ric):.2e}, within 1e-4: {np.mean(ric < 1e-4):.2f}")
+    del tomo, f, s0, t0
+    torch.cuda.empty_cache()
+    # the same survey layout at 64^3 against the CPU adjoint
+    if threads:
+        from oracle.oracle import Oracle
+        o = Oracle()
+        o.set_threads(threads)
+        m_ = 64
+        f = tomography_fields(m_, xp=torch, dtype=torch.float64, device="cuda")
+        ft = tomography_fields(m_, xp=torch, dtype=torch.float64, device="cuda", anomalies=True)
+        S = top_face_survey(m_, 4, 64)
+        tomo = Tomography(solver, S, f["phi"], f["ra"], f["rb"], batch=16)
+        tomo.observe(ft["slowness"], ft["theta"])
+        s0 = f["slowness"].clone().requires_grad_(True)
+        t0 = f["theta"].clone().requires_grad_(True)
+        _, res, ckpt = tomo.forward_pass(s0.detach(), t0.detach())
+        tomo.misfit(s0, t0).backward()
+        hf = {k: v.cpu().numpy() for k, v in f.items()}
+        cs = np.zeros((m_ + 2) ** 3)
+        ct = np.zeros((m_ + 2) ** 3)
+        tc = time.perf_counter()
+        for k in range(len(S.sources)):
+            a, b, _ = o.ellip_adjoint(ckpt[k].cpu().numpy(), m_, S.h, S.rec_idx.astype(np.int64),
+                                      res[k].cpu().numpy(), speed_field=1.0 / hf["slowness"],
+                                      theta=hf["theta"], phi=hf["phi"], ra=hf["ra"], rb=hf["rb"])
+            cs += a
+            ct += b
+        out["cpu_adjoint_64"] = {
+            "n": m_, "sources": 16, "receivers": S.nrec, "cpu_s": time.perf_counter() - tc,
+            "max_abs_diff_over_max_slowness": float(np.abs(s0.grad.cpu().numpy() - cs).max() /
+                                                    np.abs(cs).max()),
+            "max_abs_diff_over_max_theta": float(np.abs(t0.grad.cpu().numpy() - ct).max() /
+                                                 np.abs(ct).max())}
+        del tomo, f, ft, s0, t0, ckpt
+        torch.cuda.empty_cache()
+    out["check_s"] = time.perf_counter() - t_start
     return out
 
 
@@ -384,6 +550,9 @@ def main():
     ap.add_argument("--field-n", "--c2-n", dest="field_n", type=int, default=1280,
                     help="side of the per-node tilt-field run (0: skip)")
     ap.add_argument("--iters", type=int, default=8, help="iterations per solve (the reference runs 8)")
+    ap.add_argument("--fd-cells", type=int, default=32,
+                    help="seeded cells per parameter of the tomography gradient's finite-difference "
+                         "check at 256^3 (0: skip the check)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this solver runs on one GPU only (asked for --gpus {args.gpus}); "
@@ -592,6 +761,13 @@ def main():
         log(f"bench: gradient {adjoint['gradient_s']:.2f} s (forward {adjoint['forward_batched_s']:.2f} s, "
             f"adjoint {adjoint['adjoint_device_ms_total'] / 1e3:.2f} s)")
 
+    adjoint_fd = None
+    if not args.no_adjoint and args.fd_cells:
+        log(f"bench: tomography gradient vs finite differences, 256^3 FP64, {args.fd_cells} cells "
+            "per parameter; vs the CPU adjoint at 64^3")
+        adjoint_fd = adjoint_fd_run(solver, torch, cells=args.fd_cells, threads=cpu_threads)
+        log(f"bench: adjoint checks took {adjoint_fd['check_s']:.0f} s")
+
     field = None
     if args.field_n:
         del solver
@@ -654,6 +830,7 @@ def main():
         "cpu_baseline": cpu,
         "batched": batch,
         "adjoint": adjoint,
+        "adjoint_check": adjoint_fd,
         "reference_128": ref_small,
         "tilted_homogeneous_128": homog,
         "tilted_field": field,

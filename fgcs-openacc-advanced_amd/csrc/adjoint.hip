@@ -15,7 +15,14 @@
 // so a sweep visits only the neighbours that feed it.  Then
 //   dJ/ds_X = mu_X dT_X/ds_X,  dJ/dtheta_X = mu_X dT_X/dtheta_X.
 // Nodes no candidate reproduces (the closed-form start around a source) are
-// fixed: they pass nothing upstream and contribute no gradient.
+// fixed: they pass nothing upstream and contribute no gradient; they are
+// counted (held_out) so that a lost gradient path does not go unnoticed.
+//
+// A node's winning simplex may lie in another sweep direction's upwind plane,
+// so an in-edge can join two nodes of the same plane of the sweep being run.
+// Those in-plane values are read from a snapshot of mu taken before the
+// direction's sweep, never from mu itself while the plane launch writes it:
+// every pass computes the same values whatever the order of the waves.
 //
 // Several sources on one grid and model are solved together: every plane
 // launch serves all sources still running (blockIdx.z = source), which gives
@@ -97,7 +104,8 @@ struct Winner {
 
 template <typename R>
 __global__ __launch_bounds__(256) void stencil_kernel(const R *__restrict__ T, AdjArgs<R> A,
-                                                      Stencil<R> S) {
+                                                      Stencil<R> S,
+                                                      unsigned long long *__restrict__ held) {
   const long long total = (long long)A.n[0] * A.n[1] * A.n[2];
   const long long tid = blockIdx.x * (long long)blockDim.x + threadIdx.x;
   if (tid >= total) return;
@@ -257,6 +265,11 @@ __global__ __launch_bounds__(256) void stencil_kernel(const R *__restrict__ T, A
   }
   // a node is a stencil node if its best candidate reproduces its value
   const bool follows = best.nv > 0 && reached(tX) && best.val <= tX * (R(1) + A.rel_tol);
+  {  // reached nodes held fixed: one atomic per wave (the lanes still here are reached)
+    const unsigned long long hb = __builtin_amdgcn_ballot_w64(!follows);
+    if (hb && (threadIdx.x & 63) == __builtin_ctzll(__builtin_amdgcn_ballot_w64(true)))
+      atomicAdd(held, (unsigned long long)__builtin_popcountll(hb));
+  }
   int o[3] = {0, 0, 0};
   R w[3] = {R(0), R(0), R(0)};
   R gs = R(0), gth = R(0);
@@ -318,9 +331,23 @@ template <typename R>
 struct AdjSrc {
   const R *T;
   R *mu, *res;
+  R *snap;  // mu as it was before the running direction's sweep (same layout)
   unsigned long long *in_mask;
   Stencil<R> S;
 };
+
+// snap = mu for every source (blockIdx.y = source), before a direction's sweep.
+template <typename R>
+__global__ void adjoint_snapshot_kernel(const AdjSrc<R> *__restrict__ src,
+                                        const int *__restrict__ done, long long count) {
+  const int k = blockIdx.y;
+  if (done[k]) return;
+  const R *__restrict__ mu = src[k].mu;
+  R *__restrict__ snap = src[k].snap;
+  for (long long x = blockIdx.x * (long long)blockDim.x + threadIdx.x; x < count;
+       x += (long long)gridDim.x * blockDim.x)
+    snap[x] = mu[x];
+}
 
 // Largest change of each source's adjoint values in a pass: kAdjMaxSlots
 // words per source, a block updating the one its index picks, so that the
@@ -335,10 +362,12 @@ __global__ __launch_bounds__(256) void adjoint_plane_kernel(const AdjSrc<R> *__r
                                                             const int *__restrict__ done,
                                                             PlaneGeom g, long long sx,
                                                             long long sy, long long sz,
+                                                            int march,
                                                             unsigned long long *__restrict__ maxd) {
   const int zsrc = blockIdx.z;
   if (done[zsrc]) return;  // uniform over the block
   R *__restrict__ mu = src[zsrc].mu;
+  const R *__restrict__ snap = src[zsrc].snap;
   const R *__restrict__ res = src[zsrc].res;
   const Stencil<R> S = src[zsrc].S;
   const unsigned long long *__restrict__ in_mask = src[zsrc].in_mask;
@@ -353,8 +382,11 @@ __global__ __launch_bounds__(256) void adjoint_plane_kernel(const AdjSrc<R> *__r
       // decode up to kBatch in-edges first so their loads are in flight together,
       // then add them in mask order (the same order, so the same bits, as one
       // at a time); a node with more in-edges finishes one at a time
+      // an in-edge from the plane itself (no offset along the march axis) is
+      // read from the snapshot: those values are being rewritten by this launch
       constexpr int kBatch = 6;
       const R *wp[kBatch];
+      const R *mp[kBatch];
       long long zs[kBatch];
       int cnt = 0;
       while (m && cnt < kBatch) {
@@ -363,8 +395,10 @@ __global__ __launch_bounds__(256) void adjoint_plane_kernel(const AdjSrc<R> *__r
         m &= ~(3ull << (2 * c));
         const int cc = c < 13 ? c : c + 1;
         const int dx = cc % 3 - 1, dy = (cc / 3) % 3 - 1, dz = cc / 9 - 1;
+        const int along = march == 0 ? dx : march == 1 ? dy : dz;
         zs[cnt] = X + dx * sx + dy * sy + dz * sz;
         wp[cnt] = q == 0 ? S.w0 : q == 1 ? S.w1 : S.w2;
+        mp[cnt] = along == 0 ? snap : mu;
         ++cnt;
       }
       R wv[kBatch], mv[kBatch];
@@ -372,7 +406,7 @@ __global__ __launch_bounds__(256) void adjoint_plane_kernel(const AdjSrc<R> *__r
       for (int k = 0; k < kBatch; ++k)
         if (k < cnt) {
           wv[k] = wp[k][zs[k]];
-          mv[k] = mu[zs[k]];
+          mv[k] = mp[k][zs[k]];
         }
 #pragma unroll
       for (int k = 0; k < kBatch; ++k)
@@ -383,9 +417,10 @@ __global__ __launch_bounds__(256) void adjoint_plane_kernel(const AdjSrc<R> *__r
         m &= ~(3ull << (2 * c));
         const int cc = c < 13 ? c : c + 1;
         const int dx = cc % 3 - 1, dy = (cc / 3) % 3 - 1, dz = cc / 9 - 1;
+        const int along = march == 0 ? dx : march == 1 ? dy : dz;
         const long long Z = X + dx * sx + dy * sy + dz * sz;
         const R w = q == 0 ? S.w0[Z] : q == 1 ? S.w1[Z] : S.w2[Z];
-        acc += w * mu[Z];
+        acc += w * (along == 0 ? snap : mu)[Z];
       }
       const R old = mu[X];
       mu[X] = acc;
@@ -441,15 +476,17 @@ static size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
 
 // Bytes of one source's scratch per node: in-edge masks (u64), mu, res, three
 // weights, two partials (R) and three offsets (int) in the reference layout,
-// plus the masks, mu, res and weights again in the x-sweep layout.
+// plus the masks, mu, res and weights again in the x-sweep layout, and the
+// snapshot of mu (in the layout of the running direction).
 static size_t per_source_bytes(const Grid &G, size_t r) {
-  return align256((size_t)G.count * (2 * sizeof(unsigned long long) + 12 * r + 3 * sizeof(int)));
+  return align256((size_t)G.count * (2 * sizeof(unsigned long long) + 13 * r + 3 * sizeof(int)));
 }
 
 template <typename R>
 int adjoint_solve(hipStream_t st, int nsrc, const AdjointSource *srcs, const Grid &G,
                   const fold3d_model &m, const double h[3], int max_iters, R *grad_s,
-                  R *grad_th, void *scratch, int *iters_out, double *maxd_out) {
+                  R *grad_th, void *scratch, int *iters_out, double *maxd_out,
+                  long long *held_out) {
   const long long count = G.count;
   const size_t per = per_source_bytes(G, sizeof(R));
   char *base = (char *)scratch;
@@ -463,6 +500,7 @@ int adjoint_solve(hipStream_t st, int nsrc, const AdjointSource *srcs, const Gri
   unsigned long long *d_max =
       (unsigned long long *)((char *)d_src + align256(sizeof(AdjSrc<R>) * 2 * nsrc));
   int *d_done = (int *)((char *)d_max + align256(sizeof(unsigned long long) * nsrc * kAdjMaxSlots));
+  unsigned long long *d_held = (unsigned long long *)((char *)d_done + align256(sizeof(int) * nsrc));
   std::vector<AdjSrc<R>> hs(2 * nsrc);  // [0, nsrc): reference layout, then layout X
   for (int k = 0; k < nsrc; ++k) {
     char *p = base + per * k;
@@ -490,6 +528,8 @@ int adjoint_solve(hipStream_t st, int nsrc, const AdjointSource *srcs, const Gri
       *f = (R *)p;
       p += count * sizeof(R);
     }
+    A.snap = X.snap = (R *)p;
+    p += count * sizeof(R);
   }
   F3D_HIP(hipMemsetAsync(scratch, 0, adjoint_scratch_bytes(G, sizeof(R) == 8 ? FOLD3D_F64 : FOLD3D_F32, nsrc), st));
   F3D_HIP(hipMemcpyAsync(d_src, hs.data(), sizeof(AdjSrc<R>) * 2 * nsrc, hipMemcpyHostToDevice, st));
@@ -512,7 +552,8 @@ int adjoint_solve(hipStream_t st, int nsrc, const AdjointSource *srcs, const Gri
   A.rel_tol = sizeof(R) == 8 ? (R)1e-12 : (R)1e-6;
   const long long interior = (long long)G.n[0] * G.n[1] * G.n[2];
   for (int k = 0; k < nsrc; ++k) {
-    stencil_kernel<R><<<(unsigned)((interior + 255) / 256), 256, 0, st>>>(hs[k].T, A, hs[k].S);
+    stencil_kernel<R><<<(unsigned)((interior + 255) / 256), 256, 0, st>>>(hs[k].T, A, hs[k].S,
+                                                                         d_held + k);
     in_mask_kernel<R><<<2048, 256, 0, st>>>(hs[k].S, count, G.stride[0], G.stride[1],
                                             G.stride[2], hs[k].in_mask);
     const int nrec = srcs[k].nrec;
@@ -575,6 +616,8 @@ int adjoint_solve(hipStream_t st, int nsrc, const AdjointSource *srcs, const Gri
     const Grid &GL = xs ? GX : G;
     if (dd == 1)
       for (int k = 0; k < nsrc; ++k) permute_layout<R>(ws, hs[k].mu, G, hs[nsrc + k].mu, GX);
+    adjoint_snapshot_kernel<R><<<dim3(512, nsrc), 256, 0, ws>>>(xs ? d_srcX : d_src, d_done,
+                                                                count);
     const int n = GL.n[fr.m];
     for (int q = 0; q < n; ++q) {
       // sweep plane index along the march axis, against the characteristics
@@ -584,7 +627,7 @@ int adjoint_solve(hipStream_t st, int nsrc, const AdjointSource *srcs, const Gri
       const dim3 grid((g.na + 63) / 64, (g.nb + 3) / 4, nsrc);
       adjoint_plane_kernel<R><<<grid, block, 0, ws>>>(xs ? d_srcX : d_src, d_done, g,
                                                       GL.stride[0], GL.stride[1], GL.stride[2],
-                                                      d_max);
+                                                      fr.m, d_max);
     }
     if (dd == 0)
       for (int k = 0; k < nsrc; ++k) permute_layout<R>(ws, hs[nsrc + k].mu, GX, hs[k].mu, G);
@@ -618,6 +661,13 @@ int adjoint_solve(hipStream_t st, int nsrc, const AdjointSource *srcs, const Gri
   }
   adjoint_gradient_kernel<R><<<1024, 256, 0, ws>>>(d_src, nsrc, count, grad_s, grad_th);
   ADJ_HIP(hipGetLastError());
+  {
+    std::vector<unsigned long long> held(nsrc);
+    ADJ_HIP(hipMemcpyAsync(held.data(), d_held, sizeof(unsigned long long) * nsrc,
+                           hipMemcpyDeviceToHost, ws));
+    ADJ_HIP(hipStreamSynchronize(ws));
+    for (int k = 0; k < nsrc; ++k) held_out[k] = (long long)held[k];
+  }
   // later work on the caller's stream sees the gradients
   ADJ_HIP(hipEventRecord(ev, ws));
   ADJ_HIP(hipStreamWaitEvent(st, ev, 0));
@@ -643,14 +693,15 @@ size_t adjoint_scratch_bytes(const Grid &G, int precision, int nsrc) {
   const size_t tab = sizeof(AdjSrc<double>) > sizeof(AdjSrc<float>) ? sizeof(AdjSrc<double>)
                                                                     : sizeof(AdjSrc<float>);
   return per_source_bytes(G, r) * nsrc + align256(tab * 2 * nsrc) +
-         align256(sizeof(unsigned long long) * nsrc * kAdjMaxSlots) + align256(sizeof(int) * nsrc);
+         align256(sizeof(unsigned long long) * nsrc * kAdjMaxSlots) + align256(sizeof(int) * nsrc) +
+         align256(sizeof(unsigned long long) * nsrc);
 }
 
 template int adjoint_solve<double>(hipStream_t, int, const AdjointSource *, const Grid &,
                                    const fold3d_model &, const double *, int, double *, double *,
-                                   void *, int *, double *);
+                                   void *, int *, double *, long long *);
 template int adjoint_solve<float>(hipStream_t, int, const AdjointSource *, const Grid &,
                                   const fold3d_model &, const double *, int, float *, float *,
-                                  void *, int *, double *);
+                                  void *, int *, double *, long long *);
 
 }  // namespace fold3d

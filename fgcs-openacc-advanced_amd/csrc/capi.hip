@@ -372,6 +372,7 @@ int fold3d_solve(fold3d_handle *h, void *T, int nx, int ny, int nz, double dx,
     const int params = ellip ? (model->speed_field ? 1 : 0) + (model->theta ? 4 : 0) : 0;
     stats->bytes_moved = stats->updates_computed * (2 + params) *
                          (o.precision == FOLD3D_F64 ? 8.0 : 4.0);
+    stats->nodes_held = 0;
   }
   if (o.tol > 0 && !(maxdt < o.tol)) {
     set_error("fold3d_solve: max|dT| = %g >= tol %g after %d iterations", maxdt, o.tol, iters);
@@ -588,7 +589,8 @@ static int check_adjoint_source(const char *who, int k, const Grid &G, const voi
 static int run_adjoint(fold3d_handle *h, const char *who, int nsrc, const AdjointSource *srcs,
                        int nx, int ny, int nz, double dx, double dy, double dz,
                        const fold3d_model *model, const fold3d_opts &o, void *grad_slowness,
-                       void *grad_theta, int *iters, double *maxd, double *ms_out) {
+                       void *grad_theta, int *iters, double *maxd, long long *held,
+                       double *ms_out) {
   const Grid G(nx, ny, nz);
   hipStream_t st = (hipStream_t)o.stream;
   const int rc0 = ensure_scratch(h, adjoint_scratch_bytes(G, o.precision, nsrc));
@@ -599,10 +601,10 @@ static int run_adjoint(fold3d_handle *h, const char *who, int nsrc, const Adjoin
       o.precision == FOLD3D_F64
           ? adjoint_solve<double>(st, nsrc, srcs, G, *model, hsp, o.max_iters,
                                   (double *)grad_slowness, (double *)grad_theta, h->d_scratch,
-                                  iters, maxd)
+                                  iters, maxd, held)
           : adjoint_solve<float>(st, nsrc, srcs, G, *model, hsp, o.max_iters,
                                  (float *)grad_slowness, (float *)grad_theta, h->d_scratch, iters,
-                                 maxd);
+                                 maxd, held);
   F3D_HIP(hipEventRecord(h->ev1, st));
   if (rc != FOLD3D_OK) return rc;
   F3D_HIP(hipEventSynchronize(h->ev1));
@@ -643,8 +645,9 @@ int fold3d_adjoint(fold3d_handle *h, const void *T, int nx, int ny, int nz, doub
   const AdjointSource src{T, nrec, rec_idx, rec_res, o.tol};
   int iters = 0;
   double maxd = -1, ms = 0;
+  long long held = 0;
   rc = run_adjoint(h, who, 1, &src, nx, ny, nz, dx, dy, dz, model, o, grad_slowness, grad_theta,
-                   &iters, &maxd, &ms);
+                   &iters, &maxd, &held, &ms);
   if (stats && (rc == FOLD3D_OK || rc == FOLD3D_E_NOT_CONVERGED)) {
     stats->iterations = iters;
     stats->max_dT = maxd;
@@ -652,6 +655,7 @@ int fold3d_adjoint(fold3d_handle *h, const void *T, int nx, int ny, int nz, doub
     stats->sweep_ms = ms;
     stats->updates_computed = 0;
     stats->bytes_moved = 0;
+    stats->nodes_held = (double)held;
   }
   return rc;
 }
@@ -660,7 +664,7 @@ int fold3d_adjoint_batched(fold3d_handle *h, int count, const fold3d_adjoint_sou
                            int nx, int ny, int nz, double dx, double dy, double dz,
                            const fold3d_model *model, const fold3d_opts *opts,
                            void *grad_slowness, void *grad_theta, int *iterations_out,
-                           double *max_d_out, double *ms_out) {
+                           double *max_d_out, long long *held_out, double *ms_out) {
   static const char *who = "fold3d_adjoint_batched";
   F3D_CHECK(h && src, FOLD3D_E_INVALID, "fold3d_adjoint_batched: NULL handle or sources");
   F3D_CHECK(count >= 1 && count <= 4096, FOLD3D_E_INVALID,
@@ -688,13 +692,15 @@ int fold3d_adjoint_batched(fold3d_handle *h, int count, const fold3d_adjoint_sou
   }
   std::vector<int> iters(count, 0);
   std::vector<double> maxd(count, -1.0);
+  std::vector<long long> held(count, 0);
   double ms = 0;
   rc = run_adjoint(h, who, count, srcs.data(), nx, ny, nz, dx, dy, dz, model, o, grad_slowness,
-                   grad_theta, iters.data(), maxd.data(), &ms);
+                   grad_theta, iters.data(), maxd.data(), held.data(), &ms);
   if (rc == FOLD3D_OK || rc == FOLD3D_E_NOT_CONVERGED) {
     for (int k = 0; k < count; ++k) {
       if (iterations_out) iterations_out[k] = iters[k];
       if (max_d_out) max_d_out[k] = maxd[k];
+      if (held_out) held_out[k] = held[k];
     }
     if (ms_out) *ms_out = ms;
   }

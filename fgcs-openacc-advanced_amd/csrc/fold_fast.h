@@ -40,6 +40,7 @@
 
 #include <cmath>
 
+#include "common.h"
 #include "fold_local.h"
 
 namespace fold3d {
@@ -245,26 +246,55 @@ __device__ __forceinline__ void fold_unit(R &tnew, const Patch<R> &p, const Fold
 }
 
 // ---------------------------------------------------------------------------
-// Host-side construction, in the arithmetic of R (IEEE, no contraction) with
-// the expressions of fold_local.h, so the hoisted values carry the bits the
-// node loop of the literal kernel computes in the same precision.  Only the
-// pruning bounds (gap, lgap) are computed in double: they are conservative
-// bounds, not results.
+// Construction of the constants, in the arithmetic of R (IEEE, no
+// contraction, correctly rounded division and square root on the host and on
+// the device alike) with the expressions of fold_local.h, so the hoisted
+// values carry the bits the node loop of the literal kernel computes in the
+// same precision.  Only the pruning bounds (gap, lgap) are computed in
+// double: they are conservative bounds, not results.  Host and device code
+// share these functions (the allocation-free solve builds its table on the
+// device).
 
 // Least travel time from the x axis to (0, 0, dzz) in an edge site's frame:
 // dzz max_phi sin(phi) / (F + |ay| cos(phi) + az sin(phi)); the derivative
 // vanishes only at cos(phi) = -|ay| / F, which gives dzz / (sqrt(F^2 - ay^2) +
 // az) (>= the plane bound dzz / (F + az); positive because F > |a|).
-inline double edge_line_bound(double F, double ay, double az, double dzz) {
+__host__ __device__ inline double edge_line_bound(double F, double ay, double az, double dzz) {
   return dzz / (std::sqrt(F * F - ay * ay) + az);
+}
+
+// Local frame of direction `dir` from the global model (F, a) and spacings
+// h, in R as the literal kernel receives it; DXYP/DXZP/DYZP of the reference
+// driver (AN_ISO_1_OpenACC.c:1159-1161), lower axis first.
+__host__ __device__ inline double face_diagonal(const double h[3], int u, int v) {
+  const int lo = u < v ? u : v, hi = u < v ? v : u;
+  return std::sqrt(h[lo] * h[lo] + h[hi] * h[hi]);
+}
+template <typename R>
+__host__ __device__ inline FoldFrame<R> fold_frame(int dir, double F, const double a[3],
+                                                   const double h[3]) {
+  const Frame fr = kFrames[dir];
+  FoldFrame<R> f;
+  f.F = (R)F;
+  f.ax = (R)(fr.sx * a[fr.ux]);
+  f.ay = (R)(fr.sy * a[fr.uy]);
+  f.az = (R)(fr.s * a[fr.m]);
+  f.dzz = (R)h[fr.m];
+  f.dxx = (R)h[fr.ux];
+  f.dyy = (R)h[fr.uy];
+  f.dxy = (R)face_diagonal(h, fr.ux, fr.uy);
+  f.dxz = (R)face_diagonal(h, fr.ux, fr.m);
+  f.dyz = (R)face_diagonal(h, fr.uy, fr.m);
+  return f;
 }
 
 // Pruning bounds are shrunk by this factor: 99.9% in FP64, 90% in FP32.
 template <typename R>
-inline double prune_shrink() { return sizeof(R) == 8 ? 0.999 : 0.9; }
+__host__ __device__ inline double prune_shrink() { return sizeof(R) == 8 ? 0.999 : 0.9; }
 
 template <typename R>
-inline void host_edge_site(R F, R ax, R ay, R az, R dxx, R dzz, EdgeSite<R> &e) {
+__host__ __device__ inline void edge_site_consts(R F, R ax, R ay, R az, R dxx, R dzz,
+                                                 EdgeSite<R> &e) {
   e.ax = ax;
   e.az = az;
   e.dxx = dxx;
@@ -293,7 +323,7 @@ inline void host_edge_site(R F, R ax, R ay, R az, R dxx, R dzz, EdgeSite<R> &e) 
 // FoldFrame f holds (F, ax, ay, az, dzz, dxx, dyy, dxy, dxz, dyz) of the
 // direction in R, as the literal kernel receives it.
 template <typename R>
-inline void host_fold_sign(const FoldFrame<R> &f, R sgn, FoldSign<R> &S) {
+__host__ __device__ inline void fold_sign_consts(const FoldFrame<R> &f, R sgn, FoldSign<R> &S) {
   const R F = f.F;
   // a flips on the negative side exactly as `a *= -1` does
   const R ax = f.ax * sgn, ay = f.ay * sgn, az = f.az * sgn;
@@ -332,22 +362,31 @@ inline void host_fold_sign(const FoldFrame<R> &f, R sgn, FoldSign<R> &S) {
     t.sub_pos = t.ax * dzz + az * t.dxx > R(0);
     const R txz = tp[k][4];
     // sub-edge xt -> yt in the rotated frame (:170-174)
-    host_edge_site<R>(F, t.ay, (t.ax * dzz + az * t.dxx) / txz, (-t.ax * t.dxx + az * dzz) / txz,
+    edge_site_consts<R>(F, t.ay, (t.ax * dzz + az * t.dxx) / txz, (-t.ax * t.dxx + az * dzz) / txz,
                       t.dyy, txz, S.edge[8 + k]);
   }
   // the node's own edges (:242-253), order px, pxpy, py, mxpy, mx, mxmy, my, pxmy
-  host_edge_site<R>(F, ax, ay, az, dxx, dzz, S.edge[0]);
-  host_edge_site<R>(F, (ax * dxx + ay * dyy) / dxy, (-ax * dyy + ay * dxx) / dxy, az, dxy, dzz,
+  edge_site_consts<R>(F, ax, ay, az, dxx, dzz, S.edge[0]);
+  edge_site_consts<R>(F, (ax * dxx + ay * dyy) / dxy, (-ax * dyy + ay * dxx) / dxy, az, dxy, dzz,
                     S.edge[1]);
-  host_edge_site<R>(F, ay, ax, az, dyy, dzz, S.edge[2]);
-  host_edge_site<R>(F, -(ax * dxx - ay * dyy) / dxy, (ax * dyy + ay * dxx) / dxy, az, dxy, dzz,
+  edge_site_consts<R>(F, ay, ax, az, dyy, dzz, S.edge[2]);
+  edge_site_consts<R>(F, -(ax * dxx - ay * dyy) / dxy, (ax * dyy + ay * dxx) / dxy, az, dxy, dzz,
                     S.edge[3]);
-  host_edge_site<R>(F, -ax, ay, az, dxx, dzz, S.edge[4]);
-  host_edge_site<R>(F, -(ax * dxx + ay * dyy) / dxy, (-ax * dyy + ay * dxx) / dxy, az, dxy, dzz,
+  edge_site_consts<R>(F, -ax, ay, az, dxx, dzz, S.edge[4]);
+  edge_site_consts<R>(F, -(ax * dxx + ay * dyy) / dxy, (-ax * dyy + ay * dxx) / dxy, az, dxy, dzz,
                     S.edge[5]);
-  host_edge_site<R>(F, -ay, ax, az, dyy, dzz, S.edge[6]);
-  host_edge_site<R>(F, (ax * dxx - ay * dyy) / dxy, (ax * dyy + ay * dxx) / dxy, az, dxy, dzz,
+  edge_site_consts<R>(F, -ay, ax, az, dyy, dzz, S.edge[6]);
+  edge_site_consts<R>(F, (ax * dxx - ay * dyy) / dxy, (ax * dyy + ay * dxx) / dxy, az, dxy, dzz,
                     S.edge[7]);
+}
+
+// Both signs of direction `dir`.
+template <typename R>
+__host__ __device__ inline void fold_fast_consts(int dir, double F, const double a[3],
+                                                 const double h[3], FoldFast<R> &out) {
+  const FoldFrame<R> f = fold_frame<R>(dir, F, a, h);
+  fold_sign_consts<R>(f, R(1), out.sgn[0]);
+  fold_sign_consts<R>(f, R(-1), out.sgn[1]);
 }
 
 }  // namespace fold3d

@@ -79,30 +79,6 @@ __global__ __launch_bounds__(kLaneX *kRowsY) void fold_plane_kernel(
   }
 }
 
-// Local frame of direction `dir` from the global model (F, a) and spacings.
-template <typename R>
-static FoldFrame<R> fold_frame(int dir, double F, const double a[3],
-                               const double h[3]) {
-  const Frame &fr = kFrames[dir];
-  auto diag = [&](int u, int v) {
-    // DXYP/DXZP/DYZP of the reference driver (:1159-1161), lower axis first
-    int lo = u < v ? u : v, hi = u < v ? v : u;
-    return std::sqrt(h[lo] * h[lo] + h[hi] * h[hi]);
-  };
-  FoldFrame<R> f;
-  f.F = (R)F;
-  f.ax = (R)(fr.sx * a[fr.ux]);
-  f.ay = (R)(fr.sy * a[fr.uy]);
-  f.az = (R)(fr.s * a[fr.m]);
-  f.dzz = (R)h[fr.m];
-  f.dxx = (R)h[fr.ux];
-  f.dyy = (R)h[fr.uy];
-  f.dxy = (R)diag(fr.ux, fr.uy);
-  f.dxz = (R)diag(fr.ux, fr.m);
-  f.dyz = (R)diag(fr.uy, fr.m);
-  return f;
-}
-
 PlaneGeom plane_geom(const Grid &G, int dir, int plane) {
   const Frame &fr = kFrames[dir];
   // lane axis: the in-plane axis with the smaller stride
@@ -236,9 +212,7 @@ __global__ __launch_bounds__(kLaneX *kRowsY) void fold_plane_fast_kernel(
 template <typename R>
 void build_fold_fast(int dir, const fold3d_model &m, const double h[3], FoldFast<R> &out) {
   // the frame in R, as the literal kernel receives it, then everything in R
-  const FoldFrame<R> f = fold_frame<R>(dir, m.F, m.a, h);
-  host_fold_sign<R>(f, R(1), out.sgn[0]);
-  host_fold_sign<R>(f, R(-1), out.sgn[1]);
+  fold_fast_consts<R>(dir, m.F, m.a, h, out);
 }
 template void build_fold_fast<double>(int, const fold3d_model &, const double *, FoldFast<double> &);
 template void build_fold_fast<float>(int, const fold3d_model &, const double *, FoldFast<float> &);
@@ -574,6 +548,40 @@ int fold_sweeps_tiled_engine(hipStream_t st, R *T, R *Y, const Grid &G,
   *max_dt_out = last;
   return FOLD3D_OK;
 }
+// The split kernel on the caller's field in the reference layout for all six
+// directions: no second field and no layout moves (the x-sweeps' rows then
+// run along y, n + 2 values apart, so their staging loads are not
+// coalesced).  Same bits as the other engines: the kernel reads each plane
+// through its strides.
+template <typename R>
+int fold_sweeps_inplace_engine(hipStream_t st, R *T, const Grid &G, const FoldFast<R> *d_consts,
+                               int max_iters, double tol, int groups, unsigned long long *d_max,
+                               int *iters_out, double *max_dt_out) {
+  const LockState none{nullptr, 0, nullptr};
+  int it = 0;
+  double last = -1.0;
+  for (; it < max_iters;) {
+    unsigned long long *md = tol > 0 ? d_max : nullptr;
+    if (md) max_dt_reset_kernel<<<1, kMaxSlots, 0, st>>>(d_max);
+    for (int d = 0; d < 6; ++d) tiled_direction<R>(st, T, G, d, d_consts + d, md, groups, none);
+    F3D_HIP(hipGetLastError());
+    ++it;
+    if (md) {
+      F3D_HIP(max_dt_read(st, d_max, &last));
+      if (last < tol) break;
+    }
+  }
+  *iters_out = it;
+  *max_dt_out = last;
+  return FOLD3D_OK;
+}
+template int fold_sweeps_inplace_engine<double>(hipStream_t, double *, const Grid &,
+                                                const FoldFast<double> *, int, double, int,
+                                                unsigned long long *, int *, double *);
+template int fold_sweeps_inplace_engine<float>(hipStream_t, float *, const Grid &,
+                                               const FoldFast<float> *, int, double, int,
+                                               unsigned long long *, int *, double *);
+
 template int fold_sweeps_tiled_engine<double>(hipStream_t, double *, double *, const Grid &,
                                               const FoldFast<double> *, int, double, int,
                                               unsigned long long *, int *, double *, int *, int *,

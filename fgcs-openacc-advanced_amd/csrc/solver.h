@@ -81,6 +81,14 @@ int fold_sweeps_tiled_engine(hipStream_t st, R *T, R *Y, const Grid &G,
                              int *stamp = nullptr, int *stampY = nullptr,
                              unsigned long long *updates_done = nullptr);
 
+// The split kernel (`groups` = 2, 4 or 8 waves per row) on T in the reference
+// layout for all six directions: no scratch field.  d_consts: the six
+// directions' tables; d_max: kMaxSlots words (tol > 0).
+template <typename R>
+int fold_sweeps_inplace_engine(hipStream_t st, R *T, const Grid &G, const FoldFast<R> *d_consts,
+                               int max_iters, double tol, int groups, unsigned long long *d_max,
+                               int *iters_out, double *max_dt_out);
+
 // Batched elliptic solves (batch.hip).  `blocks` = persistent grid per launch,
 // `groups` = waves per row (1, 2, 4), `rows_per_pull` = rows taken from the
 // queue at a time (0 = automatic); `scratch` holds batch_scratch_bytes(resident):
@@ -94,7 +102,7 @@ size_t batch_scratch_bytes(int count, const fold3d_instance *inst, int precision
 
 // Adjoint state and gradients of `nsrc` sources on one grid and model
 // (adjoint.hip); `scratch` holds adjoint_scratch_bytes(G, precision, nsrc)
-// bytes.  iters_out / maxd_out: per source.
+// bytes.  iters_out / maxd_out / held_out (reached nodes held fixed): per source.
 struct AdjointSource {
   const void *T;        // converged forward field (device)
   int nrec;
@@ -105,7 +113,8 @@ struct AdjointSource {
 template <typename R>
 int adjoint_solve(hipStream_t st, int nsrc, const AdjointSource *srcs, const Grid &G,
                   const fold3d_model &m, const double h[3], int max_iters, R *grad_s,
-                  R *grad_th, void *scratch, int *iters_out, double *maxd_out);
+                  R *grad_th, void *scratch, int *iters_out, double *maxd_out,
+                  long long *held_out);
 size_t adjoint_scratch_bytes(const Grid &G, int precision, int nsrc);
 
 // The reference's input field built on the device, same bits as the host

@@ -78,6 +78,7 @@ class Stats(C.Structure):
         ("sweep_ms", C.c_double),
         ("updates_computed", C.c_double),
         ("bytes_moved", C.c_double),
+        ("nodes_held", C.c_double),
     ]
 
 
@@ -145,7 +146,7 @@ def lib() -> C.CDLL:
     L.fold3d_adjoint_batched.argtypes = [vp, i, C.POINTER(AdjointSource), i, i, i, d, d, d,
                                          C.POINTER(Model), C.POINTER(Opts), vp, vp,
                                          C.POINTER(C.c_int), C.POINTER(C.c_double),
-                                         C.POINTER(C.c_double)]
+                                         C.POINTER(C.c_longlong), C.POINTER(C.c_double)]
     L.fold3d_synchronize.argtypes = [vp]
     L.fold3d_point_source.argtypes = [vp, vp, i, i, i, d, d, d, dp, dp, d, d, i, vp]
     L.fold3d_initialiser_device.argtypes = [vp, vp, i, i, i, d, d, d, vp, C.POINTER(C.c_longlong)]
@@ -467,7 +468,8 @@ class Solver:
         """adjoint() for several sources on one grid and model at once: Ts, rec_idx,
         rec_res and tols are per-source lists (rec_idx may be one tensor shared by
         all).  Gradients of all sources are added in list order.  Returns
-        (sweeps per source, last max change per source, device ms)."""
+        (sweeps per source, last max change per source, device ms); the reached
+        nodes held fixed per source are left in self.last_held."""
         import torch
         count = len(Ts)
         nx, ny, nz = (n, n, n) if np.isscalar(n) else n
@@ -496,14 +498,16 @@ class Solver:
         o.stream = stream if stream is not None else torch.cuda.current_stream(Ts[0].device).cuda_stream
         its = (C.c_int * count)()
         mds = (C.c_double * count)()
+        held = (C.c_longlong * count)()
         ms = C.c_double()
         rc = lib().fold3d_adjoint_batched(
             self.h, count, arr, nx, ny, nz, dx, dy, dz, C.byref(model), C.byref(o),
             C.c_void_p(grad_slowness.data_ptr()),
             None if grad_theta is None else C.c_void_p(grad_theta.data_ptr()), its, mds,
-            C.byref(ms))
+            held, C.byref(ms))
         if not (rc == FOLD3D_E_NOT_CONVERGED and not raise_not_converged):
             check(rc)
+        self.last_held = list(held)
         return list(its), list(mds), ms.value
 
     def set_tuning(self, groups: int, kind: int = MODEL_FOLD):

@@ -125,6 +125,11 @@ typedef struct fold3d_stats {
                             target, read P per-node fields: speed_field, and
                             theta, phi, ra, rb when given); layout moves are not
                             counted.  0 from the adjoint entry points */
+  double nodes_held;     /* fold3d_adjoint: reached nodes whose value no candidate
+                            reproduces (within 1e-12 relative in FP64, 1e-6 in
+                            FP32), held fixed: they pass no gradient upstream.
+                            The closed-form start around a source is such a set;
+                            more means lost gradient paths.  0 from fold3d_solve */
 } fold3d_stats;
 
 typedef struct fold3d_handle fold3d_handle;
@@ -204,9 +209,10 @@ int fold3d_adjoint(fold3d_handle *h, const void *T, int nx, int ny, int nz,
  * opts->tol of fold3d_adjoint; opts->tol is not used) and stops on its own.
  * The gradients of all sources are added to grad_slowness / grad_theta in
  * source order, with the same bits as `count` calls of fold3d_adjoint in that
- * order.  Scratch: the handle holds 76 B per node per source in FP32 (124 B
- * in FP64).  Outputs (may be NULL): sweeps run and last max change per source
- * (arrays of length `count`), device time.  FOLD3D_E_NOT_CONVERGED if any
+ * order.  Scratch: the handle holds 80 B per node per source in FP32 (132 B
+ * in FP64).  Outputs (may be NULL): sweeps run, last max change and reached
+ * nodes held fixed (see fold3d_stats.nodes_held) per source (arrays of
+ * length `count`), device time.  FOLD3D_E_NOT_CONVERGED if any
  * source ran out of opts->max_iters sweeps (gradients written). */
 typedef struct fold3d_adjoint_source {
   const void *T;          /* converged field of the source (device, opts->precision) */
@@ -220,7 +226,7 @@ int fold3d_adjoint_batched(fold3d_handle *h, int count, const fold3d_adjoint_sou
                            int nx, int ny, int nz, double dx, double dy, double dz,
                            const fold3d_model *model, const fold3d_opts *opts,
                            void *grad_slowness, void *grad_theta, int *iterations_out,
-                           double *max_d_out, double *ms_out);
+                           double *max_d_out, long long *held_out, double *ms_out);
 
 /* Tuning of the split engines, per model kind: how many wavefronts share one
  * row of 64 nodes (fold model: 2, 4 or 8, default 4; elliptic model with a

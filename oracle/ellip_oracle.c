@@ -334,17 +334,31 @@ static void node_metric(const double *M6, const double *theta, const double *phi
 /* Value of one candidate (type/id of direction d) at node X with inverse
  * metric N and slowness s; weights of its vertices in w (if non-NULL).
  * Returns +inf when the candidate is not admissible. */
+static double eval_candidate_k(const double *T, const int n[3], long X, int d, int type,
+                               int id, const orc_ellip_consts *Kp, double s, long v[3],
+                               double w[3]);
+
 static double eval_candidate(const double *T, const int n[3], const double h[3], long X,
                              int d, int type, int id, const double N[6], double s,
                              long v[3], double w[3])
+{
+  orc_ellip_consts K;
+  orc_ellip_constants(d, N, h, &K);
+  return eval_candidate_k(T, n, X, d, type, id, &K, s, v, w);
+}
+
+/* The same with the direction's simplex constants K (orc_ellip_constants of
+ * direction d and metric N) given. */
+static double eval_candidate_k(const double *T, const int n[3], long X, int d, int type,
+                               int id, const orc_ellip_consts *Kp, double s, long v[3],
+                               double w[3])
 {
   const int *f = FR[d];
   const long H0 = n[0] + 2, H1 = n[1] + 2;
   const long stride[3] = {1, H0, H0 * H1};
   const long sm = f[1] * stride[f[0]], su = f[3] * stride[f[2]], sv = f[5] * stride[f[4]];
   const long c = X - sm;
-  orc_ellip_consts K;
-  orc_ellip_constants(d, N, h, &K);
+  const orc_ellip_consts *K = Kp;
   long pidx[9];
   int k;
   pidx[0] = c;
@@ -354,14 +368,14 @@ static double eval_candidate(const double *T, const int n[3], const double h[3],
     const double t0 = T[pidx[id]];
     if (!(t0 < BIG)) return INFINITY;
     if (v) { v[0] = pidx[id]; w[0] = 1.0; }
-    return t0 + s * K.vert[id];
+    return t0 + s * K->vert[id];
   }
   if (type == 2) {
     const int q0 = id < 8 ? 0 : 1 + (id - 8), q1 = id < 8 ? 1 + id : 1 + ((id - 8 + 1) % 8);
     const double t0 = T[pidx[q0]], t1 = T[pidx[q1]];
-    const double val = orc_ellip_edge(t0, t1, s2, K.edge[id]);
+    const double val = orc_ellip_edge(t0, t1, s2, K->edge[id]);
     if (v && isfinite(val)) {
-      const double P = K.edge[id][0], l0 = K.edge[id][1], kap = K.edge[id][2];
+      const double P = K->edge[id][0], l0 = K->edge[id][1], kap = K->edge[id][2];
       const double dd = t1 - t0, mu = P * dd, r = s2 - dd * mu;
       const double lam = l0 - sqrt(kap * r) / r * mu;
       v[0] = pidx[q0]; v[1] = pidx[q1];
@@ -372,7 +386,7 @@ static double eval_candidate(const double *T, const int n[3], const double h[3],
   {
     const int q1 = 1 + id, q2 = 1 + (id + 1) % 8;
     const double t0 = T[pidx[0]], t1 = T[pidx[q1]], t2 = T[pidx[q2]];
-    const double *Tr = K.tri[id];
+    const double *Tr = K->tri[id];
     const double val = orc_ellip_tri(t0, t1, t2, s2, Tr);
     if (v && isfinite(val)) {
       const double d1 = t1 - t0, d2 = t2 - t0;
@@ -400,10 +414,12 @@ static double node_min(const double *T, const int n[3], const double h[3], long 
   for (d = 0; d < 6; d++) {
     const long cm = ijk[FR[d][0]] - FR[d][1];
     if (cm < 1 || cm > n[FR[d][0]]) continue;
+    orc_ellip_consts K;
+    orc_ellip_constants(d, N, h, &K);
     for (t = 1; t <= 3; t++) {
       const int nid = t == 1 ? 9 : t == 2 ? 16 : 8;
       for (k = 0; k < nid; k++) {
-        const double val = eval_candidate(T, n, h, X, d, t, k, N, s, NULL, NULL);
+        const double val = eval_candidate_k(T, n, X, d, t, k, &K, s, NULL, NULL);
         if (val < best) {
           best = val;
           *dir = d; *type = t; *id = k;
@@ -413,6 +429,48 @@ static double node_min(const double *T, const int n[3], const double h[3], long 
   }
   return best;
 }
+
+/* Minimum over the candidates of one simplex of direction d and of its faces
+ * (a triangle, its three edges and three vertices; an edge and its two
+ * vertices; a vertex): where the simplex's candidate wins, this is the node's
+ * local solver restricted to that simplex, continuously differentiable across
+ * the simplex's boundary, and blind to near-ties with other simplices. */
+static double closure_min(const double *T, const int n[3], const double h[3], long X, int d,
+                          int type, int id, const double N[6], double s)
+{
+  orc_ellip_consts K;
+  orc_ellip_constants(d, N, h, &K);
+  double best = eval_candidate_k(T, n, X, d, type, id, &K, s, NULL, NULL);
+  int qv[3], nq = 0, ev[3], ne = 0, k;
+  if (type == 2) {
+    qv[nq++] = id < 8 ? 0 : 1 + (id - 8);
+    qv[nq++] = id < 8 ? 1 + id : 1 + ((id - 8 + 1) % 8);
+  } else if (type == 3) {
+    qv[nq++] = 0;
+    qv[nq++] = 1 + id;
+    qv[nq++] = 1 + (id + 1) % 8;
+    ev[ne++] = id;             /* spoke centre -> ring id */
+    ev[ne++] = (id + 1) % 8;   /* spoke centre -> ring id+1 */
+    ev[ne++] = 8 + id;         /* rim ring id -> ring id+1 */
+  }
+  for (k = 0; k < ne; k++) {
+    const double v = eval_candidate_k(T, n, X, d, 2, ev[k], &K, s, NULL, NULL);
+    if (v < best) best = v;
+  }
+  for (k = 0; k < nq; k++) {
+    const double v = eval_candidate_k(T, n, X, d, 1, qv[k], &K, s, NULL, NULL);
+    if (v < best) best = v;
+  }
+  return best;
+}
+
+/* Local derivatives of the adjoint: 1 (default) = central differences of the
+ * winning simplex's closure (closure_min), the derivative of the candidate the
+ * node follows; 0 = of the node's whole local solver (every candidate of the
+ * six patches), which at a node where two patches' candidates tie within the
+ * step averages the two slopes across the kink. */
+static int g_adj_local = 1;
+void orc_ellip_adjoint_local(int mode) { g_adj_local = mode; }
 
 int orc_ellip_adjoint(const double *T, const int n[3], const double h[3], const double M6[6],
                       double speed, const double *speed_field, const double *theta,
@@ -452,16 +510,25 @@ int orc_ellip_adjoint(const double *T, const int n[3], const double h[3], const 
      * solver (the minimum over every candidate of the six patches), which is
      * smooth across the boundary between a simplex and its faces */
     int dd, tt, ii;
+    const int wd = S[i].dir, wt = S[i].type, wi = S[i].id;
     const double es = 1e-6 * s;
-    dTds[i] = (node_min(T, n, h, i, N, s + es, &dd, &tt, &ii) -
-               node_min(T, n, h, i, N, s - es, &dd, &tt, &ii)) / (2 * es);
+    if (g_adj_local)
+      dTds[i] = (closure_min(T, n, h, i, wd, wt, wi, N, s + es) -
+                 closure_min(T, n, h, i, wd, wt, wi, N, s - es)) / (2 * es);
+    else
+      dTds[i] = (node_min(T, n, h, i, N, s + es, &dd, &tt, &ii) -
+                 node_min(T, n, h, i, N, s - es, &dd, &tt, &ii)) / (2 * es);
     if (theta) {
       const double et = 1e-6;
       double Np[6], Nm[6];
       node_metric(M6, theta, phi, ra, rb, i, et, Np);
       node_metric(M6, theta, phi, ra, rb, i, -et, Nm);
-      dTdth[i] = (node_min(T, n, h, i, Np, s, &dd, &tt, &ii) -
-                  node_min(T, n, h, i, Nm, s, &dd, &tt, &ii)) / (2 * et);
+      if (g_adj_local)
+        dTdth[i] = (closure_min(T, n, h, i, wd, wt, wi, Np, s) -
+                    closure_min(T, n, h, i, wd, wt, wi, Nm, s)) / (2 * et);
+      else
+        dTdth[i] = (node_min(T, n, h, i, Np, s, &dd, &tt, &ii) -
+                    node_min(T, n, h, i, Nm, s, &dd, &tt, &ii)) / (2 * et);
     }
   }
   (void)stride;

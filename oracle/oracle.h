@@ -99,6 +99,9 @@ int orc_ellip_sweeps(double *T, const int n[3], int max_iters, double tol,
  * reproduces T within rel_tol, else the node is held fixed.  Optional outputs:
  * the adjoint state mu and a flag per node that is reached but held fixed.
  * Returns the Jacobi passes used after the ordered push. */
+/* 1 (default): local derivatives of the winning simplex's closure; 0: of the
+ * node's whole local solver. */
+void orc_ellip_adjoint_local(int mode);
 int orc_ellip_adjoint(const double *T, const int n[3], const double h[3], const double M6[6],
                       double speed, const double *speed_field, const double *theta,
                       const double *phi, const double *ra, const double *rb, int nrec,

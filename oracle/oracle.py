@@ -136,6 +136,8 @@ class Oracle:
         lib.orc_ellip_sweeps.restype = _i
         lib.orc_ellip_sweeps.argtypes = [_dp, ip, _i, _d, _dp, _dp, _d] + [_dp] * 5 + [_dp]
         lp = C.POINTER(C.c_long)
+        lib.orc_ellip_adjoint_local.argtypes = [_i]
+        lib.orc_ellip_adjoint_local.restype = None
         lib.orc_ellip_adjoint.restype = _i
         lib.orc_ellip_adjoint.argtypes = ([_dp, ip, _dp, _dp, _d] + [_dp] * 5 +
                                           [_i, lp, _dp, _dp, _dp, _d, _i, _dp, C.c_void_p])
@@ -215,10 +217,14 @@ class Oracle:
 
     def ellip_adjoint(self, T, n, h, rec_idx, rec_res, M=None, speed=1.0, speed_field=None,
                       theta=None, phi=None, ra=None, rb=None, rel_tol=1e-12, max_passes=100000,
-                      state=False):
+                      state=False, local="closure"):
         """FP64 adjoint of the converged field T: returns (dJ/ds, dJ/dtheta or
         None, Jacobi passes) for J = 1/2 sum rec_res^2 at nodes rec_idx; with
-        `state`, also the adjoint state mu and the held-fixed flags."""
+        `state`, also the adjoint state mu and the held-fixed flags.  `local`:
+        the node derivatives are central differences (step 1e-6) of the
+        winning simplex with its faces ("closure": the candidate the node
+        follows) or of the node's whole local solver ("solver": averages the
+        slopes of two near-tied candidates of different patches)."""
         n3 = (C.c_int * 3)(*((n,) * 3 if np.isscalar(n) else n))
         nn = tuple(n3)
         M = np.ascontiguousarray(np.eye(3)[[0, 1, 2, 0, 0, 1], [0, 1, 2, 1, 2, 2]]
@@ -233,6 +239,9 @@ class Oracle:
         gt = np.zeros(T.size) if theta is not None else None
         mu = np.zeros(T.size) if state else None
         fixed = np.zeros(T.size, dtype=np.uint8) if state else None
+        if local not in ("closure", "solver"):
+            raise ValueError("local must be 'closure' or 'solver'")
+        self._ellip_proto().orc_ellip_adjoint_local(1 if local == "closure" else 0)
         passes = self._ellip_proto().orc_ellip_adjoint(
             _ptr(T), n3, _ptr(h), _ptr(M), speed, *ptrs, int(idx.size),
             idx.ctypes.data_as(C.POINTER(C.c_long)), _ptr(res), _ptr(gs),

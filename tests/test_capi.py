@@ -220,7 +220,7 @@ def test_batched_adjoint_arguments_are_checked():
     def call(h=dummy, count=2, s=src, n=8, model=m, grad=dummy, gt=None):
         return L.fold3d_adjoint_batched(h, count, s, n, n, n, 0.1, 0.1, 0.1,
                                         None if model is None else C.byref(model), C.byref(o),
-                                        grad, gt, None, None, None)
+                                        grad, gt, None, None, None, None)
 
     assert call(h=None) == fold3d.FOLD3D_E_INVALID
     assert call(s=None) == fold3d.FOLD3D_E_INVALID

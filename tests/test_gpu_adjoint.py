@@ -56,6 +56,51 @@ def test_adjoint_matches_cpu_fp64(solver, oracle):
     assert (np.abs(g_s - cs) <= 1e-8 * np.abs(cs).max()).all()
 
 
+def test_adjoint_matches_cpu_fp64_on_the_survey_layout_at_64(solver, oracle):
+    """The tomography survey's layout (16 sources on a 4x4 grid below the top
+    face, 64x64 top-face receivers: every top-face node at 64^3), FP64, per-node
+    speed and tilt: the GPU gradient of the misfit (batched forward, batched
+    adjoint of the 16 sources) against the sum over the sources of the CPU
+    adjoint on the same converged fields, to 1e-8 of the largest value (the
+    oracle's node derivatives are central differences, step 1e-6, of the
+    winning simplex with its faces; those of the node's whole local solver
+    average two slopes where candidates of two patches tie within the step,
+    which moves the tilt gradient of one node per source by up to 1e-6 of
+    the maximum).  The reached nodes held fixed are only those of the
+    closed-form starts."""
+    import torch
+    n = 64
+    print("adjoint vs CPU adjoint, 16 sources, 64x64 receivers, 64^3 FP64", flush=True)
+    tomo, f = _tomo(solver, n, torch.float64, 4, 64, batch=16)
+    S = tomo.survey
+    assert len(S.sources) == 16 and S.nrec == 64 * 64
+    s0 = f["slowness"].clone().requires_grad_(True)
+    t0 = f["theta"].clone().requires_grad_(True)
+    J, res, ckpt = tomo.forward_pass(s0.detach(), t0.detach())
+    tomo.misfit(s0, t0).backward()
+    held = list(solver.last_held)
+    h = _host(f)
+    cs = np.zeros((n + 2) ** 3)
+    ct = np.zeros((n + 2) ** 3)
+    starts = []
+    for k in range(16):
+        T = ckpt[k].cpu().numpy()
+        a, b, _ = oracle.ellip_adjoint(T, n, S.h, S.rec_idx.astype(np.int64),
+                                       res[k].cpu().numpy(), speed_field=1.0 / h["slowness"],
+                                       theta=h["theta"], phi=h["phi"], ra=h["ra"], rb=h["rb"])
+        cs += a
+        ct += b
+        starts.append(int((tomo.start_field(k, f["slowness"], f["theta"]) < 1e15).sum()))
+    g_s, g_t = s0.grad.cpu().numpy(), t0.grad.cpu().numpy()
+    es = np.abs(g_s - cs).max() / np.abs(cs).max()
+    et = np.abs(g_t - ct).max() / np.abs(ct).max()
+    print(f"misfit {float(J):.6e}; max |GPU - CPU| / max: slowness {es:.2e}, theta {et:.2e}; "
+          f"held per source {held}, start nodes {starts}")
+    assert np.abs(cs).max() > 0 and np.abs(ct).max() > 0
+    assert es <= 1e-8 and et <= 1e-8
+    assert all(0 <= hk <= sk for hk, sk in zip(held, starts))
+
+
 def _seeded_cells(g, srcs, H, count, seed):
     idx = np.flatnonzero(np.abs(g) > 1e-3 * np.abs(g).max())
     i, j, k = idx % H, (idx // H) % H, idx // (H * H)
@@ -268,4 +313,4 @@ def test_batched_adjoint_rejects_bad_sources(solver):
     o.precision = fold3d.F32
     assert L.fold3d_adjoint_batched(solver.h, 0, (fold3d.AdjointSource * 1)(), n, n, n, 0.1, 0.1,
                                     0.1, C.byref(m), C.byref(o), C.c_void_p(g.data_ptr()), None,
-                                    None, None, None) == fold3d.FOLD3D_E_INVALID
+                                    None, None, None, None) == fold3d.FOLD3D_E_INVALID

@@ -39,10 +39,10 @@ def case(oracle):
     return f, S, obs, T, src, res
 
 
-def _adjoint(oracle, f, S, T, res):
+def _adjoint(oracle, f, S, T, res, local="closure"):
     return oracle.ellip_adjoint(T, S.n, S.h, S.rec_idx.astype(np.int64), res,
                                 speed_field=1.0 / f["slowness"], theta=f["theta"],
-                                phi=f["phi"], ra=f["ra"], rb=f["rb"])
+                                phi=f["phi"], ra=f["ra"], rb=f["rb"], local=local)
 
 
 def _cells(g, src, H, count, seed):
@@ -57,10 +57,14 @@ def _cells(g, src, H, count, seed):
     return rng.choice(idx, min(count, idx.size), replace=False)
 
 
+@pytest.mark.parametrize("local", ["closure", "solver"])
 @pytest.mark.parametrize("name", ["slowness", "theta"])
-def test_adjoint_matches_central_differences(oracle, case, name):
+def test_adjoint_matches_central_differences(oracle, case, name, local):
+    """Both definitions of the node derivatives (the winning simplex with its
+    faces; the node's whole local solver) against central differences of the
+    oracle's own forward solve."""
     f, S, obs, T, src, res = case
-    gs, gt, passes = _adjoint(oracle, f, S, T, res)
+    gs, gt, passes = _adjoint(oracle, f, S, T, res, local)
     assert passes >= 1
     g = gs if name == "slowness" else gt
     cells = _cells(g, src, S.n + 2, 10, seed=7)
@@ -102,7 +106,8 @@ def test_adjoint_is_linear_in_residuals(oracle, case):
     assert np.abs(gab - (2.0 * ga + gb)).max() <= 1e-12 * scale
 
 
-def test_single_receiver_gradient_is_a_path_sensitivity(oracle, case):
+@pytest.mark.parametrize("local", ["closure", "solver"])
+def test_single_receiver_gradient_is_a_path_sensitivity(oracle, case, local):
     """With residual 1 at one receiver, dJ/ds_x = dT_r/ds_x >= 0 (a slower
     node can only delay the arrival), and since every candidate is
     homogeneous of degree 1 in (upwind values, slowness),
@@ -113,7 +118,7 @@ def test_single_receiver_gradient_is_a_path_sensitivity(oracle, case):
         gs, _, _, mu, fixed = oracle.ellip_adjoint(
             T, S.n, S.h, S.rec_idx[r:r + 1].astype(np.int64), np.ones(1),
             speed_field=1.0 / f["slowness"], theta=f["theta"], phi=f["phi"],
-            ra=f["ra"], rb=f["rb"], state=True)
+            ra=f["ra"], rb=f["rb"], state=True, local=local)
         assert (gs >= 0).all()
         assert fixed[src] and fixed.sum() <= 64  # within the 4^3 closed-form start
         euler = float(np.sum(f["slowness"] * gs) + np.sum(mu[fixed] * T[fixed]))

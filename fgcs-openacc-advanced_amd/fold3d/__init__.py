@@ -130,6 +130,8 @@ def lib() -> C.CDLL:
     L.fold3d_solve.argtypes = [vp, vp, i, i, i, d, d, d, C.POINTER(Model),
                                C.POINTER(Opts), C.POINTER(Stats)]
     L.fold3d_plane_step.argtypes = [vp, i, vp, i, i, i, i] + [d] * 10 + [vp]
+    L.fold3d_solve_noalloc.argtypes = [vp, i, i, i, d, d, d, C.POINTER(Model),
+                                       C.POINTER(Opts), C.POINTER(Stats)]
     L.fold3d_measure_peaks.argtypes = [vp, dp, dp, dp]
     L.fold3d_solve_batched.argtypes = [vp, i, C.POINTER(Instance), C.POINTER(Opts),
                                        C.POINTER(i), dp, dp]

@@ -161,6 +161,25 @@ int fold3d_solve(fold3d_handle *h, void *T, int nx, int ny, int nz,
                  double dx, double dy, double dz, const fold3d_model *model,
                  const fold3d_opts *opts, fold3d_stats *stats);
 
+/* fold3d_solve without a handle and without allocating: no hipMalloc, hipFree
+ * or other allocating call, for callers that cannot allocate (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid it).  No
+ * counterpart in the reference.  Fold model, device-resident T, FP64 or FP32;
+ * engine AUTO or SPLIT (others are FOLD3D_E_UNSUPPORTED).  Same bits as
+ * fold3d_solve.  Its scratch is a fixed 32 KiB per GPU in the library's static
+ * device memory (the six directions' constant tables and the tolerance stop's
+ * max|dT| words, whatever the grid), held until the process ends; grids too
+ * large for a device are FOLD3D_E_NOMEM before any launch.  Every call on one
+ * GPU shares that scratch: calls that overlap on one GPU (several streams or
+ * threads) race.  All six directions run on the caller's field in the
+ * reference layout (no second field), so the x-sweeps stage strided rows and
+ * the solve is slower than fold3d_solve's.  With tol > 0 the call waits for
+ * the stream once per iteration (the stop test), so only tol == 0 suits graph
+ * capture.  stats->sweep_ms is 0 (the call creates no events). */
+int fold3d_solve_noalloc(void *T, int nx, int ny, int nz, double dx, double dy, double dz,
+                         const fold3d_model *model, const fold3d_opts *opts,
+                         fold3d_stats *stats);
+
 /* Batched elliptic-model solves: many independent instances, each with its
  * own grid, spacings, metric and speed (constant or per node), in one stream
  * of launches (one launch per sweep direction and plane step serves every

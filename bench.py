@@ -99,8 +99,11 @@ def time_solves(solver, T0d, n, steps, warmup, dtype, torch, engine=0, iters=8, 
     return (time.perf_counter() - t0) * 1e3, T, sweep_ms
 
 
-def ellip_run(solver, n, dtype, tol, torch, threads=0):
-    """The 512^3 tilted-metric, Gaussian-blob speed workload, to tolerance."""
+def ellip_run(solver, n, dtype, tol, torch, threads=0, cpu_full=False):
+    """The 512^3 tilted-metric, Gaussian-blob speed workload, to tolerance.
+    With `threads`, the FP64 run also times the elliptic oracle on the host:
+    its first iteration (bounded sample), or with `cpu_full` the whole solve
+    to the same tolerance."""
     from fold3d import workloads as wl
     W = wl.blob_speed(n, xp=torch, dtype=dtype, device="cuda")
     T0 = wl.point_source_start(W, xp=torch, dtype=dtype, device="cuda")
@@ -130,16 +133,26 @@ def ellip_run(solver, n, dtype, tol, torch, threads=0):
         Tc = T0.cpu().numpy().copy()
         V = W.speed_field.cpu().numpy()
         t0 = time.perf_counter()
-        o.ellip_sweeps(Tc, n, 1, 0.0, W.h, M=W.M, speed=W.speed, speed_field=V)
+        if cpu_full:
+            its, _ = o.ellip_sweeps(Tc, n, 200, tol, W.h, M=W.M, speed=W.speed, speed_field=V)
+        else:
+            its = 1
+            o.ellip_sweeps(Tc, n, 1, 0.0, W.h, M=W.M, speed=W.speed, speed_field=V)
         dt = time.perf_counter() - t0
         T1 = T0.clone()
-        solver.solve(T1, n, spacing=W.h, model=W.model(), max_iters=1)
+        if cpu_full:
+            T1 = T  # the GPU's solve to the same tolerance
+        else:
+            solver.solve(T1, n, spacing=W.h, model=W.model(), max_iters=1)
+        what = (f"the whole solve to {tol:g} ({its} iterations)" if cpu_full else
+                "1 iteration (6 sweeps)")
         out["cpu_baseline"] = {
-            "value": cell_updates(n, 1) / dt / 1e9, "unit": "Gcell-updates/s", "cores": threads,
-            "kind": "port", "sample": f"1 iteration (6 sweeps) of the same {n}^3 FP64 solve, "
-            f"{cell_updates(n, 1):.3e} cell-updates in {dt:.1f} s",
-            "gpu_same_bits_after_1_iteration": bool(np.array_equal(
-                Tc.view(np.uint64), T1.cpu().numpy().view(np.uint64)))}
+            "value": cell_updates(n, its) / dt / 1e9, "unit": "Gcell-updates/s",
+            "cores": threads, "kind": "port",
+            "sample": f"{what} of the same {n}^3 FP64 solve, {cell_updates(n, its):.3e} "
+                      f"cell-updates in {dt:.1f} s",
+            "gpu_same_bits": bool(np.array_equal(Tc.view(np.uint64),
+                                                 T1.cpu().numpy().view(np.uint64)))}
         del T1
     return out, T
 
@@ -516,17 +529,18 @@ def host_cpu():
             "logical_cpus": os.cpu_count()}
 
 
-def cpu_baseline(T0_host, n, threads):
+def cpu_baseline(T0_host, n, threads, iters=1):
     """The oracle (FP64 OpenMP restatement, bit-identical to the reference)
-    on a bounded sample: one of the 8 iterations of the same 512^3 solve."""
+    on `iters` of the 8 iterations of the same 512^3 solve (one by default: a
+    bounded sample; all 8 with --cpu-full)."""
     from oracle.oracle import Oracle
     o = Oracle()
     o.set_threads(threads)
     T = T0_host.copy()
     t0 = time.perf_counter()
-    o.sweeps(T, n, 1)
+    o.sweeps(T, n, iters)
     dt = time.perf_counter() - t0
-    return cell_updates(n, 1) / dt / 1e9, dt, T
+    return cell_updates(n, iters) / dt / 1e9, dt, T
 
 
 def main():
@@ -550,6 +564,9 @@ def main():
     ap.add_argument("--field-n", "--c2-n", dest="field_n", type=int, default=1280,
                     help="side of the per-node tilt-field run (0: skip)")
     ap.add_argument("--iters", type=int, default=8, help="iterations per solve (the reference runs 8)")
+    ap.add_argument("--cpu-full", action="store_true",
+                    help="CPU baselines over the whole solves (8 iterations of the headline, the "
+                         "blob-speed FP64 solve to tolerance) instead of a one-iteration sample")
     ap.add_argument("--fd-cells", type=int, default=32,
                     help="seeded cells per parameter of the tomography gradient's finite-difference "
                          "check at 256^3 (0: skip the check)")
@@ -694,7 +711,8 @@ def main():
         ellip = {}
         for name, dt, tol in (("fp64", torch.float64, 1e-10), ("fp32", torch.float32, 1e-6)):
             log(f"bench: elliptic blob-speed workload {n}^3 {name}, tol {tol}")
-            r, Te = ellip_run(solver, n, dt, tol, torch, threads=cpu_threads)
+            r, Te = ellip_run(solver, n, dt, tol, torch, threads=cpu_threads,
+                              cpu_full=args.cpu_full)
             ellip[name] = r
             if name == "fp64":
                 ref64 = Te
@@ -708,8 +726,7 @@ def main():
             if "cpu_baseline" in r:
                 cb = r["cpu_baseline"]
                 log(f"bench: elliptic {name} CPU ({cb['cores']} threads, 1 iteration): "
-                    f"{cb['value']:.3f} Gcell-upd/s, GPU same bits: "
-                    f"{cb['gpu_same_bits_after_1_iteration']}")
+                    f"{cb['value']:.3f} Gcell-upd/s, GPU same bits: {cb['gpu_same_bits']}")
             del Te
         del ref64
 
@@ -737,17 +754,20 @@ def main():
     cpu = None
     if not args.no_cpu:
         threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or min(16, os.cpu_count() or 1)
-        log(f"bench: CPU baseline, {threads} threads, 1 iteration of {n}^3")
-        v, dt, T_cpu = cpu_baseline(T0, n, threads)
-        # the same iteration on the GPU: the oracle's field is the checker
+        ci = iters if args.cpu_full else 1
+        log(f"bench: CPU baseline, {threads} threads, {ci} iteration(s) of {n}^3")
+        v, dt, T_cpu = cpu_baseline(T0, n, threads, ci)
+        # the same iterations on the GPU: the oracle's field is the checker
         T1 = T0d.clone()
-        solver.solve(T1, n, max_iters=1)
+        solver.solve(T1, n, max_iters=ci)
         same = bool(np.array_equal(T1.cpu().numpy().view(np.uint64), T_cpu.view(np.uint64)))
         cpu = {"value": v, "unit": "Gcell-updates/s", "cores": threads, "kind": "port",
-               "sample": f"1 of the 8 iterations (6 sweeps) of the same {n}^3 FP64 solve, "
-                         f"{cell_updates(n, 1):.3e} cell-updates in {dt:.1f} s",
-               "gpu_same_bits_after_1_iteration": same, "host": host_cpu()}
-        log(f"bench: CPU {v:.3f} Gcell-upd/s; GPU field after that iteration bit-identical: {same}")
+               "sample": f"{ci} of the {iters} iterations (6 sweeps each) of the same {n}^3 "
+                         f"FP64 solve, {cell_updates(n, ci):.3e} cell-updates in {dt:.1f} s",
+               "gpu_same_bits": same, "host": host_cpu(),
+               "threads_note": "the threads one GPU's jobs are granted on the GPU host "
+                               "(OMP_NUM_THREADS there), not all of its physical cores"}
+        log(f"bench: CPU {v:.3f} Gcell-upd/s; GPU field after those iterations bit-identical: {same}")
         del T1, T_cpu
 
     batch = None

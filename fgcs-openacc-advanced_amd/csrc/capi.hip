@@ -27,6 +27,7 @@ struct fold3d_handle {
   int batch_groups;           // batched solve: waves per row (1, 2, 4)
   int batch_blocks_per_cu;    // batched solve: persistent blocks per CU per launch
   int batch_rows_per_pull;    // batched solve: rows per queue pull (0 = automatic)
+  int lock_from;              // locked engine: first locked iteration (0-based)
 };
 
 #define F3D_CHECK_DEV(p, what)                                                             \
@@ -98,6 +99,11 @@ static bool try_scratch(fold3d_handle *h, size_t bytes) {
 
 static constexpr size_t kConstBytes = 6 * sizeof(FoldFast<double>);
 
+// First locked iteration of FOLD3D_ENGINE_LOCKED by default: the reference's
+// 8 iterations run unlocked (most rows still change in them, and a locked
+// launch costs more per computed row), the iterations after them locked.
+static constexpr int kDefaultLockFrom = 8;
+
 // False when one FP64 field of the padded grid would exceed 2^56 bytes (far
 // beyond any device, and below the point where the size arithmetic of the
 // engines, 15 fields of it, could overflow 64 bits).
@@ -146,6 +152,7 @@ int fold3d_create(fold3d_handle **out, int device) {
   h->batch_groups = 1;
   h->batch_blocks_per_cu = 64;
   h->batch_rows_per_pull = -1;
+  h->lock_from = kDefaultLockFrom;
   hipError_t e = hipMalloc((void **)&h->d_max, sizeof(unsigned long long) * kMaxSlots);
   if (e == hipSuccess) e = hipEventCreate(&h->ev0);
   if (e == hipSuccess) e = hipEventCreate(&h->ev1);
@@ -253,7 +260,7 @@ int fold3d_solve(fold3d_handle *h, void *T, int nx, int ny, int nz, double dx,
   double maxdt = -1;
   F3D_HIP(hipEventRecord(h->ev0, st));
   int rc;
-  unsigned long long *done_ctr = nullptr;  // cell-updates computed (locking only)
+  unsigned long long *done_ctr = nullptr;  // cell-updates skipped (locking only)
   if (model->kind == FOLD3D_MODEL_FOLD && o.engine == FOLD3D_ENGINE_PLANE) {
     rc = o.precision == FOLD3D_F64
              ? fold_sweeps_plane_engine<double>(st, (double *)dT, G, *model, hsp, o.max_iters,
@@ -282,26 +289,25 @@ int fold3d_solve(fold3d_handle *h, void *T, int nx, int ny, int nz, double dx,
                                                 o.max_iters, o.tol, h->d_max, &iters, &maxdt);
     } else {
       const bool lock = o.engine == FOLD3D_ENGINE_LOCKED;
-      // scratch: T copy [, stamps in two layouts, computed-updates counter]
+      // scratch: T copy [, line-chunk stamps, computed-updates counter]
       const size_t tbytes = (bytes + 255) & ~(size_t)255;
-      const size_t sbytes = ((size_t)G.count * sizeof(int) + 255) & ~(size_t)255;
-      rc = ensure_scratch(h, lock ? tbytes + 2 * sbytes + 256 * 8 : bytes);
+      const size_t sbytes = (line_stamp_words(G) * sizeof(int) + 255) & ~(size_t)255;
+      rc = ensure_scratch(h, lock ? tbytes + sbytes + 256 * 8 : bytes);
       if (rc != FOLD3D_OK) return rc;
       char *scr = (char *)h->d_scratch;
-      int *stamp = lock ? (int *)(scr + tbytes) : nullptr;
-      int *stampY = lock ? (int *)(scr + tbytes + sbytes) : nullptr;
-      done_ctr = lock ? (unsigned long long *)(scr + tbytes + 2 * sbytes) : nullptr;
+      int *stamps = lock ? (int *)(scr + tbytes) : nullptr;
+      done_ctr = lock ? (unsigned long long *)(scr + tbytes + sbytes) : nullptr;
       if (lock) F3D_HIP(hipMemsetAsync(done_ctr, 0, 256 * sizeof(unsigned long long), st));
       const int groups = o.engine == FOLD3D_ENGINE_TILED ? 0 : h->split_groups;
       rc = o.precision == FOLD3D_F64
                ? fold_sweeps_tiled_engine<double>(st, (double *)dT, (double *)h->d_scratch, G,
                                                   (const FoldFast<double> *)h->d_consts,
                                                   o.max_iters, o.tol, groups, h->d_max, &iters,
-                                                  &maxdt, stamp, stampY, done_ctr)
+                                                  &maxdt, stamps, done_ctr, h->lock_from)
                : fold_sweeps_tiled_engine<float>(st, (float *)dT, (float *)h->d_scratch, G,
                                                  (const FoldFast<float> *)h->d_consts,
                                                  o.max_iters, o.tol, groups, h->d_max, &iters,
-                                                 &maxdt, stamp, stampY, done_ctr);
+                                                 &maxdt, stamps, done_ctr, h->lock_from);
     }
   } else if (model->theta == nullptr && o.engine != FOLD3D_ENGINE_PLANE &&
              o.engine != FOLD3D_ENGINE_HOISTED) {
@@ -366,7 +372,7 @@ int fold3d_solve(fold3d_handle *h, void *T, int nx, int ny, int nz, double dx,
       F3D_HIP(hipMemcpy(done, done_ctr, sizeof done, hipMemcpyDeviceToHost));
       double sum = 0;
       for (int k = 0; k < 256; ++k) sum += (double)done[k];
-      stats->updates_computed = sum;
+      stats->updates_computed = stats->cell_updates - sum;  // the counters hold the skipped ones
     }
     const bool ellip = model->kind == FOLD3D_MODEL_ELLIP;
     const int params = ellip ? (model->speed_field ? 1 : 0) + (model->theta ? 4 : 0) : 0;
@@ -720,6 +726,20 @@ int fold3d_set_tuning(fold3d_handle *h, int model_kind, int groups) {
               "fold3d_set_tuning: elliptic-model groups must be 0, 1, 2 or 4");
     h->ellip_groups = groups == 0 ? 2 : groups;
   }
+  return FOLD3D_OK;
+}
+
+int fold3d_set_lock_start(fold3d_handle *h, int iteration) {
+  F3D_CHECK(h != nullptr, FOLD3D_E_INVALID, "fold3d_set_lock_start: NULL handle");
+  F3D_CHECK(iteration >= -1, FOLD3D_E_INVALID,
+            "fold3d_set_lock_start: iteration must be >= 0 (or -1 for the default)");
+  h->lock_from = iteration < 0 ? kDefaultLockFrom : iteration;
+  return FOLD3D_OK;
+}
+
+int fold3d_get_lock_start(fold3d_handle *h, int *iteration) {
+  F3D_CHECK(h && iteration, FOLD3D_E_INVALID, "fold3d_get_lock_start: NULL argument");
+  *iteration = h->lock_from;
   return FOLD3D_OK;
 }
 

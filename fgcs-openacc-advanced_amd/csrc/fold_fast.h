@@ -70,6 +70,15 @@ __device__ __forceinline__ double fast_sqrt<double>(double x) { return sqrt_rn(x
 template <>
 __device__ __forceinline__ float fast_sqrt<float>(float x) { return __builtin_sqrtf(x); }
 
+// The smaller of two magnitudes of the field or two candidates.  These are
+// non-negative, never -0 (magnitudes are fabs() of the field; an accepted
+// candidate exceeds the smallest vertex) and never NaN for a field without
+// NaN, so fmin gives the value rmin gives, in one v_min instruction where
+// rmin's compare and two selects take three.  (In the foot-point test
+// min(fx, fy) a NaN operand makes the test fail either way.)
+__device__ __forceinline__ double vmin(double a, double b) { return fmin(a, b); }
+__device__ __forceinline__ float vmin(float a, float b) { return fminf(a, b); }
+
 // One edge call site (FoldEdge2Points with fixed model/spacing arguments).
 template <typename R>
 struct EdgeSite {
@@ -110,15 +119,15 @@ struct FoldFast {
 
 template <typename R>
 __device__ __forceinline__ void fast_edge(R &tnew, R st, R xt, R F, const EdgeSite<R> &e) {
-  if (tnew < rmin(xt, st) || (e.ax_pos && xt > st)) return;
-  if (!(rmin(xt, st) + e.lgap < tnew * prune_slack<R>())) return;  // cannot go below tnew
+  if (tnew < vmin(xt, st) || (e.ax_pos && xt > st)) return;
+  if (!(vmin(xt, st) + e.lgap < tnew * prune_slack<R>())) return;  // cannot go below tnew
   const R gx = div_const(xt - st, e.dxx, e.rdxx);
   const R q = R(1) - gx * e.ax;
   const R disc = e.ga * (q * q - gx * gx * e.gaz);
   if (R(0) < disc) {
     const R gz = div_const(-e.az * q + fast_sqrt(disc), e.gaz, e.rgaz);
     const R cand = st + e.dzz * gz;
-    if (cand < tnew && cand > rmin(st, xt)) {
+    if (cand < tnew && cand > vmin(st, xt)) {
       R t = (gx * gx + gz * gz) * F;
       t = t * F;
       const R g = fast_sqrt(div_const(t, e.ga, e.rga));
@@ -126,14 +135,14 @@ __device__ __forceinline__ void fast_edge(R &tnew, R st, R xt, R F, const EdgeSi
       if (R(0) < foot && foot < e.dxx) tnew = cand;
     }
   }
-  if (tnew > xt && e.k1_ok) tnew = rmin(tnew, xt + e.k1);
+  if (tnew > xt && e.k1_ok) tnew = vmin(tnew, xt + e.k1);
 }
 
 template <typename R>
 __device__ __forceinline__ void fast_tri(R &tnew, R st, R xt, R yt, const FoldSign<R> &S,
                                          const TriSite<R> &t, const EdgeSite<R> &sub) {
-  if (tnew < rmin(st, rmin(xt, yt)) || (t.ax_pos && xt > st) || (t.ay_pos && yt > xt)) return;
-  if (!(rmin(st, rmin(xt, yt)) + S.gap < tnew * prune_slack<R>())) return;  // nor its sub-edge
+  if (tnew < vmin(st, vmin(xt, yt)) || (t.ax_pos && xt > st) || (t.ay_pos && yt > xt)) return;
+  if (!(vmin(st, vmin(xt, yt)) + S.gap < tnew * prune_slack<R>())) return;  // nor its sub-edge
   const R F = S.F, az = S.az, dzz = S.dzz;
   const R gx = div_const(xt - st, t.dxx, t.rdxx);
   const R gy = div_const(yt - xt, t.dyy, t.rdyy);
@@ -143,15 +152,15 @@ __device__ __forceinline__ void fast_tri(R &tnew, R st, R xt, R yt, const FoldSi
   if (disc > R(0)) {
     const R gz = div_const(-az * al + F * fast_sqrt(disc), S.ga, S.rga);
     const R cand = st + dzz * gz;
-    if (cand < tnew && cand > rmin(rmin(st, xt), yt)) {
+    if (cand < tnew && cand > vmin(vmin(st, xt), yt)) {
       const R g = fast_sqrt(g2 + gz * gz);
       const R s = -dzz / (F * gz + az * g);
       const R fx = s * (F * gx + t.ax * g);
       const R fy = s * (F * gy + t.ay * g);
-      if (R(0) <= rmin(fx, fy) && fy * t.dxx <= fx * t.dyy && fx <= t.dxx) tnew = cand;
+      if (R(0) <= vmin(fx, fy) && fy * t.dxx <= fx * t.dyy && fx <= t.dxx) tnew = cand;
     }
   }
-  if (tnew > rmin(xt, yt) && (!(t.sub_pos && xt > yt))) fast_edge(tnew, xt, yt, F, sub);
+  if (tnew > vmin(xt, yt) && (!(t.sub_pos && xt > yt))) fast_edge(tnew, xt, yt, F, sub);
 }
 
 // Same result as fold_node_update(), bit for bit.
@@ -159,9 +168,9 @@ template <typename R>
 __device__ __forceinline__ R fold_node_fast(R told, const Patch<R> &p, const FoldSign<R> &S) {
   R tnew = fabs(told);
   const R st = p.c;
-  if (st < tnew && S.k0_ok) tnew = rmin(tnew, st + S.k0);
-  if (!(tnew < st && tnew < rmin(p.px, p.py) && tnew < rmin(p.pxpy, p.pxmy) &&
-        tnew < rmin(p.mxmy, p.mxpy))) {
+  if (st < tnew && S.k0_ok) tnew = vmin(tnew, st + S.k0);
+  if (!(tnew < st && tnew < vmin(p.px, p.py) && tnew < vmin(p.pxpy, p.pxmy) &&
+        tnew < vmin(p.mxmy, p.mxpy))) {
     // (centre, first, second) vertices of the eight triangles, reference order
     fast_tri(tnew, st, p.px, p.pxpy, S, S.tri[0], S.edge[8]);
     fast_tri(tnew, st, p.px, p.pxmy, S, S.tri[1], S.edge[9]);
@@ -191,7 +200,7 @@ __device__ __forceinline__ R fold_node_fast(R told, const Patch<R> &p, const Fol
 template <typename R>
 __device__ __forceinline__ R fold_start(R told, R st, const FoldSign<R> &S) {
   R tnew = fabs(told);
-  if (st < tnew && S.k0_ok) tnew = rmin(tnew, st + S.k0);
+  if (st < tnew && S.k0_ok) tnew = vmin(tnew, st + S.k0);
   return tnew;
 }
 
@@ -199,8 +208,8 @@ __device__ __forceinline__ R fold_start(R told, R st, const FoldSign<R> &S) {
 // keeps its one-point value.
 template <typename R>
 __device__ __forceinline__ bool fold_gate_open(R tnew, const Patch<R> &p) {
-  return !(tnew < p.c && tnew < rmin(p.px, p.py) && tnew < rmin(p.pxpy, p.pxmy) &&
-           tnew < rmin(p.mxmy, p.mxpy));
+  return !(tnew < p.c && tnew < vmin(p.px, p.py) && tnew < vmin(p.pxpy, p.pxmy) &&
+           tnew < vmin(p.mxmy, p.mxpy));
 }
 
 // Unit u (uniform across the wave): triangle u with sub-edge 8+u, and edge E(u)

@@ -374,72 +374,159 @@ __device__ __forceinline__ R split_dispatch(int grp, R told, const Patch<R> &p,
 
 constexpr int kLockCounters = 256;
 
-// Locking (kLock): stamp[x] is the sweep index sigma = 6*iteration + direction
-// at which node x last changed (0 before any change).  A row of targets was
-// last updated in this direction at sweep sigma - 6, after that sweep had
-// finished the source plane; if no node of its three source rows has changed
-// since (every stamp <= sigma - 6), its candidates are the ones it saw then,
-// all >= its value, so the update would return the value unchanged and the
-// row is skipped.  Same bits as without locking.
-template <typename R, bool kReduce, int kGroups, bool kLock>
-__global__ __launch_bounds__(64 * kGroups) void fold_split_kernel(
+// Shape of a split-kernel block, by waves per row.  A wave row covers 2 node
+// rows of a 32-node chunk (a wave: 64 nodes); a block holds split_rows() wave
+// rows and stages their source rows plus a halo once.  Two wave rows for 2
+// waves per row (4 waves per block), one otherwise.  Measured at 512^3 with 2
+// waves per row: 64-node chunks 311 ms per solve with one row per block, 305
+// with two; 32-node chunks 290 (16-node chunks 291); with 4 and 8 waves per
+// row (384^3, 256^3) 32-node chunks were 3 % faster than 64.  The blocks of a
+// chunk column keep their XCD (linear block index mod 8), so with 32-node
+// chunks each XCD owns every eighth chunk of every row: the work of a busy
+// region spreads over more XCDs while each chunk's source rows stay in its
+// XCD's L2 from plane to plane.
+constexpr int split_rows(int groups) { return groups == 2 ? 2 : 1; }
+constexpr int split_width(int) { return 32; }
+
+// Exact locking (kLock).  A stamp is the sweep index sigma = 6*iteration +
+// direction at which a node last changed (0 before any change), kept not per
+// node but per line chunk: 64 consecutive nodes of one x-line (j, k fixed) or
+// of one y-line (i, k fixed), in global coordinates, so the stamps need no
+// moving when the field changes layout.  A node that changes writes sigma
+// into its x-line chunk and its y-line chunk.  The rows of every layout are
+// x-lines (reference and y-sweep layouts) or y-lines (x-sweep layout); the
+// lanes of a row write one word of their own family and consecutive words of
+// the other (x-line chunks are indexed (k, i/64, j), y-line chunks (k, j/64,
+// i)).  A row of targets was last updated in this direction at sweep
+// sigma - 6, after that sweep had finished the source plane; if no chunk its
+// source patch touches has changed since (every stamp <= sigma - 6), its
+// candidates are the ones it saw then, all >= its value, so the update would
+// return the value unchanged and the row is skipped.  Same bits as without
+// locking.
+struct LineStamps {
+  int *xs, *ys;   // x-line chunks ((k chx + i/64) (ny+2) + j), y-line chunks ((k chy + j/64) (nx+2) + i)
+  int chx, chy;   // chunks per x-line, per y-line
+  int hx, hy;     // nx + 2, ny + 2
+};
+__device__ __forceinline__ int *line_stamp(const LineStamps &L, bool yline, int i, int j, int k) {
+  return yline ? L.ys + ((long long)k * L.chy + (j >> 6)) * L.hx + i
+               : L.xs + ((long long)k * L.chx + (i >> 6)) * L.hy + j;
+}
+// Where one plane launch sits in global coordinates: the lane axis la (0 or 1:
+// the rows are x-lines or y-lines), the row axis lb, the march axis m, the
+// source and target plane indices.
+struct LockGeom {
+  LineStamps L;
+  int la, lb, m;
+  int psrc, ptgt;
+  int sigma;
+  unsigned long long *updates_skipped;  // cell-updates skipped (kLockCounters words)
+};
+
+// A block owns kRows wave rows of one kW-node chunk (blockIdx.x): a wave row
+// is 64 / kW node rows of the chunk (lane l: node l % kW of row l / kW), and
+// its kGroups waves are threadIdx.y = r * kGroups + grp.  Blocks go to the
+// XCDs round-robin by their linear index, so a chunk stays on one XCD from
+// plane to plane and its source rows are still in that XCD's L2 from the
+// launch that wrote them.  (Rotating the chunk by the row group, so that each
+// XCD gets every chunk, balanced the XCDs' work but was 3-4 % slower overall.)
+template <typename R, bool kReduce, int kGroups, bool kLock, int kRows, int kW>
+__global__ __launch_bounds__(64 * kGroups * kRows) void fold_split_kernel(
     R *__restrict__ T, PlaneGeom g, const FoldFast<R> *__restrict__ C,
-    unsigned long long *__restrict__ max_dt, int *__restrict__ stamp, int sigma,
-    unsigned long long *__restrict__ updates_done) {
-  __shared__ R s[3][64 + 2];
-  __shared__ R part[kGroups][64];
+    unsigned long long *__restrict__ max_dt, LockGeom lk) {
+  constexpr int kRpw = 64 / kW;         // node rows per wave row
+  constexpr int kNR = kRows * kRpw;     // node rows per block
+  __shared__ R s[kNR + 2][kW + 2];
+  __shared__ R part[kRows][kGroups][64];
   const int tx = threadIdx.x;
-  const int grp = __builtin_amdgcn_readfirstlane(threadIdx.y);
-  const int a0 = blockIdx.x * 64, b = 1 + blockIdx.y;
+  const int wy = __builtin_amdgcn_readfirstlane(threadIdx.y);
+  const int grp = wy % kGroups, row = wy / kGroups;
+  const int a0 = blockIdx.x * kW, b0 = 1 + blockIdx.y * kNR;
   int changed = 0;
-  for (int idx = grp * 64 + tx; idx < 3 * 66; idx += 64 * kGroups) {
-    const int la = idx % 66, lb = idx / 66;
-    const int sa = a0 + la;
-    if (sa <= g.na + 1) {
-      const long long x = g.base + (long long)sa * g.Sa + (long long)(b - 1 + lb) * g.Sb;
-      s[lb][la] = fabs(T[x]);
-      if (kLock) changed |= stamp[x] > sigma - 6;
+  if (kLock && wy == 0 && tx < 2 * (kNR + 2)) {
+    // the stamps of the 64-node line chunks the staged rows touch (columns
+    // a0 .. a0 + kW + 1), loaded beside the staging loads
+    const int r = tx >> 1, ca = (tx & 1) ? a0 + kW + 1 : a0, sb = b0 - 1 + r;
+    if (ca <= g.na + 1 && sb <= g.nb + 1) {
+      int c3[3];
+      c3[lk.la] = ca;
+      c3[lk.lb] = sb;
+      c3[lk.m] = lk.psrc;
+      changed = *line_stamp(lk.L, lk.la == 1, c3[0], c3[1], c3[2]) > lk.sigma - 6;
     }
   }
-  if (kLock) {
-    if (sigma >= 6 && !__syncthreads_or(changed)) return;  // uniform over the block
-    if (tx == 0 && grp == 0)  // spread over kLockCounters addresses
-      atomicAdd(&updates_done[(blockIdx.x + 7u * blockIdx.y) % kLockCounters],
-                (unsigned long long)(g.na - a0 < 64 ? g.na - a0 : 64));
+  // Locking: the block votes on its stamps before it loads anything else, so
+  // a skipped block costs one small load.  (Locking runs once most rows are
+  // idle; loading the staged values beside the stamps instead costs a busy
+  // block nothing but leaves an idle one waiting for them.)
+  if (kLock && lk.sigma >= 6 && !__syncthreads_or(changed)) {  // uniform over the block
+    if (tx == 0 && wy == 0) {  // skipped updates, spread over kLockCounters addresses
+      const int nrows = g.nb - b0 + 1 < kNR ? g.nb - b0 + 1 : kNR;  // rows in the plane
+      atomicAdd(&lk.updates_skipped[(blockIdx.x + 7u * blockIdx.y) % kLockCounters],
+                (unsigned long long)(g.na - a0 < kW ? g.na - a0 : kW) * nrows);
+    }
+    return;
   }
-  const int a = a0 + 1 + tx;
-  const bool active = a <= g.na;
+  // Staging: the source rows go through registers into LDS, their loads in
+  // flight together with the load of the target value.
+  constexpr int kStage = (kNR + 2) * (kW + 2), kThreads = 64 * kGroups * kRows;
+  constexpr int kPer = (kStage + kThreads - 1) / kThreads;
+  R sv[kPer];
+#pragma unroll
+  for (int q = 0; q < kPer; ++q) {
+    const int idx = wy * 64 + tx + q * kThreads;
+    const int la = idx % (kW + 2), lb = idx / (kW + 2);
+    const int sa = a0 + la, sb = b0 - 1 + lb;
+    sv[q] = (idx < kStage && sa <= g.na + 1 && sb <= g.nb + 1)
+                ? T[g.base + (long long)sa * g.Sa + (long long)sb * g.Sb]
+                : R(0);
+  }
+  const int la_ = tx % kW, lb_ = row * kRpw + tx / kW;  // the lane's node in the block
+  const int a = a0 + 1 + la_, b = b0 + lb_;
+  const bool active = a <= g.na && b <= g.nb;
   const long long c = g.base + (long long)a * g.Sa + (long long)b * g.Sb;
   const R told = active ? T[c + g.sm] : R(0);
+#pragma unroll
+  for (int q = 0; q < kPer; ++q) {
+    const int idx = wy * 64 + tx + q * kThreads;
+    if (idx < kStage) s[idx / (kW + 2)][idx % (kW + 2)] = fabs(sv[q]);
+  }
   __syncthreads();
   R tn = fabs(told);
   if (active) {
-    const int ca = tx + 1;
+    const int ca = la_ + 1, cb = lb_ + 1;
     const int ua = g.ua, ub = g.ub, va = g.va, vb = g.vb;
     Patch<R> p;
-    p.c = s[1][ca];
-    p.px = s[1 + ub][ca + ua];
-    p.py = s[1 + vb][ca + va];
-    p.mx = s[1 - ub][ca - ua];
-    p.my = s[1 - vb][ca - va];
-    p.pxpy = s[1 + ub + vb][ca + ua + va];
-    p.pxmy = s[1 + ub - vb][ca + ua - va];
-    p.mxpy = s[1 - ub + vb][ca - ua + va];
-    p.mxmy = s[1 - ub - vb][ca - ua - va];
+    p.c = s[cb][ca];
+    p.px = s[cb + ub][ca + ua];
+    p.py = s[cb + vb][ca + va];
+    p.mx = s[cb - ub][ca - ua];
+    p.my = s[cb - vb][ca - va];
+    p.pxpy = s[cb + ub + vb][ca + ua + va];
+    p.pxmy = s[cb + ub - vb][ca + ua - va];
+    p.mxpy = s[cb - ub + vb][ca - ua + va];
+    p.mxmy = s[cb - ub - vb][ca - ua - va];
     tn = split_dispatch<R, kGroups>(grp, told, p, C);  // grp is wave-uniform
   }
-  part[grp][tx] = tn;
+  part[row][grp][tx] = tn;
   __syncthreads();
   if (grp == 0) {
     R d = R(0);
     if (active) {
-      R m = part[0][tx];
+      R m = part[row][0][tx];
 #pragma unroll
-      for (int k = 1; k < kGroups; ++k) m = rmin(part[k][tx], m);
+      for (int k = 1; k < kGroups; ++k) m = vmin(part[row][k][tx], m);
       const R res = ((told > R(0)) ? R(1) : R(-1)) * m;
       if (res != told) {
         T[c + g.sm] = res;
-        if (kLock) stamp[c + g.sm] = sigma;
+        if (kLock) {
+          int c3[3];
+          c3[lk.la] = a;
+          c3[lk.lb] = b;
+          c3[lk.m] = lk.ptgt;
+          *line_stamp(lk.L, false, c3[0], c3[1], c3[2]) = lk.sigma;
+          *line_stamp(lk.L, true, c3[0], c3[1], c3[2]) = lk.sigma;
+        }
       }
       if (kReduce) d = fabs(res - told);
     }
@@ -452,31 +539,39 @@ __global__ __launch_bounds__(64 * kGroups) void fold_split_kernel(
   }
 }
 
+__global__ void fill_int_kernel(int *__restrict__ p, long long n, int v) {
+  for (long long k = blockIdx.x * (long long)blockDim.x + threadIdx.x; k < n;
+       k += (long long)gridDim.x * blockDim.x)
+    p[k] = v;
+}
+
 // Locking state of one solve (null stamps: no locking).
 struct LockState {
-  int *stamp;                          // current layout
+  LineStamps L;                        // L.xs == nullptr: no locking
   int sigma;                           // sweep index of the launch
-  unsigned long long *updates_done;    // cell-updates computed (device counter)
+  unsigned long long *updates_skipped;    // cell-updates skipped (device counter)
 };
 
 template <typename R, int kGroups>
 static void launch_split(hipStream_t st, R *T, const PlaneGeom &g, const FoldFast<R> *C,
-                         unsigned long long *max_dt, const LockState &L) {
-  const dim3 block(64, kGroups);
-  const dim3 grid((g.na + 63) / 64, g.nb);
-  if (L.stamp) {
+                         unsigned long long *max_dt, const LockState &L, const LockGeom &lk) {
+  constexpr int kR = split_rows(kGroups), kW = split_width(kGroups);
+  constexpr int kNR = kR * (64 / kW);
+  const dim3 block(64, kGroups * kR);
+  const dim3 grid((g.na + kW - 1) / kW, (g.nb + kNR - 1) / kNR);
+  if (L.L.xs) {
     if (max_dt)
-      fold_split_kernel<R, true, kGroups, true><<<grid, block, 0, st>>>(T, g, C, max_dt, L.stamp,
-                                                                         L.sigma, L.updates_done);
+      fold_split_kernel<R, true, kGroups, true, kR, kW><<<grid, block, 0, st>>>(T, g, C, max_dt,
+                                                                              lk);
     else
-      fold_split_kernel<R, false, kGroups, true><<<grid, block, 0, st>>>(T, g, C, nullptr, L.stamp,
-                                                                          L.sigma, L.updates_done);
+      fold_split_kernel<R, false, kGroups, true, kR, kW><<<grid, block, 0, st>>>(T, g, C, nullptr,
+                                                                               lk);
   } else if (max_dt) {
-    fold_split_kernel<R, true, kGroups, false><<<grid, block, 0, st>>>(T, g, C, max_dt, nullptr, 0,
-                                                                        nullptr);
+    fold_split_kernel<R, true, kGroups, false, kR, kW><<<grid, block, 0, st>>>(T, g, C, max_dt,
+                                                                             lk);
   } else {
-    fold_split_kernel<R, false, kGroups, false><<<grid, block, 0, st>>>(T, g, C, nullptr, nullptr,
-                                                                         0, nullptr);
+    fold_split_kernel<R, false, kGroups, false, kR, kW><<<grid, block, 0, st>>>(T, g, C, nullptr,
+                                                                              lk);
   }
 }
 
@@ -487,15 +582,21 @@ static void tiled_direction(hipStream_t st, R *T, const Grid &G, int d,
                             const LockState &L) {
   const Frame &fr = kFrames[d];
   const int n = G.n[fr.m];
+  // lane and row axes of this layout's planes (as plane_geom picks them)
+  const bool u_first = G.stride[fr.ux] < G.stride[fr.uy];
+  LockGeom lk{L.L, u_first ? fr.ux : fr.uy, u_first ? fr.uy : fr.ux, fr.m, 0, 0, L.sigma,
+              L.updates_skipped};
   for (int q = 0; q < n - 1; ++q) {
     const int p = fr.s > 0 ? 1 + q : n - q;
     const PlaneGeom g = plane_geom(G, d, p);
+    lk.psrc = p;
+    lk.ptgt = p + fr.s;
     if (groups == 2) {
-      launch_split<R, 2>(st, T, g, C, max_dt, L);
+      launch_split<R, 2>(st, T, g, C, max_dt, L, lk);
     } else if (groups == 4) {
-      launch_split<R, 4>(st, T, g, C, max_dt, L);
+      launch_split<R, 4>(st, T, g, C, max_dt, L, lk);
     } else if (groups == 8) {
-      launch_split<R, 8>(st, T, g, C, max_dt, L);
+      launch_split<R, 8>(st, T, g, C, max_dt, L, lk);
     } else {
       const dim3 block(kTileA, kTileB);
       const dim3 grid((g.na + kTileA - 1) / kTileA, (g.nb + kTileB - 1) / kTileB);
@@ -511,30 +612,46 @@ template <typename R>
 int fold_sweeps_tiled_engine(hipStream_t st, R *T, R *Y, const Grid &G,
                              const FoldFast<R> *d_consts, int max_iters, double tol, int groups,
                              unsigned long long *d_max, int *iters_out, double *max_dt_out,
-                             int *stamp, int *stampY, unsigned long long *updates_done) {
+                             int *stamps, unsigned long long *updates_skipped,
+                             int lock_from) {
   const Grid GX = G.with_order(kOrderX), GZ = G.with_order(kOrderZ), GY = G.with_order(kOrderY);
-  const bool lock = stamp != nullptr && groups > 0;
-  if (lock) F3D_HIP(hipMemsetAsync(stamp, 0, sizeof(int) * G.count, st));
-  FieldMover<R> mv(st, T, Y, GZ, lock ? stamp : nullptr, lock ? stampY : nullptr);
+  const bool lock = stamps != nullptr && groups > 0;
+  LineStamps L{nullptr, nullptr, 0, 0, G.n[0] + 2, G.n[1] + 2};
+  if (lock) {
+    L.chx = line_chunks(G.n[0]);
+    L.chy = line_chunks(G.n[1]);
+    L.xs = stamps;
+    L.ys = stamps + line_stamp_count(G, false);
+  }
+  const LineStamps none{nullptr, nullptr, 0, 0, 0, 0};
+  FieldMover<R> mv(st, T, Y, GZ);
   const bool extra = FieldMover<R>::extra_move_first(max_iters, tol);
-  auto dir = [&](const Grid &Gd, int it, int d, unsigned long long *md) {
-    tiled_direction<R>(st, mv.field(), Gd, d, d_consts + d, md, groups,
-                       LockState{lock ? mv.stamps() : nullptr, 6 * it + d, updates_done});
-  };
   int it = 0;
+  auto dir = [&](const Grid &Gd, int d, unsigned long long *md) {
+    tiled_direction<R>(st, mv.field(), Gd, d, d_consts + d, md, groups,
+                       LockState{lock && it >= lock_from ? L : none, 6 * it + d,
+                                 updates_skipped});
+  };
   double last = -1.0;
   for (; it < max_iters;) {
     unsigned long long *md = tol > 0 ? d_max : nullptr;
     if (md) max_dt_reset_kernel<<<1, kMaxSlots, 0, st>>>(d_max);
+    if (lock && it == lock_from) {
+      // Locking starts: every chunk counts as changed at the last sweep before
+      // (6 it - 1; 0 at the start, when sigma < 6 forbids skipping anyway), a
+      // bound of the sweep of any change so far.
+      fill_int_kernel<<<1024, 256, 0, st>>>(stamps, (long long)line_stamp_words(G),
+                                            it > 0 ? 6 * it - 1 : 0);
+    }
     mv.to(GX);  // x_down, x_up with x slowest
-    dir(GX, it, FOLD3D_X_DOWN, md);
-    dir(GX, it, FOLD3D_X_UP, md);
+    dir(GX, FOLD3D_X_DOWN, md);
+    dir(GX, FOLD3D_X_UP, md);
     mv.to(GZ);  // k_up, k_down in the reference layout
-    dir(GZ, it, FOLD3D_K_UP, md);
-    dir(GZ, it, FOLD3D_K_DOWN, md);
+    dir(GZ, FOLD3D_K_UP, md);
+    dir(GZ, FOLD3D_K_DOWN, md);
     mv.to(GY);  // y_up, y_down with y slowest
-    dir(GY, it, FOLD3D_Y_UP, md);
-    dir(GY, it, FOLD3D_Y_DOWN, md);
+    dir(GY, FOLD3D_Y_UP, md);
+    dir(GY, FOLD3D_Y_DOWN, md);
     if (it == 0 && extra) mv.to(GZ);
     F3D_HIP(hipGetLastError());
     ++it;
@@ -557,7 +674,7 @@ template <typename R>
 int fold_sweeps_inplace_engine(hipStream_t st, R *T, const Grid &G, const FoldFast<R> *d_consts,
                                int max_iters, double tol, int groups, unsigned long long *d_max,
                                int *iters_out, double *max_dt_out) {
-  const LockState none{nullptr, 0, nullptr};
+  const LockState none{LineStamps{nullptr, nullptr, 0, 0, 0, 0}, 0, nullptr};
   int it = 0;
   double last = -1.0;
   for (; it < max_iters;) {
@@ -584,11 +701,11 @@ template int fold_sweeps_inplace_engine<float>(hipStream_t, float *, const Grid 
 
 template int fold_sweeps_tiled_engine<double>(hipStream_t, double *, double *, const Grid &,
                                               const FoldFast<double> *, int, double, int,
-                                              unsigned long long *, int *, double *, int *, int *,
-                                              unsigned long long *);
+                                              unsigned long long *, int *, double *, int *,
+                                              unsigned long long *, int);
 template int fold_sweeps_tiled_engine<float>(hipStream_t, float *, float *, const Grid &,
                                              const FoldFast<float> *, int, double, int,
-                                             unsigned long long *, int *, double *, int *, int *,
-                                             unsigned long long *);
+                                             unsigned long long *, int *, double *, int *,
+                                             unsigned long long *, int);
 
 }  // namespace fold3d

@@ -70,16 +70,28 @@ int fold_sweeps_fast_engine(hipStream_t st, R *T, const Grid &G, const FoldFast<
 // scratch field of G.count elements.
 // `groups`: 0 = one thread per node (fold_tile_kernel); 2, 4 or 8 = the
 // node's eight candidate units spread over that many waves (fold_split_kernel).
-// `stamp`, `stampY` (G.count ints each, or null): exact locking of rows whose
-// source rows have not changed since their last update in the same direction;
-// `updates_done` (device, 256 counters zeroed by the caller) counts the
-// cell-updates computed.
+// `stamps` (line_stamp_words(G) ints, or null): exact locking of rows whose
+// source patch has not changed since their last update in the same direction;
+// `updates_skipped` (device, 256 counters zeroed by the caller) counts the
+// cell-updates the locking skipped; locking starts at iteration `lock_from`
+// (0-based; the iterations before it run unlocked).
 template <typename R>
 int fold_sweeps_tiled_engine(hipStream_t st, R *T, R *Y, const Grid &G,
                              const FoldFast<R> *d_consts, int max_iters, double tol, int groups,
                              unsigned long long *d_max, int *iters_out, double *max_dt_out,
-                             int *stamp = nullptr, int *stampY = nullptr,
-                             unsigned long long *updates_done = nullptr);
+                             int *stamps = nullptr, unsigned long long *updates_skipped = nullptr,
+                             int lock_from = 0);
+
+// Locking stamps: one int per 64-node chunk of every x-line and every y-line
+// of the padded grid.
+inline int line_chunks(int n) { return (n + 2 + 63) / 64; }
+inline size_t line_stamp_count(const Grid &G, bool ylines) {
+  return ylines ? (size_t)(G.n[0] + 2) * (G.n[2] + 2) * line_chunks(G.n[1])
+                : (size_t)(G.n[1] + 2) * (G.n[2] + 2) * line_chunks(G.n[0]);
+}
+inline size_t line_stamp_words(const Grid &G) {
+  return line_stamp_count(G, false) + line_stamp_count(G, true);
+}
 
 // The split kernel (`groups` = 2, 4 or 8 waves per row) on T in the reference
 // layout for all six directions: no scratch field.  d_consts: the six
@@ -139,25 +151,19 @@ void permute_layout(hipStream_t st, const R *in, const Grid &Gin, R *out, const 
 // The buffer the field ends in follows from the number of moves; the engines
 // move it back to Z once more after the first iteration when that makes a
 // fixed iteration count end in the caller's buffer, and finish() copies it
-// there otherwise (a solve stopped by its tolerance).  Optional int fields
-// (locking stamps) travel with it.
+// there otherwise (a solve stopped by its tolerance).
 template <typename R>
 struct FieldMover {
   hipStream_t st;
   R *buf[2];
-  int *sbuf[2];
   const Grid *lay;
   int cur = 0;
 
-  FieldMover(hipStream_t s, R *T, R *scratch, const Grid &GZ, int *stamp = nullptr,
-             int *stamp_scratch = nullptr)
-      : st(s), buf{T, scratch}, sbuf{stamp, stamp_scratch}, lay(&GZ) {}
+  FieldMover(hipStream_t s, R *T, R *scratch, const Grid &GZ) : st(s), buf{T, scratch}, lay(&GZ) {}
   R *field() const { return buf[cur]; }
-  int *stamps() const { return sbuf[cur]; }
   void to(const Grid &G) {
     if (lay == &G) return;
     permute_layout<R>(st, buf[cur], *lay, buf[1 - cur], G);
-    if (sbuf[0]) permute_layout<float>(st, (const float *)sbuf[cur], *lay, (float *)sbuf[1 - cur], G);
     cur = 1 - cur;
     lay = &G;
   }

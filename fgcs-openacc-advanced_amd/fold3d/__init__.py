@@ -136,6 +136,8 @@ def lib() -> C.CDLL:
     L.fold3d_solve_batched.argtypes = [vp, i, C.POINTER(Instance), C.POINTER(Opts),
                                        C.POINTER(i), dp, dp]
     L.fold3d_set_tuning.argtypes = [vp, i, i]
+    L.fold3d_set_lock_start.argtypes = [vp, i]
+    L.fold3d_get_lock_start.argtypes = [vp, C.POINTER(i)]
     L.fold3d_get_tuning.argtypes = [vp, i, C.POINTER(i)]
     L.fold3d_set_batch_tuning.argtypes = [vp, i, i, i]
     L.fold3d_get_batch_tuning.argtypes = [vp, C.POINTER(i), C.POINTER(i), C.POINTER(i)]
@@ -511,6 +513,15 @@ class Solver:
             check(rc)
         self.last_held = list(held)
         return list(its), list(mds), ms.value
+
+    def set_lock_start(self, iteration: int = -1):
+        """First iteration the locked engine runs locked (-1: the default, 8)."""
+        check(lib().fold3d_set_lock_start(self.h, iteration))
+
+    def lock_start(self) -> int:
+        k = C.c_int()
+        check(lib().fold3d_get_lock_start(self.h, C.byref(k)))
+        return k.value
 
     def set_tuning(self, groups: int, kind: int = MODEL_FOLD):
         check(lib().fold3d_set_tuning(self.h, kind, groups))

@@ -105,12 +105,14 @@ enum fold3d_engine {
                                 over several wavefronts whose minima meet in LDS
                                 (same bits; the default for the fold model; for the
                                 elliptic model the split or metric engine) */
-  FOLD3D_ENGINE_LOCKED = 5   /* fold model: SPLIT plus exact locking — a row of 64
-                                targets whose source rows have not changed since
-                                its previous update in the same direction is not
-                                recomputed (its update provably returns the value
-                                it has).  Same bits as SPLIT; stats->updates_computed
-                                tells how many cell-updates were evaluated */
+  FOLD3D_ENGINE_LOCKED = 5   /* fold model: SPLIT plus exact locking — from the
+                                iteration set by fold3d_set_lock_start (default
+                                8, 0-based), a block of target rows whose source
+                                patch has not changed since its previous update
+                                in the same direction is not recomputed (its
+                                update provably returns the value it has).  Same
+                                bits as SPLIT; stats->updates_computed tells how
+                                many cell-updates were evaluated */
 };
 
 typedef struct fold3d_stats {
@@ -149,6 +151,9 @@ void fold3d_reference_model(fold3d_model *m);       /* F = 1.1, a = (-0.35, 0.4,
  * stats written) when tol > 0 and max_iters ran out first.  During the solve
  * T also serves as one of the two buffers the field moves between layouts
  * through, so after any other error its contents are unspecified.
+ * T must hold no NaN (the reference's unreached value is FOLD3D_MAXT); the
+ * engines take minima of magnitudes in one instruction, which equals the
+ * reference's compare-and-select only for numbers.
  * Precision, fold model: FP64 gives the reference's bits with every engine;
  * FP32 gives, with every engine, the bits of the reference's arithmetic run
  * in float.  On the reference workload (8 iterations, up to 512^3) the FP32
@@ -254,6 +259,12 @@ int fold3d_adjoint_batched(fold3d_handle *h, int count, const fold3d_adjoint_sou
  * scratch copy of T (T itself is not modified), keeps the fastest on the
  * handle for that model kind and returns its sweep time. */
 int fold3d_set_tuning(fold3d_handle *h, int model_kind, int groups);
+/* First iteration (0-based) that FOLD3D_ENGINE_LOCKED runs locked; the ones
+ * before run as FOLD3D_ENGINE_SPLIT.  Default 8 (-1 restores it): in the
+ * reference's 8 iterations most rows still change, and a locked launch costs
+ * more per computed row; locking pays on the way to the fixed point. */
+int fold3d_set_lock_start(fold3d_handle *h, int iteration);
+int fold3d_get_lock_start(fold3d_handle *h, int *iteration);
 int fold3d_get_tuning(fold3d_handle *h, int model_kind, int *groups);
 int fold3d_autotune(fold3d_handle *h, const void *T, int nx, int ny, int nz,
                     double dx, double dy, double dz, const fold3d_model *model,

@@ -33,22 +33,30 @@ HBM_PEAK_GBS = 8000.0          # MI355X HBM3E
 # two rocprofv3 --pmc passes over the first iteration: FETCH_SIZE x 2.0 plus
 # WRITE_SIZE x 1.0 KiB, the factors read off a plain device copy of known
 # size (profiles/fold_split_f64_512_pmc_traffic.md)
-PMC_TRAFFIC_512_F64 = (2961.6 * 2.0 + 849.6 * 1.0) * 1024
+PMC_TRAFFIC_512_F64 = (3474.0 * 2.0 + 898.2 * 1.0) * 1024
 # The VALU view of the same launches, all from one profiled run over the first
 # iteration (profiles/fold_split_f64_512_pmc_traffic.md): wave-instructions
-# per plane launch (SQ_INSTS_VALU: 8192 waves of 523.2), the launch's average
+# per plane launch (SQ_INSTS_VALU: 8192 waves of 456.4), the launch's average
 # duration (kernel trace) and the clock implied by that run's FP64 FMA rate.
-PMC_VALU_INSTS_512_F64 = 4285692.1
-PMC_VALU_LAUNCH_US_512_F64 = 15.322
+PMC_VALU_INSTS_512_F64 = 3738916.2
+PMC_VALU_LAUNCH_US_512_F64 = 13.561
 PMC_VALU_CLOCK_GHZ = 1.955
 # FP32 fold solve vs FP64 on the reference workload (8 iterations, up to 512^3)
 FP32_BOUND_TEXT = ("signs and reached set equal; relative error < 1e-2 everywhere, above 1e-3 "
                    "at <= 0.5% of reached nodes, above 1e-4 at <= 10%, above 1e-5 at <= 25%")
-# HBM traffic per launch of the other rows' dominant kernels, from rocprofv3
-# --pmc passes (FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 KiB, the factors of the
-# copy-kernel calibration above), keyed by row; None until profiled.  Each
-# entry: (bytes per launch, the profile file that holds the passes).
-PMC_ROW_TRAFFIC = {}
+# Traffic below L2 per launch of the other rows' dominant kernels, from
+# rocprofv3 --pmc passes (FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 KiB, the factors
+# of the copy-kernel calibration above), keyed by row.  Each entry: (bytes per
+# launch, the kernel's average launch in the kernel trace of the same command
+# in us, which launches the passes counted, the profile file).
+PMC_ROW_TRAFFIC = {
+    "blob_speed_f64": ((3969.2 * 2.0 + 697.4) * 1024, 8.93,
+                       "iteration 1, the autotuner's 2-wave trial (no max|dT| reduction)",
+                       "profiles/ellip_split_f64_512_pmc.md"),
+    "adjoint": ((25560.1 * 2.0 + 4285.3) * 1024, 14.39,
+                "all 6144 launches of one gradient's adjoint solves",
+                "profiles/adjoint_plane_256_f32_pmc.md"),
+}
 
 
 def roofline_row(updates, bytes_per_update, sweep_ms, launches, nodes_per_launch, kernel,
@@ -64,7 +72,12 @@ def roofline_row(updates, bytes_per_update, sweep_ms, launches, nodes_per_launch
            "bytes_per_launch": bytes_per_update * nodes_per_launch,
            "kernel_us": sweep_ms * 1e3 / launches if launches else None, "traffic": None}
     if row in PMC_ROW_TRAFFIC:
-        out["traffic"], out["traffic_source"] = PMC_ROW_TRAFFIC[row]
+        tb, us, scope, src = PMC_ROW_TRAFFIC[row]
+        out.update({"traffic": tb, "traffic_scope": scope, "traffic_source": src,
+                    "profiled_kernel_us": us,
+                    "profiled_traffic_gbs": tb / (us * 1e-6) / 1e9,
+                    "profiled_algorithmic_gbs": bytes_per_update * nodes_per_launch /
+                                                (us * 1e-6) / 1e9})
     return out
 
 
@@ -157,7 +170,7 @@ def ellip_run(solver, n, dtype, tol, torch, threads=0, cpu_full=False):
     return out, T
 
 
-def batch_run(solver, torch, count=96, check=6):
+def batch_run(solver, torch, count=96, check=6, tuning=None):
     """96 instances, sides uniform in {64, ..., 320}: per-instance blob speed,
     tilt/axis-ratio fields and interior source, FP32 to max|dT| < 1e-6 —
     solved as one batch and as a loop of plain solves."""
@@ -170,12 +183,20 @@ def batch_run(solver, torch, count=96, check=6):
         T0s.append(wl.point_source_start(W, xp=torch, dtype=torch.float32, device="cuda"))
     cells = sum(n ** 3 for n in sides)
     log(f"bench: batch of {count} instances, {cells / 1e9:.2f} G nodes")
-    # warm-up on three instances (code load, graph path), then the schedule autotuner
+    # warm-up on three instances (code load, graph path), then the schedule
+    # autotuner; a schedule that is set is in force from the warm-up on
+    if tuning:
+        solver.set_batch_tuning(*tuning)
     warm = [t.clone() for t in T0s[:3]]
     solver.solve_batched(Ws[:3], warm, max_iters=2, tol=0.0)
-    tuned, tune_ms = solver.autotune_batched(Ws, T0s, iterations=1)
-    log(f"bench: batch autotuner picked (waves per row, blocks per CU, rows per pull) = {tuned} "
-        f"({tune_ms:.0f} ms/iteration)")
+    if tuning:
+        tuned = solver.batch_tuning()
+        log(f"bench: batch schedule (waves per row, blocks per CU, rows per pull) = {tuned} "
+            "(set, no autotuning)")
+    else:
+        tuned, tune_ms = solver.autotune_batched(Ws, T0s, iterations=1)
+        log(f"bench: batch autotuner picked (waves per row, blocks per CU, rows per pull) = "
+            f"{tuned} ({tune_ms:.0f} ms/iteration)")
     Ts = [t.clone() for t in T0s]
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -380,10 +401,12 @@ def adjoint_run(solver, torch, n=256):
            "forward_iterations": [min(st["forward_iters"]), max(st["forward_iters"])],
            "checkpoint_bytes": st["checkpoint_bytes"], "recomputed": st["recomputed"],
            "nodes_held_per_source": list(solver.last_held),
-           # adjoint plane steps: per node visit at least the in-edge mask (8 B),
-           # the residual and the new value (2 x 4 B); the in-edges' weights and
-           # values gathered on top are not counted, so this is a lower bound
-           "roofline": roofline_row(nsrc * 6 * n ** 3 * float(np.mean(st["adjoint_iters"])), 16,
+           # adjoint plane steps, per node visit: the in-edge mask (8 B), the
+           # residual (4 B), the adjoint value read and written (8 B) and the
+           # node's three stencil weights (12 B), each read once per pass by the
+           # node its edge points to; the neighbours' adjoint values gathered
+           # on top are taken as cache hits
+           "roofline": roofline_row(nsrc * 6 * n ** 3 * float(np.mean(st["adjoint_iters"])), 32,
                                     adj_ms, 6 * n * max(st["adjoint_iters"]), nsrc * n * n,
                                     "adjoint_plane_kernel<float>", "adjoint"),
            "grad_norm_slowness": float(s0.grad.norm()), "grad_norm_theta": float(t0_.grad.norm())}
@@ -567,6 +590,9 @@ def main():
     ap.add_argument("--cpu-full", action="store_true",
                     help="CPU baselines over the whole solves (8 iterations of the headline, the "
                          "blob-speed FP64 solve to tolerance) instead of a one-iteration sample")
+    ap.add_argument("--batch-tuning", default="",
+                    help="schedule of the batched solve as 'waves,blocks_per_cu,rows_per_pull' "
+                         "(default: the batch autotuner picks)")
     ap.add_argument("--fd-cells", type=int, default=32,
                     help="seeded cells per parameter of the tomography gradient's finite-difference "
                          "check at 256^3 (0: skip the check)")
@@ -772,7 +798,10 @@ def main():
 
     batch = None
     if not args.no_batch:
-        batch = batch_run(solver, torch)
+        bt = tuple(int(v) for v in args.batch_tuning.split(",")) if args.batch_tuning else None
+        if bt is not None and len(bt) != 3:
+            ap.error("--batch-tuning takes three integers: waves,blocks_per_cu,rows_per_pull")
+        batch = batch_run(solver, torch, tuning=bt)
 
     adjoint = None
     if not args.no_adjoint:
@@ -805,8 +834,7 @@ def main():
     # plane launch over the issue rate of 256 CUs x 4 SIMDs at one
     # wave-instruction per 4 cycles.  Counts, launch time and clock come from
     # one profiled run of the first iteration (the counters cannot be taken
-    # live); that iteration is the heaviest (its launches average 15.3 us, all
-    # eight 13.3 us).
+    # live); that iteration is the heaviest (most nodes change in it).
     valu_issue = None
     if n == 512 and groups == 2:
         clock_hz = PMC_VALU_CLOCK_GHZ * 1e9
@@ -839,6 +867,8 @@ def main():
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
                      "traffic": PMC_TRAFFIC_512_F64 if (n == 512 and groups == 2) else None,
+                     "traffic_iteration": 1 if (n == 512 and groups == 2) else None,
+                     "traffic_source": "profiles/fold_split_f64_512_pmc_traffic.md",
                      "kernel": "fold_split_kernel<double>", "kernel_us": us,
                      "bytes_per_launch": bytes_per_launch, "valu_issue": valu_issue},
         "engines_fp64": engines,

@@ -53,6 +53,12 @@ PMC_ROW_TRAFFIC = {
     "blob_speed_f64": ((3969.2 * 2.0 + 697.4) * 1024, 8.93,
                        "iteration 1, the autotuner's 2-wave trial (no max|dT| reduction)",
                        "profiles/ellip_split_f64_512_pmc.md"),
+    "tilted_field": ((48118.1 * 2.0 + 2531.9) * 1024, 33.67,
+                     "iteration 1 of the solve (ellip_metric_kernel<float, true, 1>)",
+                     "profiles/metric_tilted_field_1280_f32_pmc.md"),
+    "batch": ((78665.5 * 2.0 + 4453.3) * 1024, 73.97,
+              "the 1914 launches of iteration 1 (all 96 instances live; bytes per launch "
+              "are their mean)", "profiles/batch_plane_96inst_f32_pmc.md"),
     "adjoint": ((25560.1 * 2.0 + 4285.3) * 1024, 14.39,
                 "all 6144 launches of one gradient's adjoint solves",
                 "profiles/adjoint_plane_256_f32_pmc.md"),
@@ -236,9 +242,11 @@ def batch_run(solver, torch, count=96, check=6, tuning=None):
             "iterations_min": min(its), "iterations_max": max(its),
             "batched_cell_updates": upd, "batched_gcups": upd / batch_s / 1e9,
             # 7 x 4 B per cell-update (T read and write; v, theta, phi, ra, rb),
-            # over the batch's device time; a launch serves every instance
+            # over the batch's device time; a launch serves every instance that
+            # has its plane step: with all instances live, sum (n-1) n^2 nodes
+            # over the max(n) - 1 steps of a direction
             "roofline": roofline_row(upd, 28, dev_ms, batch_launches(sides, its),
-                                     sum(n * n for n in sides) / len(sides),
+                                     sum((n - 1) * n * n for n in sides) / (max(sides) - 1),
                                      "batch_plane_kernel<float>", "batch"),
             "batch_tuning": {"waves_per_row": tuned[0], "blocks_per_cu": tuned[1],
                              "rows_per_pull": tuned[2]},

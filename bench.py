@@ -88,6 +88,10 @@ def roofline_row(updates, bytes_per_update, sweep_ms, launches, nodes_per_launch
 
 
 A100_V2_512_GCUPS = 8 * 6 * 511 * 512 ** 2 / 3.038 / 1e9  # plots/perfect-2-gpu.py:20,24
+# the reference's published single-A100 times (s) of its version 2 by grid side
+A100_V2_S = dict(zip([128, 256, 384, 512, 640, 768, 896, 1024, 1152, 1280],
+                     [0.182216, 0.607416, 1.502711, 3.037851, 5.495726, 8.727377, 13.411582,
+                      19.195663, 26.060296, 35.163471]))  # plots/perfect-2-gpu.py:20,24
 
 
 def log(msg):
@@ -251,6 +255,40 @@ def batch_run(solver, torch, count=96, check=6, tuning=None):
             "batch_tuning": {"waves_per_row": tuned[0], "blocks_per_cu": tuned[1],
                              "rows_per_pull": tuned[2]},
             "fp32_max_rel_err_vs_fp64_on_smallest": {"instances": check, "value": worst}}
+
+
+def size_sweep(torch, sizes, steps=3):
+    """The headline solve (fold model, reference input built on the device,
+    FP64, 8 iterations, autotuned waves per row) at other grid sides: median
+    sweep time of `steps` solves, against the reference's published A100 time
+    for the same side."""
+    import fold3d
+    out = []
+    solver = fold3d.Solver(0)
+    try:
+        for n in sizes:
+            T0 = torch.empty((n + 2) ** 3, dtype=torch.float64, device="cuda")
+            solver.initialiser_device(T0, n)
+            groups, _ = solver.autotune(T0, n, iterations=1)
+            T = T0.clone()
+            solver.solve(T, n, max_iters=8)  # warm
+            ms = []
+            for _ in range(steps):
+                T.copy_(T0)
+                ms.append(solver.solve(T, n, max_iters=8).sweep_ms)
+            med = sorted(ms)[len(ms) // 2]
+            r = {"n": n, "groups": groups, "sweep_ms": med,
+                 "value": cell_updates(n, 8) / (med / 1e3) / 1e9, "unit": "Gcell-updates/s"}
+            if n in A100_V2_S:
+                r["a100_v2_s"] = A100_V2_S[n]
+                r["speedup_vs_a100_v2"] = A100_V2_S[n] / (med / 1e3)
+            log(f"bench: {n}^3: {med:.1f} ms, {r['value']:.2f} Gcell-upd/s ({groups} waves per row)")
+            out.append(r)
+            del T0, T
+            torch.cuda.empty_cache()
+    finally:
+        solver.close()
+    return out
 
 
 def batch_launches(sides, its):
@@ -601,6 +639,9 @@ def main():
     ap.add_argument("--batch-tuning", default="",
                     help="schedule of the batched solve as 'waves,blocks_per_cu,rows_per_pull' "
                          "(default: the batch autotuner picks)")
+    ap.add_argument("--sizes", default="",
+                    help="comma-separated grid sides at which to time the headline solve as well "
+                         "(device-built input; default: none)")
     ap.add_argument("--fd-cells", type=int, default=32,
                     help="seeded cells per parameter of the tomography gradient's finite-difference "
                          "check at 256^3 (0: skip the check)")
@@ -759,7 +800,7 @@ def main():
             log(f"bench: elliptic {name}: {r['iterations']} iterations, {r['value']:.2f} Gcell-upd/s")
             if "cpu_baseline" in r:
                 cb = r["cpu_baseline"]
-                log(f"bench: elliptic {name} CPU ({cb['cores']} threads, 1 iteration): "
+                log(f"bench: elliptic {name} CPU ({cb['cores']} threads; {cb['sample']}): "
                     f"{cb['value']:.3f} Gcell-upd/s, GPU same bits: {cb['gpu_same_bits']}")
             del Te
         del ref64
@@ -833,6 +874,11 @@ def main():
         field = tilted_field_run(torch, args.field_n)
         log(f"bench: tilted field: {field['iterations']} iterations, {field['value']:.2f} Gcell-upd/s")
 
+    sizes = None
+    if args.sizes:
+        log(f"bench: the headline solve at sides {args.sizes}")
+        sizes = size_sweep(torch, [int(v) for v in args.sizes.split(",")])
+
     log("bench: measured ceilings of this GPU (FMA rate, device copy)")
     peaks = fold3d.measure_peaks()
     log(f"bench: {peaks}")
@@ -892,6 +938,7 @@ def main():
         "reference_128": ref_small,
         "tilted_homogeneous_128": homog,
         "tilted_field": field,
+        "sizes_fp64": sizes,
         "device_ceilings": valu,
     }
     print(json.dumps(out), flush=True)

@@ -377,10 +377,11 @@ constexpr int kLockCounters = 256;
 // Shape of a split-kernel block, by waves per row.  A wave row covers 2 node
 // rows of a 32-node chunk (a wave: 64 nodes); a block holds split_rows() wave
 // rows and stages their source rows plus a halo once.  Two wave rows for 2
-// waves per row (4 waves per block), one otherwise.  Measured at 512^3 with 2
-// waves per row: 64-node chunks 311 ms per solve with one row per block, 305
-// with two; 32-node chunks 290 (16-node chunks 291); with 4 and 8 waves per
-// row (384^3, 256^3) 32-node chunks were 3 % faster than 64.  The blocks of a
+// waves per row (4 waves per block), one otherwise.  Relative sweep times at
+// 512^3 with 2 waves per row: 64-node chunks with one row per block 1.00, with
+// two 0.98; 32-node chunks 0.93 (16-node chunks 0.94; 3 or 4 wave rows per
+// block 1.14 and 1.06 against 2); with 4 and 8 waves per row (384^3, 256^3)
+// 32-node chunks were 3 % faster than 64.  The blocks of a
 // chunk column keep their XCD (linear block index mod 8), so with 32-node
 // chunks each XCD owns every eighth chunk of every row: the work of a busy
 // region spreads over more XCDs while each chunk's source rows stay in its

@@ -260,7 +260,7 @@ int fold3d_solve(fold3d_handle *h, void *T, int nx, int ny, int nz, double dx,
   double maxdt = -1;
   F3D_HIP(hipEventRecord(h->ev0, st));
   int rc;
-  unsigned long long *done_ctr = nullptr;  // cell-updates skipped (locking only)
+  unsigned long long *done_ctr = nullptr;  // cell-updates computed by locked iterations
   if (model->kind == FOLD3D_MODEL_FOLD && o.engine == FOLD3D_ENGINE_PLANE) {
     rc = o.precision == FOLD3D_F64
              ? fold_sweeps_plane_engine<double>(st, (double *)dT, G, *model, hsp, o.max_iters,
@@ -372,7 +372,10 @@ int fold3d_solve(fold3d_handle *h, void *T, int nx, int ny, int nz, double dx,
       F3D_HIP(hipMemcpy(done, done_ctr, sizeof done, hipMemcpyDeviceToHost));
       double sum = 0;
       for (int k = 0; k < 256; ++k) sum += (double)done[k];
-      stats->updates_computed = stats->cell_updates - sum;  // the counters hold the skipped ones
+      // the counters hold what the locked iterations computed; the iterations
+      // before the lock start computed everything
+      const int unlocked = iters < h->lock_from ? iters : h->lock_from;
+      stats->updates_computed = w * unlocked + sum;
     }
     const bool ellip = model->kind == FOLD3D_MODEL_ELLIP;
     const int params = ellip ? (model->speed_field ? 1 : 0) + (model->theta ? 4 : 0) : 0;

@@ -117,9 +117,15 @@ struct FoldFast {
   FoldSign<R> sgn[2];  // [0]: T_target >= 0 (a as given), [1]: T_target < 0 (a negated)
 };
 
+// Both candidate functions leave at once when tnew is at most the smallest of
+// their vertices (the reference tests tnew < smallest): a candidate is taken
+// only above its smallest vertex, and the one-point values lie above their
+// vertex, so none can go below tnew.  On ties this skips work the reference
+// does and rejects; in particular it skips the unreached regions, where the
+// target and its whole patch hold MAXT.
 template <typename R>
 __device__ __forceinline__ void fast_edge(R &tnew, R st, R xt, R F, const EdgeSite<R> &e) {
-  if (tnew < vmin(xt, st) || (e.ax_pos && xt > st)) return;
+  if (tnew <= vmin(xt, st) || (e.ax_pos && xt > st)) return;
   if (!(vmin(xt, st) + e.lgap < tnew * prune_slack<R>())) return;  // cannot go below tnew
   const R gx = div_const(xt - st, e.dxx, e.rdxx);
   const R q = R(1) - gx * e.ax;
@@ -141,7 +147,7 @@ __device__ __forceinline__ void fast_edge(R &tnew, R st, R xt, R F, const EdgeSi
 template <typename R>
 __device__ __forceinline__ void fast_tri(R &tnew, R st, R xt, R yt, const FoldSign<R> &S,
                                          const TriSite<R> &t, const EdgeSite<R> &sub) {
-  if (tnew < vmin(st, vmin(xt, yt)) || (t.ax_pos && xt > st) || (t.ay_pos && yt > xt)) return;
+  if (tnew <= vmin(st, vmin(xt, yt)) || (t.ax_pos && xt > st) || (t.ay_pos && yt > xt)) return;
   if (!(vmin(st, vmin(xt, yt)) + S.gap < tnew * prune_slack<R>())) return;  // nor its sub-edge
   const R F = S.F, az = S.az, dzz = S.dzz;
   const R gx = div_const(xt - st, t.dxx, t.rdxx);

@@ -421,7 +421,7 @@ struct LockGeom {
   int la, lb, m;
   int psrc, ptgt;
   int sigma;
-  unsigned long long *updates_skipped;  // cell-updates skipped (kLockCounters words)
+  unsigned long long *updates_computed;  // cell-updates computed (kLockCounters words)
 };
 
 // A block owns kRows wave rows of one kW-node chunk (blockIdx.x): a wave row
@@ -431,18 +431,21 @@ struct LockGeom {
 // plane to plane and its source rows are still in that XCD's L2 from the
 // launch that wrote them.  (Rotating the chunk by the row group, so that each
 // XCD gets every chunk, balanced the XCDs' work but was 3-4 % slower overall.)
+// One block's rows (row group `by` of chunk column blockIdx.x); s and part
+// are the block's LDS.
 template <typename R, bool kReduce, int kGroups, bool kLock, int kRows, int kW>
-__global__ __launch_bounds__(64 * kGroups * kRows) void fold_split_kernel(
-    R *__restrict__ T, PlaneGeom g, const FoldFast<R> *__restrict__ C,
-    unsigned long long *__restrict__ max_dt, LockGeom lk) {
+__device__ __forceinline__ void split_rowgroup(R *__restrict__ T, const PlaneGeom &g,
+                                               const FoldFast<R> *__restrict__ C,
+                                               unsigned long long *__restrict__ max_dt,
+                                               const LockGeom &lk, int by,
+                                               R (&s)[kRows * (64 / kW) + 2][kW + 2],
+                                               R (&part)[kRows][kGroups][64]) {
   constexpr int kRpw = 64 / kW;         // node rows per wave row
   constexpr int kNR = kRows * kRpw;     // node rows per block
-  __shared__ R s[kNR + 2][kW + 2];
-  __shared__ R part[kRows][kGroups][64];
   const int tx = threadIdx.x;
   const int wy = __builtin_amdgcn_readfirstlane(threadIdx.y);
   const int grp = wy % kGroups, row = wy / kGroups;
-  const int a0 = blockIdx.x * kW, b0 = 1 + blockIdx.y * kNR;
+  const int a0 = blockIdx.x * kW, b0 = 1 + by * kNR;
   int changed = 0;
   if (kLock && wy == 0 && tx < 2 * (kNR + 2)) {
     // the stamps of the 64-node line chunks the staged rows touch (columns
@@ -457,17 +460,13 @@ __global__ __launch_bounds__(64 * kGroups * kRows) void fold_split_kernel(
     }
   }
   // Locking: the block votes on its stamps before it loads anything else, so
-  // a skipped block costs one small load.  (Locking runs once most rows are
-  // idle; loading the staged values beside the stamps instead costs a busy
-  // block nothing but leaves an idle one waiting for them.)
-  if (kLock && lk.sigma >= 6 && !__syncthreads_or(changed)) {  // uniform over the block
-    if (tx == 0 && wy == 0) {  // skipped updates, spread over kLockCounters addresses
-      const int nrows = g.nb - b0 + 1 < kNR ? g.nb - b0 + 1 : kNR;  // rows in the plane
-      atomicAdd(&lk.updates_skipped[(blockIdx.x + 7u * blockIdx.y) % kLockCounters],
-                (unsigned long long)(g.na - a0 < kW ? g.na - a0 : kW) * nrows);
-    }
-    return;
-  }
+  // a skipped block costs one small load and no store.  (Locking runs once
+  // most rows are idle; loading the staged values beside the stamps instead
+  // costs a busy block nothing but leaves an idle one waiting for them.)
+  // Only the blocks that go on count their updates, at the end (counting the
+  // skipped ones, one atomic per idle block, cost 6 % of a solve to the fixed
+  // point).
+  if (kLock && lk.sigma >= 6 && !__syncthreads_or(changed)) return;  // uniform over the block
   // Staging: the source rows go through registers into LDS, their loads in
   // flight together with the load of the target value.
   constexpr int kStage = (kNR + 2) * (kW + 2), kThreads = 64 * kGroups * kRows;
@@ -511,6 +510,13 @@ __global__ __launch_bounds__(64 * kGroups * kRows) void fold_split_kernel(
   }
   part[row][grp][tx] = tn;
   __syncthreads();
+  if (kLock && wy == kGroups - 1 && tx == 0) {
+    // computed updates, spread over kLockCounters addresses; issued by a wave
+    // that has nothing left to do, so that no load waits behind the atomic
+    const int nrows = g.nb - b0 + 1 < kNR ? g.nb - b0 + 1 : kNR;  // rows in the plane
+    atomicAdd(&lk.updates_computed[(blockIdx.x + 7u * by) % kLockCounters],
+              (unsigned long long)(g.na - a0 < kW ? g.na - a0 : kW) * nrows);
+  }
   if (grp == 0) {
     R d = R(0);
     if (active) {
@@ -540,6 +546,20 @@ __global__ __launch_bounds__(64 * kGroups * kRows) void fold_split_kernel(
   }
 }
 
+template <typename R, bool kReduce, int kGroups, bool kLock, int kRows, int kW>
+__global__ __launch_bounds__(64 * kGroups * kRows) void fold_split_kernel(
+    R *__restrict__ T, PlaneGeom g, const FoldFast<R> *__restrict__ C,
+    unsigned long long *__restrict__ max_dt, LockGeom lk) {
+  constexpr int kNR = kRows * (64 / kW);
+  __shared__ R s[kNR + 2][kW + 2];
+  __shared__ R part[kRows][kGroups][64];
+  // (Locked launches with fewer blocks, each taking 2, 4 or 8 row groups in a
+  // loop, were tried for the mostly idle late iterations: the loop raised the
+  // kernel to 99 VGPRs, and even the gain of 2 row groups in the last
+  // iterations was below what the lost occupancy cost the busy ones.)
+  split_rowgroup<R, kReduce, kGroups, kLock, kRows, kW>(T, g, C, max_dt, lk, blockIdx.y, s, part);
+}
+
 __global__ void fill_int_kernel(int *__restrict__ p, long long n, int v) {
   for (long long k = blockIdx.x * (long long)blockDim.x + threadIdx.x; k < n;
        k += (long long)gridDim.x * blockDim.x)
@@ -550,7 +570,7 @@ __global__ void fill_int_kernel(int *__restrict__ p, long long n, int v) {
 struct LockState {
   LineStamps L;                        // L.xs == nullptr: no locking
   int sigma;                           // sweep index of the launch
-  unsigned long long *updates_skipped;    // cell-updates skipped (device counter)
+  unsigned long long *updates_computed;   // cell-updates computed by locked launches
 };
 
 template <typename R, int kGroups>
@@ -567,7 +587,9 @@ static void launch_split(hipStream_t st, R *T, const PlaneGeom &g, const FoldFas
     else
       fold_split_kernel<R, false, kGroups, true, kR, kW><<<grid, block, 0, st>>>(T, g, C, nullptr,
                                                                                lk);
-  } else if (max_dt) {
+    return;
+  }
+  if (max_dt) {
     fold_split_kernel<R, true, kGroups, false, kR, kW><<<grid, block, 0, st>>>(T, g, C, max_dt,
                                                                              lk);
   } else {
@@ -586,7 +608,7 @@ static void tiled_direction(hipStream_t st, R *T, const Grid &G, int d,
   // lane and row axes of this layout's planes (as plane_geom picks them)
   const bool u_first = G.stride[fr.ux] < G.stride[fr.uy];
   LockGeom lk{L.L, u_first ? fr.ux : fr.uy, u_first ? fr.uy : fr.ux, fr.m, 0, 0, L.sigma,
-              L.updates_skipped};
+              L.updates_computed};
   for (int q = 0; q < n - 1; ++q) {
     const int p = fr.s > 0 ? 1 + q : n - q;
     const PlaneGeom g = plane_geom(G, d, p);
@@ -613,7 +635,7 @@ template <typename R>
 int fold_sweeps_tiled_engine(hipStream_t st, R *T, R *Y, const Grid &G,
                              const FoldFast<R> *d_consts, int max_iters, double tol, int groups,
                              unsigned long long *d_max, int *iters_out, double *max_dt_out,
-                             int *stamps, unsigned long long *updates_skipped,
+                             int *stamps, unsigned long long *updates_computed,
                              int lock_from) {
   const Grid GX = G.with_order(kOrderX), GZ = G.with_order(kOrderZ), GY = G.with_order(kOrderY);
   const bool lock = stamps != nullptr && groups > 0;
@@ -631,7 +653,7 @@ int fold_sweeps_tiled_engine(hipStream_t st, R *T, R *Y, const Grid &G,
   auto dir = [&](const Grid &Gd, int d, unsigned long long *md) {
     tiled_direction<R>(st, mv.field(), Gd, d, d_consts + d, md, groups,
                        LockState{lock && it >= lock_from ? L : none, 6 * it + d,
-                                 updates_skipped});
+                                 updates_computed});
   };
   double last = -1.0;
   for (; it < max_iters;) {

@@ -72,14 +72,15 @@ int fold_sweeps_fast_engine(hipStream_t st, R *T, const Grid &G, const FoldFast<
 // node's eight candidate units spread over that many waves (fold_split_kernel).
 // `stamps` (line_stamp_words(G) ints, or null): exact locking of rows whose
 // source patch has not changed since their last update in the same direction;
-// `updates_skipped` (device, 256 counters zeroed by the caller) counts the
-// cell-updates the locking skipped; locking starts at iteration `lock_from`
-// (0-based; the iterations before it run unlocked).
+// `updates_computed` (device, 256 counters zeroed by the caller) counts the
+// cell-updates the locked iterations computed; locking starts at iteration
+// `lock_from` (0-based; the iterations before it run unlocked and are not
+// counted).
 template <typename R>
 int fold_sweeps_tiled_engine(hipStream_t st, R *T, R *Y, const Grid &G,
                              const FoldFast<R> *d_consts, int max_iters, double tol, int groups,
                              unsigned long long *d_max, int *iters_out, double *max_dt_out,
-                             int *stamps = nullptr, unsigned long long *updates_skipped = nullptr,
+                             int *stamps = nullptr, unsigned long long *updates_computed = nullptr,
                              int lock_from = 0);
 
 // Locking stamps: one int per 64-node chunk of every x-line and every y-line

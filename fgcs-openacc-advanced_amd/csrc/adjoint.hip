@@ -427,9 +427,7 @@ __global__ __launch_bounds__(256) void adjoint_plane_kernel(const AdjSrc<R> *__r
       d = fabs(acc - old);
     }
   }
-  double v = (double)d;
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, 64));
+  const double v = wave_max_dt((double)d);
   if ((threadIdx.x & 63) == 0 && v > 0.0)  // the source's word for this block
     atomicMax(&maxd[zsrc * kAdjMaxSlots + (blockIdx.x + 7u * blockIdx.y) % kAdjMaxSlots],
               (unsigned long long)__double_as_longlong(v));

@@ -68,6 +68,16 @@ constexpr int kOrderY[3] = {0, 2, 1};
 // a plane launch then share each word with a few dozen others rather than
 // all meeting on one, whose serialised atomics cost more than the update.
 constexpr int kMaxSlots = 256;
+// Largest |dT| of a whole wave (every lane's d >= 0; inactive lanes pass 0).
+// A wave in which no lane changed skips the shuffles: near convergence most
+// waves change nothing.
+__device__ __forceinline__ double wave_max_dt(double d) {
+  if (__builtin_amdgcn_ballot_w64(d > 0.0) == 0) return 0.0;
+  double v = d;
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, 64));
+  return v;
+}
 __device__ __forceinline__ void max_dt_update(unsigned long long *slots, double v) {
   const unsigned b = blockIdx.x + 7u * blockIdx.y + 13u * blockIdx.z;
   atomicMax(&slots[b % kMaxSlots], (unsigned long long)__double_as_longlong(v));

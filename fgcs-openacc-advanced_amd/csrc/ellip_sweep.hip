@@ -57,9 +57,7 @@ __global__ __launch_bounds__(kLaneE *kRowsE) void ellip_plane_kernel(
     if (kReduce) d = told - tn;
   }
   if (kReduce) {
-    double v = (double)d;
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, 64));
+    const double v = wave_max_dt((double)d);
     if ((threadIdx.x & 63) == 0 && v > 0.0)
       max_dt_update(max_dt, v);
   }
@@ -285,9 +283,7 @@ __global__ __launch_bounds__(64 * kGroups) void ellip_split_kernel(
       if (kReduce) d = told - m;
     }
     if (kReduce) {
-      double v = (double)d;
-#pragma unroll
-      for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, 64));
+      const double v = wave_max_dt((double)d);
       if (tx == 0 && v > 0.0) max_dt_update(max_dt, v);
     }
   }
@@ -506,9 +502,7 @@ __global__ __launch_bounds__(64 * kGroups) void ellip_metric_kernel(
       if (kReduce) d = told - m;
     }
     if (kReduce) {
-      double v = (double)d;
-#pragma unroll
-      for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, 64));
+      const double v = wave_max_dt((double)d);
       if (tx == 0 && v > 0.0) max_dt_update(max_dt, v);
     }
   }

@@ -71,9 +71,7 @@ __global__ __launch_bounds__(kLaneX *kRowsY) void fold_plane_kernel(
   if (kReduce) {
     // wavefront max, then one atomic per wave (|dT| >= 0, so the IEEE bit
     // pattern orders like an unsigned integer)
-    double v = (double)d;
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, 64));
+    const double v = wave_max_dt((double)d);
     if ((threadIdx.x & 63) == 0 && v > 0.0)
       max_dt_update(max_dt, v);
   }
@@ -201,9 +199,7 @@ __global__ __launch_bounds__(kLaneX *kRowsY) void fold_plane_fast_kernel(
     if (kReduce) d = fabs(tn - told);
   }
   if (kReduce) {
-    double v = (double)d;
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, 64));
+    const double v = wave_max_dt((double)d);
     if ((threadIdx.x & 63) == 0 && v > 0.0)
       max_dt_update(max_dt, v);
   }
@@ -308,9 +304,7 @@ __global__ __launch_bounds__(kTileA *kTileB) void fold_tile_kernel(
     if (kReduce) d = fabs(tn - told);
   }
   if (kReduce) {
-    double v = (double)d;
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, 64));
+    const double v = wave_max_dt((double)d);
     if (tx == 0 && v > 0.0) max_dt_update(max_dt, v);
   }
 }
@@ -538,9 +532,7 @@ __device__ __forceinline__ void split_rowgroup(R *__restrict__ T, const PlaneGeo
       if (kReduce) d = fabs(res - told);
     }
     if (kReduce) {
-      double v = (double)d;
-#pragma unroll
-      for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, 64));
+      const double v = wave_max_dt((double)d);
       if (tx == 0 && v > 0.0) max_dt_update(max_dt, v);
     }
   }

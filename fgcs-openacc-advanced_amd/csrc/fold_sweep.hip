@@ -640,7 +640,6 @@ int fold_sweeps_tiled_engine(hipStream_t st, R *T, R *Y, const Grid &G,
   }
   const LineStamps none{nullptr, nullptr, 0, 0, 0, 0};
   FieldMover<R> mv(st, T, Y, GZ);
-  const bool extra = FieldMover<R>::extra_move_first(max_iters, tol);
   int it = 0;
   auto dir = [&](const Grid &Gd, int d, unsigned long long *md) {
     tiled_direction<R>(st, mv.field(), Gd, d, d_consts + d, md, groups,
@@ -658,16 +657,20 @@ int fold_sweeps_tiled_engine(hipStream_t st, R *T, R *Y, const Grid &G,
       fill_int_kernel<<<1024, 256, 0, st>>>(stamps, (long long)line_stamp_words(G),
                                             it > 0 ? 6 * it - 1 : 0);
     }
-    mv.to(GX);  // x_down, x_up with x slowest
+    // Two moves per iteration: x_down, x_up with x slowest; then the z- and
+    // the y-sweeps with y slowest (a z = const plane there is rows of x-lines,
+    // each contiguous, as the kernel needs), except in the last iteration of a
+    // fixed-count solve, which runs them in the reference layout so that the
+    // field ends in the caller's buffer after an even number of moves.
+    mv.to(GX);
     dir(GX, FOLD3D_X_DOWN, md);
     dir(GX, FOLD3D_X_UP, md);
-    mv.to(GZ);  // k_up, k_down in the reference layout
-    dir(GZ, FOLD3D_K_UP, md);
-    dir(GZ, FOLD3D_K_DOWN, md);
-    mv.to(GY);  // y_up, y_down with y slowest
-    dir(GY, FOLD3D_Y_UP, md);
-    dir(GY, FOLD3D_Y_DOWN, md);
-    if (it == 0 && extra) mv.to(GZ);
+    const Grid &GP = tol <= 0 && it == max_iters - 1 ? GZ : GY;
+    mv.to(GP);
+    dir(GP, FOLD3D_K_UP, md);
+    dir(GP, FOLD3D_K_DOWN, md);
+    dir(GP, FOLD3D_Y_UP, md);
+    dir(GP, FOLD3D_Y_DOWN, md);
     F3D_HIP(hipGetLastError());
     ++it;
     if (md) {

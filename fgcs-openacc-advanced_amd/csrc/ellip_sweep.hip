@@ -335,22 +335,25 @@ int ellip_sweeps_split_engine(hipStream_t st, R *T, R *Y, R *VX, R *VY, const Gr
     permute_layout<R>(st, V, GZ, VY, GY);
   }
   FieldMover<R> mv(st, T, Y, GZ);
-  const bool extra = FieldMover<R>::extra_move_first(max_iters, tol);
   int it = 0;
   double last = -1.0;
   for (; it < max_iters;) {
     unsigned long long *md = tol > 0 ? d_max : nullptr;
     if (md) max_dt_reset_kernel<<<1, kMaxSlots, 0, st>>>(d_max);
+    // two moves per iteration, as in the fold engines: the x-sweeps with x
+    // slowest, the z- and y-sweeps with y slowest (the reference layout in the
+    // last iteration of a fixed-count solve)
     mv.to(GX);
     ellip_split_direction<R>(st, mv.field(), V ? VX : nullptr, GX, FOLD3D_X_DOWN, args[FOLD3D_X_DOWN], md, groups);
     ellip_split_direction<R>(st, mv.field(), V ? VX : nullptr, GX, FOLD3D_X_UP, args[FOLD3D_X_UP], md, groups);
-    mv.to(GZ);
-    ellip_split_direction<R>(st, mv.field(), V, GZ, FOLD3D_K_UP, args[FOLD3D_K_UP], md, groups);
-    ellip_split_direction<R>(st, mv.field(), V, GZ, FOLD3D_K_DOWN, args[FOLD3D_K_DOWN], md, groups);
-    mv.to(GY);
-    ellip_split_direction<R>(st, mv.field(), V ? VY : nullptr, GY, FOLD3D_Y_UP, args[FOLD3D_Y_UP], md, groups);
-    ellip_split_direction<R>(st, mv.field(), V ? VY : nullptr, GY, FOLD3D_Y_DOWN, args[FOLD3D_Y_DOWN], md, groups);
-    if (it == 0 && extra) mv.to(GZ);
+    const bool in_z = tol <= 0 && it == max_iters - 1;
+    const Grid &GP = in_z ? GZ : GY;
+    const R *VP = V ? (in_z ? V : VY) : nullptr;
+    mv.to(GP);
+    ellip_split_direction<R>(st, mv.field(), VP, GP, FOLD3D_K_UP, args[FOLD3D_K_UP], md, groups);
+    ellip_split_direction<R>(st, mv.field(), VP, GP, FOLD3D_K_DOWN, args[FOLD3D_K_DOWN], md, groups);
+    ellip_split_direction<R>(st, mv.field(), VP, GP, FOLD3D_Y_UP, args[FOLD3D_Y_UP], md, groups);
+    ellip_split_direction<R>(st, mv.field(), VP, GP, FOLD3D_Y_DOWN, args[FOLD3D_Y_DOWN], md, groups);
     F3D_HIP(hipGetLastError());
     ++it;
     if (md) {
@@ -563,24 +566,27 @@ int ellip_sweeps_metric_engine(hipStream_t st, R *T, R *scratch, const Grid &G,
     permute_metric(MY, GY);
   }
   FieldMover<R> mv(st, T, Y, GZ);
-  const bool extra = FieldMover<R>::extra_move_first(max_iters, tol);
   int it = 0;
   double last = -1.0;
   for (; it < max_iters;) {
     unsigned long long *md = tol > 0 ? d_max : nullptr;
     if (md) max_dt_reset_kernel<<<1, kMaxSlots, 0, st>>>(d_max);
+    // two moves per iteration, as in the fold engines: the x-sweeps with x
+    // slowest, the z- and y-sweeps with y slowest (the reference layout and
+    // its metric fields in the last iteration of a fixed-count solve)
     mv.to(GX);
     if (!resident) permute_metric(MX, GX);
     ellip_metric_direction<R>(st, mv.field(), MX, GX, FOLD3D_X_DOWN, h, md, groups);
     ellip_metric_direction<R>(st, mv.field(), MX, GX, FOLD3D_X_UP, h, md, groups);
-    mv.to(GZ);
-    ellip_metric_direction<R>(st, mv.field(), MZ, GZ, FOLD3D_K_UP, h, md, groups);
-    ellip_metric_direction<R>(st, mv.field(), MZ, GZ, FOLD3D_K_DOWN, h, md, groups);
-    mv.to(GY);
-    if (!resident) permute_metric(MY, GY);
-    ellip_metric_direction<R>(st, mv.field(), MY, GY, FOLD3D_Y_UP, h, md, groups);
-    ellip_metric_direction<R>(st, mv.field(), MY, GY, FOLD3D_Y_DOWN, h, md, groups);
-    if (it == 0 && extra) mv.to(GZ);
+    const bool in_z = tol <= 0 && it == max_iters - 1;
+    const Grid &GP = in_z ? GZ : GY;
+    mv.to(GP);
+    if (!resident && !in_z) permute_metric(MY, GY);
+    const R *MP = in_z ? MZ : MY;
+    ellip_metric_direction<R>(st, mv.field(), MP, GP, FOLD3D_K_UP, h, md, groups);
+    ellip_metric_direction<R>(st, mv.field(), MP, GP, FOLD3D_K_DOWN, h, md, groups);
+    ellip_metric_direction<R>(st, mv.field(), MP, GP, FOLD3D_Y_UP, h, md, groups);
+    ellip_metric_direction<R>(st, mv.field(), MP, GP, FOLD3D_Y_DOWN, h, md, groups);
     F3D_HIP(hipGetLastError());
     ++it;
     if (md) {

@@ -168,11 +168,6 @@ struct FieldMover {
     cur = 1 - cur;
     lay = &G;
   }
-  // true: one more move to Z after the first iteration, so that a solve of
-  // exactly max_iters iterations (3 max_iters + 2 moves) ends in buf[0]
-  static bool extra_move_first(int max_iters, double tol) {
-    return tol <= 0 && max_iters > 1 && max_iters % 2 == 0;
-  }
   hipError_t finish(const Grid &GZ) {
     to(GZ);
     if (cur == 0) return hipSuccess;

@@ -253,7 +253,7 @@ def batch_run(solver, torch, count=96, check=6, tuning=None):
                                      sum((n - 1) * n * n for n in sides) / (max(sides) - 1),
                                      "batch_plane_kernel<float>", "batch"),
             "batch_tuning": {"waves_per_row": tuned[0], "blocks_per_cu": tuned[1],
-                             "rows_per_pull": tuned[2]},
+                             "rows_per_pull": tuned[2], "sub_batches": solver.batch_streams()},
             "fp32_max_rel_err_vs_fp64_on_smallest": {"instances": check, "value": worst}}
 
 

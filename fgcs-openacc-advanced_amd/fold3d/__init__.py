@@ -140,6 +140,8 @@ def lib() -> C.CDLL:
     L.fold3d_get_lock_start.argtypes = [vp, C.POINTER(i)]
     L.fold3d_get_tuning.argtypes = [vp, i, C.POINTER(i)]
     L.fold3d_set_batch_tuning.argtypes = [vp, i, i, i]
+    L.fold3d_set_batch_streams.argtypes = [vp, i]
+    L.fold3d_get_batch_streams.argtypes = [vp, C.POINTER(i)]
     L.fold3d_get_batch_tuning.argtypes = [vp, C.POINTER(i), C.POINTER(i), C.POINTER(i)]
     L.fold3d_autotune_batched.argtypes = [vp, i, C.POINTER(Instance), C.POINTER(Opts), i,
                                           C.POINTER(C.c_float)]
@@ -560,6 +562,15 @@ class Solver:
 
     def set_batch_tuning(self, groups: int = 0, blocks_per_cu: int = 0, rows_per_pull: int = 0):
         check(lib().fold3d_set_batch_tuning(self.h, groups, blocks_per_cu, rows_per_pull))
+
+    def set_batch_streams(self, streams: int = 0):
+        """Sub-batches the batched solve runs at once (0: the default, 3)."""
+        check(lib().fold3d_set_batch_streams(self.h, streams))
+
+    def batch_streams(self) -> int:
+        k = C.c_int()
+        check(lib().fold3d_get_batch_streams(self.h, C.byref(k)))
+        return k.value
 
     def batch_tuning(self):
         g, b, r = C.c_int(), C.c_int(), C.c_int()

@@ -306,6 +306,17 @@ int fold3d_get_batch_tuning(fold3d_handle *h, int *groups, int *blocks_per_cu,
                             int *rows_per_pull);
 int fold3d_autotune_batched(fold3d_handle *h, int count, const fold3d_instance *inst,
                             const fold3d_opts *opts, int iterations, float *ms_out);
+/* Sub-batches the batched solve runs at once (1..8; 0 restores the default,
+ * 3).  The instances are cut into that many bands of about equal work, the
+ * largest instances first, and each band is solved as a batch of its own on
+ * a stream of its own, from a host thread of its own, all ordered after
+ * opts->stream; fold3d_solve_batched returns when all are done.  Every
+ * instance gets the bits it gets in one batch; the bands keep the GPU busy
+ * while another waits on its convergence check or runs a plane step that few
+ * of its instances have.  No counterpart in the reference (no batched
+ * solve there). */
+int fold3d_set_batch_streams(fold3d_handle *h, int streams);
+int fold3d_get_batch_streams(fold3d_handle *h, int *streams);
 
 /* One plane step of one direction on a device FP64 field, fold model, with
  * explicit local-frame arguments exactly as the reference kernels take them

@@ -99,6 +99,8 @@ def test_invalid_arguments_are_codes_not_crashes():
     dummy = C.c_void_p(0x1000)  # never dereferenced: validation fails first
     assert _solve(None, dummy) == fold3d.FOLD3D_E_INVALID
     assert "NULL" in fold3d.last_error()
+    assert L.fold3d_set_batch_streams(None, 2) == fold3d.FOLD3D_E_INVALID
+    assert L.fold3d_get_batch_streams(None, None) == fold3d.FOLD3D_E_INVALID
     assert _solve(dummy, None) == fold3d.FOLD3D_E_INVALID
     assert _solve(dummy, dummy, n=0) == fold3d.FOLD3D_E_INVALID
     assert "too small" in fold3d.last_error()

@@ -193,3 +193,35 @@ def test_autotune_batched_keeps_fields_and_picks_a_schedule(solver):
             assert torch.equal(a, c)
     finally:
         solver.set_batch_tuning(0, 0, 0)
+
+
+@pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
+def test_sub_batches_on_several_streams_give_the_same_bits(solver, dtype):
+    """Cut into 2, 3 or 8 bands solved at once on their own streams, the batch
+    gives every instance the bits, iteration count and max|dT| of one batch,
+    with instances retiring at different iterations (tolerance stop)."""
+    sides = [30, 9, 24, 17, 12, 28, 21, 26, 14]
+    Ws = make_batch(sides, dtype_np=np.float64 if dtype == torch.float64 else np.float32)
+    T0s = [wl.point_source_start(W) for W in Ws]
+    Wd = [to_dev_workload(W, dtype) for W in Ws]
+    tol = 1e-10 if dtype == torch.float64 else 1e-6
+    results = {}
+    try:
+        for k in (1, 2, 3, 8):
+            solver.set_batch_streams(k)
+            assert solver.batch_streams() == k
+            Ts = [dev(T0, dtype) for T0 in T0s]
+            its, mds, ms = solver.solve_batched(Wd, Ts, max_iters=60, tol=tol)
+            assert ms > 0
+            results[k] = (its, mds, Ts)
+        its1, mds1, T1 = results[1]
+        assert len(set(its1)) > 1  # instances retire at different iterations
+        for k in (2, 3, 8):
+            its, mds, Ts = results[k]
+            assert its == its1, k
+            assert mds == mds1, k
+            for a, b in zip(Ts, T1):
+                assert torch.equal(a, b), k
+    finally:
+        solver.set_batch_streams(0)
+    assert solver.batch_streams() == 3

@@ -4,10 +4,10 @@
 // Once per batch the slowness and the inverse metric N = M^-1 of every
 // instance are written to seven fields of its own (tilt fields or a constant
 // metric alike).  An iteration then runs, for all instances together, the
-// single-instance metric engine's schedule: move T into the x-sweep layout
-// (one launch for the whole batch; the metric fields are kept in every layout,
-// or moved along when memory is short), run the x-sweeps, move T back, run the
-// z-sweeps in the reference layout, and the same for y.  One launch per (sweep
+// single-instance metric engine's schedule of two moves: move T into the
+// x-sweep layout (one launch for the whole batch; the metric fields are kept
+// in every layout, or moved along when memory is short), run the x-sweeps,
+// move T into the y-sweep layout (y slowest), run the z- and the y-sweeps.  One launch per (sweep
 // direction d, plane step q) serves every instance that still has a plane q in
 // direction d; instances are sorted by their plane count along d, so those are
 // a prefix of the sorted list.  A launch runs a fixed grid of persistent
@@ -36,12 +36,13 @@
 
 namespace fold3d {
 
-// Layouts of a batch: every instance's fields move between the reference
-// layout Z and the layout of the sweep pair being run (X or Y), as in the
-// single-instance engines.
+// Layouts of a batch: the reference layout Z, the x-sweep layout X (x
+// slowest) and the layout Y (y slowest) in which the z- and y-sweeps run, as
+// in the single-instance engines.
 enum BatchLayout { kLayZ = 0, kLayX = 1, kLayY = 2 };
 
-// Device view of one instance.  Scratch per instance: Y (T in layout X or Y)
+// Device view of one instance.  Scratch per instance: Y (T in layout X; T
+// itself holds layout Y between iterations)
 // and kMetricFields fields (slowness and N = M^-1) per layout: MZ in the
 // reference layout, MX and MY in the x- and y-sweep layouts (made once per
 // batch), or MX = MY remade before each sweep pair when memory is short.
@@ -124,7 +125,8 @@ constexpr int kBatchMaxSlots = 32;
 // launch's counter, `chunk` consecutive rows per pull.
 template <typename R, int kGroups>
 __global__ __launch_bounds__(64 * kGroups) void batch_plane_kernel(
-    const BatchInst<R> *__restrict__ inst, BatchDir bd, int dir, int lay, LatticeIdx LI, int q,
+    const BatchInst<R> *__restrict__ inst, BatchDir bd, int dir, int lay, int on_T, LatticeIdx LI,
+    int q,
     long long total, int chunk, unsigned int *__restrict__ counter,
     const int *__restrict__ done, unsigned long long *__restrict__ max_dt) {
   __shared__ R s[3][64 + 2];
@@ -173,7 +175,7 @@ __global__ __launch_bounds__(64 * kGroups) void batch_plane_kernel(
     const long long r = cur - bd.prefix[lo];
     const int tiles_a = (na + 63) / 64;
     const int a0 = (int)(r % tiles_a) * 64, b = 1 + (int)(r / tiles_a);
-    R *T = lay == kLayZ ? I.T : I.Y;
+    R *T = on_T ? I.T : I.Y;
     const R *M = lay == kLayZ ? I.MZ : lay == kLayX ? I.MX : I.MY;
     for (int idx = grp * 64 + tx; idx < 3 * 66; idx += 64 * kGroups) {
       const int l_a = idx % 66, l_b = idx / 66;
@@ -242,6 +244,7 @@ struct PermuteJob {
   int which;          // 0: T -> Y, 1: Y -> T, 2: MZ -> MX, 3: MZ -> MY
   int lin, lout;      // layouts
   int fields;
+  int all;            // also the instances that are done
 };
 
 template <typename R>
@@ -256,7 +259,7 @@ __global__ __launch_bounds__(kBTile *kBTileRows) void batch_permute_kernel(
     if (tiles_prefix[mid] <= item) lo = mid;
     else hi = mid - 1;
   }
-  if (done[lo]) return;
+  if (!job.all && done[lo]) return;
   const BatchInst<R> &I = inst[lo];
   const int Hx = I.n[0] + 2, Hy = I.n[1] + 2;
   const int tx_n = (Hx + kBTile - 1) / kBTile, ty_n = (Hy + kBTile - 1) / kBTile;
@@ -415,8 +418,8 @@ int batch_solve(hipStream_t user_st, int count, const fold3d_instance *inst, int
   long long *d_prefix = nullptr, *d_tiles = nullptr;
   unsigned int *d_counters = nullptr;
   unsigned long long *d_max = nullptr;
-  hipGraph_t graph = nullptr;
-  hipGraphExec_t exec = nullptr;
+  hipGraph_t graph[2] = {nullptr, nullptr};
+  hipGraphExec_t exec[2] = {nullptr, nullptr};
   hipEvent_t e0 = nullptr, e1 = nullptr;
   int rc = FOLD3D_OK;
   auto fail = [&](hipError_t e, const char *what) {
@@ -463,7 +466,8 @@ int batch_solve(hipStream_t user_st, int count, const fold3d_instance *inst, int
           (R)N[3], (R)N[4], (R)N[5], hinst[i].MZ);
     }
     if (resident) {  // the x- and y-sweep layouts of the metric fields, once per batch
-      const PermuteJob<R> jx{2, kLayZ, kLayX, kMetricFields}, jy{3, kLayZ, kLayY, kMetricFields};
+      const PermuteJob<R> jx{2, kLayZ, kLayX, kMetricFields, 0},
+          jy{3, kLayZ, kLayY, kMetricFields, 0};
       batch_permute_kernel<R><<<(unsigned)total_tiles, dim3(kBTile, kBTileRows), 0, st>>>(
           d_inst, d_tiles, count, jx, d_done);
       batch_permute_kernel<R><<<(unsigned)total_tiles, dim3(kBTile, kBTileRows), 0, st>>>(
@@ -472,17 +476,22 @@ int batch_solve(hipStream_t user_st, int count, const fold3d_instance *inst, int
     BATCH_HIP(hipGetLastError());
     BATCH_HIP(hipStreamSynchronize(st));
 
-    // one iteration as a launch sequence
-    auto enqueue_iteration = [&](hipStream_t s) {
+    // One iteration as a launch sequence, two moves of T: into the x-sweep
+    // layout in the scratch field Y, the x-sweeps there, back into the
+    // caller's T in the y-sweep layout (y slowest), the z- and y-sweeps there.
+    // Between iterations T stays in the y-sweep layout (the first iteration
+    // starts from the reference layout); after the last, every instance's T
+    // goes back to the reference layout.
+    auto enqueue_iteration = [&](hipStream_t s, bool first) {
       batch_reset_kernel<<<(launches + count * kBatchMaxSlots + 255) / 256, 256, 0, s>>>(
           d_counters, launches, d_max, count * kBatchMaxSlots);
       int slot = 0;
       auto permute = [&](int which, int lin, int lout, int fields) {
-        PermuteJob<R> job{which, lin, lout, fields};
+        PermuteJob<R> job{which, lin, lout, fields, 0};
         batch_permute_kernel<R><<<(unsigned)total_tiles, dim3(kBTile, kBTileRows), 0, s>>>(
             d_inst, d_tiles, count, job, d_done);
       };
-      auto direction = [&](int d, int lay) {
+      auto direction = [&](int d, int lay, int on_T) {
         const int m = kFrames[d].m;
         const int *ord = &order_all[d * count];
         const long long *pre = &prefix_all[d * (count + 1)];
@@ -503,36 +512,36 @@ int batch_solve(hipStream_t user_st, int count, const fold3d_instance *inst, int
               (int)std::min<long long>(chunk > 0 ? (total + chunk - 1) / chunk : total, blocks);
           if (groups == 1)
             batch_plane_kernel<R, 1><<<grid, dim3(64, 1), 0, s>>>(
-                d_inst, bd, d, lay, LI, q, total, chunk, d_counters + slot, d_done, d_max);
+                d_inst, bd, d, lay, on_T, LI, q, total, chunk, d_counters + slot, d_done, d_max);
           else if (groups == 2)
             batch_plane_kernel<R, 2><<<grid, dim3(64, 2), 0, s>>>(
-                d_inst, bd, d, lay, LI, q, total, chunk, d_counters + slot, d_done, d_max);
+                d_inst, bd, d, lay, on_T, LI, q, total, chunk, d_counters + slot, d_done, d_max);
           else
             batch_plane_kernel<R, 4><<<grid, dim3(64, 4), 0, s>>>(
-                d_inst, bd, d, lay, LI, q, total, chunk, d_counters + slot, d_done, d_max);
+                d_inst, bd, d, lay, on_T, LI, q, total, chunk, d_counters + slot, d_done, d_max);
         }
       };
-      permute(0, kLayZ, kLayX, 1);
+      permute(0, first ? kLayZ : kLayY, kLayX, 1);
       if (!resident) permute(2, kLayZ, kLayX, kMetricFields);
-      direction(FOLD3D_X_DOWN, kLayX);
-      direction(FOLD3D_X_UP, kLayX);
-      permute(1, kLayX, kLayZ, 1);
-      direction(FOLD3D_K_UP, kLayZ);
-      direction(FOLD3D_K_DOWN, kLayZ);
-      permute(0, kLayZ, kLayY, 1);
+      direction(FOLD3D_X_DOWN, kLayX, 0);
+      direction(FOLD3D_X_UP, kLayX, 0);
+      permute(1, kLayX, kLayY, 1);
       if (!resident) permute(3, kLayZ, kLayY, kMetricFields);
-      direction(FOLD3D_Y_UP, kLayY);
-      direction(FOLD3D_Y_DOWN, kLayY);
-      permute(1, kLayY, kLayZ, 1);
+      direction(FOLD3D_K_UP, kLayY, 1);
+      direction(FOLD3D_K_DOWN, kLayY, 1);
+      direction(FOLD3D_Y_UP, kLayY, 1);
+      direction(FOLD3D_Y_DOWN, kLayY, 1);
     };
 
-    if (use_graph) {
-      BATCH_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-      capturing = true;
-      enqueue_iteration(st);
-      capturing = false;
-      BATCH_HIP(hipStreamEndCapture(st, &graph));
-      BATCH_HIP(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+    if (use_graph) {  // the first iteration and the others
+      for (int g = 0; g < 2; ++g) {
+        BATCH_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+        capturing = true;
+        enqueue_iteration(st, g == 0);
+        capturing = false;
+        BATCH_HIP(hipStreamEndCapture(st, &graph[g]));
+        BATCH_HIP(hipGraphInstantiate(&exec[g], graph[g], nullptr, nullptr, 0));
+      }
     }
     BATCH_HIP(hipEventCreate(&e0));
     BATCH_HIP(hipEventCreate(&e1));
@@ -541,9 +550,10 @@ int batch_solve(hipStream_t user_st, int count, const fold3d_instance *inst, int
     std::vector<int> iters(count, 0);
     std::vector<double> last(count, -1.0);
     int remaining = count;
-    for (int it = 0; it < max_iters && remaining > 0; ++it) {
-      if (use_graph) BATCH_HIP(hipGraphLaunch(exec, st));
-      else enqueue_iteration(st);
+    int ran = 0;  // iterations launched
+    for (int it = 0; it < max_iters && remaining > 0; ++it, ++ran) {
+      if (use_graph) BATCH_HIP(hipGraphLaunch(exec[it == 0 ? 0 : 1], st));
+      else enqueue_iteration(st, it == 0);
       BATCH_HIP(hipGetLastError());
       BATCH_HIP(hipMemcpyAsync(hmax.data(), d_max, sizeof(unsigned long long) * hmax.size(),
                                hipMemcpyDeviceToHost, st));
@@ -568,6 +578,14 @@ int batch_solve(hipStream_t user_st, int count, const fold3d_instance *inst, int
         BATCH_HIP(hipMemcpyAsync(d_done, done.data(), sizeof(int) * count, hipMemcpyHostToDevice,
                                  st));
     }
+    if (ran > 0) {  // every instance's T back into the reference layout, through Y
+      const PermuteJob<R> back{0, kLayY, kLayZ, 1, 1}, home{1, kLayZ, kLayZ, 1, 1};
+      batch_permute_kernel<R><<<(unsigned)total_tiles, dim3(kBTile, kBTileRows), 0, st>>>(
+          d_inst, d_tiles, count, back, d_done);
+      batch_permute_kernel<R><<<(unsigned)total_tiles, dim3(kBTile, kBTileRows), 0, st>>>(
+          d_inst, d_tiles, count, home, d_done);
+      BATCH_HIP(hipGetLastError());
+    }
     BATCH_HIP(hipEventRecord(e1, st));
     BATCH_HIP(hipEventSynchronize(e1));
     float ms = 0;
@@ -590,8 +608,10 @@ cleanup:
     (void)hipStreamEndCapture(st, &g2);
     if (g2) (void)hipGraphDestroy(g2);
   }
-  if (exec) (void)hipGraphExecDestroy(exec);
-  if (graph) (void)hipGraphDestroy(graph);
+  for (int g = 0; g < 2; ++g) {
+    if (exec[g]) (void)hipGraphExecDestroy(exec[g]);
+    if (graph[g]) (void)hipGraphDestroy(graph[g]);
+  }
   if (e0) (void)hipEventDestroy(e0);
   if (e1) (void)hipEventDestroy(e1);
   (void)hipStreamSynchronize(st);

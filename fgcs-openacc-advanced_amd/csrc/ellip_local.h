@@ -51,10 +51,6 @@ __device__ __forceinline__ float ellip_sqrt<float>(float x) { return __builtin_a
 template <typename R>
 __device__ __forceinline__ bool reached(R t) { return t < R(1e15); }
 
-// correctly rounded square root of a lattice length
-__device__ __forceinline__ double vertex_sqrt(double x) { return sqrt_rn(x); }
-__device__ __forceinline__ float vertex_sqrt(float x) { return sqrtf(x); }
-
 // The pruning test: can a simplex with smallest vertex `lo` improve on `best`?
 // kGap: the bound adds `gap` (a lower bound of s |X - Y|_N over the upwind
 // plane, already shrunk by plane_gap) and allows 16 ulps of `best` for the
@@ -96,6 +92,77 @@ __device__ __forceinline__ void ellip_tri(R &best, R t0, R t1, R t2, R s2, const
   if (!(Tr[4] * r - root * m2 >= R(0))) return;
   if (!((Tr[3] + Tr[4] - R(1)) * r - root * (m1 + m2) <= R(0))) return;
   const R c = t0 + (d1 * Tr[3] + d2 * Tr[4]) + root;
+  if (c < best) best = c;
+}
+
+// ---------------------------------------------------------------------------
+// Candidates of the per-node-metric (lattice) path: the same candidates as
+// ellip_edge and ellip_tri, evaluated with fused multiply-adds, with
+// reciprocals of the hardware (one ulp in FP32, two Newton steps in FP64) in
+// place of correctly rounded divisions, and with the hardware square root in
+// FP32.  Every kernel of a per-node metric (the metric engine, the per-node
+// plane kernel, the batched solve) evaluates these, so they give each other's
+// bits; the constant-metric kernels keep ellip_edge and ellip_tri, whose
+// operation order the elliptic oracle restates.  In FP32 the candidates move
+// by a few ulps against the correctly rounded form, far inside the plane
+// bound's 10 % shrink (plane_gap); in FP64 by about an ulp.
+template <typename R>
+__device__ __forceinline__ R lat_rcp(R x);
+template <>
+__device__ __forceinline__ float lat_rcp<float>(float x) { return __builtin_amdgcn_rcpf(x); }
+template <>
+__device__ __forceinline__ double lat_rcp<double>(double x) {
+  double r = __builtin_amdgcn_rcp(x);
+  double e = fma(-x, r, 1.0);
+  r = fma(r, e, r);
+  e = fma(-x, r, 1.0);
+  return fma(r, e, r);
+}
+template <typename R>
+__device__ __forceinline__ R lat_fma(R a, R b, R c);
+template <>
+__device__ __forceinline__ float lat_fma<float>(float a, float b, float c) { return fmaf(a, b, c); }
+template <>
+__device__ __forceinline__ double lat_fma<double>(double a, double b, double c) {
+  return fma(a, b, c);
+}
+// the square root of a lattice length or a candidate's radicand
+__device__ __forceinline__ double lat_sqrt(double x) { return sqrt_rn(x); }
+__device__ __forceinline__ float lat_sqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
+
+template <typename R, bool kGap>
+__device__ __forceinline__ void ellip_edge_lat(R &best, R t0, R t1, R s2, R P, R l0, R kap,
+                                               R gap) {
+  if (!may_win<R, kGap>(rmin(t0, t1), gap, best) || !(reached(t0) && reached(t1))) return;
+  const R d = t1 - t0;
+  const R mu = P * d;
+  const R r = lat_fma(-d, mu, s2);
+  if (!(r > R(0))) return;
+  const R root = lat_sqrt(kap * r);
+  const R rm = -root * mu;
+  if (!(lat_fma(l0, r, rm) >= R(0))) return;           // foot point past t0
+  if (!(lat_fma(l0 - R(1), r, rm) <= R(0))) return;    // and before t1
+  const R c = lat_fma(d, l0, t0) + root;
+  if (c < best) best = c;
+}
+
+template <typename R, bool kGap>
+__device__ __forceinline__ void ellip_tri_lat(R &best, R t0, R t1, R t2, R s2, R P11, R P12,
+                                              R P22, R l1, R l2, R kap, R gap) {
+  if (!may_win<R, kGap>(rmin(t0, rmin(t1, t2)), gap, best) ||
+      !(reached(t0) && reached(t1) && reached(t2)))
+    return;
+  const R d1 = t1 - t0, d2 = t2 - t0;
+  const R m1 = lat_fma(P11, d1, P12 * d2);
+  const R m2 = lat_fma(P12, d1, P22 * d2);
+  const R r = lat_fma(-d2, m2, lat_fma(-d1, m1, s2));
+  if (!(r > R(0))) return;
+  const R root = lat_sqrt(kap * r);
+  const R rm1 = -root * m1, rm2 = -root * m2;
+  if (!(lat_fma(l1, r, rm1) >= R(0))) return;
+  if (!(lat_fma(l2, r, rm2) >= R(0))) return;
+  if (!(lat_fma(l1 + l2 - R(1), r, rm1 + rm2) <= R(0))) return;
+  const R c = lat_fma(d2, l2, lat_fma(d1, l1, t0)) + root;
   if (c < best) best = c;
 }
 
@@ -198,6 +265,44 @@ __device__ __forceinline__ R lat_dot(const LatticeN<R> &L, int a1, int b1, int c
          R(a1 * c2 + a2 * c1) * L.Fx + R(b1 * c2 + b2 * c1) * L.Fy;
 }
 
+// lat_dot with compile-time lattice vectors: the same six products summed in
+// the same order, with the terms whose integer coefficient is zero left out.
+// Those terms are exact zeros (the lattice products are finite), and adding a
+// zero leaves a nonzero partial sum unchanged, so the result is lat_dot's
+// (only the sign of an all-zero sum could differ, and no candidate depends on
+// it).  Under IEEE rules the compiler may not drop 0 * x itself: lat_dot
+// spends eleven instructions on every dot product, this form one to three for
+// the patch's lattice vectors.
+template <typename R, int k>
+__device__ __forceinline__ R lat_term(R x) {
+  if constexpr (k == 1) return x;
+  else if constexpr (k == -1) return -x;
+  else return R(k) * x;
+}
+template <typename R, int k, int kBefore>
+__device__ __forceinline__ void lat_add(R &acc, R x) {
+  if constexpr (k != 0) {
+    if constexpr (kBefore == 0) acc = lat_term<R, k>(x);
+    else acc = acc + lat_term<R, k>(x);
+  }
+}
+template <int a1, int b1, int c1, int a2, int b2, int c2, typename R>
+__device__ __forceinline__ R lat_dot_c(const LatticeN<R> &L) {
+  constexpr int kA = a1 * a2, kB = b1 * b2, kC = c1 * c2, kE = a1 * b2 + a2 * b1,
+                kFx = a1 * c2 + a2 * c1, kFy = b1 * c2 + b2 * c1;
+  constexpr int n1 = kA != 0, n2 = n1 + (kB != 0), n3 = n2 + (kC != 0), n4 = n3 + (kE != 0),
+                n5 = n4 + (kFx != 0), n6 = n5 + (kFy != 0);
+  static_assert(n6 > 0, "a lattice dot product with no term");
+  R acc = R(0);
+  lat_add<R, kA, 0>(acc, L.A);
+  lat_add<R, kB, n1>(acc, L.B);
+  lat_add<R, kC, n2>(acc, L.C);
+  lat_add<R, kE, n3>(acc, L.E);
+  lat_add<R, kFx, n4>(acc, L.Fx);
+  lat_add<R, kFy, n5>(acc, L.Fy);
+  return acc;
+}
+
 template <typename R>
 __device__ __forceinline__ LatticeN<R> lattice_of(const R N[6], const int u[3], const R sg[3],
                                                   const R hl[3]) {
@@ -241,44 +346,44 @@ __device__ __forceinline__ void ellip_unit_lattice(R &best, const R p[9], R s2,
   const R t0 = p[0], t1 = p[k1], t2 = p[k2];
   if (!may_win<R, kGap>(rmin(t0, rmin(t1, t2)), gap, best)) return;  // nothing here can win
   {  // spoke centre -> ring U: E = (a1, b1, 0), D = (0, 0, 1)
-    const R g = lat_dot(L, a1, b1, 0, a1, b1, 0);
-    const R b = lat_dot(L, a1, b1, 0, 0, 0, 1);
-    const R P = R(1) / g, l0 = P * b;
-    const R K[3] = {P, l0, L.C - b * l0};
-    ellip_edge<R, kGap>(best, t0, t1, s2, K, gap);
+    const R g = lat_dot_c<a1, b1, 0, a1, b1, 0>(L);
+    const R b = lat_dot_c<a1, b1, 0, 0, 0, 1>(L);
+    const R P = lat_rcp(g), l0 = P * b;
+    ellip_edge_lat<R, kGap>(best, t0, t1, s2, P, l0, lat_fma(-b, l0, L.C), gap);
   }
   {  // rim ring U -> ring U+1: E = (a2-a1, b2-b1, 0), D = (-a1, -b1, 1)
     constexpr int ea = a2 - a1, eb = b2 - b1;
-    const R g = lat_dot(L, ea, eb, 0, ea, eb, 0);
-    const R b = lat_dot(L, ea, eb, 0, -a1, -b1, 1);
-    const R c = lat_dot(L, -a1, -b1, 1, -a1, -b1, 1);
-    const R P = R(1) / g, l0 = P * b;
-    const R K[3] = {P, l0, c - b * l0};
-    ellip_edge<R, kGap>(best, t1, t2, s2, K, gap);
+    const R g = lat_dot_c<ea, eb, 0, ea, eb, 0>(L);
+    const R b = lat_dot_c<ea, eb, 0, -a1, -b1, 1>(L);
+    const R c = lat_dot_c<-a1, -b1, 1, -a1, -b1, 1>(L);
+    const R P = lat_rcp(g), l0 = P * b;
+    ellip_edge_lat<R, kGap>(best, t1, t2, s2, P, l0, lat_fma(-b, l0, c), gap);
   }
   {  // triangle centre, ring U, ring U+1
-    const R g11 = lat_dot(L, a1, b1, 0, a1, b1, 0), g22 = lat_dot(L, a2, b2, 0, a2, b2, 0);
-    const R g12 = lat_dot(L, a1, b1, 0, a2, b2, 0);
-    const R q1 = lat_dot(L, a1, b1, 0, 0, 0, 1), q2 = lat_dot(L, a2, b2, 0, 0, 0, 1);
-    const R rdet = R(1) / (g11 * g22 - g12 * g12);
+    const R g11 = lat_dot_c<a1, b1, 0, a1, b1, 0>(L), g22 = lat_dot_c<a2, b2, 0, a2, b2, 0>(L);
+    const R g12 = lat_dot_c<a1, b1, 0, a2, b2, 0>(L);
+    const R q1 = lat_dot_c<a1, b1, 0, 0, 0, 1>(L), q2 = lat_dot_c<a2, b2, 0, 0, 0, 1>(L);
+    const R rdet = lat_rcp(lat_fma(g11, g22, -g12 * g12));
     const R P11 = g22 * rdet, P12 = -g12 * rdet, P22 = g11 * rdet;
-    const R l1 = P11 * q1 + P12 * q2, l2 = P12 * q1 + P22 * q2;
-    const R K[6] = {P11, P12, P22, l1, l2, L.C - (q1 * l1 + q2 * l2)};
-    ellip_tri<R, kGap>(best, t0, t1, t2, s2, K, gap);
+    const R l1 = lat_fma(P11, q1, P12 * q2), l2 = lat_fma(P12, q1, P22 * q2);
+    ellip_tri_lat<R, kGap>(best, t0, t1, t2, s2, P11, P12, P22, l1, l2,
+                           lat_fma(-q2, l2, lat_fma(-q1, l1, L.C)), gap);
   }
 }
 
 // The nine one-point candidates of a node with a per-node metric.
+template <typename R, int K>
+__device__ __forceinline__ void ellip_vertex_lattice(R &best, const R p[9], R s,
+                                                     const LatticeN<R> &L) {
+  constexpr int a = K == 0 ? 0 : kRing[K - 1][0], b = K == 0 ? 0 : kRing[K - 1][1];
+  if (reached(p[K]))
+    best = rmin(lat_fma(s, lat_sqrt(lat_dot_c<-a, -b, 1, -a, -b, 1>(L)), p[K]), best);
+  if constexpr (K < 8) ellip_vertex_lattice<R, K + 1>(best, p, s, L);
+}
 template <typename R>
 __device__ __forceinline__ void ellip_vertices_lattice(R &best, const R p[9], R s,
                                                        const LatticeN<R> &L) {
-#pragma unroll
-  for (int k = 0; k < 9; ++k) {
-    if (reached(p[k])) {
-      const int a = k == 0 ? 0 : kRing[k - 1][0], b = k == 0 ? 0 : kRing[k - 1][1];
-      best = rmin(p[k] + s * vertex_sqrt(lat_dot(L, -a, -b, 1, -a, -b, 1)), best);
-    }
-  }
+  ellip_vertex_lattice<R, 0>(best, p, s, L);
 }
 
 // Whole node with a per-node metric (all eight units; the nine one-point

@@ -40,13 +40,6 @@ struct EllipConsts {
   R tri[8][6];    // P11, P12, P22, l1, l2, kappa; centre, ring k, ring k+1
 };
 
-template <typename R>
-__device__ __forceinline__ R ellip_sqrt(R x);
-template <>
-__device__ __forceinline__ double ellip_sqrt<double>(double x) { return sqrt_rn(x); }
-template <>
-__device__ __forceinline__ float ellip_sqrt<float>(float x) { return __builtin_amdgcn_sqrtf(x); }
-
 // Unreached nodes carry MAXT (1e16); a simplex touching one is not used.
 template <typename R>
 __device__ __forceinline__ bool reached(R t) { return t < R(1e15); }
@@ -61,51 +54,20 @@ __device__ __forceinline__ bool may_win(R lo, R gap, R best) {
   return lo < best;
 }
 
-template <typename R, bool kGap = false>
-__device__ __forceinline__ void ellip_edge(R &best, R t0, R t1, R s2, const R *E, R gap = R(0)) {
-  if (!may_win<R, kGap>(rmin(t0, t1), gap, best) || !(reached(t0) && reached(t1))) return;
-  const R P = E[0], l0 = E[1], kap = E[2];
-  const R d = t1 - t0;
-  const R mu = P * d;
-  const R r = s2 - d * mu;
-  if (!(r > R(0))) return;
-  const R root = ellip_sqrt(kap * r);
-  if (!(l0 * r - root * mu >= R(0))) return;
-  if (!((l0 - R(1)) * r - root * mu <= R(0))) return;
-  const R c = t0 + d * l0 + root;
-  if (c < best) best = c;
-}
-
-template <typename R, bool kGap = false>
-__device__ __forceinline__ void ellip_tri(R &best, R t0, R t1, R t2, R s2, const R *Tr,
-                                          R gap = R(0)) {
-  if (!may_win<R, kGap>(rmin(t0, rmin(t1, t2)), gap, best) ||
-      !(reached(t0) && reached(t1) && reached(t2)))
-    return;
-  const R d1 = t1 - t0, d2 = t2 - t0;
-  const R m1 = Tr[0] * d1 + Tr[1] * d2;
-  const R m2 = Tr[1] * d1 + Tr[2] * d2;
-  const R r = s2 - (d1 * m1 + d2 * m2);
-  if (!(r > R(0))) return;
-  const R root = ellip_sqrt(Tr[5] * r);
-  if (!(Tr[3] * r - root * m1 >= R(0))) return;
-  if (!(Tr[4] * r - root * m2 >= R(0))) return;
-  if (!((Tr[3] + Tr[4] - R(1)) * r - root * (m1 + m2) <= R(0))) return;
-  const R c = t0 + (d1 * Tr[3] + d2 * Tr[4]) + root;
-  if (c < best) best = c;
-}
-
 // ---------------------------------------------------------------------------
-// Candidates of the per-node-metric (lattice) path: the same candidates as
-// ellip_edge and ellip_tri, evaluated with fused multiply-adds, with
-// reciprocals of the hardware (one ulp in FP32, two Newton steps in FP64) in
-// place of correctly rounded divisions, and with the hardware square root in
-// FP32.  Every kernel of a per-node metric (the metric engine, the per-node
-// plane kernel, the batched solve) evaluates these, so they give each other's
-// bits; the constant-metric kernels keep ellip_edge and ellip_tri, whose
-// operation order the elliptic oracle restates.  In FP32 the candidates move
-// by a few ulps against the correctly rounded form, far inside the plane
-// bound's 10 % shrink (plane_gap); in FP64 by about an ulp.
+// Candidate arithmetic.  The elliptic kernels evaluate a simplex's candidate
+// with fused multiply-adds at fixed places (ellip_edge_lat, ellip_tri_lat
+// below; all of them in FP64, the per-node-metric ones in FP32), and the
+// elliptic oracle restates the same operations, so the FP64 constant-metric
+// kernels and the oracle agree bit for bit.  The square root is the correctly
+// rounded one in FP64 and the hardware's in FP32.  The
+// per-node-metric (lattice) path also forms its simplex constants per node,
+// with reciprocals of the hardware (one ulp in FP32, two Newton steps in
+// FP64) in place of correctly rounded divisions, and its vertex lengths with
+// the same square root; all per-node-metric kernels (the metric engine, the
+// per-node plane kernel, the batched solve) do so alike and give each other's
+// bits.  Against correctly rounded constants a candidate moves by a few ulps,
+// far inside the plane bound's shrink (plane_gap).
 template <typename R>
 __device__ __forceinline__ R lat_rcp(R x);
 template <>
@@ -166,6 +128,59 @@ __device__ __forceinline__ void ellip_tri_lat(R &best, R t0, R t1, R t2, R s2, R
   if (c < best) best = c;
 }
 
+// The same candidates with the constants of a table (EllipConsts), for a
+// constant metric.  FP64 runs the fused form above, which the oracle restates.
+// FP32 keeps one rounding per operation: with the fused form the per-iteration
+// max|dT| of a converged FP32 solve creeps at 1.0e-6 (512^3 blob speed, some
+// 8 ulps of T), just above that workload's 1e-6 stop, where the unfused form
+// stays at 3-4e-7, so the stop would come two iterations later.
+template <typename R, bool kGap = false>
+__device__ __forceinline__ void ellip_edge(R &best, R t0, R t1, R s2, const R *E, R gap = R(0)) {
+  if constexpr (sizeof(R) == 8) {
+    ellip_edge_lat<R, kGap>(best, t0, t1, s2, E[0], E[1], E[2], gap);
+  } else {
+    if (!may_win<R, kGap>(rmin(t0, t1), gap, best) || !(reached(t0) && reached(t1))) return;
+    const R P = E[0], l0 = E[1], kap = E[2];
+    const R d = t1 - t0;
+    const R mu = P * d;
+    const R r = s2 - d * mu;
+    if (!(r > R(0))) return;
+    const R root = lat_sqrt(kap * r);
+    if (!(l0 * r - root * mu >= R(0))) return;
+    if (!((l0 - R(1)) * r - root * mu <= R(0))) return;
+    const R c = t0 + d * l0 + root;
+    if (c < best) best = c;
+  }
+}
+template <typename R, bool kGap = false>
+__device__ __forceinline__ void ellip_tri(R &best, R t0, R t1, R t2, R s2, const R *Tr,
+                                          R gap = R(0)) {
+  if constexpr (sizeof(R) == 8) {
+    ellip_tri_lat<R, kGap>(best, t0, t1, t2, s2, Tr[0], Tr[1], Tr[2], Tr[3], Tr[4], Tr[5], gap);
+  } else {
+    if (!may_win<R, kGap>(rmin(t0, rmin(t1, t2)), gap, best) ||
+        !(reached(t0) && reached(t1) && reached(t2)))
+      return;
+    const R d1 = t1 - t0, d2 = t2 - t0;
+    const R m1 = Tr[0] * d1 + Tr[1] * d2;
+    const R m2 = Tr[1] * d1 + Tr[2] * d2;
+    const R r = s2 - (d1 * m1 + d2 * m2);
+    if (!(r > R(0))) return;
+    const R root = lat_sqrt(Tr[5] * r);
+    if (!(Tr[3] * r - root * m1 >= R(0))) return;
+    if (!(Tr[4] * r - root * m2 >= R(0))) return;
+    if (!((Tr[3] + Tr[4] - R(1)) * r - root * (m1 + m2) <= R(0))) return;
+    const R c = t0 + (d1 * Tr[3] + d2 * Tr[4]) + root;
+    if (c < best) best = c;
+  }
+}
+// A one-point candidate t + s |X - Y|_N with a table's length, in the same form.
+template <typename R>
+__device__ __forceinline__ R ellip_vertex(R t, R s, R len) {
+  if constexpr (sizeof(R) == 8) return lat_fma(s, len, t);
+  else return t + s * len;
+}
+
 // p[0] = centre, p[1..8] = ring.  Returns the new (non-increasing) value.
 template <typename R>
 __device__ __forceinline__ R ellip_node_update(R told, const R p[9], R s,
@@ -174,7 +189,7 @@ __device__ __forceinline__ R ellip_node_update(R told, const R p[9], R s,
   const R s2 = s * s;
 #pragma unroll
   for (int k = 0; k < 9; ++k)
-    if (reached(p[k])) best = rmin(p[k] + s * K.vert[k], best);
+    if (reached(p[k])) best = rmin(ellip_vertex(p[k], s, K.vert[k]), best);
 #pragma unroll
   for (int k = 0; k < 8; ++k) ellip_edge(best, p[0], p[1 + k], s2, K.edge[k]);
 #pragma unroll

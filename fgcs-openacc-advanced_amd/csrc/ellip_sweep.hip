@@ -268,7 +268,7 @@ __global__ __launch_bounds__(64 * kGroups) void ellip_split_kernel(
     }
 #pragma unroll
     for (int k = 0; k < 9; ++k)
-      if (reached(p[k])) best = rmin(p[k] + sl * A.K.vert[k], best);
+      if (reached(p[k])) best = rmin(ellip_vertex(p[k], sl, A.K.vert[k]), best);
     ellip_dispatch_units<R, kGroups>(grp, best, p, sl * sl, A.K, sl * A.gk);
   }
   part[grp][tx] = best;

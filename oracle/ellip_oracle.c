@@ -19,6 +19,12 @@
  *     candidate     T_0 + delta.l0 + root                      (r > 0)
  * (stationary point of the convex function above).  Here every candidate of
  * every simplex is evaluated; the GPU kernel prunes, and must agree.
+ * Candidates are evaluated with fused multiply-adds at fixed places (fma()
+ * below: one rounding for a*b + c), the operations the GPU's kernels run, so
+ * that a constant-metric solve agrees with them bit for bit in FP64.  The
+ * per-node constants are correctly rounded divisions here; the GPU's
+ * per-node-metric kernels use hardware reciprocals there and agree to a few
+ * ulps.
  */
 #include "oracle.h"
 
@@ -151,12 +157,13 @@ double orc_ellip_edge(double t0, double t1, double s2, const double E[3])
   if (!(t0 < BIG && t1 < BIG)) return INFINITY;
   const double d = t1 - t0;
   const double mu = P * d;
-  const double r = s2 - d * mu;
+  const double r = fma(-d, mu, s2);
   if (!(r > 0.0)) return INFINITY;
   const double root = sqrt(kap * r);
-  if (!(l0 * r - root * mu >= 0.0)) return INFINITY;
-  if (!((l0 - 1.0) * r - root * mu <= 0.0)) return INFINITY;
-  return t0 + d * l0 + root;
+  const double rm = -root * mu;
+  if (!(fma(l0, r, rm) >= 0.0)) return INFINITY;
+  if (!(fma(l0 - 1.0, r, rm) <= 0.0)) return INFINITY;
+  return fma(d, l0, t0) + root;
 }
 
 /* Triangle candidate over (centre t0, ring t1, ring t2). */
@@ -164,15 +171,16 @@ double orc_ellip_tri(double t0, double t1, double t2, double s2, const double Tr
 {
   if (!(t0 < BIG && t1 < BIG && t2 < BIG)) return INFINITY;
   const double d1 = t1 - t0, d2 = t2 - t0;
-  const double m1 = Tr[0] * d1 + Tr[1] * d2;
-  const double m2 = Tr[1] * d1 + Tr[2] * d2;
-  const double r = s2 - (d1 * m1 + d2 * m2);
+  const double m1 = fma(Tr[0], d1, Tr[1] * d2);
+  const double m2 = fma(Tr[1], d1, Tr[2] * d2);
+  const double r = fma(-d2, m2, fma(-d1, m1, s2));
   if (!(r > 0.0)) return INFINITY;
   const double root = sqrt(Tr[5] * r);
-  if (!(Tr[3] * r - root * m1 >= 0.0)) return INFINITY;
-  if (!(Tr[4] * r - root * m2 >= 0.0)) return INFINITY;
-  if (!((Tr[3] + Tr[4] - 1.0) * r - root * (m1 + m2) <= 0.0)) return INFINITY;
-  return t0 + (d1 * Tr[3] + d2 * Tr[4]) + root;
+  const double rm1 = -root * m1, rm2 = -root * m2;
+  if (!(fma(Tr[3], r, rm1) >= 0.0)) return INFINITY;
+  if (!(fma(Tr[4], r, rm2) >= 0.0)) return INFINITY;
+  if (!(fma(Tr[3] + Tr[4] - 1.0, r, rm1 + rm2) <= 0.0)) return INFINITY;
+  return fma(d2, Tr[4], fma(d1, Tr[3], t0)) + root;
 }
 
 /* New value of one node from its patch values p[9] (centre + ring order). */
@@ -184,7 +192,7 @@ double orc_ellip_node(double told, const double p[9], double s,
   int k;
   for (k = 0; k < 9; k++) {
     if (p[k] < BIG) {
-      const double c = p[k] + s * K->vert[k];
+      const double c = fma(s, K->vert[k], p[k]);
       if (c < best) best = c;
     }
   }
@@ -368,7 +376,7 @@ static double eval_candidate_k(const double *T, const int n[3], long X, int d, i
     const double t0 = T[pidx[id]];
     if (!(t0 < BIG)) return INFINITY;
     if (v) { v[0] = pidx[id]; w[0] = 1.0; }
-    return t0 + s * K->vert[id];
+    return fma(s, K->vert[id], t0);
   }
   if (type == 2) {
     const int q0 = id < 8 ? 0 : 1 + (id - 8), q1 = id < 8 ? 1 + id : 1 + ((id - 8 + 1) % 8);

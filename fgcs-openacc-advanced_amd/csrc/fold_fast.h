@@ -123,10 +123,35 @@ struct FoldFast {
 // vertex, so none can go below tnew.  On ties this skips work the reference
 // does and rejects; in particular it skips the unreached regions, where the
 // target and its whole patch hold MAXT.
+// Constants held in scalar registers from one point on.  The node update
+// reads its per-direction constants with wave-uniform addresses (scalar
+// loads); left alone, the compiler sinks each load into the branch that first
+// uses it, and a wave then waits on one scalar-cache round trip per branch.
+// Naming a group of constants in one empty asm statement issues their loads
+// together and waits once.
+template <typename R>
+__device__ __forceinline__ void pin_edge_body(const EdgeSite<R> &e) {
+  asm volatile("" ::"s"(e.dxx), "s"(e.rdxx), "s"(e.ax), "s"(e.az), "s"(e.ga), "s"(e.gaz),
+               "s"(e.rgaz), "s"(e.dzz), "s"(e.rga), "s"(e.k1));
+}
+template <typename R>
+__device__ __forceinline__ void pin_tri_body(const FoldSign<R> &S, const TriSite<R> &t,
+                                             const EdgeSite<R> &sub) {
+  asm volatile("" ::"s"(t.dxx), "s"(t.rdxx), "s"(t.dyy), "s"(t.rdyy), "s"(t.ax), "s"(t.ay),
+               "s"(S.ga), "s"(S.rga), "s"(sub.ax_pos), "s"(sub.lgap));
+}
+template <typename R>
+__device__ __forceinline__ void pin_unit_gate(const FoldSign<R> &S, const TriSite<R> &t,
+                                              const EdgeSite<R> &e) {
+  asm volatile("" ::"s"(t.ax_pos), "s"(t.ay_pos), "s"(t.sub_pos), "s"(S.gap), "s"(e.ax_pos),
+               "s"(e.lgap), "s"(e.k1_ok));
+}
+
 template <typename R>
 __device__ __forceinline__ void fast_edge(R &tnew, R st, R xt, R F, const EdgeSite<R> &e) {
   if (tnew <= vmin(xt, st) || (e.ax_pos && xt > st)) return;
   if (!(vmin(xt, st) + e.lgap < tnew * prune_slack<R>())) return;  // cannot go below tnew
+  pin_edge_body(e);
   const R gx = div_const(xt - st, e.dxx, e.rdxx);
   const R q = R(1) - gx * e.ax;
   const R disc = e.ga * (q * q - gx * gx * e.gaz);
@@ -149,6 +174,7 @@ __device__ __forceinline__ void fast_tri(R &tnew, R st, R xt, R yt, const FoldSi
                                          const TriSite<R> &t, const EdgeSite<R> &sub) {
   if (tnew <= vmin(st, vmin(xt, yt)) || (t.ax_pos && xt > st) || (t.ay_pos && yt > xt)) return;
   if (!(vmin(st, vmin(xt, yt)) + S.gap < tnew * prune_slack<R>())) return;  // nor its sub-edge
+  pin_tri_body(S, t, sub);
   const R F = S.F, az = S.az, dzz = S.dzz;
   const R gx = div_const(xt - st, t.dxx, t.rdxx);
   const R gy = div_const(yt - xt, t.dyy, t.rdyy);
@@ -224,6 +250,10 @@ template <typename R>
 __device__ __forceinline__ void fold_unit(R &tnew, const Patch<R> &p, const FoldSign<R> &S,
                                           int u) {
   const R st = p.c;
+  {  // the unit's gate constants, loaded together (u is a compile-time constant)
+    constexpr int kEdgeOf[8] = {0, 7, 4, 3, 5, 6, 1, 2};
+    pin_unit_gate(S, S.tri[u], S.edge[kEdgeOf[u]]);
+  }
   switch (u) {
     case 0:
       fast_tri(tnew, st, p.px, p.pxpy, S, S.tri[0], S.edge[8]);

@@ -201,6 +201,12 @@ __device__ __forceinline__ void ellip_unit(R &best, const R p[9], R s2, const El
                                            R gap) {
   constexpr int k1 = 1 + U, k2 = 1 + (U + 1) % 8;
   if (!may_win<R, true>(rmin(p[0], rmin(p[k1], p[k2])), gap, best)) return;
+  // the unit's twelve constants in scalar registers at once (one wait, not
+  // one per simplex: the compiler would sink each load into its branch)
+  asm volatile("" ::"s"(K.edge[U][0]), "s"(K.edge[U][1]), "s"(K.edge[U][2]),
+               "s"(K.edge[8 + U][0]), "s"(K.edge[8 + U][1]), "s"(K.edge[8 + U][2]),
+               "s"(K.tri[U][0]), "s"(K.tri[U][1]), "s"(K.tri[U][2]), "s"(K.tri[U][3]),
+               "s"(K.tri[U][4]), "s"(K.tri[U][5]));
   ellip_edge<R, true>(best, p[0], p[k1], s2, K.edge[U], gap);
   ellip_edge<R, true>(best, p[k1], p[k2], s2, K.edge[8 + U], gap);
   ellip_tri<R, true>(best, p[0], p[k1], p[k2], s2, K.tri[U], gap);

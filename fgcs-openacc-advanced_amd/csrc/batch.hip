@@ -12,8 +12,8 @@
 // direction d; instances are sorted by their plane count along d, so those are
 // a prefix of the sorted list.  A launch runs a fixed grid of persistent
 // blocks (`groups` waves each, which share the candidate units of a row); a
-// work item is one row of 64 target nodes of one instance, and a row -> instance
-// table finds its owner in one load.  Blocks take items either from a
+// work item is one row of 64 target nodes of one instance, and a row table
+// gives its instance and position in one load.  Blocks take items either from a
 // per-launch device counter (`rows_per_pull` at a time) or by a static stride
 // (block b takes items b, b + grid, ...); the batch autotuner picks.  The
 // launches of one iteration (plus the counter and
@@ -55,12 +55,15 @@ struct BatchInst {
   R h[3];
 };
 
-// Per (direction): instances sorted by plane count, prefix sums of their rows
-// and, for every row of one plane step, the sorted position of its instance.
+// One row of 64 target nodes of a plane step: its instance, the row's first
+// lane-axis index and its row-axis index.  Per direction the host lists the
+// rows of one plane step, instance by instance in the sorted order, so that a
+// block finds everything about its row in one load.
+struct BatchRow {
+  int inst, a0, b, pad;
+};
 struct BatchDir {
-  const int *order;         // [count]
-  const long long *prefix;  // [count + 1], rows of order[0..k)
-  const int *owner;         // [prefix[count]]
+  const BatchRow *row;  // [rows of one plane step, all instances]
 };
 
 // Lattice products of direction d from the metric fields (as lattice_map in
@@ -155,13 +158,19 @@ __global__ __launch_bounds__(64 * kGroups) void batch_plane_kernel(
       cur = item;
       item += gridDim.x;
     }
-    const int lo = bd.owner[cur];
-    const int i = bd.order[lo];
-    if (done[i]) continue;  // uniform over the block
+    const BatchRow rw = bd.row[cur];
+    const int i = rw.inst;
     const BatchInst<R> &I = inst[i];
     const long long *st = I.stride[lay];
-    // plane geometry (lanes along the in-plane axis of smaller stride)
     const int u = fr.ux, v = fr.uy;
+    {  // the row's instance data as one group of scalar loads (one wait, not
+       // one per use)
+      const int dn = done[i];
+      asm volatile("" ::"s"(dn), "s"(st[u]), "s"(st[v]), "s"(st[fr.m]), "s"(I.n[u]), "s"(I.n[v]),
+                   "s"(I.n[fr.m]), "s"(I.count), "s"(I.h[0]), "s"(I.h[1]), "s"(I.h[2]));
+      if (dn) continue;  // uniform over the block
+    }
+    // plane geometry (lanes along the in-plane axis of smaller stride)
     const bool u_first = st[u] < st[v];
     const int la = u_first ? u : v, lb = u_first ? v : u;
     const int na = I.n[la];
@@ -172,9 +181,7 @@ __global__ __launch_bounds__(64 * kGroups) void batch_plane_kernel(
     const long long sm = fr.s * st[fr.m];
     const int ua = fr.ux == la ? fr.sx : 0, ub = fr.ux == la ? 0 : fr.sx;
     const int va = fr.uy == la ? fr.sy : 0, vb = fr.uy == la ? 0 : fr.sy;
-    const long long r = cur - bd.prefix[lo];
-    const int tiles_a = (na + 63) / 64;
-    const int a0 = (int)(r % tiles_a) * 64, b = 1 + (int)(r / tiles_a);
+    const int a0 = rw.a0, b = rw.b;
     R *T = on_T ? I.T : I.Y;
     const R *M = lay == kLayZ ? I.MZ : lay == kLayX ? I.MX : I.MY;
     for (int idx = grp * 64 + tx; idx < 3 * 66; idx += 64 * kGroups) {
@@ -316,15 +323,22 @@ __global__ void batch_reset_kernel(unsigned int *counters, int nc, unsigned long
   }
 }
 
-// rows of 64 target nodes in one plane of direction d of instance x, in the
-// layout that direction runs in
-static long long rows_of(const fold3d_instance &x, int d) {
+// Rows of 64 target nodes in one plane of direction d of instance x, in the
+// layout that direction runs in (lanes along the in-plane axis of smaller
+// stride, as the kernel picks them): `tiles` per line, `lines` lines.
+static void row_shape(const fold3d_instance &x, int d, int &tiles, int &lines) {
   const Frame &fr = kFrames[d];
   const int *order = fr.m == 0 ? kOrderX : fr.m == 2 ? kOrderZ : kOrderY;
   const Grid G(x.n[0], x.n[1], x.n[2], order);
   const bool u_first = G.stride[fr.ux] < G.stride[fr.uy];
   const int la = u_first ? fr.ux : fr.uy, lb = u_first ? fr.uy : fr.ux;
-  return (long long)((x.n[la] + 63) / 64) * x.n[lb];
+  tiles = (x.n[la] + 63) / 64;
+  lines = x.n[lb];
+}
+static long long rows_of(const fold3d_instance &x, int d) {
+  int tiles, lines;
+  row_shape(x, d, tiles, lines);
+  return (long long)tiles * lines;
 }
 
 size_t batch_scratch_bytes(int count, const fold3d_instance *inst, int precision,
@@ -391,7 +405,7 @@ int batch_solve(hipStream_t user_st, int count, const fold3d_instance *inst, int
   std::vector<int> order_all(6 * count);
   std::vector<long long> prefix_all(6 * (count + 1));
   std::vector<int> steps(6);
-  std::vector<long long> owner_off(7, 0);
+  std::vector<long long> row_off(7, 0);
   for (int d = 0; d < 6; ++d) {
     const int m = kFrames[d].m;
     int *ord = &order_all[d * count];
@@ -401,21 +415,29 @@ int batch_solve(hipStream_t user_st, int count, const fold3d_instance *inst, int
     pre[0] = 0;
     for (int k = 0; k < count; ++k) pre[k + 1] = pre[k] + rows_of(inst[ord[k]], d);
     steps[d] = inst[ord[0]].n[m] - 1;
-    owner_off[d + 1] = owner_off[d] + pre[count];
+    row_off[d + 1] = row_off[d] + pre[count];
   }
-  std::vector<int> owner_all(owner_off[6]);
+  std::vector<BatchRow> row_all(row_off[6]);
   for (int d = 0; d < 6; ++d) {
+    const int *ord = &order_all[d * count];
     const long long *pre = &prefix_all[d * (count + 1)];
-    int *own = &owner_all[owner_off[d]];
-    for (int k = 0; k < count; ++k)
-      for (long long r = pre[k]; r < pre[k + 1]; ++r) own[r] = k;
+    BatchRow *row = &row_all[row_off[d]];
+    for (int k = 0; k < count; ++k) {
+      int tiles, lines;
+      row_shape(inst[ord[k]], d, tiles, lines);
+      for (long long r = pre[k]; r < pre[k + 1]; ++r) {
+        const long long q = r - pre[k];
+        row[r] = BatchRow{ord[k], (int)(q % tiles) * 64, 1 + (int)(q / tiles), 0};
+      }
+    }
   }
   int launches = 0;
   for (int d = 0; d < 6; ++d) launches += steps[d];
   // device tables (freed at the end of the solve)
   BatchInst<R> *d_inst = nullptr;
-  int *d_order = nullptr, *d_done = nullptr, *d_owner = nullptr;
-  long long *d_prefix = nullptr, *d_tiles = nullptr;
+  int *d_done = nullptr;
+  BatchRow *d_rows = nullptr;
+  long long *d_tiles = nullptr;
   unsigned int *d_counters = nullptr;
   unsigned long long *d_max = nullptr;
   hipGraph_t graph[2] = {nullptr, nullptr};
@@ -436,22 +458,16 @@ int batch_solve(hipStream_t user_st, int count, const fold3d_instance *inst, int
   } while (0)
   {
     BATCH_HIP(hipMalloc(&d_inst, sizeof(BatchInst<R>) * count));
-    BATCH_HIP(hipMalloc(&d_order, sizeof(int) * order_all.size()));
-    BATCH_HIP(hipMalloc(&d_prefix, sizeof(long long) * prefix_all.size()));
     BATCH_HIP(hipMalloc(&d_done, sizeof(int) * count));
     BATCH_HIP(hipMalloc(&d_counters, sizeof(unsigned int) * launches));
     BATCH_HIP(hipMalloc(&d_max, sizeof(unsigned long long) * count * kBatchMaxSlots));
     BATCH_HIP(hipMalloc(&d_tiles, sizeof(long long) * (count + 1)));
-    BATCH_HIP(hipMalloc(&d_owner, sizeof(int) * owner_all.size()));
-    BATCH_HIP(hipMemcpyAsync(d_owner, owner_all.data(), sizeof(int) * owner_all.size(),
+    BATCH_HIP(hipMalloc(&d_rows, sizeof(BatchRow) * row_all.size()));
+    BATCH_HIP(hipMemcpyAsync(d_rows, row_all.data(), sizeof(BatchRow) * row_all.size(),
                              hipMemcpyHostToDevice, st));
     BATCH_HIP(hipMemcpyAsync(d_tiles, tiles_prefix.data(), sizeof(long long) * (count + 1),
                              hipMemcpyHostToDevice, st));
     BATCH_HIP(hipMemcpyAsync(d_inst, hinst.data(), sizeof(BatchInst<R>) * count,
-                             hipMemcpyHostToDevice, st));
-    BATCH_HIP(hipMemcpyAsync(d_order, order_all.data(), sizeof(int) * order_all.size(),
-                             hipMemcpyHostToDevice, st));
-    BATCH_HIP(hipMemcpyAsync(d_prefix, prefix_all.data(), sizeof(long long) * prefix_all.size(),
                              hipMemcpyHostToDevice, st));
     std::vector<int> done(count, 0);
     BATCH_HIP(hipMemcpyAsync(d_done, done.data(), sizeof(int) * count, hipMemcpyHostToDevice, st));
@@ -495,7 +511,7 @@ int batch_solve(hipStream_t user_st, int count, const fold3d_instance *inst, int
         const int m = kFrames[d].m;
         const int *ord = &order_all[d * count];
         const long long *pre = &prefix_all[d * (count + 1)];
-        BatchDir bd{d_order + d * count, d_prefix + d * (count + 1), d_owner + owner_off[d]};
+        BatchDir bd{d_rows + row_off[d]};
         const LatticeIdx LI = lattice_idx(d);
         int active = count;
         for (int q = 0; q < steps[d]; ++q, ++slot) {
@@ -616,13 +632,11 @@ cleanup:
   if (e1) (void)hipEventDestroy(e1);
   (void)hipStreamSynchronize(st);
   (void)hipFree(d_inst);
-  (void)hipFree(d_order);
-  (void)hipFree(d_prefix);
   (void)hipFree(d_done);
   (void)hipFree(d_counters);
   (void)hipFree(d_max);
   (void)hipFree(d_tiles);
-  (void)hipFree(d_owner);
+  (void)hipFree(d_rows);
   // later work on the caller's stream sees the solved fields
   (void)hipEventRecord(order_ev, st);
   (void)hipStreamWaitEvent(user_st, order_ev, 0);

@@ -74,12 +74,12 @@ struct LatticeIdx {
   int sg[6];
 };
 
-static LatticeIdx lattice_idx(int d) {
-  const Frame &fr = kFrames[d];
+constexpr LatticeIdx lattice_idx(int d) {
+  const Frame fr = kFrames[d];
   const int u[3] = {fr.ux, fr.uy, fr.m};
   const int sgn[3] = {fr.sx, fr.sy, fr.s};
   const int pairs[6][2] = {{0, 0}, {1, 1}, {2, 2}, {0, 1}, {0, 2}, {1, 2}};
-  LatticeIdx L;
+  LatticeIdx L{};
   for (int e = 0; e < 6; ++e) {
     const int gi = u[pairs[e][0]], gj = u[pairs[e][1]];
     L.q[e] = gi == gj ? gi : (gi + gj == 1) ? 3 : (gi + gj == 2) ? 4 : 5;
@@ -123,19 +123,23 @@ __device__ __forceinline__ void batch_dispatch_units(int grp, R &best, const R p
 // plane do not all meet on one word (the host takes the largest).
 constexpr int kBatchMaxSlots = 32;
 
-// One plane step q of direction `dir` for every instance that has it:
+// One plane step q of direction kDir for every instance that has it:
 // persistent blocks of kGroups waves pull rows of 64 target nodes from the
-// launch's counter, `chunk` consecutive rows per pull.
-template <typename R, int kGroups>
+// launch's counter, `chunk` consecutive rows per pull.  The direction is a
+// compile-time constant, so its frame and lattice indices take no registers.
+template <typename R, int kGroups, int kDir>
 __global__ __launch_bounds__(64 * kGroups) void batch_plane_kernel(
-    const BatchInst<R> *__restrict__ inst, BatchDir bd, int dir, int lay, int on_T, LatticeIdx LI,
-    int q,
-    long long total, int chunk, unsigned int *__restrict__ counter,
-    const int *__restrict__ done, unsigned long long *__restrict__ max_dt) {
+    const BatchInst<R> *__restrict__ inst, BatchDir bd, int q, long long total, int chunk,
+    unsigned int *__restrict__ counter, const int *__restrict__ done,
+    unsigned long long *__restrict__ max_dt) {
   __shared__ R s[3][64 + 2];
   __shared__ R part[kGroups][64];
   __shared__ long long item_s;
-  const Frame fr = kFrames[dir];
+  constexpr Frame fr = kFrames[kDir];
+  constexpr LatticeIdx LI = lattice_idx(kDir);
+  // the x-sweeps run in layout X on the scratch copy Y, the others in layout Y on T
+  constexpr int lay = fr.m == 0 ? kLayX : kLayY;
+  constexpr bool on_T = fr.m != 0;
   const int tx = threadIdx.x;
   const int grp = __builtin_amdgcn_readfirstlane(threadIdx.y);
   // chunk > 0: pull `chunk` rows at a time from the launch's counter;
@@ -239,6 +243,35 @@ __global__ __launch_bounds__(64 * kGroups) void batch_plane_kernel(
                   (unsigned long long)__double_as_longlong(vmax));
     }
     __syncthreads();  // s[][] and part[][] are reused by the next item
+  }
+}
+
+// Arguments of one plane-step launch, and the launch by direction.
+template <typename R>
+struct BatchLaunch {
+  const BatchInst<R> *inst;
+  BatchDir bd;
+  int q;
+  long long total;
+  int chunk;
+  unsigned int *counter;
+  const int *done;
+  unsigned long long *max_dt;
+};
+template <typename R, int kGroups, int kDir>
+static void launch_batch_plane_dir(int grid, hipStream_t s, const BatchLaunch<R> &a) {
+  batch_plane_kernel<R, kGroups, kDir><<<grid, dim3(64, kGroups), 0, s>>>(
+      a.inst, a.bd, a.q, a.total, a.chunk, a.counter, a.done, a.max_dt);
+}
+template <typename R, int kGroups>
+static void launch_batch_plane(int d, int grid, hipStream_t s, const BatchLaunch<R> &a) {
+  switch (d) {
+    case 0: launch_batch_plane_dir<R, kGroups, 0>(grid, s, a); break;
+    case 1: launch_batch_plane_dir<R, kGroups, 1>(grid, s, a); break;
+    case 2: launch_batch_plane_dir<R, kGroups, 2>(grid, s, a); break;
+    case 3: launch_batch_plane_dir<R, kGroups, 3>(grid, s, a); break;
+    case 4: launch_batch_plane_dir<R, kGroups, 4>(grid, s, a); break;
+    default: launch_batch_plane_dir<R, kGroups, 5>(grid, s, a); break;
   }
 }
 
@@ -507,12 +540,13 @@ int batch_solve(hipStream_t user_st, int count, const fold3d_instance *inst, int
         batch_permute_kernel<R><<<(unsigned)total_tiles, dim3(kBTile, kBTileRows), 0, s>>>(
             d_inst, d_tiles, count, job, d_done);
       };
-      auto direction = [&](int d, int lay, int on_T) {
+      // (the x-sweeps run in layout X on the scratch copy, the others in
+      // layout Y on the caller's T: batch_plane_kernel derives both from d)
+      auto direction = [&](int d) {
         const int m = kFrames[d].m;
         const int *ord = &order_all[d * count];
         const long long *pre = &prefix_all[d * (count + 1)];
         BatchDir bd{d_rows + row_off[d]};
-        const LatticeIdx LI = lattice_idx(d);
         int active = count;
         for (int q = 0; q < steps[d]; ++q, ++slot) {
           while (active > 0 && inst[ord[active - 1]].n[m] - 1 <= q) --active;
@@ -526,27 +560,22 @@ int batch_solve(hipStream_t user_st, int count, const fold3d_instance *inst, int
                                         1, std::min<long long>(8, total / (4 * blocks)));
           const int grid =
               (int)std::min<long long>(chunk > 0 ? (total + chunk - 1) / chunk : total, blocks);
-          if (groups == 1)
-            batch_plane_kernel<R, 1><<<grid, dim3(64, 1), 0, s>>>(
-                d_inst, bd, d, lay, on_T, LI, q, total, chunk, d_counters + slot, d_done, d_max);
-          else if (groups == 2)
-            batch_plane_kernel<R, 2><<<grid, dim3(64, 2), 0, s>>>(
-                d_inst, bd, d, lay, on_T, LI, q, total, chunk, d_counters + slot, d_done, d_max);
-          else
-            batch_plane_kernel<R, 4><<<grid, dim3(64, 4), 0, s>>>(
-                d_inst, bd, d, lay, on_T, LI, q, total, chunk, d_counters + slot, d_done, d_max);
+          const BatchLaunch<R> bl{d_inst, bd, q, total, chunk, d_counters + slot, d_done, d_max};
+          if (groups == 1) launch_batch_plane<R, 1>(d, grid, s, bl);
+          else if (groups == 2) launch_batch_plane<R, 2>(d, grid, s, bl);
+          else launch_batch_plane<R, 4>(d, grid, s, bl);
         }
       };
       permute(0, first ? kLayZ : kLayY, kLayX, 1);
       if (!resident) permute(2, kLayZ, kLayX, kMetricFields);
-      direction(FOLD3D_X_DOWN, kLayX, 0);
-      direction(FOLD3D_X_UP, kLayX, 0);
+      direction(FOLD3D_X_DOWN);
+      direction(FOLD3D_X_UP);
       permute(1, kLayX, kLayY, 1);
       if (!resident) permute(3, kLayZ, kLayY, kMetricFields);
-      direction(FOLD3D_K_UP, kLayY, 1);
-      direction(FOLD3D_K_DOWN, kLayY, 1);
-      direction(FOLD3D_Y_UP, kLayY, 1);
-      direction(FOLD3D_Y_DOWN, kLayY, 1);
+      direction(FOLD3D_K_UP);
+      direction(FOLD3D_K_DOWN);
+      direction(FOLD3D_Y_UP);
+      direction(FOLD3D_Y_DOWN);
     };
 
     if (use_graph) {  // the first iteration and the others

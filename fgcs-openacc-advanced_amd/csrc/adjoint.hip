@@ -112,7 +112,6 @@ __global__ __launch_bounds__(256) void stencil_kernel(const R *__restrict__ T, A
   const int i = 1 + (int)(tid % A.n[0]);
   const int j = 1 + (int)((tid / A.n[0]) % A.n[1]);
   const int k = 1 + (int)(tid / ((long long)A.n[0] * A.n[1]));
-  const int ijk[3] = {i, j, k};
   const long long X = i * A.stride[0] + j * A.stride[1] + k * A.stride[2];
   const R tX = T[X];
   if (!reached(tX)) {  // held fixed: no stencil
@@ -146,7 +145,7 @@ __global__ __launch_bounds__(256) void stencil_kernel(const R *__restrict__ T, A
   best.nv = 0;
   for (int d = 0; d < 6; ++d) {
     const Frame fr = kFrames[d];
-    const int cm = ijk[fr.m] - fr.s;  // upwind centre along the march axis
+    const int cm = (fr.m == 0 ? i : fr.m == 1 ? j : k) - fr.s;  // upwind centre along the march axis
     if (cm < 1 || cm > A.n[fr.m]) continue;
     const int u[3] = {fr.ux, fr.uy, fr.m};
     const R sg[3] = {R(fr.sx), R(fr.sy), R(fr.s)};
@@ -158,8 +157,10 @@ __global__ __launch_bounds__(256) void stencil_kernel(const R *__restrict__ T, A
     R p[9];
     long long po[9];
     po[0] = -sm;
+    #pragma unroll
     for (int q = 0; q < 8; ++q) po[1 + q] = -sm + kRing[q][0] * su + kRing[q][1] * sv;
     R minv = R(1e30);
+    #pragma unroll
     for (int q = 0; q < 9; ++q) {
       p[q] = T[X + po[q]];
       if (reached(p[q])) minv = rmin(p[q], minv);
@@ -172,10 +173,15 @@ __global__ __launch_bounds__(256) void stencil_kernel(const R *__restrict__ T, A
       a = q == 0 ? 0 : kRing[q - 1][0];
       b = q == 0 ? 0 : kRing[q - 1][1];
     };
+    // (by selects: a local array indexed by the frame's axes would live in
+    // scratch memory)
     auto to_global = [&](const R wl[3], R wg[3]) {
-      for (int q = 0; q < 3; ++q) wg[u[q]] = sg[q] * wl[q];
+      const R g0 = sg[0] * wl[0], g1 = sg[1] * wl[1], g2 = sg[2] * wl[2];
+#pragma unroll
+      for (int a = 0; a < 3; ++a) wg[a] = u[0] == a ? g0 : u[1] == a ? g1 : g2;
     };
     // vertices
+    #pragma unroll
     for (int q = 0; q < 9; ++q) {
       if (!reached(p[q]) || p[q] + gap > thr) continue;
       int a, b;
@@ -194,6 +200,7 @@ __global__ __launch_bounds__(256) void stencil_kernel(const R *__restrict__ T, A
     }
     const R s2 = s * s;
     // edges: spokes 0..7 (centre -> ring q), rims 8..15 (ring q -> ring q+1)
+    #pragma unroll
     for (int e = 0; e < 16; ++e) {
       const int q0 = e < 8 ? 0 : 1 + (e - 8), q1 = e < 8 ? 1 + e : 1 + ((e - 8 + 1) % 8);
       const R t0 = p[q0], t1 = p[q1];
@@ -225,6 +232,7 @@ __global__ __launch_bounds__(256) void stencil_kernel(const R *__restrict__ T, A
       }
     }
     // triangles: centre, ring q, ring q+1
+    #pragma unroll
     for (int t = 0; t < 8; ++t) {
       const int q1 = 1 + t, q2 = 1 + (t + 1) % 8;
       const R t0 = p[0], t1 = p[q1], t2 = p[q2];
@@ -274,9 +282,12 @@ __global__ __launch_bounds__(256) void stencil_kernel(const R *__restrict__ T, A
   R w[3] = {R(0), R(0), R(0)};
   R gs = R(0), gth = R(0);
   if (follows) {
-    for (int q = 0; q < best.nv; ++q) {
-      o[q] = (int)best.off[q];
-      w[q] = best.wgt[q];
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+      if (q < best.nv) {
+        o[q] = (int)best.off[q];
+        w[q] = best.wgt[q];
+      }
     }
     gs = best.ell;
     gth = A.th ? s * quad6(dN, best.wg) / (R(2) * best.ell) : R(0);

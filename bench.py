@@ -659,6 +659,8 @@ def main():
     ap.add_argument("--batch-tuning", default="",
                     help="schedule of the batched solve as 'waves,blocks_per_cu,rows_per_pull' "
                          "(default: the batch autotuner picks)")
+    ap.add_argument("--batch-streams", type=int, default=0,
+                    help="sub-batches the batched solve runs at once (default: the library's)")
     ap.add_argument("--sizes", default="",
                     help="comma-separated grid sides at which to time the headline solve as well "
                          "(device-built input; default: none)")
@@ -870,6 +872,8 @@ def main():
         bt = tuple(int(v) for v in args.batch_tuning.split(",")) if args.batch_tuning else None
         if bt is not None and len(bt) != 3:
             ap.error("--batch-tuning takes three integers: waves,blocks_per_cu,rows_per_pull")
+        if args.batch_streams:
+            solver.set_batch_streams(args.batch_streams)
         batch = batch_run(solver, torch, tuning=bt)
 
     adjoint = None

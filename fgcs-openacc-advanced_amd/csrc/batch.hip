@@ -215,14 +215,13 @@ __global__ __launch_bounds__(64 * kGroups) void batch_plane_kernel(
     __syncthreads();
     R best = told;
     if (act) {
-      const int ca = tx + 1;
+      // the patch (uniform offsets of +x^ and +y^ within the staged rows)
+      const R *c0 = &s[1][tx + 1];
+      const int ox = ub * 66 + ua, oy = vb * 66 + va;
       R pv[9];
-      pv[0] = s[1][ca];
+      pv[0] = c0[0];
 #pragma unroll
-      for (int k = 0; k < 8; ++k) {
-        const int ii = kRing[k][0], jj = kRing[k][1];
-        pv[1 + k] = s[1 + ii * ub + jj * vb][ca + ii * ua + jj * va];
-      }
+      for (int k = 0; k < 8; ++k) pv[1 + k] = c0[kRing[k][0] * ox + kRing[k][1] * oy];
       ellip_vertices_lattice<R>(best, pv, sl, L);
       batch_dispatch_units<R, kGroups>(grp, best, pv, sl * sl, L, plane_gap(L, sl));
     }

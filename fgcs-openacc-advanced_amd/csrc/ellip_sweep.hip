@@ -263,15 +263,14 @@ __global__ __launch_bounds__(64 * kGroups) void ellip_split_kernel(
   __syncthreads();
   R best = told;
   if (active) {
-    const int ca = tx + 1;
-    const int ua = g.ua, ub = g.ub, va = g.va, vb = g.vb;
+    // the patch: the offsets of +x^ and +y^ within the staged rows are the
+    // same for every lane (uniform products)
+    const R *c0 = &s[1][tx + 1];
+    const int ox = g.ub * 66 + g.ua, oy = g.vb * 66 + g.va;
     R p[9];
-    p[0] = s[1][ca];
+    p[0] = c0[0];
 #pragma unroll
-    for (int k = 0; k < 8; ++k) {
-      const int i = kRing[k][0], j = kRing[k][1];
-      p[1 + k] = s[1 + i * ub + j * vb][ca + i * ua + j * va];
-    }
+    for (int k = 0; k < 8; ++k) p[1 + k] = c0[kRing[k][0] * ox + kRing[k][1] * oy];
 #pragma unroll
     for (int k = 0; k < 9; ++k)
       if (reached(p[k])) best = rmin(ellip_vertex(p[k], sl, A.K.vert[k]), best);
@@ -487,15 +486,14 @@ __global__ __launch_bounds__(64 * kGroups) void ellip_metric_kernel(
   __syncthreads();
   R best = told;
   if (active) {
-    const int ca = tx + 1;
-    const int ua = g.ua, ub = g.ub, va = g.va, vb = g.vb;
+    // the patch: the offsets of +x^ and +y^ within the staged rows are the
+    // same for every lane (uniform products)
+    const R *c0 = &s[1][tx + 1];
+    const int ox = g.ub * 66 + g.ua, oy = g.vb * 66 + g.va;
     R p[9];
-    p[0] = s[1][ca];
+    p[0] = c0[0];
 #pragma unroll
-    for (int k = 0; k < 8; ++k) {
-      const int i = kRing[k][0], j = kRing[k][1];
-      p[1 + k] = s[1 + i * ub + j * vb][ca + i * ua + j * va];
-    }
+    for (int k = 0; k < 8; ++k) p[1 + k] = c0[kRing[k][0] * ox + kRing[k][1] * oy];
     ellip_vertices_lattice<R>(best, p, sl, L);
     lattice_dispatch_units<R, kGroups>(grp, best, p, sl * sl, L, plane_gap(L, sl));
   }

@@ -488,18 +488,20 @@ __device__ __forceinline__ void split_rowgroup(R *__restrict__ T, const PlaneGeo
   __syncthreads();
   R tn = fabs(told);
   if (active) {
-    const int ca = la_ + 1, cb = lb_ + 1;
-    const int ua = g.ua, ub = g.ub, va = g.va, vb = g.vb;
+    // the patch around the lane's node: the offsets of +x^ and +y^ within the
+    // staged rows are the same for every lane (uniform products)
+    const R *c0 = &s[lb_ + 1][la_ + 1];
+    const int ox = g.ub * (kW + 2) + g.ua, oy = g.vb * (kW + 2) + g.va;
     Patch<R> p;
-    p.c = s[cb][ca];
-    p.px = s[cb + ub][ca + ua];
-    p.py = s[cb + vb][ca + va];
-    p.mx = s[cb - ub][ca - ua];
-    p.my = s[cb - vb][ca - va];
-    p.pxpy = s[cb + ub + vb][ca + ua + va];
-    p.pxmy = s[cb + ub - vb][ca + ua - va];
-    p.mxpy = s[cb - ub + vb][ca - ua + va];
-    p.mxmy = s[cb - ub - vb][ca - ua - va];
+    p.c = c0[0];
+    p.px = c0[ox];
+    p.py = c0[oy];
+    p.mx = c0[-ox];
+    p.my = c0[-oy];
+    p.pxpy = c0[ox + oy];
+    p.pxmy = c0[ox - oy];
+    p.mxpy = c0[oy - ox];
+    p.mxmy = c0[-ox - oy];
     tn = split_dispatch<R, kGroups>(grp, told, p, C);  // grp is wave-uniform
   }
   part[row][grp][tx] = tn;

@@ -2,12 +2,12 @@
 // stream of launches.
 //
 // Once per batch the slowness and the inverse metric N = M^-1 of every
-// instance are written to seven fields of its own (tilt fields or a constant
-// metric alike).  An iteration then runs, for all instances together, the
-// single-instance metric engine's schedule of two moves: move T into the
-// x-sweep layout (one launch for the whole batch; the metric fields are kept
-// in every layout, or moved along when memory is short), run the x-sweeps,
-// move T into the y-sweep layout (y slowest), run the z- and the y-sweeps.  One launch per (sweep
+// instance are written to seven fields of its own in each sweep layout (tilt
+// fields or a constant metric alike).  An iteration then runs, for all
+// instances together, the single-instance metric engine's schedule of two
+// moves: move T into the x-sweep layout (one launch for the whole batch), run
+// the x-sweeps, move T into the y-sweep layout (y slowest), run the z- and the
+// y-sweeps.  One launch per (sweep
 // direction d, plane step q) serves every instance that still has a plane q in
 // direction d; instances are sorted by their plane count along d, so those are
 // a prefix of the sorted list.  A launch runs a fixed grid of persistent
@@ -32,6 +32,7 @@
 
 #include "common.h"
 #include "ellip_local.h"
+#include "metric_prep.h"
 #include "solver.h"
 
 namespace fold3d {
@@ -42,13 +43,12 @@ namespace fold3d {
 enum BatchLayout { kLayZ = 0, kLayX = 1, kLayY = 2 };
 
 // Device view of one instance.  Scratch per instance: Y (T in layout X; T
-// itself holds layout Y between iterations)
-// and kMetricFields fields (slowness and N = M^-1) per layout: MZ in the
-// reference layout, MX and MY in the x- and y-sweep layouts (made once per
-// batch), or MX = MY remade before each sweep pair when memory is short.
+// itself holds layout Y between iterations) and kMetricFields fields
+// (slowness and N = M^-1) in each sweep layout, MX and MY, made once per
+// batch (metric_prep_kernel).
 template <typename R>
 struct BatchInst {
-  R *T, *Y, *MZ, *MX, *MY;
+  R *T, *Y, *MX, *MY;
   long long count;
   long long stride[3][3];  // [layout][axis]
   int n[3];
@@ -187,7 +187,7 @@ __global__ __launch_bounds__(64 * kGroups) void batch_plane_kernel(
     const int va = fr.uy == la ? fr.sy : 0, vb = fr.uy == la ? 0 : fr.sy;
     const int a0 = rw.a0, b = rw.b;
     R *T = on_T ? I.T : I.Y;
-    const R *M = lay == kLayZ ? I.MZ : lay == kLayX ? I.MX : I.MY;
+    const R *M = lay == kLayX ? I.MX : I.MY;
     for (int idx = grp * 64 + tx; idx < 3 * 66; idx += 64 * kGroups) {
       const int l_a = idx % 66, l_b = idx / 66;
       const int sa = a0 + l_a;
@@ -280,37 +280,34 @@ constexpr int kBTile = 32, kBTileRows = 8;
 
 template <typename R>
 struct PermuteJob {
-  int which;          // 0: T -> Y, 1: Y -> T, 2: MZ -> MX, 3: MZ -> MY
+  int which;          // 0: T -> Y, 1: Y -> T
   int lin, lout;      // layouts
   int fields;
   int all;            // also the instances that are done
 };
 
+// One tile of a layout move: its instance, its 32x32 column (bx, by) and its
+// plane z, listed by the host so that a block finds them in one load.
+struct BatchTile {
+  int inst, bx, by, z;
+};
+
 template <typename R>
 __global__ __launch_bounds__(kBTile *kBTileRows) void batch_permute_kernel(
-    const BatchInst<R> *__restrict__ inst, const long long *__restrict__ tiles_prefix, int count,
+    const BatchInst<R> *__restrict__ inst, const BatchTile *__restrict__ tiles,
     PermuteJob<R> job, const int *__restrict__ done) {
   __shared__ R tile[kBTile][kBTile + 1];  // [y - y0][x - x0]
-  const long long item = blockIdx.x;
-  int lo = 0, hi = count - 1;
-  while (lo < hi) {
-    const int mid = (lo + hi + 1) >> 1;
-    if (tiles_prefix[mid] <= item) lo = mid;
-    else hi = mid - 1;
-  }
+  const BatchTile tl = tiles[blockIdx.x];
+  const int lo = tl.inst;
   if (!job.all && done[lo]) return;
   const BatchInst<R> &I = inst[lo];
   const int Hx = I.n[0] + 2, Hy = I.n[1] + 2;
-  const int tx_n = (Hx + kBTile - 1) / kBTile, ty_n = (Hy + kBTile - 1) / kBTile;
-  long long r = item - tiles_prefix[lo];
-  const int bx = (int)(r % tx_n);
-  r /= tx_n;
-  const int by = (int)(r % ty_n);
-  const long long z = r / ty_n;
+  const int bx = tl.bx, by = tl.by;
+  const long long z = tl.z;
   const long long *is = I.stride[job.lin], *os = I.stride[job.lout];
   const bool in_x = is[0] == 1, out_x = os[0] == 1;
-  const R *src = job.which == 0 ? I.T : job.which == 1 ? I.Y : I.MZ;
-  R *dst = job.which == 0 ? I.Y : job.which == 1 ? I.T : job.which == 2 ? I.MX : I.MY;
+  const R *src = job.which == 0 ? I.T : I.Y;
+  R *dst = job.which == 0 ? I.Y : I.T;
   const int x0 = bx * kBTile, y0 = by * kBTile;
   const int tx = threadIdx.x;
   for (int f = 0; f < job.fields; ++f) {
@@ -328,22 +325,6 @@ __global__ __launch_bounds__(kBTile *kBTileRows) void batch_permute_kernel(
       if (x < Hx && y < Hy) out[x * os[0] + y * os[1]] = tile[y - y0][x - x0];
     }
     __syncthreads();
-  }
-}
-
-// Slowness and N = M^-1 of one instance in the reference layout.
-template <typename R>
-__global__ void batch_prep_kernel(long long count, const R *__restrict__ V, R speed,
-                                  const R *__restrict__ th, const R *__restrict__ ph,
-                                  const R *__restrict__ ra, const R *__restrict__ rb, R Nc0,
-                                  R Nc1, R Nc2, R Nc3, R Nc4, R Nc5, R *__restrict__ out) {
-  for (long long x = blockIdx.x * (long long)blockDim.x + threadIdx.x; x < count;
-       x += (long long)gridDim.x * blockDim.x) {
-    R N[6] = {Nc0, Nc1, Nc2, Nc3, Nc4, Nc5};
-    if (th) tilted_inverse_metric<R>(th[x], ph[x], ra[x], rb[x], N);
-    out[x] = V ? R(1) / V[x] : R(1) / speed;
-#pragma unroll
-    for (int k = 0; k < 6; ++k) out[(k + 1) * count + x] = N[k];
   }
 }
 
@@ -373,10 +354,9 @@ static long long rows_of(const fold3d_instance &x, int d) {
   return (long long)tiles * lines;
 }
 
-size_t batch_scratch_bytes(int count, const fold3d_instance *inst, int precision,
-                           bool resident) {
+size_t batch_scratch_bytes(int count, const fold3d_instance *inst, int precision) {
   const size_t r = precision == FOLD3D_F64 ? 8 : 4;
-  const int fields = 1 + (resident ? 3 : 2) * kMetricFields;
+  const int fields = 1 + 2 * kMetricFields;
   size_t total = 0;
   for (int i = 0; i < count; ++i) {
     const Grid G(inst[i].n[0], inst[i].n[1], inst[i].n[2]);
@@ -389,8 +369,7 @@ size_t batch_scratch_bytes(int count, const fold3d_instance *inst, int precision
 template <typename R>
 int batch_solve(hipStream_t user_st, int count, const fold3d_instance *inst, int max_iters,
                 double tol, int blocks, int groups, int rows_per_pull, int use_graph,
-                void *scratch, bool resident, int *iters_out, double *max_dt_out,
-                double *ms_out) {
+                void *scratch, int *iters_out, double *max_dt_out, double *ms_out) {
   int dev = 0;
   F3D_HIP(hipGetDevice(&dev));
   // The batch runs on a stream of its own (the legacy default stream cannot be
@@ -405,6 +384,7 @@ int batch_solve(hipStream_t user_st, int count, const fold3d_instance *inst, int
   // host tables
   std::vector<BatchInst<R>> hinst(count);
   std::vector<long long> tiles_prefix(count + 1, 0);
+  std::vector<int> tiles_nx(count), tiles_ny(count);
   {
     char *p = (char *)scratch;
     for (int i = 0; i < count; ++i) {
@@ -416,10 +396,9 @@ int batch_solve(hipStream_t user_st, int count, const fold3d_instance *inst, int
       b.T = (R *)x.T;
       b.count = GZ.count;
       b.Y = (R *)p;
-      b.MZ = b.Y + GZ.count;
-      b.MX = b.MZ + kMetricFields * GZ.count;
-      b.MY = resident ? b.MX + kMetricFields * GZ.count : b.MX;
-      p += (size_t)GZ.count * sizeof(R) * (1 + (resident ? 3 : 2) * kMetricFields);
+      b.MX = b.Y + GZ.count;
+      b.MY = b.MX + kMetricFields * GZ.count;
+      p += (size_t)GZ.count * sizeof(R) * (1 + 2 * kMetricFields);
       p = (char *)(((uintptr_t)p + 255) & ~(uintptr_t)255);
       for (int a = 0; a < 3; ++a) {
         b.stride[kLayZ][a] = GZ.stride[a];
@@ -431,9 +410,19 @@ int batch_solve(hipStream_t user_st, int count, const fold3d_instance *inst, int
       const long long tiles = (long long)((x.n[0] + 2 + kBTile - 1) / kBTile) *
                               ((x.n[1] + 2 + kBTile - 1) / kBTile) * (x.n[2] + 2);
       tiles_prefix[i + 1] = tiles_prefix[i] + tiles;
+      tiles_nx[i] = (x.n[0] + 2 + kBTile - 1) / kBTile;
+      tiles_ny[i] = (x.n[1] + 2 + kBTile - 1) / kBTile;
     }
   }
   const long long total_tiles = tiles_prefix[count];
+  std::vector<BatchTile> tile_all(total_tiles);
+  for (int i = 0; i < count; ++i) {
+    BatchTile *t = &tile_all[tiles_prefix[i]];
+    const int planes = inst[i].n[2] + 2;
+    for (int z = 0; z < planes; ++z)
+      for (int by = 0; by < tiles_ny[i]; ++by)
+        for (int bx = 0; bx < tiles_nx[i]; ++bx) *t++ = BatchTile{i, bx, by, z};
+  }
   std::vector<int> order_all(6 * count);
   std::vector<long long> prefix_all(6 * (count + 1));
   std::vector<int> steps(6);
@@ -469,7 +458,7 @@ int batch_solve(hipStream_t user_st, int count, const fold3d_instance *inst, int
   BatchInst<R> *d_inst = nullptr;
   int *d_done = nullptr;
   BatchRow *d_rows = nullptr;
-  long long *d_tiles = nullptr;
+  BatchTile *d_tiles = nullptr;
   unsigned int *d_counters = nullptr;
   unsigned long long *d_max = nullptr;
   hipGraph_t graph[2] = {nullptr, nullptr};
@@ -493,33 +482,27 @@ int batch_solve(hipStream_t user_st, int count, const fold3d_instance *inst, int
     BATCH_HIP(hipMalloc(&d_done, sizeof(int) * count));
     BATCH_HIP(hipMalloc(&d_counters, sizeof(unsigned int) * launches));
     BATCH_HIP(hipMalloc(&d_max, sizeof(unsigned long long) * count * kBatchMaxSlots));
-    BATCH_HIP(hipMalloc(&d_tiles, sizeof(long long) * (count + 1)));
+    BATCH_HIP(hipMalloc(&d_tiles, sizeof(BatchTile) * tile_all.size()));
     BATCH_HIP(hipMalloc(&d_rows, sizeof(BatchRow) * row_all.size()));
     BATCH_HIP(hipMemcpyAsync(d_rows, row_all.data(), sizeof(BatchRow) * row_all.size(),
                              hipMemcpyHostToDevice, st));
-    BATCH_HIP(hipMemcpyAsync(d_tiles, tiles_prefix.data(), sizeof(long long) * (count + 1),
+    BATCH_HIP(hipMemcpyAsync(d_tiles, tile_all.data(), sizeof(BatchTile) * tile_all.size(),
                              hipMemcpyHostToDevice, st));
     BATCH_HIP(hipMemcpyAsync(d_inst, hinst.data(), sizeof(BatchInst<R>) * count,
                              hipMemcpyHostToDevice, st));
     std::vector<int> done(count, 0);
     BATCH_HIP(hipMemcpyAsync(d_done, done.data(), sizeof(int) * count, hipMemcpyHostToDevice, st));
-    // slowness and inverse metric of every instance, once per batch
+    // slowness and inverse metric of every instance in both sweep layouts,
+    // once per batch
     for (int i = 0; i < count; ++i) {
       const fold3d_instance &x = inst[i];
       double N[6];
       sym3_inverse(x.M, N);
-      batch_prep_kernel<R><<<1024, 256, 0, st>>>(
-          hinst[i].count, (const R *)x.speed_field, (R)x.speed, (const R *)x.theta,
-          (const R *)x.phi, (const R *)x.ra, (const R *)x.rb, (R)N[0], (R)N[1], (R)N[2],
-          (R)N[3], (R)N[4], (R)N[5], hinst[i].MZ);
-    }
-    if (resident) {  // the x- and y-sweep layouts of the metric fields, once per batch
-      const PermuteJob<R> jx{2, kLayZ, kLayX, kMetricFields, 0},
-          jy{3, kLayZ, kLayY, kMetricFields, 0};
-      batch_permute_kernel<R><<<(unsigned)total_tiles, dim3(kBTile, kBTileRows), 0, st>>>(
-          d_inst, d_tiles, count, jx, d_done);
-      batch_permute_kernel<R><<<(unsigned)total_tiles, dim3(kBTile, kBTileRows), 0, st>>>(
-          d_inst, d_tiles, count, jy, d_done);
+      launch_metric_prep<R>(st, hinst[i].count, x.n, hinst[i].stride[kLayZ],
+                            hinst[i].stride[kLayX], hinst[i].stride[kLayY],
+                            (const R *)x.speed_field, (R)x.speed, (const R *)x.theta,
+                            (const R *)x.phi, (const R *)x.ra, (const R *)x.rb, N, hinst[i].MX,
+                            hinst[i].MY);
     }
     BATCH_HIP(hipGetLastError());
     BATCH_HIP(hipStreamSynchronize(st));
@@ -537,7 +520,7 @@ int batch_solve(hipStream_t user_st, int count, const fold3d_instance *inst, int
       auto permute = [&](int which, int lin, int lout, int fields) {
         PermuteJob<R> job{which, lin, lout, fields, 0};
         batch_permute_kernel<R><<<(unsigned)total_tiles, dim3(kBTile, kBTileRows), 0, s>>>(
-            d_inst, d_tiles, count, job, d_done);
+            d_inst, d_tiles, job, d_done);
       };
       // (the x-sweeps run in layout X on the scratch copy, the others in
       // layout Y on the caller's T: batch_plane_kernel derives both from d)
@@ -566,11 +549,9 @@ int batch_solve(hipStream_t user_st, int count, const fold3d_instance *inst, int
         }
       };
       permute(0, first ? kLayZ : kLayY, kLayX, 1);
-      if (!resident) permute(2, kLayZ, kLayX, kMetricFields);
       direction(FOLD3D_X_DOWN);
       direction(FOLD3D_X_UP);
       permute(1, kLayX, kLayY, 1);
-      if (!resident) permute(3, kLayZ, kLayY, kMetricFields);
       direction(FOLD3D_K_UP);
       direction(FOLD3D_K_DOWN);
       direction(FOLD3D_Y_UP);
@@ -625,9 +606,9 @@ int batch_solve(hipStream_t user_st, int count, const fold3d_instance *inst, int
     if (ran > 0) {  // every instance's T back into the reference layout, through Y
       const PermuteJob<R> back{0, kLayY, kLayZ, 1, 1}, home{1, kLayZ, kLayZ, 1, 1};
       batch_permute_kernel<R><<<(unsigned)total_tiles, dim3(kBTile, kBTileRows), 0, st>>>(
-          d_inst, d_tiles, count, back, d_done);
+          d_inst, d_tiles, back, d_done);
       batch_permute_kernel<R><<<(unsigned)total_tiles, dim3(kBTile, kBTileRows), 0, st>>>(
-          d_inst, d_tiles, count, home, d_done);
+          d_inst, d_tiles, home, d_done);
       BATCH_HIP(hipGetLastError());
     }
     BATCH_HIP(hipEventRecord(e1, st));
@@ -675,8 +656,8 @@ cleanup:
 }
 
 template int batch_solve<double>(hipStream_t, int, const fold3d_instance *, int, double, int, int,
-                                 int, int, void *, bool, int *, double *, double *);
+                                 int, int, void *, int *, double *, double *);
 template int batch_solve<float>(hipStream_t, int, const fold3d_instance *, int, double, int, int,
-                                int, int, void *, bool, int *, double *, double *);
+                                int, int, void *, int *, double *, double *);
 
 }  // namespace fold3d

@@ -84,24 +84,6 @@ static int ensure_scratch(fold3d_handle *h, size_t bytes) {
   return FOLD3D_OK;
 }
 
-// As ensure_scratch, but a failed allocation is not an error: returns false
-// (with the handle's scratch released) so the caller can ask for less.
-static bool try_scratch(fold3d_handle *h, size_t bytes) {
-  if (h->scratch_bytes >= bytes) return true;
-  if (h->d_scratch) {
-    if (hipDeviceSynchronize() != hipSuccess || hipFree(h->d_scratch) != hipSuccess) return false;
-    h->d_scratch = nullptr;
-    h->scratch_bytes = 0;
-  }
-  if (hipMalloc(&h->d_scratch, bytes) != hipSuccess) {
-    (void)hipGetLastError();  // clear the failed allocation's error state
-    h->d_scratch = nullptr;
-    return false;
-  }
-  h->scratch_bytes = bytes;
-  return true;
-}
-
 static constexpr size_t kConstBytes = 6 * sizeof(FoldFast<double>);
 
 // First locked iteration of FOLD3D_ENGINE_LOCKED by default: the reference's
@@ -339,21 +321,18 @@ int fold3d_solve(fold3d_handle *h, void *T, int nx, int ny, int nz, double dx,
                    h->d_max, &iters, &maxdt);
   } else if (model->theta != nullptr && o.engine != FOLD3D_ENGINE_PLANE &&
              o.engine != FOLD3D_ENGINE_HOISTED) {
-    // per-node metric: metric engine; scratch = T copy + three sets of metric
-    // fields (reference, x- and y-sweep layouts) when they fit, else two
-    const bool resident = try_scratch(h, ellip_metric_scratch_fields(true) * bytes);
-    if (!resident) {
-      rc = ensure_scratch(h, ellip_metric_scratch_fields(false) * bytes);
-      if (rc != FOLD3D_OK) return rc;
-    }
+    // per-node metric: metric engine; scratch = T copy + the metric fields in
+    // the x- and y-sweep layouts
+    rc = ensure_scratch(h, ellip_metric_scratch_fields() * bytes);
+    if (rc != FOLD3D_OK) return rc;
     const int groups = h->ellip_groups;
     rc = o.precision == FOLD3D_F64
              ? ellip_sweeps_metric_engine<double>(st, (double *)dT, (double *)h->d_scratch, G,
                                                   *model, hsp, o.max_iters, o.tol, groups,
-                                                  resident, h->d_max, &iters, &maxdt)
+                                                  h->d_max, &iters, &maxdt)
              : ellip_sweeps_metric_engine<float>(st, (float *)dT, (float *)h->d_scratch, G,
                                                  *model, hsp, o.max_iters, o.tol, groups,
-                                                 resident, h->d_max, &iters, &maxdt);
+                                                 h->d_max, &iters, &maxdt);
   } else
     rc = o.precision == FOLD3D_F64
              ? ellip_sweeps_plane_engine<double>(st, (double *)dT, G, *model, hsp, o.max_iters,
@@ -473,9 +452,8 @@ static int run_batch(fold3d_handle *h, int count, const fold3d_instance *inst,
   F3D_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, h->device));
   const int use_graph = o.engine != FOLD3D_ENGINE_PLANE;
   const int blocks = cus * blocks_per_cu;
-  const bool resident = try_scratch(h, batch_scratch_bytes(count, inst, o.precision, true));
-  if (!resident) {
-    const int rc = ensure_scratch(h, batch_scratch_bytes(count, inst, o.precision, false));
+  {
+    const int rc = ensure_scratch(h, batch_scratch_bytes(count, inst, o.precision));
     if (rc != FOLD3D_OK) return rc;
   }
   hipStream_t st = (hipStream_t)o.stream;
@@ -483,9 +461,9 @@ static int run_batch(fold3d_handle *h, int count, const fold3d_instance *inst,
                    double *ms) {
     return o.precision == FOLD3D_F64
                ? batch_solve<double>(st, n, in, o.max_iters, o.tol, blocks, groups,
-                                     rows_per_pull, use_graph, scr, resident, its, mds, ms)
+                                     rows_per_pull, use_graph, scr, its, mds, ms)
                : batch_solve<float>(st, n, in, o.max_iters, o.tol, blocks, groups,
-                                    rows_per_pull, use_graph, scr, resident, its, mds, ms);
+                                    rows_per_pull, use_graph, scr, its, mds, ms);
   };
   const int k = std::min(h->batch_streams, count);
   if (k <= 1) return solve(count, inst, h->d_scratch, iterations_out, max_dt_out, ms_out);
@@ -509,7 +487,7 @@ static int run_batch(fold3d_handle *h, int count, const fold3d_instance *inst,
     its[b].assign(sub[b].size(), 0);
     mds[b].assign(sub[b].size(), -1.0);
     scr[b] = p;
-    p += batch_scratch_bytes((int)sub[b].size(), sub[b].data(), o.precision, resident);
+    p += batch_scratch_bytes((int)sub[b].size(), sub[b].data(), o.precision);
   }
   hipEvent_t e0 = nullptr, e1 = nullptr;
   if (ms_out) {

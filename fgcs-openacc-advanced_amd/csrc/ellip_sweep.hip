@@ -9,6 +9,7 @@
 
 #include "common.h"
 #include "ellip_local.h"
+#include "metric_prep.h"
 #include "solver.h"
 
 namespace fold3d {
@@ -374,29 +375,13 @@ int ellip_sweeps_split_engine(hipStream_t st, R *T, R *Y, R *VX, R *VY, const Gr
 
 // ---------------------------------------------------------------------------
 // Metric engine: per-node metric (tilt and axis-ratio fields).  The slowness
-// and the six entries of N = M^-1 are computed once per solve into seven
-// fields of the reference layout (metric_prep_kernel) and copied into the x-
-// and y-sweep layouts — once per solve when the scratch holds both copies
-// (`resident`), else before each pair of x- and y-sweeps; the plane steps are
-// the split kernel's (LDS rows, units over kGroups waves) with the node's
-// lattice products read from those fields.  Same bits as ellip_plane_kernel
-// with kMetric.
+// and the six entries of N = M^-1 are computed once per solve straight into
+// seven fields of the x-sweep layout and seven of the y-sweep layout
+// (metric_prep_kernel, metric_prep.h); the plane steps are the split
+// kernel's (LDS rows, units over kGroups waves) with the node's lattice
+// products read from those fields.  Same bits as ellip_plane_kernel with
+// kMetric.
 // ---------------------------------------------------------------------------
-template <typename R>
-__global__ void metric_prep_kernel(long long count, const R *__restrict__ V, R speed,
-                                   const R *__restrict__ th, const R *__restrict__ ph,
-                                   const R *__restrict__ ra, const R *__restrict__ rb,
-                                   R *__restrict__ out) {
-  for (long long x = blockIdx.x * (long long)blockDim.x + threadIdx.x; x < count;
-       x += (long long)gridDim.x * blockDim.x) {
-    R N[6];
-    tilted_inverse_metric<R>(th[x], ph[x], ra[x], rb[x], N);
-    out[x] = V ? R(1) / V[x] : R(1) / speed;
-#pragma unroll
-    for (int q = 0; q < 6; ++q) out[(q + 1) * count + x] = N[q];
-  }
-}
-
 // Where each lattice product of one direction comes from: L.X = (sg * N_q) * h1 * h2.
 template <typename R>
 struct LatticeMap {
@@ -541,34 +526,24 @@ static void ellip_metric_direction(hipStream_t st, R *T, const R *fields, const 
   }
 }
 
-size_t ellip_metric_scratch_fields(bool resident) {
-  return 1 + (resident ? 3 : 2) * kMetricFields;
-}
+size_t ellip_metric_scratch_fields() { return 1 + 2 * kMetricFields; }
 
 template <typename R>
 int ellip_sweeps_metric_engine(hipStream_t st, R *T, R *scratch, const Grid &G,
                                const fold3d_model &m, const double h[3], int max_iters,
-                               double tol, int groups, bool resident, unsigned long long *d_max,
+                               double tol, int groups, unsigned long long *d_max,
                                int *iters_out, double *max_dt_out) {
   const long long count = G.count;
   R *Y = scratch;                        // T in the x- or y-sweep layout
-  R *MZ = scratch + count;               // metric fields, reference layout
-  R *MX = MZ + kMetricFields * count;    // metric fields, x-sweep layout (or x/y, reused)
-  R *MY = resident ? MX + kMetricFields * count : MX;  // metric fields, y-sweep layout
-  metric_prep_kernel<R><<<2048, 256, 0, st>>>(count, (const R *)m.speed_field, (R)m.speed,
-                                              (const R *)m.theta, (const R *)m.phi,
-                                              (const R *)m.ra, (const R *)m.rb, MZ);
-  F3D_HIP(hipGetLastError());
+  R *MX = scratch + count;               // metric fields, x-sweep layout
+  R *MY = MX + kMetricFields * count;    // metric fields, y-sweep layout
   const Grid GX = G.with_order(kOrderX), GZ = G.with_order(kOrderZ), GY = G.with_order(kOrderY);
-  auto permute_metric = [&](R *out, const Grid &Gout) {
-    for (int q = 0; q < kMetricFields; ++q)
-      permute_layout<R>(st, MZ + q * count, GZ, out + q * count, Gout);
-  };
-  // with room for both, the x- and y-layout metric fields are made once per solve
-  if (resident) {
-    permute_metric(MX, GX);
-    permute_metric(MY, GY);
-  }
+  double N[6];
+  sym3_inverse(m.M, N);
+  launch_metric_prep<R>(st, count, G.n, GZ.stride, GX.stride, GY.stride,
+                        (const R *)m.speed_field, (R)m.speed, (const R *)m.theta,
+                        (const R *)m.phi, (const R *)m.ra, (const R *)m.rb, N, MX, MY);
+  F3D_HIP(hipGetLastError());
   FieldMover<R> mv(st, T, Y, GZ);
   int it = 0;
   double last = -1.0;
@@ -576,21 +551,15 @@ int ellip_sweeps_metric_engine(hipStream_t st, R *T, R *scratch, const Grid &G,
     unsigned long long *md = tol > 0 ? d_max : nullptr;
     if (md) max_dt_reset_kernel<<<1, kMaxSlots, 0, st>>>(d_max);
     // two moves per iteration, as in the fold engines: the x-sweeps with x
-    // slowest, the z- and y-sweeps with y slowest (the reference layout and
-    // its metric fields in the last iteration of a fixed-count solve)
+    // slowest, the z- and y-sweeps with y slowest
     mv.to(GX);
-    if (!resident) permute_metric(MX, GX);
     ellip_metric_direction<R>(st, mv.field(), MX, GX, FOLD3D_X_DOWN, h, md, groups);
     ellip_metric_direction<R>(st, mv.field(), MX, GX, FOLD3D_X_UP, h, md, groups);
-    const bool in_z = tol <= 0 && it == max_iters - 1;
-    const Grid &GP = in_z ? GZ : GY;
-    mv.to(GP);
-    if (!resident && !in_z) permute_metric(MY, GY);
-    const R *MP = in_z ? MZ : MY;
-    ellip_metric_direction<R>(st, mv.field(), MP, GP, FOLD3D_K_UP, h, md, groups);
-    ellip_metric_direction<R>(st, mv.field(), MP, GP, FOLD3D_K_DOWN, h, md, groups);
-    ellip_metric_direction<R>(st, mv.field(), MP, GP, FOLD3D_Y_UP, h, md, groups);
-    ellip_metric_direction<R>(st, mv.field(), MP, GP, FOLD3D_Y_DOWN, h, md, groups);
+    mv.to(GY);
+    ellip_metric_direction<R>(st, mv.field(), MY, GY, FOLD3D_K_UP, h, md, groups);
+    ellip_metric_direction<R>(st, mv.field(), MY, GY, FOLD3D_K_DOWN, h, md, groups);
+    ellip_metric_direction<R>(st, mv.field(), MY, GY, FOLD3D_Y_UP, h, md, groups);
+    ellip_metric_direction<R>(st, mv.field(), MY, GY, FOLD3D_Y_DOWN, h, md, groups);
     F3D_HIP(hipGetLastError());
     ++it;
     if (md) {
@@ -606,11 +575,11 @@ int ellip_sweeps_metric_engine(hipStream_t st, R *T, R *scratch, const Grid &G,
 
 template int ellip_sweeps_metric_engine<double>(hipStream_t, double *, double *, const Grid &,
                                                 const fold3d_model &, const double *, int,
-                                                double, int, bool, unsigned long long *, int *,
+                                                double, int, unsigned long long *, int *,
                                                 double *);
 template int ellip_sweeps_metric_engine<float>(hipStream_t, float *, float *, const Grid &,
                                                const fold3d_model &, const double *, int, double,
-                                               int, bool, unsigned long long *, int *, double *);
+                                               int, unsigned long long *, int *, double *);
 
 template int ellip_sweeps_split_engine<double>(hipStream_t, double *, double *, double *,
                                                double *, const Grid &, const fold3d_model &,

@@ -43,17 +43,16 @@ int ellip_sweeps_split_engine(hipStream_t st, R *T, R *Y, R *VX, R *VY, const Gr
                               int *iters_out, double *max_dt_out);
 
 // Per-node metric (tilt and axis-ratio fields): slowness and N = M^-1 hoisted
-// into kMetricFields fields once per solve, layouts + LDS rows + units over
-// `groups` (1, 2, 4) waves.  `scratch` holds ellip_metric_scratch_fields(resident)
-// fields of G.count elements; `resident`: the x- and y-layout copies of the
-// metric fields both stay (made once per solve) instead of being remade per
-// iteration.
+// into kMetricFields fields in the x- and in the y-sweep layout once per
+// solve, layouts + LDS rows + units over `groups` (1, 2, 4) waves.  `scratch`
+// holds ellip_metric_scratch_fields() fields of G.count elements (a T copy
+// and the two sets of metric fields).
 constexpr int kMetricFields = 7;
-size_t ellip_metric_scratch_fields(bool resident);
+size_t ellip_metric_scratch_fields();
 template <typename R>
 int ellip_sweeps_metric_engine(hipStream_t st, R *T, R *scratch, const Grid &G,
                                const fold3d_model &m, const double h[3], int max_iters,
-                               double tol, int groups, bool resident, unsigned long long *d_max,
+                               double tol, int groups, unsigned long long *d_max,
                                int *iters_out, double *max_dt_out);
 
 void sym3_inverse(const double M[6], double N[6]);
@@ -104,14 +103,13 @@ int fold_sweeps_inplace_engine(hipStream_t st, R *T, const Grid &G, const FoldFa
 
 // Batched elliptic solves (batch.hip).  `blocks` = persistent grid per launch,
 // `groups` = waves per row (1, 2, 4), `rows_per_pull` = rows taken from the
-// queue at a time (0 = automatic); `scratch` holds batch_scratch_bytes(resident):
-// with `resident` the metric fields are kept in all three layouts.
+// queue at a time (0 = automatic); `scratch` holds batch_scratch_bytes(): per
+// instance a T copy and the metric fields in the x- and y-sweep layouts.
 template <typename R>
 int batch_solve(hipStream_t st, int count, const fold3d_instance *inst, int max_iters,
                 double tol, int blocks, int groups, int rows_per_pull, int use_graph,
-                void *scratch, bool resident, int *iters_out, double *max_dt_out,
-                double *ms_out);
-size_t batch_scratch_bytes(int count, const fold3d_instance *inst, int precision, bool resident);
+                void *scratch, int *iters_out, double *max_dt_out, double *ms_out);
+size_t batch_scratch_bytes(int count, const fold3d_instance *inst, int precision);
 
 // Adjoint state and gradients of `nsrc` sources on one grid and model
 // (adjoint.hip); `scratch` holds adjoint_scratch_bytes(G, precision, nsrc)

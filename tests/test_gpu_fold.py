@@ -314,8 +314,12 @@ FAST_ENGINES = [fold3d.ENGINE_HOISTED, fold3d.ENGINE_TILED, fold3d.ENGINE_SPLIT,
                 fold3d.ENGINE_LOCKED]
 
 
-@pytest.mark.parametrize("groups", [2, 4, 8])
-@pytest.mark.parametrize("engine", FAST_ENGINES)
+# waves per row (groups) apply to the split and locked engines only
+FP32_ENGINE_CASES = [(e, g) for e in FAST_ENGINES for g in (2, 4, 8)
+                     if g == 2 or e in (fold3d.ENGINE_SPLIT, fold3d.ENGINE_LOCKED)]
+
+
+@pytest.mark.parametrize("engine,groups", FP32_ENGINE_CASES)
 def test_fp32_engines_equal_the_fp32_literal_kernel(solver, engine, groups):
     """FP32: hoisted constants (computed in float with the literal kernel's
     expressions), pruning by the plane and edge-line bounds (shrunk to 90 %),
@@ -323,8 +327,6 @@ def test_fp32_engines_equal_the_fp32_literal_kernel(solver, engine, groups):
     literal FP32 kernel (fold_local.h run in float, which prunes nothing).
     On the reference workload at 64^3 (8 iterations and to the fixed point),
     and on a box grid with a random signed field and another model."""
-    if groups != 2 and engine not in (fold3d.ENGINE_SPLIT, fold3d.ENGINE_LOCKED):
-        pytest.skip("waves per row apply to the split and locked engines")
     print(f"FP32 engine {engine}, {groups} waves per row, vs the literal FP32 kernel", flush=True)
     solver.set_tuning(groups, fold3d.MODEL_FOLD)
     try:

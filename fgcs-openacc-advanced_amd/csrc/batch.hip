@@ -188,30 +188,31 @@ __global__ __launch_bounds__(64 * kGroups) void batch_plane_kernel(
     const int a0 = rw.a0, b = rw.b;
     R *T = on_T ? I.T : I.Y;
     const R *M = lay == kLayX ? I.MX : I.MY;
-    for (int idx = grp * 64 + tx; idx < 3 * 66; idx += 64 * kGroups) {
-      const int l_a = idx % 66, l_b = idx / 66;
-      const int sa = a0 + l_a;
-      if (sa <= na + 1) s[l_b][l_a] = T[base + (long long)sa * Sa + (long long)(b - 1 + l_b) * Sb];
-    }
+    // all loads of the row in flight together (RowStage); a lane outside the
+    // plane reads the row's first target node instead, and drops it
     const int a = a0 + 1 + tx;
     const bool act = a <= na;
     const long long t = base + (long long)a * Sa + (long long)b * Sb + sm;
-    R told = R(0), sl = R(1);
-    LatticeN<R> L;
-    if (act) {
-      told = T[t];
-      sl = M[t];
-      R Lv[6];
+    const long long tl = act ? t : t + (long long)(a0 + 1 - a) * Sa;
+    const R tload = T[tl], sload = M[tl];
+    R Mv[6];
 #pragma unroll
-      for (int e = 0; e < 6; ++e)
-        Lv[e] = (R(LI.sg[e]) * M[(1 + LI.q[e]) * I.count + t]) * I.h[LI.ai[e]] * I.h[LI.aj[e]];
-      L.A = Lv[0];
-      L.B = Lv[1];
-      L.C = Lv[2];
-      L.E = Lv[3];
-      L.Fx = Lv[4];
-      L.Fy = Lv[5];
-    }
+    for (int e = 0; e < 6; ++e) Mv[e] = M[(1 + LI.q[e]) * I.count + tl];
+    RowStage<R, 64 * kGroups> rs;
+    rs.load(T, base, Sa, Sb, a0, b, na, grp * 64 + tx);
+    rs.store(s, grp * 64 + tx);
+    const R told = act ? tload : R(0), sl = act ? sload : R(1);
+    R Lv[6];
+#pragma unroll
+    for (int e = 0; e < 6; ++e)
+      Lv[e] = (R(LI.sg[e]) * Mv[e]) * I.h[LI.ai[e]] * I.h[LI.aj[e]];
+    LatticeN<R> L;
+    L.A = Lv[0];
+    L.B = Lv[1];
+    L.C = Lv[2];
+    L.E = Lv[3];
+    L.Fx = Lv[4];
+    L.Fy = Lv[5];
     __syncthreads();
     R best = told;
     if (act) {

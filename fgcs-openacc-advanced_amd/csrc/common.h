@@ -96,6 +96,38 @@ static inline hipError_t max_dt_read(hipStream_t st, const unsigned long long *s
   return e;
 }
 
+// The source rows b - 1, b, b + 1 of a plane step (66 values each, from
+// lane-axis index a0) staged through registers into LDS by kThreads threads.
+// load() only issues the loads, so the caller can issue its other loads of
+// the step (target value, speed, metric) before anything waits: every thread
+// loads, from an address inside the padded grid ((a0, b - 1) always is), and
+// the threads outside the rows or the plane drop what they read.  (A load
+// under a branch lets the compiler wait for it inside the branch, before the
+// next load is issued.)
+template <typename R, int kThreads>
+struct RowStage {
+  static constexpr int kStage = 3 * 66, kPer = (kStage + kThreads - 1) / kThreads;
+  R v[kPer];
+  bool ok[kPer];
+  __device__ __forceinline__ void load(const R *__restrict__ T, long long base, long long Sa,
+                                       long long Sb, int a0, int b, int na, int thread) {
+#pragma unroll
+    for (int q = 0; q < kPer; ++q) {
+      const int idx = thread + q * kThreads;
+      const int la = idx % 66, lb = idx / 66;
+      ok[q] = idx < kStage && a0 + la <= na + 1;
+      v[q] = T[base + (long long)(ok[q] ? a0 + la : a0) * Sa + (long long)(b - 1 + (ok[q] ? lb : 0)) * Sb];
+    }
+  }
+  __device__ __forceinline__ void store(R (&s)[3][66], int thread) const {
+#pragma unroll
+    for (int q = 0; q < kPer; ++q) {
+      const int idx = thread + q * kThreads;
+      if (ok[q]) s[idx / 66][idx % 66] = v[q];
+    }
+  }
+};
+
 // Padded grid: nodes 1..n[a] on each axis, ghosts at 0 and n[a]+1.
 struct Grid {
   int n[3];

@@ -250,17 +250,20 @@ __global__ __launch_bounds__(64 * kGroups) void ellip_split_kernel(
   const int tx = threadIdx.x;
   const int grp = __builtin_amdgcn_readfirstlane(threadIdx.y);
   const int a0 = blockIdx.x * 64, b = 1 + blockIdx.y;
-  for (int idx = grp * 64 + tx; idx < 3 * 66; idx += 64 * kGroups) {
-    const int la = idx % 66, lb = idx / 66;
-    const int sa = a0 + la;
-    if (sa <= g.na + 1) s[lb][la] = T[g.base + (long long)sa * g.Sa + (long long)(b - 1 + lb) * g.Sb];
-  }
+  // all loads of the step in flight together (RowStage); a lane outside the
+  // plane reads the block's first target node instead, and drops it
   const int a = a0 + 1 + tx;
   const bool active = a <= g.na;
   const long long c = g.base + (long long)a * g.Sa + (long long)b * g.Sb;
   const long long t = c + g.sm;
-  const R told = active ? T[t] : R(0);
-  const R sl = kSpeed ? (active ? R(1) / V[t] : R(1)) : R(1) / A.speed;
+  const long long tl = active ? t : t + (long long)(a0 + 1 - a) * g.Sa;
+  const R tload = T[tl];
+  const R vload = kSpeed ? V[tl] : R(1);
+  RowStage<R, 64 * kGroups> rs;
+  rs.load(T, g.base, g.Sa, g.Sb, a0, b, g.na, grp * 64 + tx);
+  rs.store(s, grp * 64 + tx);
+  const R told = active ? tload : R(0);
+  const R sl = kSpeed ? (active ? R(1) / vload : R(1)) : R(1) / A.speed;
   __syncthreads();
   R best = told;
   if (active) {
@@ -447,27 +450,28 @@ __global__ __launch_bounds__(64 * kGroups) void ellip_metric_kernel(
   const int tx = threadIdx.x;
   const int grp = __builtin_amdgcn_readfirstlane(threadIdx.y);
   const int a0 = blockIdx.x * 64, b = 1 + blockIdx.y;
-  for (int idx = grp * 64 + tx; idx < 3 * 66; idx += 64 * kGroups) {
-    const int la = idx % 66, lb = idx / 66;
-    const int sa = a0 + la;
-    if (sa <= g.na + 1) s[lb][la] = T[g.base + (long long)sa * g.Sa + (long long)(b - 1 + lb) * g.Sb];
-  }
+  // all loads of the step in flight together (RowStage); a lane outside the
+  // plane reads the block's first target node instead, and drops it
   const int a = a0 + 1 + tx;
   const bool active = a <= g.na;
   const long long c = g.base + (long long)a * g.Sa + (long long)b * g.Sb;
   const long long t = c + g.sm;
-  R told = R(0), sl = R(1);
+  const long long tl = active ? t : t + (long long)(a0 + 1 - a) * g.Sa;
+  const R tload = T[tl], sload = S[tl];
+  R Lv[6];
+#pragma unroll
+  for (int e = 0; e < 6; ++e) Lv[e] = Lm.src[e][tl];
+  RowStage<R, 64 * kGroups> rs;
+  rs.load(T, g.base, g.Sa, g.Sb, a0, b, g.na, grp * 64 + tx);
+  rs.store(s, grp * 64 + tx);
+  const R told = active ? tload : R(0), sl = active ? sload : R(1);
   LatticeN<R> L;
-  if (active) {
-    told = T[t];
-    sl = S[t];
-    L.A = (Lm.sg[0] * Lm.src[0][t]) * Lm.h1[0] * Lm.h2[0];
-    L.B = (Lm.sg[1] * Lm.src[1][t]) * Lm.h1[1] * Lm.h2[1];
-    L.C = (Lm.sg[2] * Lm.src[2][t]) * Lm.h1[2] * Lm.h2[2];
-    L.E = (Lm.sg[3] * Lm.src[3][t]) * Lm.h1[3] * Lm.h2[3];
-    L.Fx = (Lm.sg[4] * Lm.src[4][t]) * Lm.h1[4] * Lm.h2[4];
-    L.Fy = (Lm.sg[5] * Lm.src[5][t]) * Lm.h1[5] * Lm.h2[5];
-  }
+  L.A = (Lm.sg[0] * Lv[0]) * Lm.h1[0] * Lm.h2[0];
+  L.B = (Lm.sg[1] * Lv[1]) * Lm.h1[1] * Lm.h2[1];
+  L.C = (Lm.sg[2] * Lv[2]) * Lm.h1[2] * Lm.h2[2];
+  L.E = (Lm.sg[3] * Lv[3]) * Lm.h1[3] * Lm.h2[3];
+  L.Fx = (Lm.sg[4] * Lv[4]) * Lm.h1[4] * Lm.h2[4];
+  L.Fy = (Lm.sg[5] * Lv[5]) * Lm.h1[5] * Lm.h2[5];
   __syncthreads();
   R best = told;
   if (active) {

@@ -62,9 +62,6 @@ struct BatchInst {
 struct BatchRow {
   int inst, a0, b, pad;
 };
-struct BatchDir {
-  const BatchRow *row;  // [rows of one plane step, all instances]
-};
 
 // Lattice products of direction d from the metric fields (as lattice_map in
 // the single-instance engine): L.X = (sg * N_q) * h_i * h_j.
@@ -73,6 +70,12 @@ struct LatticeIdx {
   int ai[6], aj[6];  // global axes of the spacings
   int sg[6];
 };
+
+// Rank of an axis in a batch layout's axis order (0: fastest).
+constexpr int axis_rank(int lay, int axis) {
+  const int *o = lay == kLayX ? kOrderX : lay == kLayY ? kOrderY : kOrderZ;
+  return o[0] == axis ? 0 : o[1] == axis ? 1 : 2;
+}
 
 constexpr LatticeIdx lattice_idx(int d) {
   const Frame fr = kFrames[d];
@@ -129,7 +132,8 @@ constexpr int kBatchMaxSlots = 32;
 // compile-time constant, so its frame and lattice indices take no registers.
 template <typename R, int kGroups, int kDir>
 __global__ __launch_bounds__(64 * kGroups) void batch_plane_kernel(
-    const BatchInst<R> *__restrict__ inst, BatchDir bd, int q, long long total, int chunk,
+    const BatchInst<R> *__restrict__ inst, const BatchRow *__restrict__ rows, int q,
+    long long total, int chunk,
     unsigned int *__restrict__ counter, const int *__restrict__ done,
     unsigned long long *__restrict__ max_dt) {
   __shared__ R s[3][64 + 2];
@@ -140,52 +144,52 @@ __global__ __launch_bounds__(64 * kGroups) void batch_plane_kernel(
   // the x-sweeps run in layout X on the scratch copy Y, the others in layout Y on T
   constexpr int lay = fr.m == 0 ? kLayX : kLayY;
   constexpr bool on_T = fr.m != 0;
+  // plane geometry: lanes along the in-plane axis of smaller stride, which the
+  // layout's axis order fixes (strides grow strictly along it)
+  constexpr bool kUFirst = axis_rank(lay, fr.ux) < axis_rank(lay, fr.uy);
+  constexpr int la = kUFirst ? fr.ux : fr.uy, lb = kUFirst ? fr.uy : fr.ux;
+  constexpr int ua = fr.ux == la ? fr.sx : 0, ub = fr.ux == la ? 0 : fr.sx;
+  constexpr int va = fr.uy == la ? fr.sy : 0, vb = fr.uy == la ? 0 : fr.sy;
   const int tx = threadIdx.x;
   const int grp = __builtin_amdgcn_readfirstlane(threadIdx.y);
   // chunk > 0: pull `chunk` rows at a time from the launch's counter;
   // chunk == 0: static schedule, rows blockIdx.x, blockIdx.x + gridDim.x, ...
-  long long item = chunk > 0 ? 0 : blockIdx.x, end = 0;
+  // rows: the plane step's row table ([rows of one plane step, all
+  // instances]), read with scalar loads; row indices are below 2^31.
+  int item = chunk > 0 ? 0 : blockIdx.x, end = 0;
   for (;;) {
-    long long cur;
     if (chunk > 0) {
       if (item == end) {
         if (tx == 0 && grp == 0) item_s = (long long)atomicAdd(counter, (unsigned)chunk);
         __syncthreads();
-        item = item_s;
+        item = __builtin_amdgcn_readfirstlane((int)item_s);
         __syncthreads();
         if (item >= total) break;
-        end = item + chunk < total ? item + chunk : total;
+        end = item + chunk < total ? item + chunk : (int)total;
       }
-      cur = item++;
-    } else {
-      if (item >= total) break;
-      cur = item;
-      item += gridDim.x;
+    } else if (item >= total) {
+      break;
     }
-    const BatchRow rw = bd.row[cur];
-    const int i = rw.inst;
+    const BatchRow rw = rows[item];
+    item += chunk > 0 ? 1 : gridDim.x;
+    const int i = __builtin_amdgcn_readfirstlane(rw.inst);
     const BatchInst<R> &I = inst[i];
     const long long *st = I.stride[lay];
-    const int u = fr.ux, v = fr.uy;
     {  // the row's instance data as one group of scalar loads (one wait, not
        // one per use)
       const int dn = done[i];
-      asm volatile("" ::"s"(dn), "s"(st[u]), "s"(st[v]), "s"(st[fr.m]), "s"(I.n[u]), "s"(I.n[v]),
-                   "s"(I.n[fr.m]), "s"(I.count), "s"(I.h[0]), "s"(I.h[1]), "s"(I.h[2]));
+      asm volatile("" ::"s"(dn), "s"(st[la]), "s"(st[lb]), "s"(st[fr.m]), "s"(I.n[la]),
+                   "s"(I.n[fr.m]), "s"(I.count), "s"(I.h[0]), "s"(I.h[1]), "s"(I.h[2]),
+                   "s"(on_T ? I.T : I.Y), "s"(lay == kLayX ? I.MX : I.MY));
       if (dn) continue;  // uniform over the block
     }
-    // plane geometry (lanes along the in-plane axis of smaller stride)
-    const bool u_first = st[u] < st[v];
-    const int la = u_first ? u : v, lb = u_first ? v : u;
     const int na = I.n[la];
     const int nm = I.n[fr.m];
     const int p = fr.s > 0 ? 1 + q : nm - q;
     const long long base = (long long)p * st[fr.m];
     const long long Sa = st[la], Sb = st[lb];
     const long long sm = fr.s * st[fr.m];
-    const int ua = fr.ux == la ? fr.sx : 0, ub = fr.ux == la ? 0 : fr.sx;
-    const int va = fr.uy == la ? fr.sy : 0, vb = fr.uy == la ? 0 : fr.sy;
-    const int a0 = rw.a0, b = rw.b;
+    const int a0 = __builtin_amdgcn_readfirstlane(rw.a0), b = __builtin_amdgcn_readfirstlane(rw.b);
     R *T = on_T ? I.T : I.Y;
     const R *M = lay == kLayX ? I.MX : I.MY;
     // all loads of the row in flight together (RowStage); a lane outside the
@@ -250,7 +254,7 @@ __global__ __launch_bounds__(64 * kGroups) void batch_plane_kernel(
 template <typename R>
 struct BatchLaunch {
   const BatchInst<R> *inst;
-  BatchDir bd;
+  const BatchRow *rows;
   int q;
   long long total;
   int chunk;
@@ -261,7 +265,7 @@ struct BatchLaunch {
 template <typename R, int kGroups, int kDir>
 static void launch_batch_plane_dir(int grid, hipStream_t s, const BatchLaunch<R> &a) {
   batch_plane_kernel<R, kGroups, kDir><<<grid, dim3(64, kGroups), 0, s>>>(
-      a.inst, a.bd, a.q, a.total, a.chunk, a.counter, a.done, a.max_dt);
+      a.inst, a.rows, a.q, a.total, a.chunk, a.counter, a.done, a.max_dt);
 }
 template <typename R, int kGroups>
 static void launch_batch_plane(int d, int grid, hipStream_t s, const BatchLaunch<R> &a) {
@@ -529,7 +533,7 @@ int batch_solve(hipStream_t user_st, int count, const fold3d_instance *inst, int
         const int m = kFrames[d].m;
         const int *ord = &order_all[d * count];
         const long long *pre = &prefix_all[d * (count + 1)];
-        BatchDir bd{d_rows + row_off[d]};
+        const BatchRow *rows_d = d_rows + row_off[d];
         int active = count;
         for (int q = 0; q < steps[d]; ++q, ++slot) {
           while (active > 0 && inst[ord[active - 1]].n[m] - 1 <= q) --active;
@@ -543,7 +547,7 @@ int batch_solve(hipStream_t user_st, int count, const fold3d_instance *inst, int
                                         1, std::min<long long>(8, total / (4 * blocks)));
           const int grid =
               (int)std::min<long long>(chunk > 0 ? (total + chunk - 1) / chunk : total, blocks);
-          const BatchLaunch<R> bl{d_inst, bd, q, total, chunk, d_counters + slot, d_done, d_max};
+          const BatchLaunch<R> bl{d_inst, rows_d, q, total, chunk, d_counters + slot, d_done, d_max};
           if (groups == 1) launch_batch_plane<R, 1>(d, grid, s, bl);
           else if (groups == 2) launch_batch_plane<R, 2>(d, grid, s, bl);
           else launch_batch_plane<R, 4>(d, grid, s, bl);

@@ -33,14 +33,14 @@ HBM_PEAK_GBS = 8000.0          # MI355X HBM3E
 # two rocprofv3 --pmc passes over the first iteration: FETCH_SIZE x 2.0 plus
 # WRITE_SIZE x 1.0 KiB, the factors read off a plain device copy of known
 # size (profiles/fold_split_f64_512_pmc_traffic.md)
-PMC_TRAFFIC_512_F64 = (3453.9 * 2.0 + 898.2 * 1.0) * 1024
+PMC_TRAFFIC_512_F64 = (3454.1 * 2.0 + 898.2 * 1.0) * 1024
 # The VALU view of the same launches, all from one profiled run over the first
 # iteration (profiles/fold_split_f64_512_pmc_traffic.md): wave-instructions
-# per plane launch (SQ_INSTS_VALU: 8192 waves of 430.5), the launch's average
+# per plane launch (SQ_INSTS_VALU: 8192 waves of 436.0), the launch's average
 # duration (kernel trace of the same command) and the clock implied by the
 # FP64 FMA rate of an earlier run of the same passes.
-PMC_VALU_INSTS_512_F64 = 3526342.2
-PMC_VALU_LAUNCH_US_512_F64 = 12.923
+PMC_VALU_INSTS_512_F64 = 3571925.6
+PMC_VALU_LAUNCH_US_512_F64 = 12.867
 PMC_VALU_CLOCK_GHZ = 1.955
 # FP32 fold solve vs FP64 on the reference workload (8 iterations, up to 512^3)
 FP32_BOUND_TEXT = ("signs and reached set equal; relative error < 1e-2 everywhere, above 1e-3 "
@@ -51,16 +51,18 @@ FP32_BOUND_TEXT = ("signs and reached set equal; relative error < 1e-2 everywher
 # launch, the kernel's average launch in the kernel trace of the same command
 # in us, which launches the passes counted, the profile file).
 PMC_ROW_TRAFFIC = {
-    "blob_speed_f64": ((3962.5 * 2.0 + 697.4) * 1024, 8.35,
+    "blob_speed_f64": ((3961.6 * 2.0 + 697.4) * 1024, 7.89,
                        "iteration 1, the autotuner's 2-wave trial (no max|dT| reduction)",
                        "profiles/ellip_split_f64_512_pmc.md"),
-    "blob_speed_f32": ((2370.6 * 2.0 + 371.1) * 1024, 6.95,
+    "blob_speed_f32": ((2373.6 * 2.0 + 371.1) * 1024, 6.24,
                        "iteration 1, the autotuner's 2-wave trial (no max|dT| reduction)",
                        "profiles/ellip_split_f32_512_pmc.md"),
     "tilted_field": ((48087.4 * 2.0 + 2531.8) * 1024, 25.46,
-                     "iteration 1 of the solve (ellip_metric_kernel<float, true, 1>)",
+                     "iteration 1 of the solve (ellip_metric_kernel<float, true, 1>), counted "
+                     "on the kernel before its loads were issued ahead of the first wait (the "
+                     "same bytes; 25.46 us per launch then, 24.0 us now)",
                      "profiles/metric_tilted_field_1280_f32_pmc.md"),
-    "batch": ((79880.8 * 2.0 + 4453.3) * 1024, 62.62,
+    "batch": ((79928.0 * 2.0 + 4453.3) * 1024, 50.16,
               "the 1914 launches of iteration 1 of one batch of all 96 instances (bytes per "
               "launch are their mean)", "profiles/batch_plane_96inst_f32_pmc.md"),
     "adjoint": ((25560.1 * 2.0 + 4285.3) * 1024, 14.39,

@@ -463,17 +463,16 @@ __device__ __forceinline__ void split_rowgroup(R *__restrict__ T, const PlaneGeo
   if (kLock && lk.sigma >= 6 && !__syncthreads_or(changed)) return;  // uniform over the block
   // Loads: the target value and the source rows (which go through registers
   // into LDS) are all in flight together.  Every lane loads, from an address
-  // inside the padded grid ((a0 + 1, b0) and (a0, b0 - 1) always are), and the
-  // lanes outside the plane or the stage drop what they read: a conditional
-  // load is a branch, and the compiler waits for a load inside its branch
-  // when it moves the value's first use (fabs) there, which made the two
-  // loads one after the other.
+  // clamped into the padded grid, and the lanes outside the plane or the
+  // stage drop what they read: a conditional load is a branch, and the
+  // compiler waits for a load inside its branch when it moves the value's
+  // first use (fabs) there, which made the two loads one after the other.
   const int la_ = tx % kW, lb_ = row * kRpw + tx / kW;  // the lane's node in the block
   const int a = a0 + 1 + la_, b = b0 + lb_;
   const bool active = a <= g.na && b <= g.nb;
-  const long long c = g.base + (long long)a * g.Sa + (long long)b * g.Sb;
-  const R tload = T[g.base + (long long)(active ? a : a0 + 1) * g.Sa +
-                    (long long)(active ? b : b0) * g.Sb + g.sm];
+  // (clamped: the target of an active lane, where only active lanes store)
+  const long long c = g.base + (long long)min(a, g.na) * g.Sa + (long long)min(b, g.nb) * g.Sb;
+  const R tload = T[c + g.sm];
   constexpr int kStage = (kNR + 2) * (kW + 2), kThreads = 64 * kGroups * kRows;
   constexpr int kPer = (kStage + kThreads - 1) / kThreads;
   R sv[kPer];
@@ -484,7 +483,7 @@ __device__ __forceinline__ void split_rowgroup(R *__restrict__ T, const PlaneGeo
     const int la = idx % (kW + 2), lb = idx / (kW + 2);
     const int sa = a0 + la, sb = b0 - 1 + lb;
     sok[q] = idx < kStage && sa <= g.na + 1 && sb <= g.nb + 1;
-    sv[q] = T[g.base + (long long)(sok[q] ? sa : a0) * g.Sa + (long long)(sok[q] ? sb : b0 - 1) * g.Sb];
+    sv[q] = T[g.base + (long long)min(sa, g.na + 1) * g.Sa + (long long)min(sb, g.nb + 1) * g.Sb];
   }
   const R told = active ? tload : R(0);
 #pragma unroll

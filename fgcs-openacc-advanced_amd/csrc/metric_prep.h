@@ -38,17 +38,33 @@ __global__ __launch_bounds__(kPrepTile *kPrepRows) void metric_prep_kernel(
   const long long z = blockIdx.x;
   const int x0 = blockIdx.y * kPrepTile, y0 = blockIdx.z * kPrepTile;
   const int tx = threadIdx.x, ty = threadIdx.y;
+  // Every input of the thread's kQ nodes is loaded before any is used (from
+  // addresses clamped into the grid; a node outside it drops its values):
+  // loads under the node's bounds test were each waited for before the next
+  // node's were issued.
+  R tv[kQ], pv[kQ], av[kQ], bv[kQ], vv[kQ];
+#pragma unroll
+  for (int q = 0; q < kQ; ++q) {
+    const int x = min(x0 + tx, g.hx - 1), y = min(y0 + ty + q * kPrepRows, g.hy - 1);
+    const long long iz = x * g.sz[0] + y * g.sz[1] + z * g.sz[2];
+    if (th) {  // (uniform)
+      tv[q] = th[iz];
+      pv[q] = ph[iz];
+      av[q] = ra[iz];
+      bv[q] = rb[iz];
+    }
+    vv[q] = V ? V[iz] : speed;
+  }
   R val[kQ][7];
 #pragma unroll
   for (int q = 0; q < kQ; ++q) {
     const int x = x0 + tx, y = y0 + ty + q * kPrepRows;
-    if (x < g.hx && y < g.hy) {
-      const long long iz = x * g.sz[0] + y * g.sz[1] + z * g.sz[2];
-      R N[6] = {Nc0, Nc1, Nc2, Nc3, Nc4, Nc5};
-      if (th) tilted_inverse_metric<R>(th[iz], ph[iz], ra[iz], rb[iz], N);
-      val[q][0] = V ? R(1) / V[iz] : R(1) / speed;
+    R N[6] = {Nc0, Nc1, Nc2, Nc3, Nc4, Nc5};
+    if (th) tilted_inverse_metric<R>(tv[q], pv[q], av[q], bv[q], N);
+    val[q][0] = R(1) / vv[q];
 #pragma unroll
-      for (int k = 0; k < 6; ++k) val[q][1 + k] = N[k];
+    for (int k = 0; k < 6; ++k) val[q][1 + k] = N[k];
+    if (x < g.hx && y < g.hy) {
       const long long iy = x * g.sy[0] + y * g.sy[1] + z * g.sy[2];
 #pragma unroll
       for (int f = 0; f < 7; ++f) MY[f * g.count + iy] = val[q][f];

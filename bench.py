@@ -757,7 +757,8 @@ def main():
         log("bench: FP64 solves to the exact fixed point, split and locked engines")
         fixpt = {}
         res = {}
-        for name, eng in (("split", fold3d.ENGINE_SPLIT), ("locked", fold3d.ENGINE_LOCKED)):
+        for name, eng in (("split", fold3d.ENGINE_SPLIT), ("locked", fold3d.ENGINE_LOCKED),
+                          ("auto", fold3d.ENGINE_AUTO)):
             Tf = T0d.clone()
             torch.cuda.synchronize()
             t0 = time.perf_counter()
@@ -767,11 +768,14 @@ def main():
                            "computed_fraction": st_f.updates_computed / st_f.cell_updates}
             res[name] = Tf
         fixpt["same_bits"] = bool(torch.equal(res["split"].view(torch.int64),
-                                              res["locked"].view(torch.int64)))
+                                              res["locked"].view(torch.int64)) and
+                                  torch.equal(res["split"].view(torch.int64),
+                                              res["auto"].view(torch.int64)))
         fixpt["locked_speedup"] = fixpt["split"]["s"] / fixpt["locked"]["s"]
         log(f"bench: fixed point after {fixpt['split']['iterations']} iterations: split "
             f"{fixpt['split']['s']:.2f} s, locked {fixpt['locked']['s']:.2f} s "
-            f"({fixpt['locked']['computed_fraction']:.3f} of the updates computed), same bits: "
+            f"({fixpt['locked']['computed_fraction']:.3f} of the updates computed), default "
+            f"engine (locked from iteration 24) {fixpt['auto']['s']:.2f} s, same bits: "
             f"{fixpt['same_bits']}")
         del res, Tf
 

@@ -90,6 +90,12 @@ static constexpr size_t kConstBytes = 6 * sizeof(FoldFast<double>);
 // 8 iterations run unlocked (most rows still change in them, and a locked
 // launch costs more per computed row), the iterations after them locked.
 static constexpr int kDefaultLockFrom = 8;
+// FOLD3D_ENGINE_AUTO on the fold model: the split engine, locked from this
+// iteration on when the solve may run longer.  Measured at 512^3 FP64: while
+// 80-98 % of the rows still change (the first 24 iterations) a locked
+// iteration costs 3-4 % more than a split one; beyond that locking wins
+// (1.17x over 80 iterations, 1.47x to the fixed point).  Same bits either way.
+static constexpr int kAutoLockFrom = 24;
 // Batched solve: sub-batches run at once by default, and at most.  On the
 // 96-instance workload (sides 64..320, FP32 to 1e-6) 1, 2, 3, 4 and 6 bands
 // ran at 117.5, 134.1, 135.5, 119.7 and 134.7 solves/s.
@@ -281,7 +287,9 @@ int fold3d_solve(fold3d_handle *h, void *T, int nx, int ny, int nz, double dx,
                                                 (const FoldFast<float> *)h->d_consts,
                                                 o.max_iters, o.tol, h->d_max, &iters, &maxdt);
     } else {
-      const bool lock = o.engine == FOLD3D_ENGINE_LOCKED;
+      const bool lock = o.engine == FOLD3D_ENGINE_LOCKED ||
+                        (o.engine == FOLD3D_ENGINE_AUTO && o.max_iters > kAutoLockFrom);
+      const int lock_from = o.engine == FOLD3D_ENGINE_LOCKED ? h->lock_from : kAutoLockFrom;
       // scratch: T copy [, line-chunk stamps, computed-updates counter]
       const size_t tbytes = (bytes + 255) & ~(size_t)255;
       const size_t sbytes = (line_stamp_words(G) * sizeof(int) + 255) & ~(size_t)255;
@@ -296,11 +304,11 @@ int fold3d_solve(fold3d_handle *h, void *T, int nx, int ny, int nz, double dx,
                ? fold_sweeps_tiled_engine<double>(st, (double *)dT, (double *)h->d_scratch, G,
                                                   (const FoldFast<double> *)h->d_consts,
                                                   o.max_iters, o.tol, groups, h->d_max, &iters,
-                                                  &maxdt, stamps, done_ctr, h->lock_from)
+                                                  &maxdt, stamps, done_ctr, lock_from)
                : fold_sweeps_tiled_engine<float>(st, (float *)dT, (float *)h->d_scratch, G,
                                                  (const FoldFast<float> *)h->d_consts,
                                                  o.max_iters, o.tol, groups, h->d_max, &iters,
-                                                 &maxdt, stamps, done_ctr, h->lock_from);
+                                                 &maxdt, stamps, done_ctr, lock_from);
     }
   } else if (model->theta == nullptr && o.engine != FOLD3D_ENGINE_PLANE &&
              o.engine != FOLD3D_ENGINE_HOISTED) {

@@ -92,7 +92,11 @@ typedef struct fold3d_opts {
 } fold3d_opts;
 
 enum fold3d_engine {
-  FOLD3D_ENGINE_AUTO = 0,    /* the fastest engine available for the model */
+  FOLD3D_ENGINE_AUTO = 0,    /* the fastest engine available for the model;
+                                for the fold model SPLIT, and on a solve that
+                                may run more than 24 iterations LOCKED from
+                                iteration 24 on (same bits; locking pays only
+                                once most rows have settled) */
   FOLD3D_ENGINE_PLANE = 1,   /* one launch per plane, node update written as the
                                 reference's local solvers (fold model) */
   FOLD3D_ENGINE_HOISTED = 2, /* one launch per plane, per-direction invariants
@@ -103,8 +107,9 @@ enum fold3d_engine {
                                 the handle) */
   FOLD3D_ENGINE_SPLIT = 4,   /* as TILED, with each node's candidate units spread
                                 over several wavefronts whose minima meet in LDS
-                                (same bits; the default for the fold model; for the
-                                elliptic model the split or metric engine) */
+                                (same bits; what AUTO runs for the fold model in
+                                its first 24 iterations; for the elliptic model
+                                the split or metric engine) */
   FOLD3D_ENGINE_LOCKED = 5   /* fold model: SPLIT plus exact locking — from the
                                 iteration set by fold3d_set_lock_start (default
                                 8, 0-based), a block of target rows whose source

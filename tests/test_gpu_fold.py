@@ -203,8 +203,12 @@ def test_solves_64_match_reference_digests(solver, golden_large, engine):
     assert matches_digest(T.cpu().numpy(), golden_large, "fixed_64")
 
 
-@pytest.mark.parametrize("engine", [fold3d.ENGINE_SPLIT, fold3d.ENGINE_LOCKED])
+@pytest.mark.parametrize("engine", [fold3d.ENGINE_SPLIT, fold3d.ENGINE_LOCKED,
+                                    fold3d.ENGINE_AUTO])
 def test_fixed_point_128_matches_reference_digest(solver, golden_large, engine):
+    """The reference's fixed point at 128^3 (34 iterations).  ENGINE_AUTO runs
+    the split engine and, on a solve allowed past 24 iterations, locks from
+    iteration 24 on: the same bits, with rows skipped in the last ten."""
     from conftest import matches_digest
     n = 128
     print(f"128^3 to the fixed point, engine {engine}", flush=True)
@@ -212,6 +216,11 @@ def test_fixed_point_128_matches_reference_digest(solver, golden_large, engine):
     st = solver.solve(T, n, max_iters=100, tol=1e-300, engine=engine)
     assert st.iterations == int(golden_large["fixed_iters_128"][0])
     assert matches_digest(T.cpu().numpy(), golden_large, "fixed_128")
+    if engine == fold3d.ENGINE_AUTO:
+        assert st.updates_computed < st.cell_updates
+        T = to_dev(fold3d.implicit_initialiser(n))
+        st = solver.solve(T, n, max_iters=24, engine=engine)  # no locking within 24
+        assert st.updates_computed == st.cell_updates
 
 
 def test_solve_64_matches_oracle(solver, oracle):

@@ -1,9 +1,9 @@
 // batch.hip — many independent elliptic-model solves of uneven size in one
 // stream of launches.
 //
-// Once per batch the slowness and the inverse metric N = M^-1 of every
-// instance are written to seven fields of its own in each sweep layout (tilt
-// fields or a constant metric alike).  An iteration then runs, for all
+// Once per batch the slowness and the six lattice products of the inverse
+// metric N = M^-1 of every instance are written to seven fields of its own in
+// each sweep layout (tilt fields or a constant metric alike).  An iteration then runs, for all
 // instances together, the single-instance metric engine's schedule of two
 // moves: move T into the x-sweep layout (one launch for the whole batch), run
 // the x-sweeps, move T into the y-sweep layout (y slowest), run the z- and the
@@ -44,15 +44,14 @@ enum BatchLayout { kLayZ = 0, kLayX = 1, kLayY = 2 };
 
 // Device view of one instance.  Scratch per instance: Y (T in layout X; T
 // itself holds layout Y between iterations) and kMetricFields fields
-// (slowness and N = M^-1) in each sweep layout, MX and MY, made once per
-// batch (metric_prep_kernel).
+// (the slowness and the lattice products of N = M^-1) in each sweep layout,
+// MX and MY, made once per batch (metric_prep_kernel).
 template <typename R>
 struct BatchInst {
   R *T, *Y, *MX, *MY;
   long long count;
   long long stride[3][3];  // [layout][axis]
   int n[3];
-  R h[3];
 };
 
 // One row of 64 target nodes of a plane step: its instance, the row's first
@@ -63,11 +62,11 @@ struct BatchRow {
   int inst, a0, b, pad;
 };
 
-// Lattice products of direction d from the metric fields (as lattice_map in
-// the single-instance engine): L.X = (sg * N_q) * h_i * h_j.
+// Lattice products of direction d from the metric fields, which hold the
+// products P_q = (N_ab h_a) h_b (as lattice_map in the single-instance
+// engine): L.X = sg * P_q.
 struct LatticeIdx {
-  int q[6];      // index of N_q (0..5) for A, B, C, E, Fx, Fy
-  int ai[6], aj[6];  // global axes of the spacings
+  int q[6];   // index of P_q (0..5) for A, B, C, E, Fx, Fy
   int sg[6];
 };
 
@@ -86,8 +85,6 @@ constexpr LatticeIdx lattice_idx(int d) {
   for (int e = 0; e < 6; ++e) {
     const int gi = u[pairs[e][0]], gj = u[pairs[e][1]];
     L.q[e] = gi == gj ? gi : (gi + gj == 1) ? 3 : (gi + gj == 2) ? 4 : 5;
-    L.ai[e] = gi;
-    L.aj[e] = gj;
     L.sg[e] = sgn[pairs[e][0]] * sgn[pairs[e][1]];
   }
   return L;
@@ -179,8 +176,8 @@ __global__ __launch_bounds__(64 * kGroups) void batch_plane_kernel(
        // one per use)
       const int dn = done[i];
       asm volatile("" ::"s"(dn), "s"(st[la]), "s"(st[lb]), "s"(st[fr.m]), "s"(I.n[la]),
-                   "s"(I.n[fr.m]), "s"(I.count), "s"(I.h[0]), "s"(I.h[1]), "s"(I.h[2]),
-                   "s"(on_T ? I.T : I.Y), "s"(lay == kLayX ? I.MX : I.MY));
+                   "s"(I.n[fr.m]), "s"(I.count), "s"(on_T ? I.T : I.Y),
+                   "s"(lay == kLayX ? I.MX : I.MY));
       if (dn) continue;  // uniform over the block
     }
     const int na = I.n[la];
@@ -208,8 +205,7 @@ __global__ __launch_bounds__(64 * kGroups) void batch_plane_kernel(
     const R told = act ? tload : R(0), sl = act ? sload : R(1);
     R Lv[6];
 #pragma unroll
-    for (int e = 0; e < 6; ++e)
-      Lv[e] = (R(LI.sg[e]) * Mv[e]) * I.h[LI.ai[e]] * I.h[LI.aj[e]];
+    for (int e = 0; e < 6; ++e) Lv[e] = LI.sg[e] > 0 ? Mv[e] : -Mv[e];  // sg * P_q
     LatticeN<R> L;
     L.A = Lv[0];
     L.B = Lv[1];
@@ -410,7 +406,6 @@ int batch_solve(hipStream_t user_st, int count, const fold3d_instance *inst, int
         b.stride[kLayX][a] = GX.stride[a];
         b.stride[kLayY][a] = GY.stride[a];
         b.n[a] = x.n[a];
-        b.h[a] = (R)x.h[a];
       }
       const long long tiles = (long long)((x.n[0] + 2 + kBTile - 1) / kBTile) *
                               ((x.n[1] + 2 + kBTile - 1) / kBTile) * (x.n[2] + 2);
@@ -497,8 +492,8 @@ int batch_solve(hipStream_t user_st, int count, const fold3d_instance *inst, int
                              hipMemcpyHostToDevice, st));
     std::vector<int> done(count, 0);
     BATCH_HIP(hipMemcpyAsync(d_done, done.data(), sizeof(int) * count, hipMemcpyHostToDevice, st));
-    // slowness and inverse metric of every instance in both sweep layouts,
-    // once per batch
+    // slowness and lattice products of every instance in both sweep
+    // layouts, once per batch
     for (int i = 0; i < count; ++i) {
       const fold3d_instance &x = inst[i];
       double N[6];
@@ -506,8 +501,8 @@ int batch_solve(hipStream_t user_st, int count, const fold3d_instance *inst, int
       launch_metric_prep<R>(st, hinst[i].count, x.n, hinst[i].stride[kLayZ],
                             hinst[i].stride[kLayX], hinst[i].stride[kLayY],
                             (const R *)x.speed_field, (R)x.speed, (const R *)x.theta,
-                            (const R *)x.phi, (const R *)x.ra, (const R *)x.rb, N, hinst[i].MX,
-                            hinst[i].MY);
+                            (const R *)x.phi, (const R *)x.ra, (const R *)x.rb, N, x.h,
+                            hinst[i].MX, hinst[i].MY);
     }
     BATCH_HIP(hipGetLastError());
     BATCH_HIP(hipStreamSynchronize(st));

@@ -318,18 +318,39 @@ __device__ __forceinline__ R lat_dot_c(const LatticeN<R> &L) {
   return acc;
 }
 
+// The six products P_ab = (N_ab h_a) h_b of N with the spacings of the global
+// axes a <= b, in the symmetric storage order (xx, yy, zz, xy, xz, yz).  A
+// frame's lattice products are these up to sign, and every per-node-metric
+// kernel forms them this way (the metric fields hold them), so all of them
+// give each other's bits.
+template <typename R>
+__host__ __device__ inline void lattice_products(const R N[6], const R h[3], R P[6]) {
+  P[0] = (N[0] * h[0]) * h[0];
+  P[1] = (N[1] * h[1]) * h[1];
+  P[2] = (N[2] * h[2]) * h[2];
+  P[3] = (N[3] * h[0]) * h[1];
+  P[4] = (N[4] * h[0]) * h[2];
+  P[5] = (N[5] * h[1]) * h[2];
+}
+
+// Lattice products of a frame (axes u, signs sg, spacings hl in frame order)
+// from N: sg_i sg_j P_(u_i u_j).
 template <typename R>
 __device__ __forceinline__ LatticeN<R> lattice_of(const R N[6], const int u[3], const R sg[3],
                                                   const R hl[3]) {
   const R Ng[3][3] = {{N[0], N[3], N[4]}, {N[3], N[1], N[5]}, {N[4], N[5], N[2]}};
-  auto nl = [&](int i, int j) { return sg[i] * sg[j] * Ng[u[i]][u[j]]; };
+  auto nl = [&](int i, int j) {
+    const R n = Ng[u[i]][u[j]];
+    const R p = u[i] <= u[j] ? (n * hl[i]) * hl[j] : (n * hl[j]) * hl[i];
+    return (sg[i] * sg[j]) * p;
+  };
   LatticeN<R> L;
-  L.A = nl(0, 0) * hl[0] * hl[0];
-  L.B = nl(1, 1) * hl[1] * hl[1];
-  L.C = nl(2, 2) * hl[2] * hl[2];
-  L.E = nl(0, 1) * hl[0] * hl[1];
-  L.Fx = nl(0, 2) * hl[0] * hl[2];
-  L.Fy = nl(1, 2) * hl[1] * hl[2];
+  L.A = nl(0, 0);
+  L.B = nl(1, 1);
+  L.C = nl(2, 2);
+  L.E = nl(0, 1);
+  L.Fx = nl(0, 2);
+  L.Fy = nl(1, 2);
   return L;
 }
 

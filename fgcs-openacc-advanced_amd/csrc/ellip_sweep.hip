@@ -378,22 +378,22 @@ int ellip_sweeps_split_engine(hipStream_t st, R *T, R *Y, R *VX, R *VY, const Gr
 
 // ---------------------------------------------------------------------------
 // Metric engine: per-node metric (tilt and axis-ratio fields).  The slowness
-// and the six entries of N = M^-1 are computed once per solve straight into
-// seven fields of the x-sweep layout and seven of the y-sweep layout
-// (metric_prep_kernel, metric_prep.h); the plane steps are the split
-// kernel's (LDS rows, units over kGroups waves) with the node's lattice
-// products read from those fields.  Same bits as ellip_plane_kernel with
-// kMetric.
+// and the six products P_ab = (N_ab h_a) h_b of N = M^-1 with the spacings
+// (lattice_products) are computed once per solve straight into seven fields
+// of the x-sweep layout and seven of the y-sweep layout (metric_prep_kernel,
+// metric_prep.h); the plane steps are the split kernel's (LDS rows, units
+// over kGroups waves) with the node's lattice products read from those
+// fields.  Same bits as ellip_plane_kernel with kMetric.
 // ---------------------------------------------------------------------------
-// Where each lattice product of one direction comes from: L.X = (sg * N_q) * h1 * h2.
+// Where each lattice product of one direction comes from: L.X = sg * P_q.
 template <typename R>
 struct LatticeMap {
-  const R *src[6];  // N_q field in the current layout, for A, B, C, E, Fx, Fy
-  R sg[6], h1[6], h2[6];
+  const R *src[6];  // P_q field in the current layout, for A, B, C, E, Fx, Fy
+  R sg[6];
 };
 
 template <typename R>
-static LatticeMap<R> lattice_map(int d, const R *fields, long long count, const double h[3]) {
+static LatticeMap<R> lattice_map(int d, const R *fields, long long count) {
   const Frame &fr = kFrames[d];
   const int u[3] = {fr.ux, fr.uy, fr.m};
   const int sgn[3] = {fr.sx, fr.sy, fr.s};
@@ -407,8 +407,6 @@ static LatticeMap<R> lattice_map(int d, const R *fields, long long count, const 
     else q = (gi + gj == 1) ? 3 : (gi + gj == 2) ? 4 : 5;
     M.src[e] = fields + (1 + q) * count;
     M.sg[e] = R(sgn[i] * sgn[j]);
-    M.h1[e] = (R)h[gi];
-    M.h2[e] = (R)h[gj];
   }
   return M;
 }
@@ -466,12 +464,12 @@ __global__ __launch_bounds__(64 * kGroups) void ellip_metric_kernel(
   rs.store(s, grp * 64 + tx);
   const R told = active ? tload : R(0), sl = active ? sload : R(1);
   LatticeN<R> L;
-  L.A = (Lm.sg[0] * Lv[0]) * Lm.h1[0] * Lm.h2[0];
-  L.B = (Lm.sg[1] * Lv[1]) * Lm.h1[1] * Lm.h2[1];
-  L.C = (Lm.sg[2] * Lv[2]) * Lm.h1[2] * Lm.h2[2];
-  L.E = (Lm.sg[3] * Lv[3]) * Lm.h1[3] * Lm.h2[3];
-  L.Fx = (Lm.sg[4] * Lv[4]) * Lm.h1[4] * Lm.h2[4];
-  L.Fy = (Lm.sg[5] * Lv[5]) * Lm.h1[5] * Lm.h2[5];
+  L.A = Lm.sg[0] * Lv[0];
+  L.B = Lm.sg[1] * Lv[1];
+  L.C = Lm.sg[2] * Lv[2];
+  L.E = Lm.sg[3] * Lv[3];
+  L.Fx = Lm.sg[4] * Lv[4];
+  L.Fy = Lm.sg[5] * Lv[5];
   __syncthreads();
   R best = told;
   if (active) {
@@ -520,7 +518,7 @@ static void ellip_metric_direction(hipStream_t st, R *T, const R *fields, const 
                                    const double h[3], unsigned long long *max_dt, int groups) {
   const Frame &fr = kFrames[d];
   const int n = G.n[fr.m];
-  const LatticeMap<R> Lm = lattice_map<R>(d, fields, G.count, h);
+  const LatticeMap<R> Lm = lattice_map<R>(d, fields, G.count);
   for (int q = 0; q < n - 1; ++q) {
     const int p = fr.s > 0 ? 1 + q : n - q;
     const PlaneGeom g = plane_geom(G, d, p);
@@ -546,7 +544,7 @@ int ellip_sweeps_metric_engine(hipStream_t st, R *T, R *scratch, const Grid &G,
   sym3_inverse(m.M, N);
   launch_metric_prep<R>(st, count, G.n, GZ.stride, GX.stride, GY.stride,
                         (const R *)m.speed_field, (R)m.speed, (const R *)m.theta,
-                        (const R *)m.phi, (const R *)m.ra, (const R *)m.rb, N, MX, MY);
+                        (const R *)m.phi, (const R *)m.ra, (const R *)m.rb, N, h, MX, MY);
   F3D_HIP(hipGetLastError());
   FieldMover<R> mv(st, T, Y, GZ);
   int it = 0;

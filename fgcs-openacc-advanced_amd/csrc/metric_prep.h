@@ -1,9 +1,9 @@
 // metric_prep.h — the per-node metric of the elliptic model, written once
 // per solve into the two layouts the sweeps read it in.
 //
-// For each node: the slowness 1/v and the six entries of N = M^-1 (from the
-// tilt and axis-ratio fields, or the model's constant metric), seven fields
-// in all.  The x-sweeps read them in the x-sweep layout (y fastest, then z,
+// For each node: the slowness 1/v and the six products P_ab = (N_ab h_a) h_b
+// of N = M^-1 (from the tilt and axis-ratio fields, or the model's constant
+// metric) with the spacings (lattice_products), seven fields in all.  The x-sweeps read them in the x-sweep layout (y fastest, then z,
 // then x), the z- and y-sweeps in the y-sweep layout (x fastest, then z, then
 // y).  The inputs are in the reference layout (x, y, z).  A block takes one
 // 32x32 tile of one (x, y) plane, evaluates each of its nodes once, writes the
@@ -15,7 +15,7 @@
 
 #include <hip/hip_runtime.h>
 
-#include "ellip_local.h"  // tilted_inverse_metric
+#include "ellip_local.h"  // tilted_inverse_metric, lattice_products
 
 namespace fold3d {
 
@@ -32,7 +32,7 @@ template <typename R>
 __global__ __launch_bounds__(kPrepTile *kPrepRows) void metric_prep_kernel(
     MetricPrep g, const R *__restrict__ V, R speed, const R *__restrict__ th,
     const R *__restrict__ ph, const R *__restrict__ ra, const R *__restrict__ rb, R Nc0, R Nc1,
-    R Nc2, R Nc3, R Nc4, R Nc5, R *__restrict__ MX, R *__restrict__ MY) {
+    R Nc2, R Nc3, R Nc4, R Nc5, R hx, R hy, R hz, R *__restrict__ MX, R *__restrict__ MY) {
   constexpr int kQ = kPrepTile / kPrepRows;  // nodes per thread
   __shared__ R tile[kPrepTile][kPrepTile + 1];  // [y - y0][x - x0]
   const long long z = blockIdx.x;
@@ -62,8 +62,8 @@ __global__ __launch_bounds__(kPrepTile *kPrepRows) void metric_prep_kernel(
     R N[6] = {Nc0, Nc1, Nc2, Nc3, Nc4, Nc5};
     if (th) tilted_inverse_metric<R>(tv[q], pv[q], av[q], bv[q], N);
     val[q][0] = R(1) / vv[q];
-#pragma unroll
-    for (int k = 0; k < 6; ++k) val[q][1 + k] = N[k];
+    const R h[3] = {hx, hy, hz};
+    lattice_products<R>(N, h, &val[q][1]);
     if (x < g.hx && y < g.hy) {
       const long long iy = x * g.sy[0] + y * g.sy[1] + z * g.sy[2];
 #pragma unroll
@@ -87,13 +87,13 @@ __global__ __launch_bounds__(kPrepTile *kPrepRows) void metric_prep_kernel(
 
 // Launch for one grid: `ref`, `lx` and `ly` are the strides of the reference,
 // x-sweep and y-sweep layouts; N holds the constant metric's inverse (used
-// when th is null).
+// when th is null); h the grid spacings.
 template <typename R>
 inline void launch_metric_prep(hipStream_t st, long long count, const int n[3],
                                const long long ref[3], const long long lx[3],
                                const long long ly[3], const R *V, R speed, const R *th,
-                               const R *ph, const R *ra, const R *rb, const double N[6], R *MX,
-                               R *MY) {
+                               const R *ph, const R *ra, const R *rb, const double N[6],
+                               const double h[3], R *MX, R *MY) {
   MetricPrep g;
   g.count = count;
   g.hx = n[0] + 2;
@@ -106,7 +106,8 @@ inline void launch_metric_prep(hipStream_t st, long long count, const int n[3],
   }
   const dim3 grid(g.hz, (g.hx + kPrepTile - 1) / kPrepTile, (g.hy + kPrepTile - 1) / kPrepTile);
   metric_prep_kernel<R><<<grid, dim3(kPrepTile, kPrepRows), 0, st>>>(
-      g, V, speed, th, ph, ra, rb, (R)N[0], (R)N[1], (R)N[2], (R)N[3], (R)N[4], (R)N[5], MX, MY);
+      g, V, speed, th, ph, ra, rb, (R)N[0], (R)N[1], (R)N[2], (R)N[3], (R)N[4], (R)N[5], (R)h[0],
+      (R)h[1], (R)h[2], MX, MY);
 }
 
 }  // namespace fold3d

@@ -42,8 +42,8 @@ int ellip_sweeps_split_engine(hipStream_t st, R *T, R *Y, R *VX, R *VY, const Gr
                               double tol, int groups, unsigned long long *d_max,
                               int *iters_out, double *max_dt_out);
 
-// Per-node metric (tilt and axis-ratio fields): slowness and N = M^-1 hoisted
-// into kMetricFields fields in the x- and in the y-sweep layout once per
+// Per-node metric (tilt and axis-ratio fields): slowness and the lattice
+// products of N = M^-1 hoisted into kMetricFields fields in the x- and in the y-sweep layout once per
 // solve, layouts + LDS rows + units over `groups` (1, 2, 4) waves.  `scratch`
 // holds ellip_metric_scratch_fields() fields of G.count elements (a T copy
 // and the two sets of metric fields).

@@ -407,14 +407,22 @@ def tilted_field_run(torch, n=1280):
         torch.cuda.empty_cache()
         groups, tune_ms = solver.autotune(T0, n, spacing=W.h, model=W.model(), iterations=1)
         log(f"bench: tilted field: autotuner picked {groups} waves per node row ({tune_ms:.0f} ms/iteration)")
-        T = T0
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        st = solver.solve(T, n, spacing=W.h, model=W.model(), max_iters=200, tol=1e-6)
-        torch.cuda.synchronize()
-        wall = time.perf_counter() - t0
+        # three solves from the same start; the median one is reported (one
+        # solve right after the autotuner has read up to 6 % slow)
+        T = torch.empty_like(T0)
+        runs = []
+        for _ in range(3):
+            T.copy_(T0)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            st = solver.solve(T, n, spacing=W.h, model=W.model(), max_iters=200, tol=1e-6)
+            torch.cuda.synchronize()
+            runs.append((st.sweep_ms, time.perf_counter() - t0, st))
+        runs.sort(key=lambda r: r[0])
+        _, wall, st = runs[1]
         out = {"n": n, "dtype": "f32", "iterations": st.iterations, "max_dT": st.max_dT,
-               "sweep_ms": st.sweep_ms, "wall_s": wall, "groups": groups,
+               "sweep_ms": st.sweep_ms, "sweep_ms_runs": [r[0] for r in runs], "wall_s": wall,
+               "groups": groups,
                "value": st.cell_updates / (st.sweep_ms / 1e3) / 1e9, "unit": "Gcell-updates/s",
                "cell_updates": st.cell_updates,
                # 7 x 4 B: T read and write, v, theta, phi, ra, rb per node
@@ -425,7 +433,7 @@ def tilted_field_run(torch, n=1280):
                # the metric engine's scratch: a T copy and the seven metric
                # fields in each of the two sweep layouts
                "solver_scratch_bytes": 15 * (n + 2) ** 3 * 4}
-        del W, T0, T
+        del W, T0, T, runs
     finally:
         solver.close()
         torch.cuda.empty_cache()

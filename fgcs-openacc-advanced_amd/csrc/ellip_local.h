@@ -367,9 +367,12 @@ template <typename R>
 __device__ __forceinline__ R plane_gap(const LatticeN<R> &L, R s) {
   const R det = L.A * L.B - L.E * L.E;
   if (!(det > R(1e-6) * L.A * L.B)) return R(0);
-  const R q = L.C - (L.B * L.Fx * L.Fx - R(2) * L.E * L.Fx * L.Fy + L.A * L.Fy * L.Fy) / det;
+  // (the hardware reciprocal and square root: a bound is all that is needed,
+  // and their few ulps lie far inside the shrink)
+  const R q =
+      L.C - (L.B * L.Fx * L.Fx - R(2) * L.E * L.Fx * L.Fy + L.A * L.Fy * L.Fy) * lat_rcp(det);
   if (!(q > R(1e-3) * L.C)) return R(0);
-  return (sizeof(R) == 8 ? R(0.999) : R(0.9)) * s * sqrt(q);
+  return (sizeof(R) == 8 ? R(0.999) : R(0.9)) * s * lat_sqrt(q);
 }
 
 // Unit U of the node (triangle U, spoke U, rim U) with constants from L.

@@ -62,7 +62,7 @@ PMC_ROW_TRAFFIC = {
                      "on the kernel before its loads were issued ahead of the first wait (the "
                      "same bytes; 25.46 us per launch then, 24.0 us now)",
                      "profiles/metric_tilted_field_1280_f32_pmc.md"),
-    "batch": ((79928.0 * 2.0 + 4453.3) * 1024, 50.16,
+    "batch": ((79909.3 * 2.0 + 4453.3) * 1024, 49.41,
               "the 1914 launches of iteration 1 of one batch of all 96 instances (bytes per "
               "launch are their mean)", "profiles/batch_plane_96inst_f32_pmc.md"),
     "adjoint": ((25560.1 * 2.0 + 4285.3) * 1024, 14.39,

@@ -60,7 +60,7 @@ PMC_ROW_TRAFFIC = {
     "tilted_field": ((48087.4 * 2.0 + 2531.8) * 1024, 25.46,
                      "iteration 1 of the solve (ellip_metric_kernel<float, true, 1>), counted "
                      "on the kernel before its loads were issued ahead of the first wait (the "
-                     "same bytes; 25.46 us per launch then, 24.0 us now)",
+                     "same bytes; 25.46 us per launch then, 22.9 us now)",
                      "profiles/metric_tilted_field_1280_f32_pmc.md"),
     "batch": ((79909.3 * 2.0 + 4453.3) * 1024, 49.41,
               "the 1914 launches of iteration 1 of one batch of all 96 instances (bytes per "

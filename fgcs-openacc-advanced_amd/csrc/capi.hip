@@ -260,6 +260,7 @@ int fold3d_solve(fold3d_handle *h, void *T, int nx, int ny, int nz, double dx,
   F3D_HIP(hipEventRecord(h->ev0, st));
   int rc;
   unsigned long long *done_ctr = nullptr;  // cell-updates computed by locked iterations
+  int locked_from = 0;                     // first locked iteration of this solve
   if (model->kind == FOLD3D_MODEL_FOLD && o.engine == FOLD3D_ENGINE_PLANE) {
     rc = o.precision == FOLD3D_F64
              ? fold_sweeps_plane_engine<double>(st, (double *)dT, G, *model, hsp, o.max_iters,
@@ -290,6 +291,7 @@ int fold3d_solve(fold3d_handle *h, void *T, int nx, int ny, int nz, double dx,
       const bool lock = o.engine == FOLD3D_ENGINE_LOCKED ||
                         (o.engine == FOLD3D_ENGINE_AUTO && o.max_iters > kAutoLockFrom);
       const int lock_from = o.engine == FOLD3D_ENGINE_LOCKED ? h->lock_from : kAutoLockFrom;
+      locked_from = lock_from;
       // scratch: T copy [, line-chunk stamps, computed-updates counter]
       const size_t tbytes = (bytes + 255) & ~(size_t)255;
       const size_t sbytes = (line_stamp_words(G) * sizeof(int) + 255) & ~(size_t)255;
@@ -372,7 +374,7 @@ int fold3d_solve(fold3d_handle *h, void *T, int nx, int ny, int nz, double dx,
       for (int k = 0; k < 256; ++k) sum += (double)done[k];
       // the counters hold what the locked iterations computed; the iterations
       // before the lock start computed everything
-      const int unlocked = iters < h->lock_from ? iters : h->lock_from;
+      const int unlocked = iters < locked_from ? iters : locked_from;
       stats->updates_computed = w * unlocked + sum;
     }
     const bool ellip = model->kind == FOLD3D_MODEL_ELLIP;

@@ -217,7 +217,9 @@ def test_fixed_point_128_matches_reference_digest(solver, golden_large, engine):
     assert st.iterations == int(golden_large["fixed_iters_128"][0])
     assert matches_digest(T.cpu().numpy(), golden_large, "fixed_128")
     if engine == fold3d.ENGINE_AUTO:
-        assert st.updates_computed < st.cell_updates
+        # the first 24 iterations compute every update, the locked ones fewer
+        w = st.cell_updates / st.iterations
+        assert 24 * w <= st.updates_computed < st.cell_updates
         T = to_dev(fold3d.implicit_initialiser(n))
         st = solver.solve(T, n, max_iters=24, engine=engine)  # no locking within 24
         assert st.updates_computed == st.cell_updates

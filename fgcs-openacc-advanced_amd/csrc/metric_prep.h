@@ -3,9 +3,9 @@
 //
 // For each node: the slowness 1/v and the six products P_ab = (N_ab h_a) h_b
 // of N = M^-1 (from the tilt and axis-ratio fields, or the model's constant
-// metric) with the spacings (lattice_products), seven fields in all.  The x-sweeps read them in the x-sweep layout (y fastest, then z,
-// then x), the z- and y-sweeps in the y-sweep layout (x fastest, then z, then
-// y).  The inputs are in the reference layout (x, y, z).  A block takes one
+// metric) with the spacings (lattice_products), seven fields in all.  The
+// x-sweeps read them in the x-sweep layout (y fastest, then z, then x), the
+// z- and y-sweeps in the y-sweep layout (x fastest, then z, then y).  The inputs are in the reference layout (x, y, z).  A block takes one
 // 32x32 tile of one (x, y) plane, evaluates each of its nodes once, writes the
 // tile into the y-sweep layout directly (x stays fastest) and, through LDS,
 // into the x-sweep layout.  The plane z is the fastest block index, so the

@@ -390,6 +390,9 @@ __global__ __launch_bounds__(256) void adjoint_plane_kernel(const AdjSrc<R> *__r
     unsigned long long m = in_mask[X];
     if (m != 0ull) {
       R acc = res[X];
+      // the node's own value, read with the residual rather than after the
+      // sum (only this thread writes it; in-plane in-edges read the snapshot)
+      const R old = mu[X];
       // decode up to kBatch in-edges first so their loads are in flight together,
       // then add them in mask order (the same order, so the same bits, as one
       // at a time); a node with more in-edges finishes one at a time
@@ -433,7 +436,6 @@ __global__ __launch_bounds__(256) void adjoint_plane_kernel(const AdjSrc<R> *__r
         const R w = q == 0 ? S.w0[Z] : q == 1 ? S.w1[Z] : S.w2[Z];
         acc += w * (along == 0 ? snap : mu)[Z];
       }
-      const R old = mu[X];
       mu[X] = acc;
       d = fabs(acc - old);
     }

@@ -440,7 +440,7 @@ __global__ __launch_bounds__(256) void adjoint_plane_kernel(const AdjSrc<R> *__r
       d = fabs(acc - old);
     }
   }
-  const double v = wave_max_dt((double)d);
+  const double v = wave_max_dt(d);
   if ((threadIdx.x & 63) == 0 && v > 0.0)  // the source's word for this block
     atomicMax(&maxd[zsrc * kAdjMaxSlots + (blockIdx.x + 7u * blockIdx.y) % kAdjMaxSlots],
               (unsigned long long)__double_as_longlong(v));

@@ -237,7 +237,7 @@ __global__ __launch_bounds__(64 * kGroups) void batch_plane_kernel(
         if (mn != told) T[t] = mn;
         d = told - mn;
       }
-      const double vmax = wave_max_dt((double)d);
+      const double vmax = wave_max_dt(d);
       if (tx == 0 && vmax > 0.0)  // the instance's word for this block (see kBatchMaxSlots)
         atomicMax(&max_dt[(long long)i * kBatchMaxSlots + blockIdx.x % kBatchMaxSlots],
                   (unsigned long long)__double_as_longlong(vmax));

@@ -78,6 +78,15 @@ __device__ __forceinline__ double wave_max_dt(double d) {
   for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, 64));
   return v;
 }
+// FP32 fields: the maximum taken in float and widened once (widening is exact
+// and monotone, so the result is the one of the double form).
+__device__ __forceinline__ double wave_max_dt(float d) {
+  if (__builtin_amdgcn_ballot_w64(d > 0.0f) == 0) return 0.0;
+  float v = d;
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v = fmaxf(v, __shfl_xor(v, off, 64));
+  return (double)v;
+}
 __device__ __forceinline__ void max_dt_update(unsigned long long *slots, double v) {
   const unsigned b = blockIdx.x + 7u * blockIdx.y + 13u * blockIdx.z;
   atomicMax(&slots[b % kMaxSlots], (unsigned long long)__double_as_longlong(v));
@@ -116,7 +125,8 @@ struct RowStage {
       const int idx = thread + q * kThreads;
       const int la = idx % 66, lb = idx / 66;
       ok[q] = idx < kStage && a0 + la <= na + 1;
-      v[q] = T[base + (long long)(ok[q] ? a0 + la : a0) * Sa + (long long)(b - 1 + (ok[q] ? lb : 0)) * Sb];
+      v[q] = T[base + (long long)(ok[q] ? a0 + la : a0) * Sa +
+               (long long)(b - 1 + (ok[q] ? lb : 0)) * Sb];
     }
   }
   __device__ __forceinline__ void store(R (&s)[3][66], int thread) const {

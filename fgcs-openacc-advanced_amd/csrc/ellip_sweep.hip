@@ -58,7 +58,7 @@ __global__ __launch_bounds__(kLaneE *kRowsE) void ellip_plane_kernel(
     if (kReduce) d = told - tn;
   }
   if (kReduce) {
-    const double v = wave_max_dt((double)d);
+    const double v = wave_max_dt(d);
     if ((threadIdx.x & 63) == 0 && v > 0.0)
       max_dt_update(max_dt, v);
   }
@@ -292,7 +292,7 @@ __global__ __launch_bounds__(64 * kGroups) void ellip_split_kernel(
       if (kReduce) d = told - m;
     }
     if (kReduce) {
-      const double v = wave_max_dt((double)d);
+      const double v = wave_max_dt(d);
       if (tx == 0 && v > 0.0) max_dt_update(max_dt, v);
     }
   }
@@ -496,7 +496,7 @@ __global__ __launch_bounds__(64 * kGroups) void ellip_metric_kernel(
       if (kReduce) d = told - m;
     }
     if (kReduce) {
-      const double v = wave_max_dt((double)d);
+      const double v = wave_max_dt(d);
       if (tx == 0 && v > 0.0) max_dt_update(max_dt, v);
     }
   }

@@ -71,7 +71,7 @@ __global__ __launch_bounds__(kLaneX *kRowsY) void fold_plane_kernel(
   if (kReduce) {
     // wavefront max, then one atomic per wave (|dT| >= 0, so the IEEE bit
     // pattern orders like an unsigned integer)
-    const double v = wave_max_dt((double)d);
+    const double v = wave_max_dt(d);
     if ((threadIdx.x & 63) == 0 && v > 0.0)
       max_dt_update(max_dt, v);
   }
@@ -199,7 +199,7 @@ __global__ __launch_bounds__(kLaneX *kRowsY) void fold_plane_fast_kernel(
     if (kReduce) d = fabs(tn - told);
   }
   if (kReduce) {
-    const double v = wave_max_dt((double)d);
+    const double v = wave_max_dt(d);
     if ((threadIdx.x & 63) == 0 && v > 0.0)
       max_dt_update(max_dt, v);
   }
@@ -304,7 +304,7 @@ __global__ __launch_bounds__(kTileA *kTileB) void fold_tile_kernel(
     if (kReduce) d = fabs(tn - told);
   }
   if (kReduce) {
-    const double v = wave_max_dt((double)d);
+    const double v = wave_max_dt(d);
     if (tx == 0 && v > 0.0) max_dt_update(max_dt, v);
   }
 }
@@ -540,7 +540,7 @@ __device__ __forceinline__ void split_rowgroup(R *__restrict__ T, const PlaneGeo
       if (kReduce) d = fabs(res - told);
     }
     if (kReduce) {
-      const double v = wave_max_dt((double)d);
+      const double v = wave_max_dt(d);
       if (tx == 0 && v > 0.0) max_dt_update(max_dt, v);
     }
   }

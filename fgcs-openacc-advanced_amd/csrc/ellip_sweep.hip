@@ -241,7 +241,18 @@ __device__ __forceinline__ void ellip_dispatch_units(int grp, R &best, const R p
   }
 }
 
-template <typename R, bool kReduce, bool kSpeed, int kGroups>
+// x = 1 / x in place (the slowness field from the speed field; the same
+// correctly rounded division the plane kernel does with kSpeed == 1).
+template <typename R>
+__global__ void reciprocal_kernel(R *__restrict__ p, long long n) {
+  for (long long k = blockIdx.x * (long long)blockDim.x + threadIdx.x; k < n;
+       k += (long long)gridDim.x * blockDim.x)
+    p[k] = R(1) / p[k];
+}
+
+// kSpeed: 0 the model's constant speed, 1 a speed field V (slowness 1 / V per
+// node), 2 a slowness field (already 1 / V, made once per solve).
+template <typename R, bool kReduce, int kSpeed, int kGroups>
 __global__ __launch_bounds__(64 * kGroups) void ellip_split_kernel(
     R *__restrict__ T, const R *__restrict__ V, PlaneGeom g, EllipArgs<R> A,
     unsigned long long *__restrict__ max_dt) {
@@ -263,7 +274,9 @@ __global__ __launch_bounds__(64 * kGroups) void ellip_split_kernel(
   rs.load(T, g.base, g.Sa, g.Sb, a0, b, g.na, grp * 64 + tx);
   rs.store(s, grp * 64 + tx);
   const R told = active ? tload : R(0);
-  const R sl = kSpeed ? (active ? R(1) / vload : R(1)) : R(1) / A.speed;
+  const R sl = kSpeed == 2   ? (active ? vload : R(1))
+               : kSpeed == 1 ? (active ? R(1) / vload : R(1))
+                             : R(1) / A.speed;
   __syncthreads();
   R best = told;
   if (active) {
@@ -298,7 +311,7 @@ __global__ __launch_bounds__(64 * kGroups) void ellip_split_kernel(
   }
 }
 
-template <typename R, bool kSpeed, int kGroups>
+template <typename R, int kSpeed, int kGroups>
 static void ellip_split_launch(hipStream_t st, R *T, const R *V, const PlaneGeom &g,
                                const EllipArgs<R> &A, unsigned long long *max_dt) {
   const dim3 block(64, kGroups);
@@ -312,20 +325,24 @@ static void ellip_split_launch(hipStream_t st, R *T, const R *V, const PlaneGeom
 template <typename R>
 static void ellip_split_direction(hipStream_t st, R *T, const R *V, const Grid &G, int d,
                                   const EllipArgs<R> &A, unsigned long long *max_dt,
-                                  int groups) {
+                                  int groups, bool slowness = false) {
   const Frame &fr = kFrames[d];
   const int n = G.n[fr.m];
   for (int q = 0; q < n - 1; ++q) {
     const int p = fr.s > 0 ? 1 + q : n - q;
     const PlaneGeom g = plane_geom(G, d, p);
-    if (V) {
-      if (groups == 1) ellip_split_launch<R, true, 1>(st, T, V, g, A, max_dt);
-      else if (groups == 2) ellip_split_launch<R, true, 2>(st, T, V, g, A, max_dt);
-      else ellip_split_launch<R, true, 4>(st, T, V, g, A, max_dt);
+    if (V && slowness) {
+      if (groups == 1) ellip_split_launch<R, 2, 1>(st, T, V, g, A, max_dt);
+      else if (groups == 2) ellip_split_launch<R, 2, 2>(st, T, V, g, A, max_dt);
+      else ellip_split_launch<R, 2, 4>(st, T, V, g, A, max_dt);
+    } else if (V) {
+      if (groups == 1) ellip_split_launch<R, 1, 1>(st, T, V, g, A, max_dt);
+      else if (groups == 2) ellip_split_launch<R, 1, 2>(st, T, V, g, A, max_dt);
+      else ellip_split_launch<R, 1, 4>(st, T, V, g, A, max_dt);
     } else {
-      if (groups == 1) ellip_split_launch<R, false, 1>(st, T, V, g, A, max_dt);
-      else if (groups == 2) ellip_split_launch<R, false, 2>(st, T, V, g, A, max_dt);
-      else ellip_split_launch<R, false, 4>(st, T, V, g, A, max_dt);
+      if (groups == 1) ellip_split_launch<R, 0, 1>(st, T, V, g, A, max_dt);
+      else if (groups == 2) ellip_split_launch<R, 0, 2>(st, T, V, g, A, max_dt);
+      else ellip_split_launch<R, 0, 4>(st, T, V, g, A, max_dt);
     }
   }
 }
@@ -339,9 +356,11 @@ int ellip_sweeps_split_engine(hipStream_t st, R *T, R *Y, R *VX, R *VY, const Gr
   for (int d = 0; d < 6; ++d) args[d] = ellip_args<R>(d, m, h);
   const Grid GX = G.with_order(kOrderX), GZ = G.with_order(kOrderZ), GY = G.with_order(kOrderY);
   const R *V = (const R *)m.speed_field;
-  if (V) {  // the speed field in the x- and y-sweep layouts, once per solve
+  if (V) {  // the slowness 1 / V in the x- and y-sweep layouts, once per solve
     permute_layout<R>(st, V, GZ, VX, GX);
     permute_layout<R>(st, V, GZ, VY, GY);
+    reciprocal_kernel<R><<<2048, 256, 0, st>>>(VX, G.count);
+    reciprocal_kernel<R><<<2048, 256, 0, st>>>(VY, G.count);
   }
   FieldMover<R> mv(st, T, Y, GZ);
   int it = 0;
@@ -353,16 +372,24 @@ int ellip_sweeps_split_engine(hipStream_t st, R *T, R *Y, R *VX, R *VY, const Gr
     // slowest, the z- and y-sweeps with y slowest (the reference layout in the
     // last iteration of a fixed-count solve)
     mv.to(GX);
-    ellip_split_direction<R>(st, mv.field(), V ? VX : nullptr, GX, FOLD3D_X_DOWN, args[FOLD3D_X_DOWN], md, groups);
-    ellip_split_direction<R>(st, mv.field(), V ? VX : nullptr, GX, FOLD3D_X_UP, args[FOLD3D_X_UP], md, groups);
+    ellip_split_direction<R>(st, mv.field(), V ? VX : nullptr, GX, FOLD3D_X_DOWN,
+                             args[FOLD3D_X_DOWN], md, groups, true);
+    ellip_split_direction<R>(st, mv.field(), V ? VX : nullptr, GX, FOLD3D_X_UP,
+                             args[FOLD3D_X_UP], md, groups, true);
     const bool in_z = tol <= 0 && it == max_iters - 1;
     const Grid &GP = in_z ? GZ : GY;
+    // (the reference layout's pass reads the caller's speed field: V, not 1 / V)
     const R *VP = V ? (in_z ? V : VY) : nullptr;
+    const bool slow = !in_z;
     mv.to(GP);
-    ellip_split_direction<R>(st, mv.field(), VP, GP, FOLD3D_K_UP, args[FOLD3D_K_UP], md, groups);
-    ellip_split_direction<R>(st, mv.field(), VP, GP, FOLD3D_K_DOWN, args[FOLD3D_K_DOWN], md, groups);
-    ellip_split_direction<R>(st, mv.field(), VP, GP, FOLD3D_Y_UP, args[FOLD3D_Y_UP], md, groups);
-    ellip_split_direction<R>(st, mv.field(), VP, GP, FOLD3D_Y_DOWN, args[FOLD3D_Y_DOWN], md, groups);
+    ellip_split_direction<R>(st, mv.field(), VP, GP, FOLD3D_K_UP, args[FOLD3D_K_UP], md, groups,
+                             slow);
+    ellip_split_direction<R>(st, mv.field(), VP, GP, FOLD3D_K_DOWN, args[FOLD3D_K_DOWN], md,
+                             groups, slow);
+    ellip_split_direction<R>(st, mv.field(), VP, GP, FOLD3D_Y_UP, args[FOLD3D_Y_UP], md, groups,
+                             slow);
+    ellip_split_direction<R>(st, mv.field(), VP, GP, FOLD3D_Y_DOWN, args[FOLD3D_Y_DOWN], md,
+                             groups, slow);
     F3D_HIP(hipGetLastError());
     ++it;
     if (md) {

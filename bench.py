@@ -51,12 +51,12 @@ FP32_BOUND_TEXT = ("signs and reached set equal; relative error < 1e-2 everywher
 # launch, the kernel's average launch in the kernel trace of the same command
 # in us, which launches the passes counted, the profile file).
 PMC_ROW_TRAFFIC = {
-    "blob_speed_f64": ((3961.6 * 2.0 + 697.4) * 1024, 7.89,
-                       "iteration 1, the autotuner's 2-wave trial (no max|dT| reduction)",
-                       "profiles/ellip_split_f64_512_pmc.md"),
-    "blob_speed_f32": ((2373.6 * 2.0 + 371.1) * 1024, 6.24,
-                       "iteration 1, the autotuner's 2-wave trial (no max|dT| reduction)",
-                       "profiles/ellip_split_f32_512_pmc.md"),
+    "blob_speed_f64": ((3963.0 * 2.0 + 742.3) * 1024, 8.28,
+                       "iteration 1 of the solve, 2 waves per row (ellip_split_kernel<double, "
+                       "true, 2, 2>)", "profiles/ellip_split_f64_512_pmc.md"),
+    "blob_speed_f32": ((2372.6 * 2.0 + 416.0) * 1024, 6.57,
+                       "iteration 1 of the solve, 2 waves per row (ellip_split_kernel<float, "
+                       "true, 2, 2>)", "profiles/ellip_split_f32_512_pmc.md"),
     "tilted_field": ((48087.4 * 2.0 + 2531.8) * 1024, 25.46,
                      "iteration 1 of the solve (ellip_metric_kernel<float, true, 1>), counted "
                      "on the kernel before its loads were issued ahead of the first wait (the "
@@ -128,15 +128,20 @@ def time_solves(solver, T0d, n, steps, warmup, dtype, torch, engine=0, iters=8, 
     return (time.perf_counter() - t0) * 1e3, T, sweep_ms
 
 
-def ellip_run(solver, n, dtype, tol, torch, threads=0, cpu_full=False):
+def ellip_run(solver, n, dtype, tol, torch, threads=0, cpu_full=False, groups=0):
     """The 512^3 tilted-metric, Gaussian-blob speed workload, to tolerance.
     With `threads`, the FP64 run also times the elliptic oracle on the host:
     its first iteration (bounded sample), or with `cpu_full` the whole solve
-    to the same tolerance."""
+    to the same tolerance.  `groups` (1, 2 or 4 waves per node row) replaces
+    the autotuner's pick, so that profiled runs all see the same kernel."""
+    import fold3d
     from fold3d import workloads as wl
     W = wl.blob_speed(n, xp=torch, dtype=dtype, device="cuda")
     T0 = wl.point_source_start(W, xp=torch, dtype=dtype, device="cuda")
-    groups, _ = solver.autotune(T0, n, spacing=W.h, model=W.model(), iterations=1)
+    if groups:
+        solver.set_tuning(groups, fold3d.MODEL_ELLIP)
+    else:
+        groups, _ = solver.autotune(T0, n, spacing=W.h, model=W.model(), iterations=1)
     T = T0.clone()
     st = solver.solve(T, n, spacing=W.h, model=W.model(), max_iters=200, tol=tol)  # warm
     T.copy_(T0)
@@ -673,6 +678,9 @@ def main():
                     help="skip the 128^3 runs")
     ap.add_argument("--groups", type=int, default=0, choices=[0, 2, 4, 8],
                     help="waves per node row of the fold split kernel (0: autotune)")
+    ap.add_argument("--ellip-groups", type=int, default=0, choices=[0, 1, 2, 4],
+                    help="waves per node row of the elliptic split kernel in the blob-speed "
+                         "rows (0: autotune)")
     ap.add_argument("--no-fixpt", action="store_true",
                     help="skip the solves to the exact fixed point (split and locked engines)")
     ap.add_argument("--field-n", "--c2-n", dest="field_n", type=int, default=1280,
@@ -838,7 +846,7 @@ def main():
         for name, dt, tol in (("fp64", torch.float64, 1e-10), ("fp32", torch.float32, 1e-6)):
             log(f"bench: elliptic blob-speed workload {n}^3 {name}, tol {tol}")
             r, Te = ellip_run(solver, n, dt, tol, torch, threads=cpu_threads,
-                              cpu_full=args.cpu_full)
+                              cpu_full=args.cpu_full, groups=args.ellip_groups)
             ellip[name] = r
             if name == "fp64":
                 ref64 = Te
